@@ -1,0 +1,386 @@
+/* oracle_raycast.c -- TEST INFRASTRUCTURE ONLY (the oracle; see r3d_oracle.h).
+ *
+ * Host transcription of the cube-mode raycaster, src/shaders/raycast.slang
+ * main() (lines 505-1039) specialised to R3D_MODE 0, over a linear mip
+ * pyramid. The texture unit's filtering is replaced by the pinned FP32
+ * sequence below (the hardware's sub-texel rounding is not reproducible):
+ *
+ *   tri(level, p):  t = clamp(p * dim - 0.5, 0, dim - 1) per axis,
+ *                   i0 = (int)t, i1 = min(i0 + 1, dim - 1), f = t - i0;
+ *                   c00 = lerp(V[z0][y0][x0], V[z0][y0][x1], fx) ... (values
+ *                   are the raw u8 as floats), then lerp over y, then z
+ *                   (the order of tests/test_gpu.c:33-49).
+ *   sample(p, lod): d = (int)lod, w = lod - d; s = tri(d); if (w > 0 and
+ *                   d + 1 < nlev) s = lerp(s, tri(d + 1), w); return s * K255
+ *                   (LINEAR mip filter, src/vk/vkbackend.c:1058-1069).
+ *   tf(v):          t = clamp(v * 256 - 0.5, 0, 255), i0 = (int)t,
+ *                   i1 = min(i0 + 1, 255), lerp per channel
+ *                   (tests/test_gpu.c:51-56).
+ *   occ(p):         nearest texel floor(p * odim) clamped, value * K255.
+ *   lerp(a,b,t) = fmaf(t, b - a, a); dot = (ax*bx + ay*by) + az*bz unfused;
+ *   length = sqrtf(dot); normalize(v) = v / length(v); min/max are selects.
+ *   K255 = 1.0f / 255.0f rounded once.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle_math.h"
+#include "r3d_oracle.h"
+
+#define K255 (1.0f / 255.0f)
+#define MAX_STEPS 4096u      /* raycast.slang:120 */
+#define OCC_BLOCK_F 8.0f     /* raycast.slang:125 */
+
+enum { MODE_FULL = 0, MODE_MIP = 1, MODE_DEPTH = 2, MODE_HEATMAP = 3, MODE_RAYDIR = 4, MODE_FLAT = 5 };
+
+typedef struct { float x, y, z; } v3f;
+static inline v3f mk(float x, float y, float z) { v3f r = {x, y, z}; return r; }
+static inline v3f vadd(v3f a, v3f b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3f vsub(v3f a, v3f b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3f vscale(v3f a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline float vdot(v3f a, v3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline float vlen(v3f a) { return sqrtf(vdot(a, a)); }
+static inline v3f vnorm(v3f a) { float l = vlen(a); return mk(a.x / l, a.y / l, a.z / l); }
+
+/* ---- sampling ---------------------------------------------------------- */
+
+static float tri(const r3do_volume *V, uint32_t l, v3f p) {
+  const uint32_t nx = V->dims[l][0], ny = V->dims[l][1], nz = V->dims[l][2];
+  float tx = om_clamp(p.x * (float)nx - 0.5f, 0.0f, (float)(nx - 1));
+  float ty = om_clamp(p.y * (float)ny - 0.5f, 0.0f, (float)(ny - 1));
+  float tz = om_clamp(p.z * (float)nz - 0.5f, 0.0f, (float)(nz - 1));
+  uint32_t x0 = (uint32_t)tx, y0 = (uint32_t)ty, z0 = (uint32_t)tz;
+  uint32_t x1 = x0 + 1 < nx ? x0 + 1 : nx - 1;
+  uint32_t y1 = y0 + 1 < ny ? y0 + 1 : ny - 1;
+  uint32_t z1 = z0 + 1 < nz ? z0 + 1 : nz - 1;
+  float fx = tx - (float)x0, fy = ty - (float)y0, fz = tz - (float)z0;
+  const uint8_t *L = V->lev[l];
+#define VV(Z, Y, X) ((float)L[((size_t)(Z) * ny + (Y)) * nx + (X)])
+  float c00 = om_lerp(VV(z0, y0, x0), VV(z0, y0, x1), fx);
+  float c10 = om_lerp(VV(z0, y1, x0), VV(z0, y1, x1), fx);
+  float c01 = om_lerp(VV(z1, y0, x0), VV(z1, y0, x1), fx);
+  float c11 = om_lerp(VV(z1, y1, x0), VV(z1, y1, x1), fx);
+#undef VV
+  return om_lerp(om_lerp(c00, c10, fy), om_lerp(c01, c11, fy), fz);
+}
+
+/* vol_sample for cube mode: volumes[0].SampleLevel(p, lod) (raycast.slang:448). */
+static float vol_sample(const r3do_volume *V, v3f p, float lod) {
+  uint32_t d = (uint32_t)lod;
+  float w = lod - (float)d;
+  float s = tri(V, d, p);
+  if (w > 0.0f && d + 1 < V->nlev) s = om_lerp(s, tri(V, d + 1, p), w);
+  return s * K255;
+}
+
+static float occ_sample(const r3do_volume *V, v3f p) {
+  float q[3] = {p.x, p.y, p.z};
+  uint32_t idx[3];
+  for (int a = 0; a < 3; a++) {
+    float f = floorf(q[a] * (float)V->odim[a]);
+    float hi = (float)(V->odim[a] - 1);
+    f = om_clamp(f, 0.0f, hi);
+    idx[a] = (uint32_t)f;
+  }
+  uint8_t m = V->occ[((size_t)idx[2] * V->odim[1] + idx[1]) * V->odim[0] + idx[0]];
+  return (float)m * K255;
+}
+
+static void tf_sample(const float *tf, float v, float out[4]) {
+  float t = om_clamp(v * 256.0f - 0.5f, 0.0f, 255.0f);
+  uint32_t i0 = (uint32_t)t;
+  uint32_t i1 = i0 + 1 < 256 ? i0 + 1 : 255;
+  float f = t - (float)i0;
+  for (int c = 0; c < 4; c++) out[c] = om_lerp(tf[i0 * 4 + c], tf[i1 * 4 + c], f);
+}
+
+/* ---- ray setup (raycast.slang:469-495, common.slang:45-52) ------------- */
+
+static v3f to_volume_dir(const r3do_frame_params *pc, v3f d) {
+  return mk(vdot(mk(pc->vol_r0[0], pc->vol_r1[0], pc->vol_r2[0]), d),
+            vdot(mk(pc->vol_r0[1], pc->vol_r1[1], pc->vol_r2[1]), d),
+            vdot(mk(pc->vol_r0[2], pc->vol_r1[2], pc->vol_r2[2]), d));
+}
+static v3f to_volume_point(const r3do_frame_params *pc, v3f p, v3f c) {
+  v3f q = vsub(vsub(p, mk(pc->vol_tx, pc->vol_ty, pc->vol_tz)), c);
+  return vadd(to_volume_dir(pc, q), c);
+}
+
+static void pix_ndc(const r3do_frame_params *pc, uint32_t px, uint32_t py, float *nx, float *ny) {
+  float u = ((float)px + 0.5f) / (float)pc->viewport[0];
+  float v = ((float)py + 0.5f) / (float)pc->viewport[1];
+  *nx = u * 2.0f - 1.0f;
+  *ny = 1.0f - v * 2.0f;
+}
+
+/* ---- one pixel ---------------------------------------------------------- */
+
+static void march(const r3do_volume *V, const float *tf, const r3do_frame_params *pc,
+                  float cutoff, uint32_t px, uint32_t py, float out[4], uint32_t *nsteps,
+                  uint64_t *nskip, uint64_t *nshade) {
+  const v3f fwd = mk(pc->cam_forward[0], pc->cam_forward[1], pc->cam_forward[2]);
+  const v3f right = mk(pc->cam_right[0], pc->cam_right[1], pc->cam_right[2]);
+  const v3f up = mk(pc->cam_up[0], pc->cam_up[1], pc->cam_up[2]);
+  const int ortho = (pc->view_flags & 1u) != 0u;
+  float ndx, ndy;
+  pix_ndc(pc, px, py, &ndx, &ndy);
+  v3f dir = ortho ? vnorm(fwd) : vnorm(vadd(vadd(fwd, vscale(right, ndx)), vscale(up, ndy)));
+  *nsteps = 0;
+  if (pc->mode == MODE_RAYDIR) {
+    out[0] = dir.x * 0.5f + 0.5f; out[1] = dir.y * 0.5f + 0.5f; out[2] = dir.z * 0.5f + 0.5f;
+    out[3] = 1.0f;
+    return;
+  }
+  uint32_t md = V->dims[0][0];
+  if (V->dims[0][1] > md) md = V->dims[0][1];
+  if (V->dims[0][2] > md) md = V->dims[0][2];
+  const float voxel = 1.0f / (float)md;
+  const float max_lod = om_min(pc->max_mip, (float)(V->nlev - 1));
+  const v3f ext = mk(1.0f, 1.0f, 1.0f);
+  v3f org = mk(pc->cam_origin[0], pc->cam_origin[1], pc->cam_origin[2]);
+  if (ortho) org = vadd(vadd(org, vscale(right, ndx)), vscale(up, ndy));
+  const v3f ro = to_volume_point(pc, org, vscale(ext, 0.5f));
+  dir = to_volume_dir(pc, dir);
+
+  /* box_hit against [0, ext] (raycast.slang:489-495) */
+  v3f inv = mk(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+  v3f t0 = mk((0.0f - ro.x) * inv.x, (0.0f - ro.y) * inv.y, (0.0f - ro.z) * inv.z);
+  v3f t1 = mk((ext.x - ro.x) * inv.x, (ext.y - ro.y) * inv.y, (ext.z - ro.z) * inv.z);
+  v3f tmin = mk(om_min(t0.x, t1.x), om_min(t0.y, t1.y), om_min(t0.z, t1.z));
+  v3f tmax = mk(om_max(t0.x, t1.x), om_max(t0.y, t1.y), om_max(t0.z, t1.z));
+  float hx = om_max(om_max(tmin.x, tmin.y), tmin.z);
+  float hy = om_min(om_min(tmax.x, tmax.y), tmax.z);
+  float tn = om_max(hx, 0.0f), tfar = om_min(hy, 1e30f);
+  if (tn >= tfar) {
+    out[0] = 0.02f; out[1] = 0.02f; out[2] = 0.03f; out[3] = 1.0f;
+    return;
+  }
+  /* raycast.slang:733-765 */
+  const float cone = 2.0f * vlen(up) / (float)pc->viewport[1];
+  const float dt = pc->step_voxels * voxel;
+  const float jitter = om_hash12(px, py, pc->frame_index);
+  float t = tn + jitter * dt;
+  float mip_max = 0.0f, first_hit = -1.0f;
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  uint32_t steps = 0;
+  const int composite = pc->mode == MODE_FULL || pc->mode == MODE_FLAT;
+  const int shaded = pc->mode == MODE_FULL;
+  const float span_step = om_max((tfar - tn) / ((float)MAX_STEPS * dt), 1.0f);
+  const float bs = OCC_BLOCK_F * voxel;
+  const float occ_gate = om_max(pc->skip_gate, 0.5f / 255.0f);
+  const float opac_k = pc->density * pc->step_voxels;
+  const v3f lvec = vscale(dir, -1.0f);
+  const v3f hvec = vnorm(vsub(lvec, dir));
+  const v3f key = mk(0x1.a20bd8p-2f, 0x1.a20bd8p-1f, -0x1.a20bd8p-2f); /* normalize(0.4,0.8,-0.4) */
+  const v3f sgn = mk(dir.x > 0.0f ? 1.0f : 0.0f, dir.y > 0.0f ? 1.0f : 0.0f, dir.z > 0.0f ? 1.0f : 0.0f);
+
+  while (t < tfar && steps < MAX_STEPS) {
+    steps++;
+    float foot = ortho ? cone : t * cone;
+    float lod = om_clamp(om_log2f(om_max(foot / voxel, 1.0f)) + pc->lod_bias, 0.0f, max_lod);
+    float lod_step = om_max(om_exp2f(lod), span_step);
+    v3f p = vadd(ro, vscale(dir, t));
+    float v = vol_sample(V, p, lod);
+    int cut = v < pc->threshold;
+    if (cut) v = 0.0f;
+    /* empty-space skip (raycast.slang:866-869) */
+    if (v < occ_gate && occ_sample(V, p) < occ_gate) {
+      v3f nextp = mk((floorf(p.x / bs) + sgn.x) * bs, (floorf(p.y / bs) + sgn.y) * bs,
+                     (floorf(p.z / bs) + sgn.z) * bs);
+      float tx = fabsf(dir.x) > 1e-8f ? (nextp.x - ro.x) / dir.x : 1e30f;
+      float ty = fabsf(dir.y) > 1e-8f ? (nextp.y - ro.y) / dir.y : 1e30f;
+      float tz = fabsf(dir.z) > 1e-8f ? (nextp.z - ro.z) / dir.z : 1e30f;
+      t = om_max(om_min(om_min(tx, ty), tz), t) + bs * 0.01f;
+      (*nskip)++;
+      continue;
+    }
+    if (v > mip_max) mip_max = v;
+    if (pc->mode == MODE_MIP && mip_max >= 0.999f) break;
+    if (first_hit < 0.0f && v > 0.05f) first_hit = t;
+    if (composite && !cut) {
+      float s[4];
+      tf_sample(tf, v, s);
+      float a = 1.0f - om_pow_unit(1.0f - s[3], opac_k * lod_step);
+      if (a > 0.003f) {
+        float col[3] = {s[0], s[1], s[2]};
+        if (shaded) {
+          (*nshade)++;
+          float e = voxel * lod_step;
+          float gx = vol_sample(V, mk(p.x + e, p.y, p.z), lod) - vol_sample(V, mk(p.x - e, p.y, p.z), lod);
+          float gy = vol_sample(V, mk(p.x, p.y + e, p.z), lod) - vol_sample(V, mk(p.x, p.y - e, p.z), lod);
+          float gz = vol_sample(V, mk(p.x, p.y, p.z + e), lod) - vol_sample(V, mk(p.x, p.y, p.z - e), lod);
+          v3f g = mk(gx, gy, gz);
+          float gm = vlen(g);
+          if (gm > 1e-4f) {
+            v3f n = mk(g.x / gm, g.y / gm, g.z / gm);
+            float diff = 0.7f * fabsf(vdot(n, lvec)) + 0.3f * fabsf(vdot(n, key));
+            float spec = om_pow24(fabsf(vdot(n, hvec))) * 0.3f;
+            float lit = 0.25f + 0.75f * diff;
+            float w = om_sat(gm * 6.0f);
+            float m = om_lerp(1.0f, lit, w);
+            float sw = spec * w;
+            col[0] = col[0] * m + sw;
+            col[1] = col[1] * m + sw;
+            col[2] = col[2] * m + sw;
+          }
+        }
+        float wa = (1.0f - acc[3]) * a;
+        acc[0] = acc[0] + wa * col[0];
+        acc[1] = acc[1] + wa * col[1];
+        acc[2] = acc[2] + wa * col[2];
+        acc[3] = acc[3] + wa;
+        if (acc[3] >= cutoff) break;
+      }
+    }
+    t += dt * lod_step;
+  }
+  *nsteps = steps;
+  if (pc->mode == MODE_HEATMAP) {
+    float x = (float)steps / 1024.0f;
+    out[0] = x; out[1] = 1.0f - fabsf(x - 0.5f) * 2.0f; out[2] = 1.0f - x;
+  } else if (pc->mode == MODE_DEPTH) {
+    float d = first_hit < 0.0f ? 0.0f : 1.0f - om_sat(first_hit / 2.0f);
+    out[0] = d; out[1] = d; out[2] = d;
+  } else if (composite) {
+    float ta = 1.0f - acc[3];
+    out[0] = acc[0] + ta * 0.02f; out[1] = acc[1] + ta * 0.02f; out[2] = acc[2] + ta * 0.03f;
+  } else {
+    float m = mip_max * pc->density;
+    out[0] = m; out[1] = m; out[2] = m;
+  }
+  out[3] = 1.0f;
+}
+
+static int view_supported(const r3do_frame_params *pc) {
+  /* cube mode only: slab / clip / bricks / virtual-slab / surf views are the
+   * other compile-time specialisations (vkbackend.c:7125-7130) */
+  if (pc->slab_grid || pc->clip_valid || pc->brick_mode) return 0;
+  if (pc->view_flags & 2u) return 0;
+  if (pc->mode > MODE_FLAT) return 0;
+  if (pc->viewport[0] == 0 || pc->viewport[1] == 0) return 0;
+  return 1;
+}
+
+int r3do_raycast(const r3do_volume *vol, const float *tf, const r3do_frame_params *views,
+                 uint32_t nviews, float alpha_cutoff, float *img, uint32_t img_w, uint32_t img_h,
+                 uint32_t *px_steps, uint32_t row0, uint32_t row_step, int nthreads,
+                 r3do_counters *cnt) {
+  for (uint32_t v = 0; v < nviews; v++)
+    if (!view_supported(&views[v])) return -1;
+  if (row_step == 0) row_step = 1;
+  if (nthreads < 1) nthreads = 1;
+  uint64_t tot_s = 0, tot_k = 0, tot_h = 0, tot_r = 0;
+  for (uint32_t v = 0; v < nviews; v++) {
+    const r3do_frame_params *pc = &views[v];
+    const int half = (pc->view_flags & 32u) != 0u;
+    const uint32_t ox = pc->view_org & 0xffffu, oy = pc->view_org >> 16;
+    const uint32_t W = pc->viewport[0], H = pc->viewport[1];
+    const uint32_t TW = half ? (W + 1) / 2 : W, TH = half ? (H + 1) / 2 : H;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(+ : tot_s, tot_k, tot_h, tot_r)
+    for (int64_t ty = 0; ty < (int64_t)TH; ty++) {
+      if ((uint32_t)ty % row_step != row0 % row_step) continue;
+      for (uint32_t tx = 0; tx < TW; tx++) {
+        uint32_t lx = half ? tx * 2 : tx, ly = half ? (uint32_t)ty * 2 : (uint32_t)ty;
+        if (lx >= W || ly >= H) continue;
+        float c[4];
+        uint32_t ns = 0;
+        uint64_t nk = 0, nh = 0;
+        march(vol, tf, pc, alpha_cutoff, lx, ly, c, &ns, &nk, &nh);
+        tot_s += ns; tot_k += nk; tot_h += nh; tot_r += 1;
+        for (uint32_t dy = 0; dy < (half ? 2u : 1u); dy++)
+          for (uint32_t dx = 0; dx < (half ? 2u : 1u); dx++) {
+            uint32_t qx = lx + dx, qy = ly + dy;
+            if (qx >= W || qy >= H) continue;
+            uint32_t gx = ox + qx, gy = oy + qy;
+            if (gx >= img_w || gy >= img_h) continue;
+            memcpy(img + ((size_t)gy * img_w + gx) * 4, c, sizeof c);
+            if (px_steps) px_steps[(size_t)gy * img_w + gx] = ns;
+          }
+      }
+    }
+  }
+  if (cnt) {
+    cnt->samples = tot_s; cnt->skips = tot_k; cnt->shaded = tot_h; cnt->rays = tot_r;
+  }
+  return 0;
+}
+
+/* ---- pyramid and occupancy --------------------------------------------- */
+
+void r3do_downsample(const uint8_t *src, uint32_t nx, uint32_t ny, uint32_t nz, uint8_t *dst) {
+  uint32_t mx = nx > 1 ? nx / 2 : 1, my = ny > 1 ? ny / 2 : 1, mz = nz > 1 ? nz / 2 : 1;
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < (int64_t)mz; z++)
+    for (uint32_t y = 0; y < my; y++)
+      for (uint32_t x = 0; x < mx; x++) {
+        uint32_t sum = 4u;
+        for (uint32_t dz = 0; dz < 2; dz++)
+          for (uint32_t dy = 0; dy < 2; dy++)
+            for (uint32_t dx = 0; dx < 2; dx++) {
+              uint32_t sx = 2 * x + dx, sy = 2 * y + dy, sz = 2 * (uint32_t)z + dz;
+              if (sx >= nx) sx = nx - 1;
+              if (sy >= ny) sy = ny - 1;
+              if (sz >= nz) sz = nz - 1;
+              sum += src[((size_t)sz * ny + sy) * nx + sx];
+            }
+        dst[((size_t)z * my + y) * mx + x] = (uint8_t)(sum / 8u);
+      }
+}
+
+void r3do_occupancy(const uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint8_t *occ) {
+  uint32_t ox = (nx + 7) / 8, oy = (ny + 7) / 8, oz = (nz + 7) / 8;
+  size_t n = (size_t)ox * oy * oz;
+  uint8_t *raw = malloc(n);
+  if (!raw) return;
+#pragma omp parallel for schedule(static)
+  for (int64_t bz = 0; bz < (int64_t)oz; bz++)
+    for (uint32_t by = 0; by < oy; by++)
+      for (uint32_t bx = 0; bx < ox; bx++) {
+        uint8_t m = 0;
+        for (uint32_t z = (uint32_t)bz * 8; z < (uint32_t)bz * 8 + 8 && z < nz; z++)
+          for (uint32_t y = by * 8; y < by * 8 + 8 && y < ny; y++)
+            for (uint32_t x = bx * 8; x < bx * 8 + 8 && x < nx; x++) {
+              uint8_t v = vol[((size_t)z * ny + y) * nx + x];
+              if (v > m) m = v;
+            }
+        raw[((size_t)bz * oy + by) * ox + bx] = m;
+      }
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < (int64_t)oz; z++)
+    for (uint32_t y = 0; y < oy; y++)
+      for (uint32_t x = 0; x < ox; x++) {
+        uint8_t m = 0;
+        for (int dz = -1; dz <= 1; dz++)
+          for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+              int64_t zz = z + dz, yy = (int64_t)y + dy, xx = (int64_t)x + dx;
+              if (zz < 0 || yy < 0 || xx < 0 || zz >= oz || yy >= oy || xx >= ox) continue;
+              uint8_t v = raw[((size_t)zz * oy + (size_t)yy) * ox + (size_t)xx];
+              if (v > m) m = v;
+            }
+        occ[((size_t)z * oy + y) * ox + x] = m;
+      }
+  free(raw);
+}
+
+/* ---- exported math ------------------------------------------------------ */
+
+float r3do_log2f(float x) { return om_log2f(x); }
+float r3do_exp2f(float x) { return om_exp2f(x); }
+float r3do_pow_unit(float b, float y) { return om_pow_unit(b, y); }
+float r3do_hash12(uint32_t px, uint32_t py, uint32_t frame) { return om_hash12(px, py, frame); }
+
+void r3do_math_batch(const float *x, uint32_t n, uint32_t fn, float *out) {
+  for (uint32_t i = 0; i < n; i++) {
+    switch (fn) {
+    case 0: out[i] = om_log2f(x[i]); break;
+    case 1: out[i] = om_exp2f(x[i]); break;
+    case 2: out[i] = om_pow_unit(x[i], 1.7f); break;
+    case 3: out[i] = sqrtf(x[i]); break;
+    case 4: out[i] = 1.0f / x[i]; break;
+    case 5: out[i] = om_pow24(x[i]); break;
+    default: { float s, c; om_sincos_turns(x[i], &s, &c); out[i] = fn == 6 ? s : c; } break;
+    }
+  }
+}

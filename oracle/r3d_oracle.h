@@ -1,0 +1,161 @@
+/* r3d_oracle.h -- TEST INFRASTRUCTURE ONLY: the CPU oracle.
+ *
+ * A plain-C restatement of the reference's two hot paths, used as the checker
+ * for the HIP path and as the timed CPU baseline. Only tests/, the smoke check
+ * and bench.py's cpu_baseline leg may load liboracle.so; the product never
+ * links or calls it.
+ *
+ *  - 3ddct decode back half (tools/3ddct/dct3d.c:38-154, 542-593, 625-639):
+ *    dequantise, separable inverse DCT-III, add dc, sparse corrections,
+ *    denormalise, roundf + clamp to u8. Pinned FP32 sequence: no fused
+ *    multiply-add anywhere (the reference built with -ffp-contract=off).
+ *  - The matching forward path (dct3d.c:52-59, 70-79, 106-111, 144-148,
+ *    442-472) so synthetic chunks can be encoded.
+ *  - Cube-mode raycast (src/shaders/raycast.slang:505-1039 with R3D_MODE 0,
+ *    common.slang:45-61), the mip pyramid (rounded 2x box,
+ *    tools/lodpack/main.c:216-227) and the dilated 8^3 occupancy grid
+ *    (src/vk/vkbackend.c:794-870).
+ *  - The sheet-volume generator that defines the synthetic workloads.
+ */
+#ifndef R3D_ORACLE_H
+#define R3D_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- 3ddct decode ------------------------------------------------------ */
+
+/* Per-chunk header, the decoded preamble of dct3d.c:550-554. */
+typedef struct r3do_chunk_hdr {
+  float vmin;
+  float vspan;
+  int32_t dcq; /* dc = dcq / 16 */
+  uint32_t table; /* step-table index (unused by the single-chunk calls) */
+} r3do_chunk_hdr;
+
+/* cm[k*16+n] = (float)(c_k * cos(pi*(2n+1)*k/32)) in double (dct3d.c:38-49). */
+void r3do_dct_basis(float cm[256]);
+/* 46-entry step table of dct3d.c:131-143: step[f] = qstep * powf(1+f, 0.65). */
+void r3do_step_table_qstep(float qstep, float step46[46]);
+/* The configured table q(i,j,k) = 1 + 2(i+j+k), indexed by f = i+j+k. */
+void r3do_step_table_linear(float step46[46]);
+
+/* Decode one chunk from its levels (raster (z*16+y)*16+x). Corrections are
+ * applied in order as norm[pos] += delta (dct3d.c:572-587, delta already
+ * signed and multiplied by cq). pre_round (optional) receives the FP32 values
+ * before roundf. Returns 1, or 0 when the header is invalid (dct3d.c:555-556);
+ * on 0 nothing is written. */
+int r3do_decode_chunk(const int16_t lvl[4096], const r3do_chunk_hdr *h, const float step46[46],
+                      const uint16_t *corr_pos, const float *corr_delta, uint32_t ncorr,
+                      uint8_t out[4096], float *pre_round);
+/* The FP32 pipeline pieces, for golden-vector checks. */
+void r3do_dequant(const int16_t lvl[4096], const float step46[46], float coef[4096]);
+void r3do_idct3(const float coef[4096], float blk[4096]);
+void r3do_fdct3(const float blk[4096], float coef[4096]);
+
+/* Batch decode: chunk i reads levels + i*4096 and writes out + i*4096.
+ * Corrections are CSR (corr_off[n+1]) or NULL. Runs on nthreads threads.
+ * status[i] = 1/0 as r3do_decode_chunk. */
+void r3do_decode_batch(uint32_t n, const int16_t *levels, const r3do_chunk_hdr *hdr,
+                       const float *tables, uint32_t ntables, const uint32_t *corr_off,
+                       const uint16_t *corr_pos, const float *corr_delta, uint8_t *out,
+                       int32_t *status, int nthreads);
+
+/* Encode one u8 chunk with a given step table into int16 levels, following
+ * dct3d.c ENC_BODY + encode_float up to the quantiser (no entropy coding).
+ * Levels are clipped to int16. */
+void r3do_encode_chunk(const uint8_t in[4096], const float step46[46], int16_t lvl[4096],
+                       r3do_chunk_hdr *h);
+/* Encode every 16^3 chunk of an n^3-multiple-of-16 volume, chunk order
+ * (cz*CY + cy)*CX + cx. */
+void r3do_encode_volume(const uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz,
+                        const float step46[46], int16_t *levels, r3do_chunk_hdr *hdr,
+                        int nthreads);
+
+/* ---- volume preprocessing --------------------------------------------- */
+
+/* Level l+1 from level l: (sum of the 2x2x2 box + 4) / 8, source index
+ * clamped to the level's last voxel. dims of level l+1 = max(d >> 1, 1). */
+void r3do_downsample(const uint8_t *src, uint32_t nx, uint32_t ny, uint32_t nz, uint8_t *dst);
+/* Per-8^3-block max (partial edge blocks), then 3^3 max dilation. */
+void r3do_occupancy(const uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint8_t *occ);
+
+/* ---- raycast ------------------------------------------------------------ */
+
+/* Byte-for-byte mirror of r3d_frame_params (src/render/render_types.h:47-94). */
+typedef struct r3do_frame_params {
+  float cam_origin[3]; float step_voxels;
+  float cam_right[3];  float density;
+  float cam_up[3];     float lod_bias;
+  float cam_forward[3]; float max_mip;
+  uint32_t viewport[2];
+  uint32_t mode;
+  uint32_t frame_index;
+  float threshold;
+  uint32_t slab_grid, slab_wz;
+  float slab_z0, slab_nx, slab_ny, slab_px, slab_py;
+  uint32_t slab_depth;
+  uint32_t clip_valid;
+  float clip_orig[6][2];
+  float vol_r0[3]; float vol_tx;
+  float vol_r1[3]; float vol_ty;
+  float vol_r2[3]; float vol_tz;
+  uint32_t brick_mode;
+  float skip_gate;
+  float slab_x0, slab_y0;
+  uint32_t view_org;
+  uint32_t view_flags;
+  float overlay_gain;
+  uint32_t overlay_flags;
+} r3do_frame_params;
+
+/* A mip pyramid as separate linear [z][y][x] levels. */
+typedef struct r3do_volume {
+  uint32_t nlev;
+  uint32_t dims[16][3];
+  const uint8_t *lev[16];
+  uint32_t odim[3];
+  const uint8_t *occ;
+} r3do_volume;
+
+typedef struct r3do_counters {
+  uint64_t samples; /* loop iterations (each fetches one volume sample) */
+  uint64_t skips;   /* iterations that took the empty-space jump */
+  uint64_t shaded;  /* samples that ran the 6-tap gradient */
+  uint64_t rays;    /* rays marched */
+} r3do_counters;
+
+/* Render views into an RGBA FP32 image of img_w x img_h (each view at its
+ * view_org). tf is 256x4 floats. Thread rows ty with ty % row_step == row0 are
+ * rendered (row_step 1 = all). px_steps (optional, img_w*img_h) receives the
+ * per-pixel loop-iteration count. Returns 0, or -1 for an unsupported view. */
+int r3do_raycast(const r3do_volume *vol, const float *tf, const r3do_frame_params *views,
+                 uint32_t nviews, float alpha_cutoff, float *img, uint32_t img_w, uint32_t img_h,
+                 uint32_t *px_steps, uint32_t row0, uint32_t row_step, int nthreads,
+                 r3do_counters *cnt);
+
+/* Pinned math, exported so tests can compare against libm and the GPU. */
+float r3do_log2f(float x);
+float r3do_exp2f(float x);
+float r3do_pow_unit(float b, float y);
+float r3do_hash12(uint32_t px, uint32_t py, uint32_t frame);
+void r3do_math_batch(const float *x, uint32_t n, uint32_t fn, float *out);
+
+/* ---- synthetic sheet volume ------------------------------------------ */
+
+/* Concentric wavy cylindrical sheets about the z axis through (n/2, n/2):
+ * sheet k has radius 20 + 9k + 3 sin(2 pi z / 40), is 3 voxels thick and has
+ * value 160 + off_k (off_k uniform in [-30,30]); background 20 + U[-5,5];
+ * plus N(0, 8) noise (Box-Muller); rounded, clamped to [0,255]. Writes voxels
+ * [z0, z1) of an nx x ny x nz volume (x fastest). */
+void r3do_synth_sheets(uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t nsheets,
+                       uint64_t seed, uint32_t z0, uint32_t z1, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* R3D_ORACLE_H */
