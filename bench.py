@@ -1,0 +1,212 @@
+"""Benchmark: the 1024^3 / 1920x1080 workload on one MI355X.
+
+One step is the full hot path over one synthetic batch:
+  1. decode all 262,144 16^3 chunks (dense int16 levels, step table
+     q = 1 + 2(i+j+k)) straight into level 0 of the bricked volume,
+  2. build the 4-level mip pyramid (128^3 bricks' worth of levels) and the
+     dilated occupancy grid,
+  3. raycast one 1920x1080 view with ray-cone LOD and full gradient shading.
+Inputs (the sheet volume and its encoded levels) are generated on the GPU and
+resident in HBM before the timed region.
+
+Printed JSON line: value = ray samples per second of the raycast (samples =
+loop iterations that fetch a volume sample, counted exactly by the kernel);
+fps = 1000 / raycast ms; decode_GBps = algorithmic decode bytes / decode ms;
+roofline = the decode kernel against HBM (12,288 B per chunk: 8,192 B of
+levels read + 4,096 B of voxels written); cpu_baseline = the oracle's host
+raycaster timed on a row sample of the same frame.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+CHUNK_BYTES = 8192 + 4096
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1024, help="volume edge (multiple of 16)")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--cpu-rows", type=int, default=16, help="CPU baseline renders every Nth row")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "decode_exact_1024.md"),
+                    help="rocprofv3 summary holding the decode kernel's measured HBM bytes per launch")
+    return ap.parse_args()
+
+
+def read_traffic(path):
+    """`traffic_bytes_per_launch: N` from a committed rocprof summary, or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+
+    import render3d_amd as r3d
+    from render3d_amd import scene
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    ctx = r3d.Context(0)
+    n, W, H = args.n, args.width, args.height
+    nsheets = max(1, min(48, (n // 2 - 20) // 9 + 1 if n >= 64 else 1))
+    seed = 20261203
+    stream = torch.cuda.current_stream()
+
+    # ---- inputs, resident in HBM before timing --------------------------------
+    t0 = time.time()
+    vol_lin = ctx.synth_sheets(n, n, n, nsheets, seed)
+    step_tab = torch.arange(46, dtype=torch.float32, device=dev) * 2 + 1  # q = 1 + 2(i+j+k)
+    levels, headers = ctx.encode_chunks(vol_lin, n, n, n, step_tab)
+    del vol_lin
+    nch = levels.shape[0]
+    vol = ctx.volume(n, n, n, args.levels)
+    offs, base, sy, sz = vol.chunk_targets()
+    batch = ctx.make_batch(headers=headers, step_tables=step_tab, out=base, levels=levels, out_offsets=offs,
+                           stride_y=sy, stride_z=sz)
+    lut = scene.tf_ramp_40_200()
+    ctx.set_transfer(lut)
+    cam = scene.orbit_camera(45.0, 0.0, 2.0)
+    view = scene.frame_params(cam, W, H, step_voxels=1.0, gate=scene.skip_gate(lut))
+    img = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    counters = torch.zeros(4, dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+
+    def step(timed):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record(stream)
+        ctx.decode(batch)
+        ev[1].record(stream)
+        vol.build()
+        ev[2].record(stream)
+        ctx.raycast(vol, [view], W, H, alpha_cutoff=0.99, out=img, counters=counters if timed else None)
+        ev[3].record(stream)
+        return ev
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    counters.zero_()
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    events = [step(True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t_start) * 1000.0
+    dec_ms = [e[0].elapsed_time(e[1]) for e in events]
+    pyr_ms = [e[1].elapsed_time(e[2]) for e in events]
+    ray_ms = [e[2].elapsed_time(e[3]) for e in events]
+    cnt = counters.cpu().numpy().astype(np.uint64) // np.uint64(args.steps)
+    samples, skips, shaded, rays = (int(x) for x in cnt)
+    ray_avg = float(np.mean(ray_ms))
+    dec_avg = float(np.mean(dec_ms))
+    value = samples / (ray_avg / 1000.0)
+    dec_bytes = CHUNK_BYTES * nch
+    dec_gbps = dec_bytes / (dec_avg / 1000.0) / 1e9
+
+    cpu = None
+    parity = None
+    if not args.no_cpu_baseline:
+        cpu, parity = cpu_baseline(ctx, vol, view, lut, img, args, levels, headers, step_tab, n, W, H)
+
+    out = {
+        "metric": "ray samples/s + fps @1024³ 1080p; chunk decode GB/s; adjoint/fwd time ratio",
+        "value": value,
+        "unit": "ray samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded sheet volume generated and encoded on the GPU)",
+        "config": {"workload": f"{n}^3 u8 sheet volume, {nch} 16^3 chunks dense int16, "
+                               f"{args.levels}-level pyramid, {W}x{H} FULL raycast, step 1.0, cutoff 0.99",
+                   "volume": n, "chunks": nch, "width": W, "height": H, "levels": args.levels},
+        "fps": 1000.0 / ray_avg,
+        "raycast_ms": ray_avg,
+        "decode_ms": dec_avg,
+        "pyramid_ms": float(np.mean(pyr_ms)),
+        "decode_GBps": dec_gbps,
+        "adjoint_fwd_ratio": None,
+        "samples_per_frame": samples,
+        "skips_per_frame": skips,
+        "shaded_per_frame": shaded,
+        "rays_per_frame": rays,
+        "setup_s": setup_s,
+        "roofline": {"kernel": "k_decode_exact", "bound": "hbm", "achieved": dec_gbps,
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": dec_gbps / HBM_PEAK_GBPS,
+                     "traffic": read_traffic(args.traffic)},
+        "cpu_baseline": cpu,
+        "parity_vs_oracle": parity,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def cpu_baseline(ctx, vol, view, lut, img, args, levels, headers, step_tab, n, W, H):
+    """The oracle's host raycaster on every `cpu_rows`-th row of the same frame
+    (same volume, pyramid and occupancy read back from HBM), on the host's
+    cores; plus the oracle decoder on a chunk sample."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    from render3d_amd import scene
+
+    cores = min(16, os.cpu_count() or 1)
+    lv = [((d), vol.read_level(l)) for l, d in enumerate(vol.level_dims)]
+    occ = (vol.odim, vol.read_occupancy())
+    ov = orc.OracleVolume(None, n, n, n, levels=lv, occ=occ)
+    rs = args.cpu_rows
+    t = time.perf_counter()
+    ref, cnt, _ = orc.raycast(ov, scene.tf_unorm(lut), [view], W, H, alpha_cutoff=0.99, row0=0, row_step=rs,
+                              nthreads=cores)
+    dt = time.perf_counter() - t
+    g = img.cpu().numpy()
+    rows = np.arange(0, H, rs)
+    err = float(np.max(np.abs(g[rows] - ref[rows])))
+    # decoder: oracle on a 4096-chunk sample of the same batch
+    k = min(4096, levels.shape[0])
+    idx = np.linspace(0, levels.shape[0] - 1, k).astype(np.int64)
+    lvs = levels[idx].cpu().numpy()
+    hd = headers[idx].cpu().numpy()
+    t2 = time.perf_counter()
+    dec, _ = orc.decode_batch(lvs, hd, step_tab.cpu().numpy(), nthreads=cores)
+    dt2 = time.perf_counter() - t2
+    cpu = {"value": cnt["samples"] / dt, "unit": "ray samples/s", "cores": cores, "kind": "port",
+           "sample": f"every {rs}th row of the {W}x{H} frame ({cnt['rays']} rays, {cnt['samples']} samples, "
+                     f"{dt:.1f} s)",
+           "decode_GBps": k * CHUNK_BYTES / dt2 / 1e9,
+           "decode_sample": f"{k} chunks, {dt2:.2f} s"}
+    parity = {"raycast_rows_max_abs": err, "raycast_rows": int(rows.size)}
+    return cpu, parity
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,284 @@
+/* r3d_hip.h -- C ABI of the MI355X (gfx950) path for render3d's two
+ * data-parallel hot paths: the 3ddct 16^3 chunk decoder and the cube-mode
+ * compute raycaster (plus its adjoint).
+ *
+ * Plain C: no HIP, torch or C++ types cross this boundary. Streams are passed
+ * as `void *` holding a hipStream_t (NULL = the device's null stream).
+ * Conventions:
+ *   - Every int-returning call returns R3D_OK (0) or a negative r3d_status.
+ *     (The reference's dct3d returns 1/0, tools/3ddct/dct3d.h:68-70; that
+ *     convention is not carried across this ABI.) r3d_hip_last_error() gives
+ *     the message of the last failure on a context.
+ *   - Nothing is launched for an invalid call: bad counts, sizes beyond the
+ *     stated limits, NULL required pointers, unsupported view kinds.
+ *   - Bulk data (levels, pairs, headers, outputs) are device pointers owned by
+ *     the caller; volumes and transfer tables are owned by their handles.
+ *   - A context is used by one host thread at a time (or under the caller's
+ *     lock), like the reference's single queue behind a mutex
+ *     (src/vk/vkctx.c:329-343).
+ */
+#ifndef R3D_HIP_H
+#define R3D_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status -------------------------------------------------------------- */
+enum {
+  R3D_OK = 0,
+  R3D_ERR_INVALID = -1,     /* bad argument or malformed batch; nothing launched */
+  R3D_ERR_NOMEM = -2,       /* device or host allocation failed */
+  R3D_ERR_DEVICE = -3,      /* a HIP runtime call failed */
+  R3D_ERR_UNSUPPORTED = -4, /* a view kind / variant this build does not implement */
+  R3D_ERR_RANGE = -5,       /* a size beyond a documented limit */
+};
+
+typedef struct r3d_hip_ctx r3d_hip_ctx;
+typedef struct r3d_hip_volume r3d_hip_volume;
+
+/* ---- shared POD types (byte-for-byte mirrors of the reference) ------------ */
+
+/* src/render/render_types.h:40-45 */
+typedef struct r3d_volume_desc {
+  uint32_t nx, ny, nz;
+  uint32_t brick_dim;
+  float voxel_um;
+} r3d_volume_desc;
+
+/* src/render/render_types.h:47-94 (248 bytes, mirrors FrameParams in
+ * src/shaders/common.slang:5-42). Only cube-mode views are rendered: a view
+ * with slab_grid, clip_valid or brick_mode nonzero, or the surf flag, is
+ * rejected with R3D_ERR_UNSUPPORTED. */
+typedef struct r3d_frame_params {
+  float cam_origin[3];  float step_voxels;
+  float cam_right[3];   float density;
+  float cam_up[3];      float lod_bias;
+  float cam_forward[3]; float max_mip;
+  uint32_t viewport[2];
+  uint32_t mode;
+  uint32_t frame_index;
+  float threshold;
+  uint32_t slab_grid;
+  uint32_t slab_wz;
+  float slab_z0;
+  float slab_nx, slab_ny;
+  float slab_px, slab_py;
+  uint32_t slab_depth;
+  uint32_t clip_valid;
+  float clip_orig[6][2];
+  float vol_r0[3]; float vol_tx;
+  float vol_r1[3]; float vol_ty;
+  float vol_r2[3]; float vol_tz;
+  uint32_t brick_mode;
+  float skip_gate;
+  float slab_x0, slab_y0;
+  uint32_t view_org;
+  uint32_t view_flags;
+  float overlay_gain;
+  uint32_t overlay_flags;
+} r3d_frame_params;
+
+/* render modes, render_types.h:25-33 */
+enum {
+  R3D_MODE_FULL = 0,
+  R3D_MODE_MIP = 1,
+  R3D_MODE_DEPTH = 2,
+  R3D_MODE_HEATMAP = 3,
+  R3D_MODE_RAYDIR = 4,
+  R3D_MODE_FLAT = 5,
+};
+#define R3D_VIEW_ORTHO 1u
+#define R3D_VIEW_HALF 32u
+
+/* ---- context ------------------------------------------------------------- */
+
+/* Replaces r3d_create (src/render/render.h:17) for the compute path: binds a
+ * HIP device and builds the lazily published constant tables (the DCT basis
+ * of tools/3ddct/dct3d.c:38-49). */
+int r3d_hip_init(r3d_hip_ctx **out, int device);
+/* r3d_destroy (render.h:18). */
+void r3d_hip_destroy(r3d_hip_ctx *ctx);
+const char *r3d_hip_last_error(const r3d_hip_ctx *ctx);
+/* Waits for `stream`. */
+int r3d_hip_sync(r3d_hip_ctx *ctx, void *stream);
+/* The 16x16 FP32 DCT basis the decoder uses, cm[k*16+n] (dct3d.c:38-49). */
+int r3d_hip_dct_basis(const r3d_hip_ctx *ctx, float out[256]);
+
+/* ---- 3ddct chunk decode ------------------------------------------------------ */
+
+#define R3D_CHUNK_VOXELS 4096u
+#define R3D_STEP_TABLE_LEN 46u        /* one step per L1 frequency 0..45 (dct3d.c:131-143) */
+#define R3D_MAX_CORRECTIONS 4096u     /* per chunk, as dct3d.c:575 clamps */
+
+/* Per-chunk header: the decoded preamble of dct3d.c:550-554. */
+typedef struct r3d_chunk_header {
+  float vmin;
+  float vspan;     /* > 0 */
+  int32_t dcq;     /* block mean in 1/16 units: dc = dcq / 16 */
+  uint32_t table;  /* index into step_tables */
+} r3d_chunk_header;
+
+enum {
+  R3D_COEF_DENSE_I16 = 0, /* levels[i*4096 + (z*16+y)*16+x], int16 */
+  R3D_COEF_SPARSE = 1,    /* pairs[pair_offsets[i] .. pair_offsets[i+1]) */
+};
+/* A sparse pair: bits 0..11 = raster position, bits 16..31 = int16 level.
+ * Each position appears at most once per chunk. */
+#define R3D_PAIR(pos, level) ((uint32_t)(pos) | ((uint32_t)(uint16_t)(int16_t)(level) << 16))
+
+/* One batch of chunks. All pointers are device pointers.
+ * Output chunk i: voxel (z,y,x) goes to
+ *   out + (out_offsets ? out_offsets[i] : i*4096) + z*out_stride_z + y*out_stride_y + x
+ * (strides 0 -> 256 and 16: flat 4096-byte chunks). */
+typedef struct r3d_chunk_batch {
+  uint32_t nchunks;
+  uint32_t coef_format;
+  const int16_t *levels;          /* dense */
+  const uint32_t *pair_offsets;   /* sparse: nchunks + 1 entries */
+  const uint32_t *pairs;          /* sparse */
+  uint64_t npairs;                /* sparse: entries in pairs; offsets are clamped to it */
+  const r3d_chunk_header *headers;
+  const float *step_tables;       /* ntables * 46 floats */
+  uint32_t ntables;
+  uint32_t reserved0;
+  /* optional sparse corrections (dct3d.c:572-587): per chunk, applied in
+   * order as norm[pos] += delta; delta carries the sign and cq. */
+  const uint32_t *corr_offsets;   /* nchunks + 1, or NULL */
+  const uint16_t *corr_pos;
+  const float *corr_delta;
+  uint64_t ncorr;                 /* entries in corr_pos / corr_delta */
+  uint8_t *out;
+  const uint64_t *out_offsets;    /* or NULL */
+  uint32_t out_stride_y;
+  uint32_t out_stride_z;
+  /* optional per-chunk status: 1 = decoded, 0 = invalid header (the chunk is
+   * zero-filled, as r3d_shard_chunk_decode does for a corrupt chunk,
+   * src/shard/shardio.c:170-189). */
+  int32_t *status;
+} r3d_chunk_batch;
+
+enum {
+  R3D_DECODE_EXACT_F32 = 0, /* the pinned FP32 sequence: bit-exact with the oracle */
+  R3D_DECODE_MFMA_F16 = 2,  /* FP16 MFMA, FP32 accumulate: approximate */
+};
+typedef struct r3d_decode_opts {
+  uint32_t variant;
+  uint32_t flags;
+} r3d_decode_opts;
+
+/* Decodes every chunk of the batch: dequantise (deq_one, dct3d.c:149-154),
+ * inverse 3-D DCT (dct3d.c:113-118), + dc, corrections, denormalise
+ * (dct3d.c:589-591), roundf and clamp to u8 (dct3d.c:625-639). The batched
+ * replacement of dct3d_decode_u8 (dct3d.h:80-91) behind
+ * r3d_shard_decode_region (src/shard/shardio.c:269-306) and of the GPU batch
+ * seam r3d_vkc5d_decode (src/vk/vkc5d.h:26-32). Asynchronous on `stream`. */
+int r3d_hip_decode_chunks(r3d_hip_ctx *ctx, const r3d_chunk_batch *batch,
+                          const r3d_decode_opts *opts, void *stream);
+
+/* ---- volumes --------------------------------------------------------------- */
+
+#define R3D_MAX_LEVELS 16u
+enum { R3D_SRC_HOST = 0, R3D_SRC_DEVICE = 1 };
+
+/* Allocates a bricked mip pyramid (16^3 bricks with a one-voxel apron per
+ * level) plus the dilated 8^3 occupancy grid. nlevels = 0 selects the full
+ * chain floor(log2(maxdim)) + 1 (src/vk/vkbackend.c:1517-1518). */
+int r3d_hip_volume_create(r3d_hip_ctx *ctx, const r3d_volume_desc *desc, uint32_t nlevels,
+                          r3d_hip_volume **out);
+void r3d_hip_volume_destroy(r3d_hip_volume *vol);
+/* Level count and dims (dims[l*3 + {0,1,2}] = x, y, z) and occupancy dims. */
+int r3d_hip_volume_info(const r3d_hip_volume *vol, uint32_t *nlevels, uint32_t dims[48],
+                        uint32_t odim[3]);
+/* r3d_upload_volume (render.h:20): copies linear [z][y][x] voxels (host or
+ * device) into level 0, then builds aprons, the pyramid and the occupancy. */
+int r3d_hip_volume_upload(r3d_hip_ctx *ctx, r3d_hip_volume *vol, const uint8_t *voxels,
+                          uint32_t src_kind, void *stream);
+/* Where chunk (cx,cy,cz) of level 0 lives, for decoding straight into the
+ * volume: writes one byte offset per 16^3 chunk in (cz*CY + cy)*CX + cx order
+ * into out_offsets (device, CX*CY*CZ entries) and returns the base pointer and
+ * strides to put in an r3d_chunk_batch. */
+int r3d_hip_volume_chunk_targets(r3d_hip_ctx *ctx, r3d_hip_volume *vol, uint64_t *out_offsets,
+                                 uint8_t **base, uint32_t *stride_y, uint32_t *stride_z,
+                                 void *stream);
+/* Rebuilds level-0 aprons, levels 1.. (rounded 2x box) and the occupancy from
+ * level 0 (after a decode into it). */
+int r3d_hip_volume_build(r3d_hip_ctx *ctx, r3d_hip_volume *vol, void *stream);
+/* Copies level `level` out as linear [z][y][x] voxels (host or device dst). */
+int r3d_hip_volume_read_level(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, uint32_t level,
+                              uint8_t *dst, uint32_t dst_kind, void *stream);
+int r3d_hip_volume_read_occupancy(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, uint8_t *dst,
+                                  uint32_t dst_kind, void *stream);
+
+/* ---- transfer function ------------------------------------------------------ */
+
+/* r3d_set_transfer (render.h:240): 256 RGBA8 entries, stored as unorm floats. */
+int r3d_hip_set_transfer(r3d_hip_ctx *ctx, const uint8_t rgba[256][4]);
+/* FP32 table (256 x 4, host memory), for the differentiable path. */
+int r3d_hip_set_transfer_f32(r3d_hip_ctx *ctx, const float *rgba);
+
+/* ---- raycast --------------------------------------------------------------- */
+
+/* Output image (device memory). rgba: FP32 RGBA, width*height*4 floats;
+ * rgba8: RGBA8 unorm (round(sat(c)*255)). Either may be NULL. */
+typedef struct r3d_hip_image {
+  float *rgba;
+  uint8_t *rgba8;
+  uint32_t width, height;
+} r3d_hip_image;
+
+typedef struct r3d_hip_stats {
+  uint64_t samples; /* loop iterations, each fetching one volume sample */
+  uint64_t skips;   /* iterations that took the empty-space jump */
+  uint64_t shaded;  /* samples that ran the 6-tap gradient */
+  uint64_t rays;
+} r3d_hip_stats;
+
+#define R3D_MAX_VIEWS_PER_CALL 256u
+typedef struct r3d_raycast_opts {
+  float alpha_cutoff;     /* early termination: raycast.slang:124 uses 0.98 */
+  uint32_t flags;
+  const float *tf;        /* optional device FP32 LUT (256x4); NULL = the context's */
+  uint64_t *counters;     /* optional device u64[4]: stats accumulate here */
+  uint32_t *px_steps;     /* optional device u32[width*height]: per-pixel iterations */
+} r3d_raycast_opts;
+
+/* Renders `nviews` cube-mode views (each at its view_org) into `out`: the
+ * compute dispatch of src/vk/vkbackend.c:7121-7173 running
+ * src/shaders/raycast.slang main with R3D_MODE 0. `views` is host memory.
+ * If `stats` is non-NULL the call waits for `stream` and fills it. */
+int r3d_hip_raycast(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_frame_params *views,
+                    uint32_t nviews, const r3d_hip_image *out, const r3d_raycast_opts *opts,
+                    r3d_hip_stats *stats, void *stream);
+
+/* ---- synthetic workloads ------------------------------------------------------- */
+
+/* The seeded sheet volume (linear [z][y][x], device dst): concentric wavy
+ * cylindrical sheets about the z axis, noise N(0,8). Bit-identical to the
+ * host generator of the same definition. */
+int r3d_hip_synth_sheets(r3d_hip_ctx *ctx, uint8_t *dst, uint32_t nx, uint32_t ny, uint32_t nz,
+                         uint32_t nsheets, uint64_t seed, void *stream);
+/* Forward 3-D DCT + dead-zone quantiser of every 16^3 chunk of a linear
+ * volume (dims multiples of 16), ENC_BODY + encode_float of
+ * tools/3ddct/dct3d.c:431-472 and 598-621 without the entropy stage.
+ * Writes int16 levels (chunk order (cz*CY+cy)*CX+cx) and headers (table 0). */
+int r3d_hip_encode_chunks(r3d_hip_ctx *ctx, const uint8_t *vol, uint32_t nx, uint32_t ny,
+                          uint32_t nz, const float *step_table, int16_t *levels,
+                          r3d_chunk_header *headers, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#ifdef __cplusplus
+static_assert(sizeof(r3d_frame_params) == 248, "r3d_frame_params must be 248 bytes");
+static_assert(sizeof(r3d_chunk_header) == 16, "r3d_chunk_header must be 16 bytes");
+#else
+_Static_assert(sizeof(r3d_frame_params) == 248, "r3d_frame_params must be 248 bytes");
+_Static_assert(sizeof(r3d_chunk_header) == 16, "r3d_chunk_header must be 16 bytes");
+#endif
+
+#endif /* R3D_HIP_H */

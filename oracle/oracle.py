@@ -1,0 +1,219 @@
+"""ctypes access to the CPU oracle (liboracle.so). TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker / the timed CPU baseline. The product never does.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C")
+i16p = np.ctypeslib.ndpointer(np.int16, flags="C")
+u16p = np.ctypeslib.ndpointer(np.uint16, flags="C")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C")
+VP = C.c_void_p
+
+
+class Counters(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("skips", C.c_uint64), ("shaded", C.c_uint64), ("rays", C.c_uint64)]
+
+
+class VolumeC(C.Structure):
+    _fields_ = [("nlev", C.c_uint32), ("dims", (C.c_uint32 * 3) * 16), ("lev", C.c_void_p * 16),
+                ("odim", C.c_uint32 * 3), ("occ", C.c_void_p)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+    L = C.CDLL(LIB_PATH)
+    L.r3do_dct_basis.argtypes = [f32p]
+    L.r3do_step_table_qstep.argtypes = [C.c_float, f32p]
+    L.r3do_step_table_linear.argtypes = [f32p]
+    L.r3do_decode_chunk.restype = C.c_int
+    L.r3do_decode_chunk.argtypes = [i16p, VP, f32p, VP, VP, C.c_uint32, u8p, VP]
+    L.r3do_dequant.argtypes = [i16p, f32p, f32p]
+    L.r3do_idct3.argtypes = [f32p, f32p]
+    L.r3do_fdct3.argtypes = [f32p, f32p]
+    L.r3do_decode_batch.argtypes = [C.c_uint32, i16p, VP, f32p, C.c_uint32, VP, VP, VP, u8p, VP, C.c_int]
+    L.r3do_encode_chunk.argtypes = [u8p, f32p, i16p, VP]
+    L.r3do_encode_volume.argtypes = [u8p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, i16p, VP, C.c_int]
+    L.r3do_downsample.argtypes = [u8p, C.c_uint32, C.c_uint32, C.c_uint32, u8p]
+    L.r3do_occupancy.argtypes = [u8p, C.c_uint32, C.c_uint32, C.c_uint32, u8p]
+    L.r3do_raycast.restype = C.c_int
+    L.r3do_raycast.argtypes = [C.POINTER(VolumeC), f32p, VP, C.c_uint32, C.c_float, f32p, C.c_uint32,
+                               C.c_uint32, VP, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Counters)]
+    for fn in ("r3do_log2f", "r3do_exp2f"):
+        getattr(L, fn).restype = C.c_float
+        getattr(L, fn).argtypes = [C.c_float]
+    L.r3do_pow_unit.restype = C.c_float
+    L.r3do_pow_unit.argtypes = [C.c_float, C.c_float]
+    L.r3do_hash12.restype = C.c_float
+    L.r3do_hash12.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    L.r3do_math_batch.argtypes = [f32p, C.c_uint32, C.c_uint32, f32p]
+    L.r3do_synth_sheets.argtypes = [u8p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
+                                    C.c_uint32, C.c_uint32, C.c_int]
+    _lib = L
+    return L
+
+
+def hdr_array(vmin, vspan, dcq, table=None):
+    """r3do_chunk_hdr[n] as int32 [n,4] (floats bit-cast)."""
+    n = np.size(vmin)
+    h = np.zeros((n, 4), np.int32)
+    h[:, 0] = np.asarray(vmin, np.float32).reshape(-1).view(np.int32)
+    h[:, 1] = np.asarray(vspan, np.float32).reshape(-1).view(np.int32)
+    h[:, 2] = np.asarray(dcq, np.int32).reshape(-1)
+    if table is not None:
+        h[:, 3] = np.asarray(table, np.uint32).reshape(-1).view(np.int32)
+    return h
+
+
+def basis():
+    out = np.zeros(256, np.float32)
+    load().r3do_dct_basis(out)
+    return out
+
+
+def step_linear():
+    s = np.zeros(46, np.float32)
+    load().r3do_step_table_linear(s)
+    return s
+
+
+def step_qstep(q):
+    s = np.zeros(46, np.float32)
+    load().r3do_step_table_qstep(q, s)
+    return s
+
+
+def decode_chunk(levels, vmin, vspan, dcq, step46, corr_pos=None, corr_delta=None):
+    """Returns (ok, u8[4096], pre_round f32[4096])."""
+    L = load()
+    h = hdr_array([vmin], [vspan], [dcq])
+    out = np.zeros(4096, np.uint8)
+    pre = np.zeros(4096, np.float32)
+    n = 0 if corr_pos is None else len(corr_pos)
+    cp = np.ascontiguousarray(corr_pos if n else np.zeros(1), np.uint16)
+    cd = np.ascontiguousarray(corr_delta if n else np.zeros(1), np.float32)
+    ok = L.r3do_decode_chunk(np.ascontiguousarray(levels, np.int16), h.ctypes.data, np.ascontiguousarray(step46, np.float32),
+                             cp.ctypes.data, cd.ctypes.data, n, out, pre.ctypes.data)
+    return ok, out, pre
+
+
+def decode_batch(levels, hdr, tables, corr_off=None, corr_pos=None, corr_delta=None, nthreads=8):
+    L = load()
+    levels = np.ascontiguousarray(levels, np.int16).reshape(-1)
+    n = levels.size // 4096
+    out = np.zeros(n * 4096, np.uint8)
+    status = np.zeros(n, np.int32)
+    tables = np.ascontiguousarray(tables, np.float32).reshape(-1)
+    hdr = np.ascontiguousarray(hdr, np.int32)
+    args = [None, None, None]
+    if corr_off is not None:
+        co = np.ascontiguousarray(corr_off, np.uint32)
+        cp = np.ascontiguousarray(corr_pos, np.uint16)
+        cd = np.ascontiguousarray(corr_delta, np.float32)
+        args = [co.ctypes.data, cp.ctypes.data, cd.ctypes.data]
+    L.r3do_decode_batch(n, levels, hdr.ctypes.data, tables, tables.size // 46, args[0], args[1], args[2], out,
+                        status.ctypes.data, nthreads)
+    return out.reshape(n, 4096), status
+
+
+def encode_volume(vol, nx, ny, nz, step46, nthreads=8):
+    L = load()
+    n = (nx // 16) * (ny // 16) * (nz // 16)
+    lv = np.zeros(n * 4096, np.int16)
+    hdr = np.zeros((n, 4), np.int32)
+    L.r3do_encode_volume(np.ascontiguousarray(vol, np.uint8).reshape(-1), nx, ny, nz,
+                         np.ascontiguousarray(step46, np.float32), lv, hdr.ctypes.data, nthreads)
+    return lv.reshape(n, 4096), hdr
+
+
+def synth_sheets(nx, ny, nz, nsheets, seed, nthreads=8, z0=0, z1=None):
+    out = np.zeros(nx * ny * nz, np.uint8)
+    load().r3do_synth_sheets(out, nx, ny, nz, nsheets, seed, z0, nz if z1 is None else z1, nthreads)
+    return out
+
+
+def pyramid(vol, nx, ny, nz, nlev):
+    """Levels [(dims, array)] by the rounded 2x box."""
+    L = load()
+    levels = [((nx, ny, nz), np.ascontiguousarray(vol, np.uint8).reshape(-1))]
+    for _ in range(1, nlev):
+        (a, b, c), src = levels[-1]
+        d = (max(a // 2, 1), max(b // 2, 1), max(c // 2, 1))
+        dst = np.zeros(d[0] * d[1] * d[2], np.uint8)
+        L.r3do_downsample(src, a, b, c, dst)
+        levels.append((d, dst))
+    return levels
+
+
+def occupancy(vol, nx, ny, nz):
+    o = ((nx + 7) // 8, (ny + 7) // 8, (nz + 7) // 8)
+    out = np.zeros(o[0] * o[1] * o[2], np.uint8)
+    load().r3do_occupancy(np.ascontiguousarray(vol, np.uint8).reshape(-1), nx, ny, nz, out)
+    return o, out
+
+
+def full_levels(nx, ny, nz):
+    md = max(nx, ny, nz)
+    n = 1
+    while (md >> n) >= 1 and n < 16:
+        n += 1
+    return n
+
+
+class OracleVolume:
+    """Linear levels + occupancy, kept alive for the C struct's pointers."""
+
+    def __init__(self, vol, nx, ny, nz, nlev=0, levels=None, occ=None):
+        nlev = nlev or full_levels(nx, ny, nz)
+        self.levels = levels if levels is not None else pyramid(vol, nx, ny, nz, nlev)
+        self.odim, self.occ = occ if occ is not None else occupancy(vol, nx, ny, nz)
+        s = VolumeC()
+        s.nlev = len(self.levels)
+        for l, (d, arr) in enumerate(self.levels):
+            s.dims[l][0], s.dims[l][1], s.dims[l][2] = d
+            s.lev[l] = arr.ctypes.data
+        s.odim[0], s.odim[1], s.odim[2] = self.odim
+        s.occ = self.occ.ctypes.data
+        self.c = s
+
+
+def raycast(ovol, tf_f32, views, width, height, alpha_cutoff=0.98, row0=0, row_step=1, nthreads=8,
+            px_steps=False):
+    """Returns (rgba float32 [h,w,4] (unrendered rows zero), counters dict, px_steps or None)."""
+    L = load()
+    n = len(views)
+    arr = (type(views[0]) * n)(*views)
+    img = np.zeros(height * width * 4, np.float32)
+    steps = np.zeros(height * width, np.uint32) if px_steps else None
+    cnt = Counters()
+    rc = L.r3do_raycast(C.byref(ovol.c), np.ascontiguousarray(tf_f32, np.float32).reshape(-1), C.cast(arr, VP), n,
+                        alpha_cutoff, img, width, height, None if steps is None else steps.ctypes.data, row0,
+                        row_step, nthreads, C.byref(cnt))
+    if rc != 0:
+        raise ValueError("oracle raycast: unsupported view")
+    return (img.reshape(height, width, 4),
+            {"samples": cnt.samples, "skips": cnt.skips, "shaded": cnt.shaded, "rays": cnt.rays},
+            None if steps is None else steps.reshape(height, width))
+
+
+def math_batch(x, fn):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(x)
+    load().r3do_math_batch(x, x.size, fn, out)
+    return out

@@ -259,6 +259,7 @@ static int view_supported(const r3do_frame_params *pc) {
   if (pc->view_flags & 2u) return 0;
   if (pc->mode > MODE_FLAT) return 0;
   if (pc->viewport[0] == 0 || pc->viewport[1] == 0) return 0;
+  if (!(pc->step_voxels > 0.0f) || !(pc->max_mip >= 0.0f)) return 0;
   return 1;
 }
 

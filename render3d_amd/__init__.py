@@ -1,0 +1,11 @@
+"""render3d_amd: MI355X (gfx950) HIP implementation of render3d's two
+data-parallel hot paths -- the 3ddct 16^3 chunk decoder and the cube-mode
+compute raycaster -- behind the C ABI in include/r3d_hip.h."""
+from . import scene
+from ._lib import (FrameParams, R3DError, MODE_FULL, MODE_MIP, MODE_DEPTH, MODE_HEATMAP, MODE_RAYDIR,
+                   MODE_FLAT, VIEW_ORTHO, VIEW_HALF, load as load_library, LIB_PATH)
+from .api import Context, Volume, headers_tensor, headers_from_tensor
+
+__all__ = ["scene", "FrameParams", "R3DError", "Context", "Volume", "headers_tensor", "headers_from_tensor",
+           "load_library", "LIB_PATH", "MODE_FULL", "MODE_MIP", "MODE_DEPTH", "MODE_HEATMAP", "MODE_RAYDIR",
+           "MODE_FLAT", "VIEW_ORTHO", "VIEW_HALF"]

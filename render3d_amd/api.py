@@ -1,0 +1,248 @@
+"""Python host API over the C ABI (include/r3d_hip.h).
+
+Device buffers are torch tensors on the GPU (torch is used for allocation and
+streams only); every computation runs in libr3d_hip.so. Mirrors the reference's
+render backend calls (src/render/render.h) and dct3d's decode
+(tools/3ddct/dct3d.h:80-91), batched.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (ChunkBatch, DecodeOpts, FrameParams, Image, R3DError, RaycastOpts, Stats,
+                   VolumeDesc)
+
+try:
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is not None:
+        return C.c_void_p(stream)
+    if torch is not None and torch.cuda.is_available():
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return None
+
+
+def headers_tensor(vmin, vspan, dcq, table=None, device="cuda"):
+    """r3d_chunk_header[n] as an int32 [n, 4] tensor (floats bit-cast)."""
+    n = len(vmin)
+    h = np.zeros((n, 4), np.int32)
+    h[:, 0] = np.asarray(vmin, np.float32).view(np.int32)
+    h[:, 1] = np.asarray(vspan, np.float32).view(np.int32)
+    h[:, 2] = np.asarray(dcq, np.int32)
+    h[:, 3] = 0 if table is None else np.asarray(table, np.uint32).view(np.int32)
+    return torch.from_numpy(h).to(device)
+
+
+def headers_from_tensor(h):
+    a = h.cpu().numpy()
+    return (a[:, 0].view(np.float32).copy(), a[:, 1].view(np.float32).copy(), a[:, 2].copy(),
+            a[:, 3].view(np.uint32).copy())
+
+
+class Context:
+    """r3d_hip_ctx: one device, one host thread at a time."""
+
+    def __init__(self, device=0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        rc = self.lib.r3d_hip_init(C.byref(h), device)
+        if rc != 0:
+            raise R3DError(rc, f"r3d_hip_init(device={device}) failed")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            self.lib.r3d_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != 0:
+            raise R3DError(rc, self.lib.r3d_hip_last_error(self.h).decode())
+        return rc
+
+    def sync(self, stream=None):
+        self.check(self.lib.r3d_hip_sync(self.h, _stream(stream)))
+
+    def dct_basis(self):
+        out = np.zeros(256, np.float32)
+        self.check(self.lib.r3d_hip_dct_basis(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    # ---- decode -----------------------------------------------------------
+    def make_batch(self, *, headers, step_tables, out, levels=None, pairs=None, pair_offsets=None,
+                   out_offsets=None, stride_y=0, stride_z=0, corr_offsets=None, corr_pos=None,
+                   corr_delta=None, status=None, nchunks=None):
+        b = ChunkBatch()
+        b.nchunks = int(headers.shape[0]) if nchunks is None else int(nchunks)
+        if levels is not None:
+            b.coef_format = _lib.R3D_COEF_DENSE_I16
+            b.levels = levels.data_ptr()
+        else:
+            b.coef_format = _lib.R3D_COEF_SPARSE
+            b.pairs = pairs.data_ptr()
+            b.pair_offsets = pair_offsets.data_ptr()
+            b.npairs = int(pairs.numel())
+        b.headers = headers.data_ptr()
+        b.step_tables = step_tables.data_ptr()
+        b.ntables = int(step_tables.numel() // 46)
+        if corr_offsets is not None:
+            b.corr_offsets = corr_offsets.data_ptr()
+            b.corr_pos = corr_pos.data_ptr()
+            b.corr_delta = corr_delta.data_ptr()
+            b.ncorr = int(corr_pos.numel())
+        b.out = out if isinstance(out, int) else out.data_ptr()
+        b.out_offsets = None if out_offsets is None else out_offsets.data_ptr()
+        b.out_stride_y = stride_y
+        b.out_stride_z = stride_z
+        b.status = None if status is None else status.data_ptr()
+        return b
+
+    def decode(self, batch, variant=_lib.R3D_DECODE_EXACT_F32, stream=None):
+        opts = DecodeOpts(variant, 0)
+        self.check(self.lib.r3d_hip_decode_chunks(self.h, C.byref(batch), C.byref(opts), _stream(stream)))
+
+    def decode_chunks(self, *, headers, step_tables, levels=None, pairs=None, pair_offsets=None,
+                      corr_offsets=None, corr_pos=None, corr_delta=None, status=None,
+                      variant=_lib.R3D_DECODE_EXACT_F32, stream=None):
+        """Decode into a fresh flat [n, 4096] uint8 tensor."""
+        n = int(headers.shape[0])
+        out = torch.empty((n, 4096), dtype=torch.uint8, device=headers.device)
+        b = self.make_batch(headers=headers, step_tables=step_tables, out=out, levels=levels, pairs=pairs,
+                            pair_offsets=pair_offsets, corr_offsets=corr_offsets, corr_pos=corr_pos,
+                            corr_delta=corr_delta, status=status)
+        self.decode(b, variant, stream)
+        return out
+
+    # ---- workloads ----------------------------------------------------------
+    def synth_sheets(self, nx, ny, nz, nsheets, seed, stream=None):
+        out = torch.empty(nx * ny * nz, dtype=torch.uint8, device=f"cuda:{self.device}")
+        self.check(self.lib.r3d_hip_synth_sheets(self.h, _ptr(out), nx, ny, nz, nsheets, seed, _stream(stream)))
+        return out
+
+    def encode_chunks(self, vol, nx, ny, nz, step_table, stream=None):
+        n = (nx // 16) * (ny // 16) * (nz // 16)
+        levels = torch.empty((n, 4096), dtype=torch.int16, device=vol.device)
+        hdr = torch.empty((n, 4), dtype=torch.int32, device=vol.device)
+        self.check(self.lib.r3d_hip_encode_chunks(self.h, _ptr(vol), nx, ny, nz, _ptr(step_table), _ptr(levels),
+                                                  _ptr(hdr), _stream(stream)))
+        return levels, hdr
+
+    # ---- transfer / volumes / raycast ------------------------------------
+    def set_transfer(self, lut_u8):
+        lut = np.ascontiguousarray(lut_u8, np.uint8).reshape(256, 4)
+        self.check(self.lib.r3d_hip_set_transfer(self.h, lut.ctypes.data_as(C.c_void_p)))
+
+    def set_transfer_f32(self, lut):
+        lut = np.ascontiguousarray(lut, np.float32).reshape(256, 4)
+        self.check(self.lib.r3d_hip_set_transfer_f32(self.h, lut.ctypes.data_as(C.c_void_p)))
+
+    def volume(self, nx, ny, nz, nlevels=0):
+        return Volume(self, nx, ny, nz, nlevels)
+
+    def raycast(self, vol, views, width, height, *, alpha_cutoff=0.98, rgba=True, rgba8=False, tf=None,
+                counters=None, px_steps=False, stats=False, out=None, stream=None):
+        """Render views into a [height, width, 4] float32 image (and/or RGBA8)."""
+        if isinstance(views, FrameParams):
+            views = [views]
+        arr = (FrameParams * len(views))(*views)
+        dev = f"cuda:{self.device}"
+        img = out if out is not None else (torch.empty((height, width, 4), dtype=torch.float32, device=dev)
+                                          if rgba else None)
+        img8 = torch.empty((height, width, 4), dtype=torch.uint8, device=dev) if rgba8 else None
+        steps = torch.zeros((height, width), dtype=torch.int32, device=dev) if px_steps else None
+        im = Image(img.data_ptr() if img is not None else None, img8.data_ptr() if img8 is not None else None,
+                   width, height)
+        opts = RaycastOpts(alpha_cutoff, 0, None if tf is None else tf.data_ptr(),
+                           None if counters is None else counters.data_ptr(),
+                           None if steps is None else steps.data_ptr())
+        st = Stats()
+        self.check(self.lib.r3d_hip_raycast(self.h, vol.h, arr, len(views), C.byref(im), C.byref(opts),
+                                            C.byref(st) if stats else None, _stream(stream)))
+        res = {"rgba": img, "rgba8": img8, "px_steps": steps}
+        if stats:
+            res["stats"] = {"samples": st.samples, "skips": st.skips, "shaded": st.shaded, "rays": st.rays}
+        return res
+
+
+class Volume:
+    """r3d_hip_volume: bricked pyramid + occupancy in HBM."""
+
+    def __init__(self, ctx, nx, ny, nz, nlevels=0):
+        self.ctx = ctx
+        d = VolumeDesc(nx, ny, nz, 128, 1.0)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.r3d_hip_volume_create(ctx.h, C.byref(d), nlevels, C.byref(h)))
+        self.h = h
+        self.dims_xyz = (nx, ny, nz)
+        nl = C.c_uint32()
+        dims = (C.c_uint32 * 48)()
+        od = (C.c_uint32 * 3)()
+        ctx.check(ctx.lib.r3d_hip_volume_info(h, C.byref(nl), dims, od))
+        self.nlevels = nl.value
+        self.level_dims = [tuple(dims[l * 3:l * 3 + 3]) for l in range(self.nlevels)]
+        self.odim = tuple(od)
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.r3d_hip_volume_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, voxels, stream=None):
+        if torch is not None and isinstance(voxels, torch.Tensor) and voxels.is_cuda:
+            self.ctx.check(self.ctx.lib.r3d_hip_volume_upload(self.ctx.h, self.h, _ptr(voxels), _lib.R3D_SRC_DEVICE,
+                                                              _stream(stream)))
+        else:
+            v = np.ascontiguousarray(voxels, np.uint8)
+            self.ctx.check(self.ctx.lib.r3d_hip_volume_upload(self.ctx.h, self.h, v.ctypes.data_as(C.c_void_p),
+                                                              _lib.R3D_SRC_HOST, _stream(stream)))
+
+    def chunk_targets(self, stream=None):
+        nx, ny, nz = self.dims_xyz
+        n = (nx // 16) * (ny // 16) * (nz // 16)
+        offs = torch.empty(n, dtype=torch.int64, device=f"cuda:{self.ctx.device}")
+        base = C.c_void_p()
+        sy, sz = C.c_uint32(), C.c_uint32()
+        self.ctx.check(self.ctx.lib.r3d_hip_volume_chunk_targets(self.ctx.h, self.h, _ptr(offs), C.byref(base),
+                                                                 C.byref(sy), C.byref(sz), _stream(stream)))
+        return offs, base.value, sy.value, sz.value
+
+    def build(self, stream=None):
+        self.ctx.check(self.ctx.lib.r3d_hip_volume_build(self.ctx.h, self.h, _stream(stream)))
+
+    def read_level(self, level, stream=None):
+        nx, ny, nz = self.level_dims[level]
+        out = np.empty(nx * ny * nz, np.uint8)
+        self.ctx.check(self.ctx.lib.r3d_hip_volume_read_level(self.ctx.h, self.h, level,
+                                                              out.ctypes.data_as(C.c_void_p), _lib.R3D_SRC_HOST,
+                                                              _stream(stream)))
+        return out
+
+    def read_occupancy(self, stream=None):
+        ox, oy, oz = self.odim
+        out = np.empty(ox * oy * oz, np.uint8)
+        self.ctx.check(self.ctx.lib.r3d_hip_volume_read_occupancy(self.ctx.h, self.h,
+                                                                  out.ctypes.data_as(C.c_void_p), _lib.R3D_SRC_HOST,
+                                                                  _stream(stream)))
+        return out

@@ -1,0 +1,84 @@
+// r3d_internal.h -- state shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "r3d_hip.h"
+
+#define R3D_BRICK 16u
+#define R3D_BRICK_STRIDE 17u                  // 16 payload + 1 apron voxel per axis
+#define R3D_BRICK_BYTES (17u * 17u * 17u)     // 4913
+#define R3D_OCC_BLOCK 8u
+
+namespace r3d {
+
+// Per-level descriptor as the kernels read it (32 bytes, staged in LDS).
+struct LevelDev {
+  uint32_t off_lo, off_hi;  // byte offset of the level's first brick
+  uint32_t bx, by;          // bricks along x and y
+  float fx, fy, fz;         // level dims as floats
+  uint32_t bz;
+};
+
+struct VolDev {
+  const uint8_t *base;      // all levels' bricks
+  const LevelDev *levels;   // device copy of the level table
+  const uint8_t *occ;       // dilated occupancy [oz][oy][ox]
+  uint32_t nlev;
+  uint32_t maxdim;
+  uint32_t ox, oy, oz;
+};
+
+struct ViewSlot {
+  void *host = nullptr;   // pinned staging
+  void *dev = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  bool recorded = false;
+};
+
+}  // namespace r3d
+
+struct r3d_hip_ctx {
+  int device = 0;
+  int num_cus = 0;
+  char err[512] = {0};
+  float basis[256];            // host copy of the DCT basis
+  float *d_tf = nullptr;       // 256 x 4 FP32 transfer table
+  unsigned long long *d_counters = nullptr;  // 4 x u64 scratch for stats
+  r3d::ViewSlot slots[4];
+  int slot_next = 0;
+};
+
+struct r3d_hip_volume {
+  r3d_hip_ctx *ctx = nullptr;
+  r3d_volume_desc desc{};
+  uint32_t nlev = 0;
+  uint32_t dims[R3D_MAX_LEVELS][3] = {};
+  uint32_t bricks[R3D_MAX_LEVELS][3] = {};
+  uint64_t level_off[R3D_MAX_LEVELS] = {};
+  uint64_t bytes = 0;
+  uint8_t *d_bricks = nullptr;
+  r3d::LevelDev *d_levels = nullptr;
+  uint32_t odim[3] = {};
+  uint8_t *d_occ = nullptr;
+  uint8_t *d_occ_raw = nullptr;
+};
+
+// error helpers
+int r3d_fail(r3d_hip_ctx *ctx, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+#define R3D_HIP_CHECK(ctx, expr)                                                        \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return r3d_fail((ctx), R3D_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_));   \
+  } while (0)
+
+namespace r3d {
+// Copies the FP32 DCT basis (dct3d.c:38-49) into the decoder's constant bank.
+int upload_basis(r3d_hip_ctx *ctx);
+r3d::VolDev vol_dev(const r3d_hip_volume *v);
+}  // namespace r3d

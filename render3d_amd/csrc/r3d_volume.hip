@@ -1,0 +1,450 @@
+// r3d_volume.hip -- bricked mip pyramid, occupancy and the synthetic sheet
+// volume.
+//
+// HBM layout. Each level is a grid of 16^3 bricks; a brick is stored as
+// 17x17x17 bytes (x fastest): the 16^3 payload plus a one-voxel apron on the
+// +x, +y and +z faces holding the neighbouring brick's first voxel (or, at the
+// volume's edge and in partial bricks, the clamped edge voxel). A trilinear
+// footprint starting at payload voxel (x0,y0,z0) therefore lies inside one
+// brick at the fixed offsets {0, 1, 17, 18, 289, 290, 306, 307}, which is the
+// clamp-to-edge sampler of src/vk/vkbackend.c:1058-1069 without per-tap
+// clamping. A 16^3 brick is also exactly one 3ddct chunk, so the decoder
+// writes level 0 in place (strides 17 / 289).
+//
+// Levels: dims(l) = max(dims(0) >> l, 1); level l+1 is the rounded 2x box of
+// level l, (sum of 8 + 4) / 8 with the source index clamped to the level's
+// last voxel (tools/lodpack/main.c:216-227). The reference builds its chain
+// with a linear blit whose rounding is implementation-defined
+// (src/vk/vkbackend.c:944-983); the exact integer box replaces it.
+//
+// Occupancy: max over each 8^3 block of level 0, then a 3^3 max dilation
+// (src/vk/vkbackend.c:794-870; occmax.comp, occdilate.comp).
+#include <new>
+#include <string.h>
+
+#include "r3d_dmath.h"
+#include "r3d_internal.h"
+
+namespace r3d {
+
+struct LevelGeom {
+  uint8_t *base;  // level's first brick
+  uint32_t nx, ny, nz;
+  uint32_t bx, by, bz;
+};
+
+__device__ __forceinline__ size_t payload_index(const LevelGeom &L, uint32_t x, uint32_t y, uint32_t z) {
+  const size_t b = ((size_t)(z >> 4) * L.by + (y >> 4)) * L.bx + (x >> 4);
+  return b * R3D_BRICK_BYTES + ((z & 15u) * 17u + (y & 15u)) * 17u + (x & 15u);
+}
+
+// Global voxel of brick byte i (its position clamped into the level).
+__device__ __forceinline__ void brick_byte_voxel(const LevelGeom &L, uint64_t i, uint32_t *gx,
+                                                 uint32_t *gy, uint32_t *gz, bool *payload) {
+  const uint64_t b = i / R3D_BRICK_BYTES;
+  const uint32_t r = (uint32_t)(i - b * R3D_BRICK_BYTES);
+  const uint32_t lz = r / 289u, ly = (r / 17u) % 17u, lx = r % 17u;
+  const uint32_t bxi = (uint32_t)(b % L.bx), byi = (uint32_t)((b / L.bx) % L.by),
+                 bzi = (uint32_t)(b / ((uint64_t)L.bx * L.by));
+  uint32_t x = bxi * 16u + lx, y = byi * 16u + ly, zz = bzi * 16u + lz;
+  *payload = lx < 16u && ly < 16u && lz < 16u && x < L.nx && y < L.ny && zz < L.nz;
+  *gx = x < L.nx ? x : L.nx - 1u;
+  *gy = y < L.ny ? y : L.ny - 1u;
+  *gz = zz < L.nz ? zz : L.nz - 1u;
+}
+
+__global__ void k_bricks_from_linear(LevelGeom L, const uint8_t *src, uint64_t total) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t x, y, z;
+    bool pl;
+    brick_byte_voxel(L, i, &x, &y, &z, &pl);
+    L.base[i] = src[((size_t)z * L.ny + y) * L.nx + x];
+  }
+}
+
+// Apron and padding bytes copy the clamped payload voxel.
+__global__ void k_apron(LevelGeom L, uint64_t total) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t x, y, z;
+    bool pl;
+    brick_byte_voxel(L, i, &x, &y, &z, &pl);
+    if (!pl) L.base[i] = L.base[payload_index(L, x, y, z)];
+  }
+}
+
+// Every byte (payload, apron, padding) of level l+1 from level l's payload.
+__global__ void k_downsample(LevelGeom S, LevelGeom D, uint64_t total) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t x, y, z;
+    bool pl;
+    brick_byte_voxel(D, i, &x, &y, &z, &pl);
+    uint32_t sum = 4u;
+#pragma unroll
+    for (uint32_t dz = 0; dz < 2; dz++)
+#pragma unroll
+      for (uint32_t dy = 0; dy < 2; dy++)
+#pragma unroll
+        for (uint32_t dx = 0; dx < 2; dx++) {
+          uint32_t sx = 2u * x + dx, sy = 2u * y + dy, sz = 2u * z + dz;
+          sx = sx < S.nx ? sx : S.nx - 1u;
+          sy = sy < S.ny ? sy : S.ny - 1u;
+          sz = sz < S.nz ? sz : S.nz - 1u;
+          sum += S.base[payload_index(S, sx, sy, sz)];
+        }
+    D.base[i] = (uint8_t)(sum / 8u);
+  }
+}
+
+__global__ void k_occ_raw(LevelGeom L, uint8_t *raw, uint32_t ox, uint32_t oy, uint32_t oz) {
+  const uint64_t n = (uint64_t)ox * oy * oz;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t bx = (uint32_t)(i % ox), by = (uint32_t)((i / ox) % oy), bz = (uint32_t)(i / ((uint64_t)ox * oy));
+    const uint32_t x0 = bx * 8u, y0 = by * 8u, z0 = bz * 8u;
+    const uint32_t x1 = x0 + 8u < L.nx ? x0 + 8u : L.nx;
+    const uint32_t y1 = y0 + 8u < L.ny ? y0 + 8u : L.ny;
+    const uint32_t z1 = z0 + 8u < L.nz ? z0 + 8u : L.nz;
+    uint32_t m = 0;
+    for (uint32_t z = z0; z < z1; z++)
+      for (uint32_t y = y0; y < y1; y++) {
+        const uint8_t *row = L.base + payload_index(L, x0, y, z);  // x0..x1 stay in one brick
+        for (uint32_t x = 0; x < x1 - x0; x++) m = row[x] > m ? row[x] : m;
+      }
+    raw[i] = (uint8_t)m;
+  }
+}
+
+__global__ void k_occ_dilate(const uint8_t *raw, uint8_t *occ, uint32_t ox, uint32_t oy, uint32_t oz) {
+  const uint64_t n = (uint64_t)ox * oy * oz;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % ox), y = (int)((i / ox) % oy), z = (int)(i / ((uint64_t)ox * oy));
+    uint32_t m = 0;
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          const int zz = z + dz, yy = y + dy, xx = x + dx;
+          if (zz < 0 || yy < 0 || xx < 0 || zz >= (int)oz || yy >= (int)oy || xx >= (int)ox) continue;
+          const uint32_t v = raw[((size_t)zz * oy + yy) * ox + xx];
+          m = v > m ? v : m;
+        }
+    occ[i] = (uint8_t)m;
+  }
+}
+
+__global__ void k_level_to_linear(LevelGeom L, uint8_t *dst) {
+  const uint64_t n = (uint64_t)L.nx * L.ny * L.nz;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t x = (uint32_t)(i % L.nx), y = (uint32_t)((i / L.nx) % L.ny),
+                   z = (uint32_t)(i / ((uint64_t)L.nx * L.ny));
+    dst[i] = L.base[payload_index(L, x, y, z)];
+  }
+}
+
+__global__ void k_chunk_offsets(uint64_t *off, uint64_t level_off, uint32_t bx, uint32_t by, uint32_t bz) {
+  const uint64_t n = (uint64_t)bx * by * bz;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    off[i] = level_off + i * R3D_BRICK_BYTES;
+}
+
+// The sheet volume; same pinned sequence as the host generator.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ void k_synth_sheets(uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t nsheets,
+                               uint64_t seed) {
+  __shared__ int32_t off[256];
+  for (uint32_t k = threadIdx.x; k < nsheets; k += blockDim.x)
+    off[k] = (int32_t)(mix64(seed ^ (0x5EED0000ull + k)) % 61u) - 30;
+  __syncthreads();
+  const float cx = (float)nx * 0.5f, cy = (float)ny * 0.5f;
+  const float LN2 = 0x1.62e430p-1f;
+  const uint64_t n = (uint64_t)nx * ny * nz;
+  for (uint64_t idx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; idx < n;
+       idx += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t x = (uint32_t)(idx % nx), y = (uint32_t)((idx / nx) % ny),
+                   z = (uint32_t)(idx / ((uint64_t)nx * ny));
+    float ws, wc;
+    sincos_turns((float)z / 40.0f, &ws, &wc);
+    const float wave = 3.0f * ws;
+    const uint64_t h = mix64(seed ^ (idx * 0xD1B54A32D192ED03ull));
+    const uint32_t a = (uint32_t)h, b = (uint32_t)(h >> 32), c = (uint32_t)mix64(h);
+    int32_t base = 20 + (int32_t)(c % 11u) - 5;
+    const float dx = ((float)x + 0.5f) - cx, dy = ((float)y + 0.5f) - cy;
+    const float r = __builtin_sqrtf(dx * dx + dy * dy);
+    const float rr = r - wave;
+    const float kf = __builtin_floorf((rr - 20.0f) / 9.0f + 0.5f);
+    if (kf >= 0.0f && kf < (float)nsheets) {
+      const int32_t k = (int32_t)kf;
+      if (__builtin_fabsf(rr - (20.0f + 9.0f * (float)k)) < 1.5f) base = 160 + off[k];
+    }
+    const float u1 = ((float)(a >> 8) + 0.5f) * 0x1p-24f;
+    const float u2 = ((float)(b >> 8) + 0.5f) * 0x1p-24f;
+    float s2, c2;
+    sincos_turns(u2, &s2, &c2);
+    const float g = __builtin_sqrtf(-2.0f * (log2_pinned(u1) * LN2)) * c2;
+    float v = __builtin_roundf((float)base + 8.0f * g);
+    v = fclamp_sel(v, 0.0f, 255.0f);
+    vol[idx] = (uint8_t)v;
+  }
+}
+
+static LevelGeom geom(const r3d_hip_volume *v, uint32_t l) {
+  LevelGeom g;
+  g.base = v->d_bricks + v->level_off[l];
+  g.nx = v->dims[l][0];
+  g.ny = v->dims[l][1];
+  g.nz = v->dims[l][2];
+  g.bx = v->bricks[l][0];
+  g.by = v->bricks[l][1];
+  g.bz = v->bricks[l][2];
+  return g;
+}
+
+static uint64_t level_bytes(const r3d_hip_volume *v, uint32_t l) {
+  return (uint64_t)v->bricks[l][0] * v->bricks[l][1] * v->bricks[l][2] * R3D_BRICK_BYTES;
+}
+
+static dim3 grid1d(const r3d_hip_ctx *ctx, uint64_t n, uint32_t block) {
+  uint64_t g = (n + block - 1) / block;
+  const uint64_t cap = (uint64_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 64u;
+  return dim3((uint32_t)(g < cap ? (g ? g : 1) : cap));
+}
+
+VolDev vol_dev(const r3d_hip_volume *v) {
+  VolDev d;
+  d.base = v->d_bricks;
+  d.levels = v->d_levels;
+  d.occ = v->d_occ;
+  d.nlev = v->nlev;
+  uint32_t md = v->dims[0][0];
+  if (v->dims[0][1] > md) md = v->dims[0][1];
+  if (v->dims[0][2] > md) md = v->dims[0][2];
+  d.maxdim = md;
+  d.ox = v->odim[0];
+  d.oy = v->odim[1];
+  d.oz = v->odim[2];
+  return d;
+}
+
+}  // namespace r3d
+
+using namespace r3d;
+
+extern "C" int r3d_hip_volume_create(r3d_hip_ctx *ctx, const r3d_volume_desc *desc, uint32_t nlevels,
+                                     r3d_hip_volume **out) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!desc || !out) return r3d_fail(ctx, R3D_ERR_INVALID, "volume_create: NULL argument");
+  *out = nullptr;
+  if (desc->nx == 0 || desc->ny == 0 || desc->nz == 0)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "volume_create: empty volume");
+  if (desc->nx > 65536 || desc->ny > 65536 || desc->nz > 65536)
+    return r3d_fail(ctx, R3D_ERR_RANGE, "volume_create: dims above 65536");
+  uint32_t maxdim = desc->nx;
+  if (desc->ny > maxdim) maxdim = desc->ny;
+  if (desc->nz > maxdim) maxdim = desc->nz;
+  uint32_t full = 1;  // src/vk/vkbackend.c:1517-1518
+  while ((maxdim >> full) >= 1 && full < R3D_MAX_LEVELS) full++;
+  if (nlevels == 0) nlevels = full;
+  if (nlevels > full) return r3d_fail(ctx, R3D_ERR_RANGE, "volume_create: %u levels > %u", nlevels, full);
+  r3d_hip_volume *v = new (std::nothrow) r3d_hip_volume();
+  if (!v) return R3D_ERR_NOMEM;
+  v->ctx = ctx;
+  v->desc = *desc;
+  v->nlev = nlevels;
+  uint64_t off = 0;
+  LevelDev tab[R3D_MAX_LEVELS];
+  memset(tab, 0, sizeof tab);
+  for (uint32_t l = 0; l < nlevels; l++) {
+    const uint32_t d[3] = {desc->nx, desc->ny, desc->nz};
+    for (int a = 0; a < 3; a++) {
+      v->dims[l][a] = (d[a] >> l) ? (d[a] >> l) : 1u;
+      v->bricks[l][a] = (v->dims[l][a] + 15u) / 16u;
+    }
+    v->level_off[l] = off;
+    off += (level_bytes(v, l) + 255u) & ~(uint64_t)255u;
+    tab[l].off_lo = (uint32_t)v->level_off[l];
+    tab[l].off_hi = (uint32_t)(v->level_off[l] >> 32);
+    tab[l].bx = v->bricks[l][0];
+    tab[l].by = v->bricks[l][1];
+    tab[l].bz = v->bricks[l][2];
+    tab[l].fx = (float)v->dims[l][0];
+    tab[l].fy = (float)v->dims[l][1];
+    tab[l].fz = (float)v->dims[l][2];
+  }
+  v->bytes = off;
+  for (int a = 0; a < 3; a++) v->odim[a] = ((&desc->nx)[a] + 7u) / 8u;
+  const uint64_t on = (uint64_t)v->odim[0] * v->odim[1] * v->odim[2];
+  int rc = R3D_OK;
+  if (hipSetDevice(ctx->device) != hipSuccess) rc = R3D_ERR_DEVICE;
+  if (rc == R3D_OK && (hipMalloc(&v->d_bricks, v->bytes) != hipSuccess ||
+                       hipMalloc(&v->d_levels, sizeof tab) != hipSuccess ||
+                       hipMalloc(&v->d_occ, on) != hipSuccess || hipMalloc(&v->d_occ_raw, on) != hipSuccess))
+    rc = r3d_fail(ctx, R3D_ERR_NOMEM, "volume_create: cannot allocate %llu bytes",
+                  (unsigned long long)(v->bytes + 2 * on));
+  if (rc == R3D_OK && hipMemcpy(v->d_levels, tab, sizeof tab, hipMemcpyHostToDevice) != hipSuccess)
+    rc = R3D_ERR_DEVICE;
+  if (rc != R3D_OK) {
+    r3d_hip_volume_destroy(v);
+    return rc;
+  }
+  *out = v;
+  return R3D_OK;
+}
+
+extern "C" void r3d_hip_volume_destroy(r3d_hip_volume *v) {
+  if (!v) return;
+  if (v->ctx) hipSetDevice(v->ctx->device);
+  if (v->d_bricks) hipFree(v->d_bricks);
+  if (v->d_levels) hipFree(v->d_levels);
+  if (v->d_occ) hipFree(v->d_occ);
+  if (v->d_occ_raw) hipFree(v->d_occ_raw);
+  delete v;
+}
+
+extern "C" int r3d_hip_volume_info(const r3d_hip_volume *v, uint32_t *nlevels, uint32_t dims[48],
+                                   uint32_t odim[3]) {
+  if (!v) return R3D_ERR_INVALID;
+  if (nlevels) *nlevels = v->nlev;
+  if (dims)
+    for (uint32_t l = 0; l < R3D_MAX_LEVELS; l++)
+      for (int a = 0; a < 3; a++) dims[l * 3 + a] = l < v->nlev ? v->dims[l][a] : 0u;
+  if (odim)
+    for (int a = 0; a < 3; a++) odim[a] = v->odim[a];
+  return R3D_OK;
+}
+
+extern "C" int r3d_hip_volume_build(r3d_hip_ctx *ctx, r3d_hip_volume *v, void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!v || v->ctx != ctx) return r3d_fail(ctx, R3D_ERR_INVALID, "volume_build: bad volume");
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  {
+    const LevelGeom g0 = geom(v, 0);
+    const uint64_t n0 = level_bytes(v, 0);
+    hipLaunchKernelGGL(k_apron, grid1d(ctx, n0, 256), dim3(256), 0, s, g0, n0);
+  }
+  for (uint32_t l = 0; l + 1 < v->nlev; l++) {
+    const uint64_t n = level_bytes(v, l + 1);
+    hipLaunchKernelGGL(k_downsample, grid1d(ctx, n, 256), dim3(256), 0, s, geom(v, l), geom(v, l + 1), n);
+  }
+  const uint64_t on = (uint64_t)v->odim[0] * v->odim[1] * v->odim[2];
+  hipLaunchKernelGGL(k_occ_raw, grid1d(ctx, on, 256), dim3(256), 0, s, geom(v, 0), v->d_occ_raw, v->odim[0],
+                     v->odim[1], v->odim[2]);
+  hipLaunchKernelGGL(k_occ_dilate, grid1d(ctx, on, 256), dim3(256), 0, s, (const uint8_t *)v->d_occ_raw,
+                     v->d_occ, v->odim[0], v->odim[1], v->odim[2]);
+  R3D_HIP_CHECK(ctx, hipGetLastError());
+  return R3D_OK;
+}
+
+extern "C" int r3d_hip_volume_upload(r3d_hip_ctx *ctx, r3d_hip_volume *v, const uint8_t *voxels,
+                                     uint32_t src_kind, void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!v || v->ctx != ctx || !voxels) return r3d_fail(ctx, R3D_ERR_INVALID, "volume_upload: bad argument");
+  if (src_kind != R3D_SRC_HOST && src_kind != R3D_SRC_DEVICE)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "volume_upload: bad src_kind %u", src_kind);
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  const uint64_t nvox = (uint64_t)v->desc.nx * v->desc.ny * v->desc.nz;
+  const uint8_t *src = voxels;
+  uint8_t *tmp = nullptr;
+  if (src_kind == R3D_SRC_HOST) {
+    if (hipMalloc(&tmp, nvox) != hipSuccess)
+      return r3d_fail(ctx, R3D_ERR_NOMEM, "volume_upload: staging %llu bytes", (unsigned long long)nvox);
+    if (hipMemcpyAsync(tmp, voxels, nvox, hipMemcpyHostToDevice, s) != hipSuccess) {
+      hipFree(tmp);
+      return r3d_fail(ctx, R3D_ERR_DEVICE, "volume_upload: copy failed");
+    }
+    src = tmp;
+  }
+  const uint64_t n0 = level_bytes(v, 0);
+  hipLaunchKernelGGL(k_bricks_from_linear, grid1d(ctx, n0, 256), dim3(256), 0, s, geom(v, 0), src, n0);
+  int rc = r3d_hip_volume_build(ctx, v, stream);
+  if (tmp) {
+    hipStreamSynchronize(s);
+    hipFree(tmp);
+  }
+  if (rc != R3D_OK) return rc;
+  R3D_HIP_CHECK(ctx, hipGetLastError());
+  return R3D_OK;
+}
+
+extern "C" int r3d_hip_volume_chunk_targets(r3d_hip_ctx *ctx, r3d_hip_volume *v, uint64_t *out_offsets,
+                                            uint8_t **base, uint32_t *stride_y, uint32_t *stride_z,
+                                            void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!v || v->ctx != ctx) return r3d_fail(ctx, R3D_ERR_INVALID, "chunk_targets: bad volume");
+  if (v->desc.nx % 16 || v->desc.ny % 16 || v->desc.nz % 16)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "chunk_targets: level-0 dims must be multiples of 16");
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (out_offsets) {
+    const uint64_t n = (uint64_t)v->bricks[0][0] * v->bricks[0][1] * v->bricks[0][2];
+    hipLaunchKernelGGL(k_chunk_offsets, grid1d(ctx, n, 256), dim3(256), 0, (hipStream_t)stream, out_offsets,
+                       v->level_off[0], v->bricks[0][0], v->bricks[0][1], v->bricks[0][2]);
+    R3D_HIP_CHECK(ctx, hipGetLastError());
+  }
+  if (base) *base = v->d_bricks;
+  if (stride_y) *stride_y = R3D_BRICK_STRIDE;
+  if (stride_z) *stride_z = R3D_BRICK_STRIDE * R3D_BRICK_STRIDE;
+  return R3D_OK;
+}
+
+static int copy_out(r3d_hip_ctx *ctx, const uint8_t *dsrc, uint8_t *dst, uint64_t n, uint32_t kind,
+                    hipStream_t s) {
+  if (kind == R3D_SRC_DEVICE) {
+    R3D_HIP_CHECK(ctx, hipMemcpyAsync(dst, dsrc, n, hipMemcpyDeviceToDevice, s));
+  } else {
+    R3D_HIP_CHECK(ctx, hipMemcpyAsync(dst, dsrc, n, hipMemcpyDeviceToHost, s));
+    R3D_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  }
+  return R3D_OK;
+}
+
+extern "C" int r3d_hip_volume_read_level(r3d_hip_ctx *ctx, const r3d_hip_volume *v, uint32_t level,
+                                         uint8_t *dst, uint32_t dst_kind, void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!v || v->ctx != ctx || !dst || level >= v->nlev)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "read_level: bad argument");
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  const LevelGeom g = geom(v, level);
+  const uint64_t n = (uint64_t)g.nx * g.ny * g.nz;
+  uint8_t *lin = nullptr;
+  if (hipMalloc(&lin, n) != hipSuccess) return r3d_fail(ctx, R3D_ERR_NOMEM, "read_level: staging");
+  hipLaunchKernelGGL(k_level_to_linear, grid1d(ctx, n, 256), dim3(256), 0, s, g, lin);
+  int rc = copy_out(ctx, lin, dst, n, dst_kind, s);
+  hipStreamSynchronize(s);
+  hipFree(lin);
+  return rc;
+}
+
+extern "C" int r3d_hip_volume_read_occupancy(r3d_hip_ctx *ctx, const r3d_hip_volume *v, uint8_t *dst,
+                                             uint32_t dst_kind, void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!v || v->ctx != ctx || !dst) return r3d_fail(ctx, R3D_ERR_INVALID, "read_occupancy: bad argument");
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const uint64_t n = (uint64_t)v->odim[0] * v->odim[1] * v->odim[2];
+  return copy_out(ctx, v->d_occ, dst, n, dst_kind, (hipStream_t)stream);
+}
+
+extern "C" int r3d_hip_synth_sheets(r3d_hip_ctx *ctx, uint8_t *dst, uint32_t nx, uint32_t ny, uint32_t nz,
+                                    uint32_t nsheets, uint64_t seed, void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!dst || nx == 0 || ny == 0 || nz == 0) return r3d_fail(ctx, R3D_ERR_INVALID, "synth: bad argument");
+  if (nsheets > 256) return r3d_fail(ctx, R3D_ERR_RANGE, "synth: at most 256 sheets");
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const uint64_t n = (uint64_t)nx * ny * nz;
+  hipLaunchKernelGGL(k_synth_sheets, grid1d(ctx, n, 256), dim3(256), 0, (hipStream_t)stream, dst, nx, ny, nz,
+                     nsheets, seed);
+  R3D_HIP_CHECK(ctx, hipGetLastError());
+  return R3D_OK;
+}

@@ -1,0 +1,163 @@
+"""HIP decoder parity: bit-exact against the reference's own outputs (golden
+vectors) and against the oracle on generated batches, through the C ABI."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "dct3d_golden.npz"))
+
+
+def cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def to_pairs(levels, rng=None):
+    """Dense [n,4096] int16 -> (offsets, pairs) with R3D_PAIR packing."""
+    offs = [0]
+    pairs = []
+    for i in range(levels.shape[0]):
+        pos = np.nonzero(levels[i])[0].astype(np.uint32)
+        if rng is not None:
+            rng.shuffle(pos)
+        pairs.append(pos | (levels[i][pos].astype(np.int16).view(np.uint16).astype(np.uint32) << 16))
+        offs.append(offs[-1] + pos.size)
+    p = np.concatenate(pairs).astype(np.uint32) if pairs else np.zeros(0, np.uint32)
+    if p.size == 0:
+        p = np.zeros(1, np.uint32)
+    return np.array(offs, np.uint32), p
+
+
+def golden_batch():
+    n = len(GOLD["blob_names"])
+    hdr = orc.hdr_array(GOLD["blob_hdr"][:, 0], GOLD["blob_hdr"][:, 1], GOLD["blob_dcq"], np.arange(n))
+    return n, hdr, GOLD["blob_steps"].reshape(-1), GOLD["blob_levels"]
+
+
+def test_golden_blobs_bit_exact_with_reference_dense(ctx):
+    n, hdr, steps, lv = golden_batch()
+    out = ctx.decode_chunks(headers=cuda(hdr), step_tables=cuda(steps), levels=cuda(lv),
+                            corr_offsets=cuda(GOLD["blob_corr_off"]), corr_pos=cuda(GOLD["blob_corr_pos"]),
+                            corr_delta=cuda(GOLD["blob_corr_delta"]))
+    got = out.cpu().numpy()
+    bad = [GOLD["blob_names"][i] for i in range(n) if not np.array_equal(got[i], GOLD["blob_u8"][i])]
+    assert not bad, f"chunks differing from dct3d_decode_u8: {bad}"
+
+
+def test_golden_blobs_bit_exact_sparse(ctx):
+    n, hdr, steps, lv = golden_batch()
+    offs, pairs = to_pairs(lv, np.random.default_rng(1))
+    out = ctx.decode_chunks(headers=cuda(hdr), step_tables=cuda(steps), pairs=cuda(pairs), pair_offsets=cuda(offs),
+                            corr_offsets=cuda(GOLD["blob_corr_off"]), corr_pos=cuda(GOLD["blob_corr_pos"]),
+                            corr_delta=cuda(GOLD["blob_corr_delta"]))
+    assert np.array_equal(out.cpu().numpy(), GOLD["blob_u8"])
+
+
+def test_linear_table_golden(ctx):
+    k = GOLD["lin_levels"].shape[0]
+    hdr = orc.hdr_array(GOLD["lin_vmin"], GOLD["lin_vspan"], GOLD["lin_dcq"])
+    out = ctx.decode_chunks(headers=cuda(hdr), step_tables=cuda(GOLD["lin_step"]), levels=cuda(GOLD["lin_levels"]))
+    assert np.array_equal(out.cpu().numpy(), GOLD["lin_u8"].reshape(k, 4096))
+
+
+@pytest.mark.parametrize("n,nsheets,seed", [(64, 3, 11), (128, 6, 1234)])
+def test_generated_batch_matches_oracle(ctx, n, nsheets, seed):
+    vol = ctx.synth_sheets(n, n, n, nsheets, seed)
+    step = cuda(orc.step_linear())
+    levels, hdr = ctx.encode_chunks(vol, n, n, n, step)
+    out = ctx.decode_chunks(headers=hdr, step_tables=step, levels=levels)
+    torch.cuda.synchronize()
+    ref, status = orc.decode_batch(levels.cpu().numpy(), hdr.cpu().numpy(), orc.step_linear())
+    assert status.all()
+    got = out.cpu().numpy()
+    assert np.array_equal(got, ref), f"{int(np.sum(got != ref))} voxels differ"
+    # decoded data resembles the volume it was encoded from (lossy codec)
+    vc = vol.cpu().numpy().reshape(n // 16, 16, n // 16, 16, n // 16, 16).transpose(0, 2, 4, 1, 3, 5)
+    assert np.mean(np.abs(vc.reshape(-1, 4096).astype(int) - got.astype(int))) < 12
+
+
+def test_status_and_zero_fill_for_invalid_headers(ctx):
+    rng = np.random.default_rng(2)
+    lv = (rng.integers(-20, 20, (6, 4096)) * (rng.random((6, 4096)) < 0.05)).astype(np.int16)
+    hdr = orc.hdr_array([0, 0, np.nan, 0, 3, 0], [255, 0, 1, 255, 100, -1], [100, 1, 2, 3, 4, 5], [0, 0, 0, 7, 0, 0])
+    status = torch.full((6,), -5, dtype=torch.int32, device="cuda")
+    out = ctx.decode_chunks(headers=cuda(hdr), step_tables=cuda(orc.step_linear()), levels=cuda(lv), status=status)
+    got = out.cpu().numpy()
+    st = status.cpu().numpy()
+    assert st.tolist() == [1, 0, 0, 0, 1, 0]
+    for i in (1, 2, 3, 5):
+        assert not got[i].any()
+    for i in (0, 4):
+        ok, ref, _ = orc.decode_chunk(lv[i], hdr[i, 0:1].view(np.float32)[0], hdr[i, 1:2].view(np.float32)[0],
+                                      hdr[i, 2], orc.step_linear())
+        assert ok and np.array_equal(got[i], ref)
+
+
+def test_extreme_levels(ctx):
+    lv = np.zeros((4, 4096), np.int16)
+    lv[0, :] = 32767
+    lv[1, :] = -32768
+    lv[2, ::7] = 32767
+    lv[3, 0] = -32768
+    hdr = orc.hdr_array([0, 0, 10, -5], [255, 255, 3, 1000], [0, 4000, -100, 12], [0, 0, 0, 0])
+    out = ctx.decode_chunks(headers=cuda(hdr), step_tables=cuda(orc.step_qstep(np.float32(3.0))), levels=cuda(lv))
+    ref, _ = orc.decode_batch(lv, hdr, orc.step_qstep(np.float32(3.0)))
+    assert np.array_equal(out.cpu().numpy(), ref)
+
+
+def test_strided_destination_matches_flat(ctx):
+    # decode 8 chunks into a 2x2x2-chunk (32^3) volume laid out with padded strides
+    vol = ctx.synth_sheets(32, 32, 32, 1, 5)
+    step = cuda(orc.step_linear())
+    levels, hdr = ctx.encode_chunks(vol, 32, 32, 32, step)
+    flat = ctx.decode_chunks(headers=hdr, step_tables=step, levels=levels).cpu().numpy()
+    sy, sz = 40, 40 * 17
+    dst = torch.zeros(8 * 16 * sz + 64, dtype=torch.uint8, device="cuda")
+    offs = cuda(np.arange(8, dtype=np.int64) * 16 * sz + 3)
+    b = ctx.make_batch(headers=hdr, step_tables=step, out=dst, levels=levels, out_offsets=offs, stride_y=sy,
+                       stride_z=sz)
+    ctx.decode(b)
+    d = dst.cpu().numpy()
+    for i in range(8):
+        for z in range(16):
+            for y in range(16):
+                o = 3 + i * 16 * sz + z * sz + y * sy
+                assert np.array_equal(d[o:o + 16], flat[i, (z * 16 + y) * 16:(z * 16 + y) * 16 + 16])
+
+
+def test_invalid_batches_are_rejected_without_launch(ctx):
+    import render3d_amd as r3d
+
+    hdr = cuda(orc.hdr_array([0], [255], [0]))
+    step = cuda(orc.step_linear())
+    lv = torch.zeros(4096 * 2, dtype=torch.int16, device="cuda")
+    out = torch.zeros(4096, dtype=torch.uint8, device="cuda")
+    b = ctx.make_batch(headers=hdr, step_tables=step, out=out, levels=lv[1:4097])  # misaligned
+    with pytest.raises(r3d.R3DError):
+        ctx.decode(b)
+    b = ctx.make_batch(headers=hdr, step_tables=step, out=out, levels=lv, stride_y=8)
+    with pytest.raises(r3d.R3DError):
+        ctx.decode(b)
+    b = ctx.make_batch(headers=hdr, step_tables=step, out=out, levels=lv)
+    with pytest.raises(r3d.R3DError):
+        ctx.decode(b, variant=7)
+    b.nchunks = 0
+    ctx.decode(b)  # empty batch is a no-op
+
+
+def test_corrupt_sparse_offsets_stay_in_bounds(ctx):
+    # offsets past npairs are clamped; the decode still completes
+    lv = np.zeros((2, 4096), np.int16)
+    lv[:, 0] = 3
+    offs, pairs = to_pairs(lv)
+    offs = np.array([0, 1, 1 << 30], np.uint32)
+    hdr = orc.hdr_array([0, 0], [255, 255], [0, 0])
+    out = ctx.decode_chunks(headers=cuda(hdr), step_tables=cuda(orc.step_linear()), pairs=cuda(pairs),
+                            pair_offsets=cuda(offs))
+    torch.cuda.synchronize()
+    assert out.shape == (2, 4096)

@@ -1,0 +1,157 @@
+"""HIP raycaster, pyramid, occupancy and generator parity against the oracle
+(host transcription of raycast.slang cube mode), through the C ABI. The
+raycast bar is max-abs <= 1e-5 in FP32 and identical per-pixel sample counts."""
+import math
+
+import numpy as np
+import pytest
+
+import render3d_amd as r3d
+from oracle import oracle as orc
+from render3d_amd import scene
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def test_synth_matches_host_generator(ctx):
+    for (nx, ny, nz, ns, seed) in [(64, 64, 64, 3, 9), (96, 80, 48, 4, 123)]:
+        g = ctx.synth_sheets(nx, ny, nz, ns, seed).cpu().numpy()
+        h = orc.synth_sheets(nx, ny, nz, ns, seed)
+        assert np.array_equal(g, h), (nx, ny, nz, int(np.sum(g != h)))
+
+
+@pytest.mark.parametrize("dims", [(64, 64, 64), (96, 80, 40), (33, 17, 70)])
+def test_pyramid_and_occupancy_exact(ctx, dims):
+    nx, ny, nz = dims
+    vol = orc.synth_sheets(nx, ny, nz, 3, 77)
+    v = ctx.volume(nx, ny, nz)
+    v.upload(vol)
+    ref = orc.pyramid(vol, nx, ny, nz, v.nlevels)
+    for l in range(v.nlevels):
+        assert v.level_dims[l] == ref[l][0]
+        assert np.array_equal(v.read_level(l), ref[l][1]), f"level {l}"
+    odim, occ = orc.occupancy(vol, nx, ny, nz)
+    assert v.odim == odim
+    assert np.array_equal(v.read_occupancy(), occ)
+
+
+def test_decode_into_bricks_equals_upload(ctx):
+    n = 64
+    vol = ctx.synth_sheets(n, n, n, 3, 21)
+    step = torch.from_numpy(orc.step_linear()).cuda()
+    levels, hdr = ctx.encode_chunks(vol, n, n, n, step)
+    flat = ctx.decode_chunks(headers=hdr, step_tables=step, levels=levels)
+    v = ctx.volume(n, n, n, 4)
+    offs, base, sy, sz = v.chunk_targets()
+    ctx.decode(ctx.make_batch(headers=hdr, step_tables=step, out=base, levels=levels, out_offsets=offs,
+                              stride_y=sy, stride_z=sz))
+    v.build()
+    lin = flat.cpu().numpy().reshape(n // 16, n // 16, n // 16, 16, 16, 16).transpose(0, 3, 1, 4, 2, 5).reshape(-1)
+    w = ctx.volume(n, n, n, 4)
+    w.upload(lin)
+    for l in range(4):
+        assert np.array_equal(v.read_level(l), w.read_level(l))
+    assert np.array_equal(v.read_occupancy(), w.read_occupancy())
+
+
+def _views(width, height):
+    lut = scene.tf_ramp_40_200()
+    gate = scene.skip_gate(lut)
+    cam = scene.orbit_camera(45.0, 10.0, 2.0)
+    base = dict(step_voxels=0.5, gate=gate)
+    out = {
+        "full": [scene.frame_params(cam, width, height, **base)],
+        "flat": [scene.frame_params(cam, width, height, mode=r3d.MODE_FLAT, **base)],
+        "mip": [scene.frame_params(cam, width, height, mode=r3d.MODE_MIP, **base)],
+        "depth": [scene.frame_params(cam, width, height, mode=r3d.MODE_DEPTH, **base)],
+        "heatmap": [scene.frame_params(cam, width, height, mode=r3d.MODE_HEATMAP, **base)],
+        "raydir": [scene.frame_params(cam, width, height, mode=r3d.MODE_RAYDIR, **base)],
+        "lodbias": [scene.frame_params(cam, width, height, lod_bias=1.5, frame_index=7, **base)],
+        "threshold": [scene.frame_params(cam, width, height, threshold=0.3, **base)],
+        "half": [scene.frame_params(cam, width, height, view_flags=r3d.VIEW_HALF, **base)],
+    }
+    ortho = scene.frame_params(cam, width, height, view_flags=r3d.VIEW_ORTHO, **base)
+    ortho.cam_right[:] = [x * 0.9 for x in ortho.cam_right]
+    out["ortho"] = [ortho]
+    rot = scene.frame_params(scene.orbit_camera(-30.0, 25.0, 1.6), width, height, step_voxels=1.0, gate=gate)
+    a = math.radians(20)
+    rot.vol_r0[:] = [math.cos(a), 0.0, math.sin(a)]
+    rot.vol_r1[:] = [0.0, 1.0, 0.0]
+    rot.vol_r2[:] = [-math.sin(a), 0.0, math.cos(a)]
+    rot.vol_tx, rot.vol_ty, rot.vol_tz = 0.05, -0.02, 0.01
+    out["rotated"] = [rot]
+    inside = scene.Camera((0.45, 0.55, 0.3))
+    inside.yaw = np.float32(0.3)
+    out["inside"] = [scene.frame_params(inside, width, height, step_voxels=0.75, gate=gate)]
+    # several views of mixed modes and sizes in one call
+    w2, h2 = width // 2, height // 2
+    out["multi"] = [
+        scene.frame_params(cam, w2, h2, view_org=(0, 0), **base),
+        scene.frame_params(scene.orbit_camera(120.0, -20.0, 2.2), w2, h2, mode=r3d.MODE_MIP, view_org=(w2, 0),
+                           **base),
+        scene.frame_params(scene.orbit_camera(200.0, 5.0, 1.8), w2, h2, mode=r3d.MODE_FLAT, view_org=(0, h2),
+                           **base),
+        scene.frame_params(scene.orbit_camera(300.0, 40.0, 2.0), w2 - 3, h2 - 5, view_org=(w2, h2), **base),
+    ]
+    return lut, out
+
+
+@pytest.fixture(scope="module")
+def scene96(ctx):
+    dims = (96, 80, 72)
+    vol = orc.synth_sheets(*dims, 4, 31)
+    v = ctx.volume(*dims)
+    v.upload(vol)
+    ov = orc.OracleVolume(vol, *dims)
+    return v, ov
+
+
+@pytest.mark.parametrize("case", ["full", "flat", "mip", "depth", "heatmap", "raydir", "lodbias", "threshold",
+                                  "half", "ortho", "rotated", "inside", "multi"])
+def test_raycast_matches_oracle(ctx, scene96, case):
+    v, ov = scene96
+    W, H = 120, 88
+    lut, views = _views(W, H)
+    vs = views[case]
+    ctx.set_transfer(lut)
+    out = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    res = ctx.raycast(v, vs, W, H, alpha_cutoff=0.99, out=out, px_steps=True, stats=True, rgba8=True)
+    img = res["rgba"].cpu().numpy()
+    ref, cnt, steps = orc.raycast(ov, scene.tf_unorm(lut), vs, W, H, alpha_cutoff=0.99, px_steps=True)
+    err = float(np.max(np.abs(img - ref)))
+    assert err <= TOL, f"{case}: max abs {err}"
+    assert np.array_equal(res["px_steps"].cpu().numpy().astype(np.uint32), steps), case
+    st = res["stats"]
+    assert (st["samples"], st["skips"], st["shaded"], st["rays"]) == \
+        (cnt["samples"], cnt["skips"], cnt["shaded"], cnt["rays"]), (case, st, cnt)
+    want8 = (np.clip(ref, 0, 1) * np.float32(255) + np.float32(0.5)).astype(np.uint8)
+    assert np.array_equal(res["rgba8"].cpu().numpy(), want8)
+
+
+def test_raycast_f32_tf_and_empty_volume(ctx):
+    n = 48
+    v = ctx.volume(n, n, n)
+    v.upload(np.zeros(n ** 3, np.uint8))
+    lut = scene.tf_unorm(scene.tf_build(scene.tf_preset(1)))
+    tf = torch.from_numpy(lut.reshape(-1)).cuda()
+    p = scene.frame_params(scene.orbit_camera(30.0, 10.0, 2.0), 64, 48, step_voxels=0.5, gate=0.0)
+    res = ctx.raycast(v, [p], 64, 48, tf=tf, stats=True)
+    img = res["rgba"].cpu().numpy()
+    bg = np.array([0.02, 0.02, 0.03, 1.0], np.float32)
+    assert np.all(img == bg)
+    assert res["stats"]["samples"] == res["stats"]["skips"]
+
+
+def test_unsupported_views_rejected(ctx, scene96):
+    v, _ = scene96
+    p = scene.frame_params(scene.orbit_camera(), 32, 32)
+    p.slab_grid = 1
+    with pytest.raises(r3d.R3DError):
+        ctx.raycast(v, [p], 32, 32)
+    p.slab_grid = 0
+    p.mode = 9
+    with pytest.raises(r3d.R3DError):
+        ctx.raycast(v, [p], 32, 32)
