@@ -164,7 +164,7 @@ class Context:
         dev = f"cuda:{self.device}"
         img = out if out is not None else (torch.empty((height, width, 4), dtype=torch.float32, device=dev)
                                           if rgba else None)
-        img8 = torch.empty((height, width, 4), dtype=torch.uint8, device=dev) if rgba8 else None
+        img8 = torch.zeros((height, width, 4), dtype=torch.uint8, device=dev) if rgba8 else None
         steps = torch.zeros((height, width), dtype=torch.int32, device=dev) if px_steps else None
         im = Image(img.data_ptr() if img is not None else None, img8.data_ptr() if img8 is not None else None,
                    width, height)

@@ -122,12 +122,14 @@ def test_raycast_matches_oracle(ctx, scene96, case):
     img = res["rgba"].cpu().numpy()
     ref, cnt, steps = orc.raycast(ov, scene.tf_unorm(lut), vs, W, H, alpha_cutoff=0.99, px_steps=True)
     err = float(np.max(np.abs(img - ref)))
+    print(f"{case}: max abs {err:g}, samples {cnt['samples']}")
     assert err <= TOL, f"{case}: max abs {err}"
     assert np.array_equal(res["px_steps"].cpu().numpy().astype(np.uint32), steps), case
     st = res["stats"]
     assert (st["samples"], st["skips"], st["shaded"], st["rays"]) == \
         (cnt["samples"], cnt["skips"], cnt["shaded"], cnt["rays"]), (case, st, cnt)
     want8 = (np.clip(ref, 0, 1) * np.float32(255) + np.float32(0.5)).astype(np.uint8)
+    want8[ref[..., 3] == 0] = 0  # pixels no view covers are left untouched (zero)
     assert np.array_equal(res["rgba8"].cpu().numpy(), want8)
 
 
