@@ -38,8 +38,11 @@ def parse():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--levels", type=int, default=4)
-    ap.add_argument("--cpu-rows", type=int, default=16, help="CPU baseline renders every Nth row")
+    ap.add_argument("--cpu-rows", type=int, default=1, help="CPU baseline renders every Nth row")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-decode", action="store_true",
+                    help="for rocprofv3 --pmc: one 1 GiB device copy (calibration) then one decode launch")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "decode_exact_1024.md"),
                     help="rocprofv3 summary holding the decode kernel's measured HBM bytes per launch")
     return ap.parse_args()
@@ -96,6 +99,16 @@ def main():
     counters = torch.zeros(4, dtype=torch.int64, device=dev)
     torch.cuda.synchronize()
     setup_s = time.time() - t0
+
+    if args.profile_decode:
+        a = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)  # a plain copy of known size: 1 GiB read + 1 GiB written
+        torch.cuda.synchronize()
+        ctx.decode(batch)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_decode": True, "chunks": nch, "copy_bytes": 1 << 30}))
+        return 0
 
     def step(timed):
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
@@ -171,9 +184,10 @@ def main():
 
 
 def cpu_baseline(ctx, vol, view, lut, img, args, levels, headers, step_tab, n, W, H):
-    """The oracle's host raycaster on every `cpu_rows`-th row of the same frame
-    (same volume, pyramid and occupancy read back from HBM), on the host's
-    cores; plus the oracle decoder on a chunk sample."""
+    """The oracle's host raycaster on the same frame (same volume, pyramid and
+    occupancy read back from HBM), repeated until at least `cpu_seconds` of
+    work, on the host's cores (at most 16, the GPU's share of the box); plus
+    the oracle decoder on a 4096-chunk sample of the same batch."""
     import numpy as np
 
     from oracle import oracle as orc
@@ -184,27 +198,38 @@ def cpu_baseline(ctx, vol, view, lut, img, args, levels, headers, step_tab, n, W
     occ = (vol.odim, vol.read_occupancy())
     ov = orc.OracleVolume(None, n, n, n, levels=lv, occ=occ)
     rs = args.cpu_rows
+    tf = scene.tf_unorm(lut)
     t = time.perf_counter()
-    ref, cnt, _ = orc.raycast(ov, scene.tf_unorm(lut), [view], W, H, alpha_cutoff=0.99, row0=0, row_step=rs,
-                              nthreads=cores)
+    reps, samples, rays = 0, 0, 0
+    while True:
+        ref, cnt, _ = orc.raycast(ov, tf, [view], W, H, alpha_cutoff=0.99, row0=0, row_step=rs, nthreads=cores)
+        reps += 1
+        samples += cnt["samples"]
+        rays += cnt["rays"]
+        if time.perf_counter() - t >= args.cpu_seconds:
+            break
     dt = time.perf_counter() - t
     g = img.cpu().numpy()
     rows = np.arange(0, H, rs)
     err = float(np.max(np.abs(g[rows] - ref[rows])))
-    # decoder: oracle on a 4096-chunk sample of the same batch
     k = min(4096, levels.shape[0])
     idx = np.linspace(0, levels.shape[0] - 1, k).astype(np.int64)
     lvs = levels[idx].cpu().numpy()
     hd = headers[idx].cpu().numpy()
+    st = step_tab.cpu().numpy()
     t2 = time.perf_counter()
-    dec, _ = orc.decode_batch(lvs, hd, step_tab.cpu().numpy(), nthreads=cores)
+    dreps = 0
+    while True:
+        orc.decode_batch(lvs, hd, st, nthreads=cores)
+        dreps += 1
+        if time.perf_counter() - t2 >= args.cpu_seconds / 2:
+            break
     dt2 = time.perf_counter() - t2
-    cpu = {"value": cnt["samples"] / dt, "unit": "ray samples/s", "cores": cores, "kind": "port",
-           "sample": f"every {rs}th row of the {W}x{H} frame ({cnt['rays']} rays, {cnt['samples']} samples, "
-                     f"{dt:.1f} s)",
-           "decode_GBps": k * CHUNK_BYTES / dt2 / 1e9,
-           "decode_sample": f"{k} chunks, {dt2:.2f} s"}
-    parity = {"raycast_rows_max_abs": err, "raycast_rows": int(rows.size)}
+    cpu = {"value": samples / dt, "unit": "ray samples/s", "cores": cores, "kind": "port",
+           "sample": f"{reps} x the {W}x{H} frame (every {rs}th row), {rays} rays, {samples} samples, {dt:.1f} s",
+           "decode_GBps": dreps * k * CHUNK_BYTES / dt2 / 1e9,
+           "decode_sample": f"{dreps} x {k} chunks of the batch, {dt2:.1f} s"}
+    parity = {"raycast_max_abs": err, "raycast_rows_checked": int(rows.size)}
     return cpu, parity
 
 

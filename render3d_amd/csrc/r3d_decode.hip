@@ -27,10 +27,24 @@
 namespace r3d {
 
 __constant__ float c_basis[256];
+__constant__ float c_mag[16];
+constexpr int basis_j(int k, int n);
+constexpr bool basis_neg(int k, int n);
 
 int upload_basis(r3d_hip_ctx *ctx) {
+  // magnitudes by fold index, then a bit-exact check of every entry
+  float mag[16] = {0};
+  for (int k = 0; k < 16; k++)
+    for (int n = 0; n < 8; n++) mag[basis_j(k, n)] = __builtin_fabsf(ctx->basis[k * 16 + n]);
+  for (int k = 0; k < 16; k++)
+    for (int n = 0; n < 8; n++) {
+      const float want = basis_neg(k, n) ? -mag[basis_j(k, n)] : mag[basis_j(k, n)];
+      if (__builtin_memcmp(&want, &ctx->basis[k * 16 + n], 4) != 0)
+        return r3d_fail(ctx, R3D_ERR_DEVICE, "DCT basis entry (%d,%d) breaks the cosine symmetry", k, n);
+    }
   R3D_HIP_CHECK(ctx, hipMemcpyToSymbol(HIP_SYMBOL(c_basis), ctx->basis, sizeof ctx->basis, 0,
                                        hipMemcpyHostToDevice));
+  R3D_HIP_CHECK(ctx, hipMemcpyToSymbol(HIP_SYMBOL(c_mag), mag, sizeof mag, 0, hipMemcpyHostToDevice));
   return R3D_OK;
 }
 
@@ -61,7 +75,24 @@ __device__ __forceinline__ float deq_one(float lv, float step) {
   return lv < 0.0f ? -r : r;
 }
 
-// dct3d.c:61-68 on a register-resident line.
+// The basis has 15 distinct magnitudes: cm[k][n] = +-M[j] with
+// j = |((2n+1)k mod 64)| folded to [0,16) (j = 0 is c_0 = 1/4, equal to
+// M[8]) and the sign negative for (2n+1)k mod 64 in (16, 48). The host
+// checks that the FP32 table it built reproduces this structure bit for bit
+// (upload_basis), so (-M)*v == cm*v exactly and e - M*v == e + cm*v.
+constexpr int basis_j(int k, int n) {
+  return k == 0 ? 0 : (((2 * n + 1) * k) & 31) < 32 - (((2 * n + 1) * k) & 31)
+                          ? (((2 * n + 1) * k) & 31)
+                          : 32 - (((2 * n + 1) * k) & 31);
+}
+constexpr bool basis_neg(int k, int n) {
+  return (((2 * n + 1) * k) & 63) > 16 && (((2 * n + 1) * k) & 63) < 48;
+}
+
+// dct3d.c:61-68 on a register-resident line. Every k term is added: a zero
+// input contributes an exact +-0, so the sums equal the reference's, which
+// skips zero inputs (the only possible difference is the sign of an exact
+// zero, which the + dc that follows removes).
 __device__ __forceinline__ void idct16(const float in[16], float out[16]) {
   float e[8], o[8];
 #pragma unroll
@@ -72,17 +103,19 @@ __device__ __forceinline__ void idct16(const float in[16], float out[16]) {
 #pragma unroll
   for (int k = 0; k < 16; k += 2) {
     const float v = in[k];
-    if (v != 0.0f) {
 #pragma unroll
-      for (int n = 0; n < 8; n++) e[n] += c_basis[k * 16 + n] * v;
+    for (int n = 0; n < 8; n++) {
+      const float p = c_mag[basis_j(k, n)] * v;
+      e[n] = basis_neg(k, n) ? e[n] - p : e[n] + p;
     }
   }
 #pragma unroll
   for (int k = 1; k < 16; k += 2) {
     const float v = in[k];
-    if (v != 0.0f) {
 #pragma unroll
-      for (int n = 0; n < 8; n++) o[n] += c_basis[k * 16 + n] * v;
+    for (int n = 0; n < 8; n++) {
+      const float p = c_mag[basis_j(k, n)] * v;
+      o[n] = basis_neg(k, n) ? o[n] - p : o[n] + p;
     }
   }
 #pragma unroll

@@ -49,6 +49,8 @@ struct r3d_hip_ctx {
   float basis[256];            // host copy of the DCT basis
   float *d_tf = nullptr;       // 256 x 4 FP32 transfer table
   unsigned long long *d_counters = nullptr;  // 4 x u64 scratch for stats
+  uint32_t *d_partials = nullptr;            // per-tile raycast counter partials
+  size_t partials_cap = 0;                   // in tiles
   r3d::ViewSlot slots[4];
   int slot_next = 0;
 };

@@ -44,7 +44,7 @@ struct RayArgs {
   uint8_t *img8;
   uint32_t img_w, img_h;
   float cutoff;
-  unsigned long long *counters;
+  uint32_t *partials;  // [tile][4] counter sums of this launch's tiles, or NULL
   uint32_t *px_steps;
 };
 
@@ -329,18 +329,38 @@ __global__ __launch_bounds__(64) void k_raycast(RayArgs a) {
         if (a.px_steps) a.px_steps[pi] = R.steps;
       }
   }
-  if (a.counters) {
+  if (a.partials) {  // one plain store per tile; k_reduce_counters sums them
     st = wave_sum(st);
     sk = wave_sum(sk);
     sh = wave_sum(sh);
     nr = wave_sum(nr);
-    if (lane == 0) {
-      atomicAdd(&a.counters[0], (unsigned long long)st);
-      atomicAdd(&a.counters[1], (unsigned long long)sk);
-      atomicAdd(&a.counters[2], (unsigned long long)sh);
-      atomicAdd(&a.counters[3], (unsigned long long)nr);
-    }
+    if (lane == 0) ((uint4 *)a.partials)[tile] = make_uint4(st, sk, sh, nr);
   }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_counters(const uint32_t *partials, uint32_t ntiles,
+                                                         unsigned long long *counters) {
+  __shared__ unsigned long long red[4][256];
+  unsigned long long acc[4] = {0, 0, 0, 0};
+  for (uint32_t i = threadIdx.x; i < ntiles; i += 256u) {
+    const uint4 p = ((const uint4 *)partials)[i];
+    acc[0] += p.x;
+    acc[1] += p.y;
+    acc[2] += p.z;
+    acc[3] += p.w;
+  }
+  for (int c = 0; c < 4; c++) red[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int c = 0; c < 4; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) counters[threadIdx.x] += red[threadIdx.x][0];
+}
+
+__global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
+  if (threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
 static int kind_of(uint32_t mode) {
@@ -439,7 +459,20 @@ extern "C" int r3d_hip_raycast(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, cons
   unsigned long long *counters = opts ? (unsigned long long *)opts->counters : nullptr;
   if (stats) {
     counters = ctx->d_counters;
-    R3D_HIP_CHECK(ctx, hipMemsetAsync(counters, 0, 4 * sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, counters, 4u);
+  }
+  const uint32_t total_tiles = gtiles[0] + gtiles[1] + gtiles[2] + gtiles[3];
+  if (counters && ctx->partials_cap < total_tiles) {
+    if (ctx->d_partials) {
+      R3D_HIP_CHECK(ctx, hipStreamSynchronize(s));
+      R3D_HIP_CHECK(ctx, hipFree(ctx->d_partials));
+      ctx->d_partials = nullptr;
+      ctx->partials_cap = 0;
+    }
+    const size_t cap = total_tiles < 65536u ? 65536u : total_tiles;
+    if (hipMalloc(&ctx->d_partials, cap * 16u) != hipSuccess)
+      return r3d_fail(ctx, R3D_ERR_NOMEM, "raycast: counter partials for %zu tiles", cap);
+    ctx->partials_cap = cap;
   }
   RayArgs a;
   a.vol = vol_dev(vol);
@@ -449,13 +482,15 @@ extern "C" int r3d_hip_raycast(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, cons
   a.img_w = out->width;
   a.img_h = out->height;
   a.cutoff = cutoff;
-  a.counters = counters;
   a.px_steps = opts ? opts->px_steps : nullptr;
   const r3d_frame_params *dviews = (const r3d_frame_params *)slot->dev;
   const uint32_t *dts = (const uint32_t *)((const uint8_t *)slot->dev + pbytes);
+  uint32_t tile_base = 0;
   for (int k = 0; k < NKINDS; k++) {
     const uint32_t nv = gstart[k + 1] - gstart[k];
     if (nv == 0 || gtiles[k] == 0) continue;
+    a.partials = counters ? ctx->d_partials + (size_t)tile_base * 4u : nullptr;
+    tile_base += gtiles[k];
     a.views = dviews + gstart[k];
     a.tile_start = dts + tpos[k];
     a.nviews = nv;
@@ -466,6 +501,11 @@ extern "C" int r3d_hip_raycast(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, cons
     case KIND_FLAT: hipLaunchKernelGGL(k_raycast<KIND_FLAT>, grid, block, 0, s, a); break;
     default: hipLaunchKernelGGL(k_raycast<KIND_OTHER>, grid, block, 0, s, a); break;
     }
+    R3D_HIP_CHECK(ctx, hipGetLastError());
+  }
+  if (counters && total_tiles) {
+    hipLaunchKernelGGL(k_reduce_counters, dim3(1), dim3(256), 0, s, (const uint32_t *)ctx->d_partials,
+                       total_tiles, counters);
     R3D_HIP_CHECK(ctx, hipGetLastError());
   }
   R3D_HIP_CHECK(ctx, hipEventRecord(slot->ev, s));

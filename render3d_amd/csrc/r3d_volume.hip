@@ -63,57 +63,94 @@ __global__ void k_bricks_from_linear(LevelGeom L, const uint8_t *src, uint64_t t
   }
 }
 
-// Apron and padding bytes copy the clamped payload voxel.
-__global__ void k_apron(LevelGeom L, uint64_t total) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t x, y, z;
-    bool pl;
-    brick_byte_voxel(L, i, &x, &y, &z, &pl);
-    if (!pl) L.base[i] = L.base[payload_index(L, x, y, z)];
+// One workgroup per brick, one lane per 17-byte row (17 x 17 rows). Apron
+// and padding bytes copy the payload voxel at the clamped global position;
+// payload bytes are only read, so concurrent bricks never race.
+__global__ __launch_bounds__(320) void k_apron(LevelGeom L) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t bxi = b % L.bx, byi = (b / L.bx) % L.by, bzi = b / (L.bx * L.by);
+  const uint32_t r = threadIdx.x;
+  if (r >= 289u) return;
+  const uint32_t lz = r / 17u, ly = r % 17u;
+  const uint32_t gy0 = byi * 16u + ly, gz0 = bzi * 16u + lz;
+  const uint32_t gy = gy0 < L.ny ? gy0 : L.ny - 1u, gz = gz0 < L.nz ? gz0 : L.nz - 1u;
+  const bool row_payload = ly < 16u && lz < 16u && gy0 < L.ny && gz0 < L.nz;
+  const uint32_t xfull = L.nx - bxi * 16u;  // payload x positions inside the level
+  uint8_t *dst = L.base + (size_t)b * R3D_BRICK_BYTES + r * 17u;
+  const uint32_t x0 = row_payload ? (xfull < 16u ? xfull : 16u) : 0u;
+  for (uint32_t lx = x0; lx < 17u; lx++) {
+    const uint32_t gx0 = bxi * 16u + lx;
+    const uint32_t gx = gx0 < L.nx ? gx0 : L.nx - 1u;
+    dst[lx] = L.base[payload_index(L, gx, gy, gz)];
   }
 }
 
-// Every byte (payload, apron, padding) of level l+1 from level l's payload.
-__global__ void k_downsample(LevelGeom S, LevelGeom D, uint64_t total) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t x, y, z;
-    bool pl;
-    brick_byte_voxel(D, i, &x, &y, &z, &pl);
+// Level l+1 from level l, one workgroup per destination brick, one lane per
+// 17-byte row; every byte (payload, apron, padding) is the rounded 2x box at
+// its clamped global position.
+__global__ __launch_bounds__(320) void k_downsample(LevelGeom S, LevelGeom D) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t bxi = b % D.bx, byi = (b / D.bx) % D.by, bzi = b / (D.bx * D.by);
+  const uint32_t r = threadIdx.x;
+  if (r >= 289u) return;
+  const uint32_t lz = r / 17u, ly = r % 17u;
+  uint32_t y = byi * 16u + ly, z = bzi * 16u + lz;
+  y = y < D.ny ? y : D.ny - 1u;
+  z = z < D.nz ? z : D.nz - 1u;
+  // the four source rows (2y + dy, 2z + dz), clamped
+  const uint8_t *rowp[4];
+#pragma unroll
+  for (uint32_t q = 0; q < 4; q++) {
+    uint32_t sy = 2u * y + (q & 1u), sz = 2u * z + (q >> 1);
+    sy = sy < S.ny ? sy : S.ny - 1u;
+    sz = sz < S.nz ? sz : S.nz - 1u;
+    rowp[q] = S.base + ((size_t)(sz >> 4) * S.by + (sy >> 4)) * S.bx * (size_t)R3D_BRICK_BYTES +
+              ((sz & 15u) * 17u + (sy & 15u)) * 17u;
+  }
+  uint8_t *dst = D.base + (size_t)b * R3D_BRICK_BYTES + r * 17u;
+  for (uint32_t lx = 0; lx < 17u; lx++) {
+    uint32_t x = bxi * 16u + lx;
+    x = x < D.nx ? x : D.nx - 1u;
+    uint32_t s0 = 2u * x, s1 = 2u * x + 1u;
+    s0 = s0 < S.nx ? s0 : S.nx - 1u;
+    s1 = s1 < S.nx ? s1 : S.nx - 1u;
+    const size_t o0 = (size_t)(s0 >> 4) * R3D_BRICK_BYTES + (s0 & 15u);
+    const size_t o1 = (size_t)(s1 >> 4) * R3D_BRICK_BYTES + (s1 & 15u);
     uint32_t sum = 4u;
 #pragma unroll
-    for (uint32_t dz = 0; dz < 2; dz++)
-#pragma unroll
-      for (uint32_t dy = 0; dy < 2; dy++)
-#pragma unroll
-        for (uint32_t dx = 0; dx < 2; dx++) {
-          uint32_t sx = 2u * x + dx, sy = 2u * y + dy, sz = 2u * z + dz;
-          sx = sx < S.nx ? sx : S.nx - 1u;
-          sy = sy < S.ny ? sy : S.ny - 1u;
-          sz = sz < S.nz ? sz : S.nz - 1u;
-          sum += S.base[payload_index(S, sx, sy, sz)];
-        }
-    D.base[i] = (uint8_t)(sum / 8u);
+    for (uint32_t q = 0; q < 4; q++) sum += (uint32_t)rowp[q][o0] + (uint32_t)rowp[q][o1];
+    dst[lx] = (uint8_t)(sum / 8u);
   }
 }
 
-__global__ void k_occ_raw(LevelGeom L, uint8_t *raw, uint32_t ox, uint32_t oy, uint32_t oz) {
-  const uint64_t n = (uint64_t)ox * oy * oz;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t bx = (uint32_t)(i % ox), by = (uint32_t)((i / ox) % oy), bz = (uint32_t)(i / ((uint64_t)ox * oy));
-    const uint32_t x0 = bx * 8u, y0 = by * 8u, z0 = bz * 8u;
-    const uint32_t x1 = x0 + 8u < L.nx ? x0 + 8u : L.nx;
-    const uint32_t y1 = y0 + 8u < L.ny ? y0 + 8u : L.ny;
-    const uint32_t z1 = z0 + 8u < L.nz ? z0 + 8u : L.nz;
-    uint32_t m = 0;
-    for (uint32_t z = z0; z < z1; z++)
-      for (uint32_t y = y0; y < y1; y++) {
-        const uint8_t *row = L.base + payload_index(L, x0, y, z);  // x0..x1 stay in one brick
-        for (uint32_t x = 0; x < x1 - x0; x++) m = row[x] > m ? row[x] : m;
-      }
-    raw[i] = (uint8_t)m;
+// Raw occupancy of level 0, one workgroup per brick (its 2x2x2 blocks of 8^3),
+// one lane per payload row.
+__global__ __launch_bounds__(256) void k_occ_raw(LevelGeom L, uint8_t *raw, uint32_t ox, uint32_t oy, uint32_t oz) {
+  __shared__ uint32_t m8[8];
+  const uint32_t b = blockIdx.x;
+  const uint32_t bxi = b % L.bx, byi = (b / L.bx) % L.by, bzi = b / (L.bx * L.by);
+  const uint32_t t = threadIdx.x;
+  if (t < 8u) m8[t] = 0u;
+  __syncthreads();
+  const uint32_t lz = t >> 4, ly = t & 15u;
+  const uint32_t gy = byi * 16u + ly, gz = bzi * 16u + lz;
+  if (gy < L.ny && gz < L.nz) {
+    const uint8_t *row = L.base + (size_t)b * R3D_BRICK_BYTES + (lz * 17u + ly) * 17u;
+    const uint32_t xn = L.nx - bxi * 16u < 16u ? L.nx - bxi * 16u : 16u;
+    uint32_t lo = 0, hi = 0;
+    for (uint32_t x = 0; x < xn; x++) {
+      const uint32_t v = row[x];
+      if (x < 8u) lo = v > lo ? v : lo;
+      else hi = v > hi ? v : hi;
+    }
+    const uint32_t q = ((lz >> 3) * 2u + (ly >> 3)) * 2u;
+    atomicMax(&m8[q], lo);
+    atomicMax(&m8[q + 1u], hi);
+  }
+  __syncthreads();
+  if (t < 8u) {
+    const uint32_t X = bxi * 2u + (t & 1u), Y = byi * 2u + ((t >> 1) & 1u), Z = bzi * 2u + (t >> 2);
+    if (X < ox && Y < oy && Z < oz) raw[((size_t)Z * oy + Y) * ox + X] = (uint8_t)m8[t];
   }
 }
 
@@ -212,6 +249,10 @@ static LevelGeom geom(const r3d_hip_volume *v, uint32_t l) {
 
 static uint64_t level_bytes(const r3d_hip_volume *v, uint32_t l) {
   return (uint64_t)v->bricks[l][0] * v->bricks[l][1] * v->bricks[l][2] * R3D_BRICK_BYTES;
+}
+
+static uint32_t nbricks(const r3d_hip_volume *v, uint32_t l) {
+  return v->bricks[l][0] * v->bricks[l][1] * v->bricks[l][2];
 }
 
 static dim3 grid1d(const r3d_hip_ctx *ctx, uint64_t n, uint32_t block) {
@@ -330,15 +371,12 @@ extern "C" int r3d_hip_volume_build(r3d_hip_ctx *ctx, r3d_hip_volume *v, void *s
   hipStream_t s = (hipStream_t)stream;
   {
     const LevelGeom g0 = geom(v, 0);
-    const uint64_t n0 = level_bytes(v, 0);
-    hipLaunchKernelGGL(k_apron, grid1d(ctx, n0, 256), dim3(256), 0, s, g0, n0);
+    hipLaunchKernelGGL(k_apron, dim3(nbricks(v, 0)), dim3(320), 0, s, g0);
   }
-  for (uint32_t l = 0; l + 1 < v->nlev; l++) {
-    const uint64_t n = level_bytes(v, l + 1);
-    hipLaunchKernelGGL(k_downsample, grid1d(ctx, n, 256), dim3(256), 0, s, geom(v, l), geom(v, l + 1), n);
-  }
+  for (uint32_t l = 0; l + 1 < v->nlev; l++)
+    hipLaunchKernelGGL(k_downsample, dim3(nbricks(v, l + 1)), dim3(320), 0, s, geom(v, l), geom(v, l + 1));
   const uint64_t on = (uint64_t)v->odim[0] * v->odim[1] * v->odim[2];
-  hipLaunchKernelGGL(k_occ_raw, grid1d(ctx, on, 256), dim3(256), 0, s, geom(v, 0), v->d_occ_raw, v->odim[0],
+  hipLaunchKernelGGL(k_occ_raw, dim3(nbricks(v, 0)), dim3(256), 0, s, geom(v, 0), v->d_occ_raw, v->odim[0],
                      v->odim[1], v->odim[2]);
   hipLaunchKernelGGL(k_occ_dilate, grid1d(ctx, on, 256), dim3(256), 0, s, (const uint8_t *)v->d_occ_raw,
                      v->d_occ, v->odim[0], v->odim[1], v->odim[2]);
