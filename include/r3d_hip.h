@@ -205,8 +205,19 @@ int r3d_hip_volume_chunk_targets(r3d_hip_ctx *ctx, r3d_hip_volume *vol, uint64_t
                                  uint8_t **base, uint32_t *stride_y, uint32_t *stride_z,
                                  void *stream);
 /* Rebuilds level-0 aprons, levels 1.. (rounded 2x box) and the occupancy from
- * level 0 (after a decode into it). */
+ * level 0 (after a decode into it through r3d_hip_volume_chunk_targets). */
 int r3d_hip_volume_build(r3d_hip_ctx *ctx, r3d_hip_volume *vol, void *stream);
+/* Builds the whole volume (level 0 with aprons, levels 1.., occupancy) from
+ * flat decoded chunks (device, 4096 bytes each, (cz*CY+cy)*CX+cx order, 16-byte
+ * aligned); level-0 dims must be multiples of 16. */
+int r3d_hip_volume_build_from_chunks(r3d_hip_ctx *ctx, r3d_hip_volume *vol, const uint8_t *chunks,
+                                     void *stream);
+/* The shard path (r3d_shard_decode_region, src/shard/shardio.c:269-306, moved
+ * onto the GPU): decodes every chunk of the volume ((cz*CY+cy)*CX+cx order;
+ * the batch's out, out_offsets and strides are ignored) into a staging area the
+ * volume owns, then builds it as r3d_hip_volume_build_from_chunks. */
+int r3d_hip_volume_decode(r3d_hip_ctx *ctx, r3d_hip_volume *vol, const r3d_chunk_batch *batch,
+                          const r3d_decode_opts *opts, void *stream);
 /* Copies level `level` out as linear [z][y][x] voxels (host or device dst). */
 int r3d_hip_volume_read_level(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, uint32_t level,
                               uint8_t *dst, uint32_t dst_kind, void *stream);

@@ -132,6 +132,8 @@ SIGNATURES = {
     "r3d_hip_volume_chunk_targets": (C.c_int, [P, P, P, C.POINTER(P), C.POINTER(C.c_uint32),
                                                C.POINTER(C.c_uint32), P]),
     "r3d_hip_volume_build": (C.c_int, [P, P, P]),
+    "r3d_hip_volume_build_from_chunks": (C.c_int, [P, P, P, P]),
+    "r3d_hip_volume_decode": (C.c_int, [P, P, C.POINTER(ChunkBatch), C.POINTER(DecodeOpts), P]),
     "r3d_hip_volume_read_level": (C.c_int, [P, P, C.c_uint32, P, C.c_uint32, P]),
     "r3d_hip_volume_read_occupancy": (C.c_int, [P, P, P, C.c_uint32, P]),
     "r3d_hip_set_transfer": (C.c_int, [P, P]),

@@ -229,7 +229,19 @@ class Volume:
         return offs, base.value, sy.value, sz.value
 
     def build(self, stream=None):
+        """Aprons, pyramid and occupancy after level 0 was filled via chunk_targets()."""
         self.ctx.check(self.ctx.lib.r3d_hip_volume_build(self.ctx.h, self.h, _stream(stream)))
+
+    def build_from_chunks(self, chunks, stream=None):
+        """Whole volume from flat decoded chunks ([n, 4096] uint8, brick order)."""
+        self.ctx.check(self.ctx.lib.r3d_hip_volume_build_from_chunks(self.ctx.h, self.h, _ptr(chunks),
+                                                                     _stream(stream)))
+
+    def decode(self, batch, variant=_lib.R3D_DECODE_EXACT_F32, stream=None):
+        """Decode every chunk of the volume (brick order) and build it."""
+        opts = DecodeOpts(variant, 0)
+        self.ctx.check(self.ctx.lib.r3d_hip_volume_decode(self.ctx.h, self.h, C.byref(batch), C.byref(opts),
+                                                          _stream(stream)))
 
     def read_level(self, level, stream=None):
         nx, ny, nz = self.level_dims[level]

@@ -10,7 +10,8 @@
 
 #define R3D_BRICK 16u
 #define R3D_BRICK_STRIDE 17u                  // 16 payload + 1 apron voxel per axis
-#define R3D_BRICK_BYTES (17u * 17u * 17u)     // 4913
+#define R3D_BRICK_CONTENT (17u * 17u * 17u)   // 4913 bytes of voxels per brick
+#define R3D_BRICK_BYTES 4928u                 // brick stride: content padded to 16 bytes
 #define R3D_OCC_BLOCK 8u
 
 namespace r3d {
@@ -56,6 +57,7 @@ struct r3d_hip_ctx {
 };
 
 struct r3d_hip_volume {
+  uint8_t *d_flat = nullptr;  // staging for r3d_hip_volume_decode: nchunks x 4096
   r3d_hip_ctx *ctx = nullptr;
   r3d_volume_desc desc{};
   uint32_t nlev = 0;

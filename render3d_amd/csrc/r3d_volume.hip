@@ -38,28 +38,114 @@ __device__ __forceinline__ size_t payload_index(const LevelGeom &L, uint32_t x, 
   return b * R3D_BRICK_BYTES + ((z & 15u) * 17u + (y & 15u)) * 17u + (x & 15u);
 }
 
-// Global voxel of brick byte i (its position clamped into the level).
-__device__ __forceinline__ void brick_byte_voxel(const LevelGeom &L, uint64_t i, uint32_t *gx,
-                                                 uint32_t *gy, uint32_t *gz, bool *payload) {
-  const uint64_t b = i / R3D_BRICK_BYTES;
-  const uint32_t r = (uint32_t)(i - b * R3D_BRICK_BYTES);
-  const uint32_t lz = r / 289u, ly = (r / 17u) % 17u, lx = r % 17u;
-  const uint32_t bxi = (uint32_t)(b % L.bx), byi = (uint32_t)((b / L.bx) % L.by),
-                 bzi = (uint32_t)(b / ((uint64_t)L.bx * L.by));
-  uint32_t x = bxi * 16u + lx, y = byi * 16u + ly, zz = bzi * 16u + lz;
-  *payload = lx < 16u && ly < 16u && lz < 16u && x < L.nx && y < L.ny && zz < L.nz;
-  *gx = x < L.nx ? x : L.nx - 1u;
-  *gy = y < L.ny ? y : L.ny - 1u;
-  *gz = zz < L.nz ? zz : L.nz - 1u;
+// Level 0 from a source of voxels, one workgroup per brick: the 17^3 brick
+// (payload, apron, padding = the voxel at the clamped global position) is
+// assembled in LDS and written with 16-byte stores; the same pass emits the
+// brick's eight raw 8^3 occupancy maxima and its octant of level 1 (the
+// rounded 2x box of its payload). SRC_CHUNKS reads flat 4096-byte chunks in
+// brick order (dims multiples of 16); SRC_LINEAR reads [z][y][x] voxels.
+enum { SRC_CHUNKS = 0, SRC_LINEAR = 1 };
+
+template <int SRC>
+__device__ __forceinline__ uint32_t src_voxel(const uint8_t *src, const LevelGeom &L, uint32_t x, uint32_t y,
+                                              uint32_t z) {
+  if (SRC == SRC_CHUNKS)
+    return src[(((size_t)(z >> 4) * L.by + (y >> 4)) * L.bx + (x >> 4)) * 4096u + ((z & 15u) * 16u + (y & 15u)) * 16u +
+               (x & 15u)];
+  return src[((size_t)z * L.ny + y) * L.nx + x];
 }
 
-__global__ void k_bricks_from_linear(LevelGeom L, const uint8_t *src, uint64_t total) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t x, y, z;
-    bool pl;
-    brick_byte_voxel(L, i, &x, &y, &z, &pl);
-    L.base[i] = src[((size_t)z * L.ny + y) * L.nx + x];
+template <int SRC>
+__global__ __launch_bounds__(256) void k_brickify(const uint8_t *src, LevelGeom L0, LevelGeom L1, int do_l1,
+                                                  uint8_t *raw, uint32_t ox, uint32_t oy, uint32_t oz) {
+  __shared__ __attribute__((aligned(16))) uint8_t tile[R3D_BRICK_BYTES];
+  __shared__ uint32_t m8[8];
+  const uint32_t b = blockIdx.x;
+  const uint32_t bxi = b % L0.bx, byi = (b / L0.bx) % L0.by, bzi = b / (L0.bx * L0.by);
+  const uint32_t t = threadIdx.x;
+  const uint32_t lz = t >> 4, ly = t & 15u;
+  const bool full = bxi * 16u + 16u <= L0.nx && byi * 16u + 16u <= L0.ny && bzi * 16u + 16u <= L0.nz;
+  const bool vec = full && (SRC == SRC_CHUNKS || (L0.nx & 15u) == 0u);
+  if (t < 8u) m8[t] = 0u;
+  if (t < R3D_BRICK_BYTES - R3D_BRICK_CONTENT) tile[R3D_BRICK_CONTENT + t] = 0;
+  if (vec) {  // own payload row as one 16-byte load
+    const uint8_t *row = SRC == SRC_CHUNKS
+                             ? src + (size_t)b * 4096u + t * 16u
+                             : src + ((size_t)(bzi * 16u + lz) * L0.ny + (byi * 16u + ly)) * L0.nx + bxi * 16u;
+    const uint4 q = *(const uint4 *)row;
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+    uint8_t *d = tile + (lz * 17u + ly) * 17u;
+#pragma unroll
+    for (int i = 0; i < 16; i++) d[i] = (uint8_t)(w[i >> 2] >> ((i & 3) * 8));
+  }
+  if (vec) {
+    // apron only: 256 z-face + 272 y-face + 289 x-face bytes, enumerated directly
+    auto put = [&](uint32_t px, uint32_t py, uint32_t pz) {
+      uint32_t x = bxi * 16u + px, y = byi * 16u + py, z = bzi * 16u + pz;
+      x = x < L0.nx ? x : L0.nx - 1u;
+      y = y < L0.ny ? y : L0.ny - 1u;
+      z = z < L0.nz ? z : L0.nz - 1u;
+      tile[(pz * 17u + py) * 17u + px] = (uint8_t)src_voxel<SRC>(src, L0, x, y, z);
+    };
+    const uint32_t a = t >> 4, c = t & 15u;
+    put(c, a, 16u);   // z = 16, y < 16, x < 16
+    put(c, 16u, a);   // y = 16, z < 16, x < 16
+    put(16u, c, a);   // x = 16, z < 16, y < 16
+    if (t < 16u) put(t, 16u, 16u);                 // y = z = 16, x < 16
+    else if (t < 33u) put(16u, t - 16u, 16u);      // x = 16, z = 16, y = 0..16
+    else if (t < 49u) put(16u, 16u, t - 33u);      // x = 16, y = 16, z < 16
+  } else {
+    for (uint32_t p = t; p < R3D_BRICK_CONTENT; p += 256u) {
+      const uint32_t pz = p / 289u, py = (p / 17u) % 17u, px = p % 17u;
+      uint32_t x = bxi * 16u + px, y = byi * 16u + py, z = bzi * 16u + pz;
+      x = x < L0.nx ? x : L0.nx - 1u;
+      y = y < L0.ny ? y : L0.ny - 1u;
+      z = z < L0.nz ? z : L0.nz - 1u;
+      tile[p] = (uint8_t)src_voxel<SRC>(src, L0, x, y, z);
+    }
+  }
+  __syncthreads();
+  uint4 *dst = (uint4 *)(L0.base + (size_t)b * R3D_BRICK_BYTES);
+  for (uint32_t i = t; i < R3D_BRICK_BYTES / 16u; i += 256u) dst[i] = ((const uint4 *)tile)[i];
+  // raw occupancy: payload voxels inside the level only
+  {
+    const uint32_t gy = byi * 16u + ly, gz = bzi * 16u + lz;
+    if (gy < L0.ny && gz < L0.nz) {
+      const uint8_t *row = tile + (lz * 17u + ly) * 17u;
+      const uint32_t xn = L0.nx - bxi * 16u < 16u ? L0.nx - bxi * 16u : 16u;
+      uint32_t lo = 0, hi = 0;
+      for (uint32_t x = 0; x < xn; x++) {
+        const uint32_t v = row[x];
+        if (x < 8u) lo = v > lo ? v : lo;
+        else hi = v > hi ? v : hi;
+      }
+      const uint32_t q = ((lz >> 3) * 2u + (ly >> 3)) * 2u;
+      atomicMax(&m8[q], lo);
+      atomicMax(&m8[q + 1u], hi);
+    }
+  }
+  // level-1 octant: 8x8 rows of 8 voxels, sources clamped through the tile
+  if (do_l1 && t < 64u && (bxi >> 1) < L1.bx && (byi >> 1) < L1.by && (bzi >> 1) < L1.bz) {
+    const uint32_t z1 = t >> 3, y1 = t & 7u;
+    uint8_t *drow = L1.base + ((size_t)((bzi >> 1) * L1.by + (byi >> 1)) * L1.bx + (bxi >> 1)) * R3D_BRICK_BYTES +
+                    (((bzi & 1u) * 8u + z1) * 17u + (byi & 1u) * 8u + y1) * 17u + (bxi & 1u) * 8u;
+    const uint8_t *r00 = tile + ((2u * z1) * 17u + 2u * y1) * 17u;
+    for (uint32_t x1 = 0; x1 < 8u; x1++) {
+      uint32_t sum = 4u;
+#pragma unroll
+      for (uint32_t dz = 0; dz < 2; dz++)
+#pragma unroll
+        for (uint32_t dy = 0; dy < 2; dy++) {
+          const uint8_t *r = r00 + (dz * 17u + dy) * 17u + 2u * x1;
+          sum += (uint32_t)r[0] + (uint32_t)r[1];
+        }
+      drow[x1] = (uint8_t)(sum / 8u);
+    }
+  }
+  __syncthreads();
+  if (t < 8u) {
+    const uint32_t X = bxi * 2u + (t & 1u), Y = byi * 2u + ((t >> 1) & 1u), Z = bzi * 2u + (t >> 2);
+    if (X < ox && Y < oy && Z < oz) raw[((size_t)Z * oy + Y) * ox + X] = (uint8_t)m8[t];
   }
 }
 
@@ -349,6 +435,7 @@ extern "C" void r3d_hip_volume_destroy(r3d_hip_volume *v) {
   if (v->d_levels) hipFree(v->d_levels);
   if (v->d_occ) hipFree(v->d_occ);
   if (v->d_occ_raw) hipFree(v->d_occ_raw);
+  if (v->d_flat) hipFree(v->d_flat);
   delete v;
 }
 
@@ -364,24 +451,78 @@ extern "C" int r3d_hip_volume_info(const r3d_hip_volume *v, uint32_t *nlevels, u
   return R3D_OK;
 }
 
+static dim3 brick_grid(const r3d_hip_volume *v, uint32_t l) { return dim3(nbricks(v, l)); }
+
+// Level 1 (from level 0, or only its apron when the level-0 pass already wrote
+// its payload), levels 2.., and the occupancy dilation.
+static int build_tail(r3d_hip_ctx *ctx, r3d_hip_volume *v, hipStream_t s, bool l1_payload_done) {
+  if (v->nlev > 1) {
+    if (l1_payload_done)
+      hipLaunchKernelGGL(k_apron, brick_grid(v, 1), dim3(320), 0, s, geom(v, 1));
+    else
+      hipLaunchKernelGGL(k_downsample, brick_grid(v, 1), dim3(320), 0, s, geom(v, 0), geom(v, 1));
+    for (uint32_t l = 1; l + 1 < v->nlev; l++)
+      hipLaunchKernelGGL(k_downsample, brick_grid(v, l + 1), dim3(320), 0, s, geom(v, l), geom(v, l + 1));
+  }
+  hipLaunchKernelGGL(k_occ_dilate, grid1d(ctx, (uint64_t)v->odim[0] * v->odim[1] * v->odim[2], 256), dim3(256), 0, s,
+                     (const uint8_t *)v->d_occ_raw, v->d_occ, v->odim[0], v->odim[1], v->odim[2]);
+  R3D_HIP_CHECK(ctx, hipGetLastError());
+  return R3D_OK;
+}
+
+template <int SRC>
+static int brickify_and_build(r3d_hip_ctx *ctx, r3d_hip_volume *v, const uint8_t *src, hipStream_t s) {
+  const LevelGeom l1 = v->nlev > 1 ? geom(v, 1) : geom(v, 0);
+  hipLaunchKernelGGL(k_brickify<SRC>, brick_grid(v, 0), dim3(256), 0, s, src, geom(v, 0), l1, v->nlev > 1 ? 1 : 0,
+                     v->d_occ_raw, v->odim[0], v->odim[1], v->odim[2]);
+  R3D_HIP_CHECK(ctx, hipGetLastError());
+  return build_tail(ctx, v, s, v->nlev > 1);
+}
+
 extern "C" int r3d_hip_volume_build(r3d_hip_ctx *ctx, r3d_hip_volume *v, void *stream) {
   if (!ctx) return R3D_ERR_INVALID;
   if (!v || v->ctx != ctx) return r3d_fail(ctx, R3D_ERR_INVALID, "volume_build: bad volume");
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   hipStream_t s = (hipStream_t)stream;
-  {
-    const LevelGeom g0 = geom(v, 0);
-    hipLaunchKernelGGL(k_apron, dim3(nbricks(v, 0)), dim3(320), 0, s, g0);
-  }
-  for (uint32_t l = 0; l + 1 < v->nlev; l++)
-    hipLaunchKernelGGL(k_downsample, dim3(nbricks(v, l + 1)), dim3(320), 0, s, geom(v, l), geom(v, l + 1));
-  const uint64_t on = (uint64_t)v->odim[0] * v->odim[1] * v->odim[2];
-  hipLaunchKernelGGL(k_occ_raw, dim3(nbricks(v, 0)), dim3(256), 0, s, geom(v, 0), v->d_occ_raw, v->odim[0],
+  hipLaunchKernelGGL(k_apron, brick_grid(v, 0), dim3(320), 0, s, geom(v, 0));
+  hipLaunchKernelGGL(k_occ_raw, brick_grid(v, 0), dim3(256), 0, s, geom(v, 0), v->d_occ_raw, v->odim[0],
                      v->odim[1], v->odim[2]);
-  hipLaunchKernelGGL(k_occ_dilate, grid1d(ctx, on, 256), dim3(256), 0, s, (const uint8_t *)v->d_occ_raw,
-                     v->d_occ, v->odim[0], v->odim[1], v->odim[2]);
   R3D_HIP_CHECK(ctx, hipGetLastError());
-  return R3D_OK;
+  return build_tail(ctx, v, s, false);
+}
+
+extern "C" int r3d_hip_volume_build_from_chunks(r3d_hip_ctx *ctx, r3d_hip_volume *v, const uint8_t *chunks,
+                                                void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!v || v->ctx != ctx || !chunks) return r3d_fail(ctx, R3D_ERR_INVALID, "build_from_chunks: bad argument");
+  if (v->desc.nx % 16 || v->desc.ny % 16 || v->desc.nz % 16)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "build_from_chunks: level-0 dims must be multiples of 16");
+  if ((uintptr_t)chunks & 15u) return r3d_fail(ctx, R3D_ERR_INVALID, "build_from_chunks: chunks not 16-byte aligned");
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  return brickify_and_build<SRC_CHUNKS>(ctx, v, chunks, (hipStream_t)stream);
+}
+
+extern "C" int r3d_hip_volume_decode(r3d_hip_ctx *ctx, r3d_hip_volume *v, const r3d_chunk_batch *batch,
+                                     const r3d_decode_opts *opts, void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!v || v->ctx != ctx || !batch) return r3d_fail(ctx, R3D_ERR_INVALID, "volume_decode: bad argument");
+  if (v->desc.nx % 16 || v->desc.ny % 16 || v->desc.nz % 16)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "volume_decode: level-0 dims must be multiples of 16");
+  const uint64_t nch = (uint64_t)v->bricks[0][0] * v->bricks[0][1] * v->bricks[0][2];
+  if (batch->nchunks != nch)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "volume_decode: %u chunks for a volume of %llu", batch->nchunks,
+                    (unsigned long long)nch);
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (!v->d_flat && hipMalloc(&v->d_flat, nch * 4096u) != hipSuccess)
+    return r3d_fail(ctx, R3D_ERR_NOMEM, "volume_decode: staging %llu bytes", (unsigned long long)(nch * 4096u));
+  r3d_chunk_batch b = *batch;
+  b.out = v->d_flat;
+  b.out_offsets = nullptr;
+  b.out_stride_y = 0;
+  b.out_stride_z = 0;
+  int rc = r3d_hip_decode_chunks(ctx, &b, opts, stream);
+  if (rc != R3D_OK) return rc;
+  return brickify_and_build<SRC_CHUNKS>(ctx, v, v->d_flat, (hipStream_t)stream);
 }
 
 extern "C" int r3d_hip_volume_upload(r3d_hip_ctx *ctx, r3d_hip_volume *v, const uint8_t *voxels,
@@ -395,25 +536,23 @@ extern "C" int r3d_hip_volume_upload(r3d_hip_ctx *ctx, r3d_hip_volume *v, const 
   const uint64_t nvox = (uint64_t)v->desc.nx * v->desc.ny * v->desc.nz;
   const uint8_t *src = voxels;
   uint8_t *tmp = nullptr;
-  if (src_kind == R3D_SRC_HOST) {
+  if (src_kind == R3D_SRC_HOST || ((uintptr_t)voxels & 15u)) {
     if (hipMalloc(&tmp, nvox) != hipSuccess)
       return r3d_fail(ctx, R3D_ERR_NOMEM, "volume_upload: staging %llu bytes", (unsigned long long)nvox);
-    if (hipMemcpyAsync(tmp, voxels, nvox, hipMemcpyHostToDevice, s) != hipSuccess) {
+    const hipError_t e = hipMemcpyAsync(tmp, voxels, nvox,
+                                        src_kind == R3D_SRC_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) {
       hipFree(tmp);
       return r3d_fail(ctx, R3D_ERR_DEVICE, "volume_upload: copy failed");
     }
     src = tmp;
   }
-  const uint64_t n0 = level_bytes(v, 0);
-  hipLaunchKernelGGL(k_bricks_from_linear, grid1d(ctx, n0, 256), dim3(256), 0, s, geom(v, 0), src, n0);
-  int rc = r3d_hip_volume_build(ctx, v, stream);
+  int rc = brickify_and_build<SRC_LINEAR>(ctx, v, src, s);
   if (tmp) {
     hipStreamSynchronize(s);
     hipFree(tmp);
   }
-  if (rc != R3D_OK) return rc;
-  R3D_HIP_CHECK(ctx, hipGetLastError());
-  return R3D_OK;
+  return rc;
 }
 
 extern "C" int r3d_hip_volume_chunk_targets(r3d_hip_ctx *ctx, r3d_hip_volume *v, uint64_t *out_offsets,

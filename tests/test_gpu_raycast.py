@@ -57,6 +57,26 @@ def test_decode_into_bricks_equals_upload(ctx):
     assert np.array_equal(v.read_occupancy(), w.read_occupancy())
 
 
+@pytest.mark.parametrize("n,nlev", [(64, 4), (48, 0), (32, 1)])
+def test_volume_decode_and_build_from_chunks(ctx, n, nlev):
+    vol = ctx.synth_sheets(n, n, n, 3, 5 + n)
+    step = torch.from_numpy(orc.step_linear()).cuda()
+    levels, hdr = ctx.encode_chunks(vol, n, n, n, step)
+    flat = ctx.decode_chunks(headers=hdr, step_tables=step, levels=levels)
+    lin = flat.cpu().numpy().reshape(n // 16, n // 16, n // 16, 16, 16, 16).transpose(0, 3, 1, 4, 2, 5).reshape(-1)
+    ref = orc.pyramid(lin, n, n, n, orc.full_levels(n, n, n) if nlev == 0 else nlev)
+    odim, occ = orc.occupancy(lin, n, n, n)
+    a = ctx.volume(n, n, n, nlev)
+    a.decode(ctx.make_batch(headers=hdr, step_tables=step, out=0, levels=levels))
+    b = ctx.volume(n, n, n, nlev)
+    b.build_from_chunks(flat)
+    for v in (a, b):
+        assert v.nlevels == len(ref)
+        for l in range(v.nlevels):
+            assert np.array_equal(v.read_level(l), ref[l][1]), l
+        assert np.array_equal(v.read_occupancy(), occ)
+
+
 def _views(width, height):
     lut = scene.tf_ramp_40_200()
     gate = scene.skip_gate(lut)
