@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=1, help="CPU baseline renders every Nth row")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adj-views", type=int, default=8)
+    ap.add_argument("--adj-grid", type=int, default=128)
     ap.add_argument("--profile-decode", action="store_true",
                     help="for rocprofv3 --pmc: one 1 GiB device copy (calibration) then one decode launch")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "decode_exact_1024.md"),
@@ -140,6 +143,8 @@ def main():
     dec_bytes = CHUNK_BYTES * nch
     dec_gbps = dec_bytes / (dec_avg / 1000.0) / 1e9
 
+    adj = None if args.no_adjoint else adjoint_ratio(ctx, vol, lut, args, W, H, dev)
+
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
@@ -166,7 +171,8 @@ def main():
         "decode_ms": dec_avg,
         "pyramid_ms": float(np.mean(pyr_ms)),
         "decode_GBps": dec_gbps,
-        "adjoint_fwd_ratio": None,
+        "adjoint_fwd_ratio": None if adj is None else adj["ratio"],
+        "adjoint": adj,
         "samples_per_frame": samples,
         "skips_per_frame": skips,
         "shaded_per_frame": shaded,
@@ -180,6 +186,50 @@ def main():
     }
     print(json.dumps(out))
     return 0
+
+
+def adjoint_ratio(ctx, vol, lut, args, W, H, dev):
+    """Differentiable forward vs forward + adjoint, `adj_views` views at WxH
+    over the same volume with an adj_grid^3 density grid and the FP32 TF."""
+    import numpy as np
+    import torch
+
+    from render3d_amd import scene
+    from render3d_amd.autograd import DiffRenderer
+
+    views = [scene.frame_params(scene.orbit_camera(45.0 + 360.0 * i / args.adj_views, 10.0 * (i % 3 - 1), 2.0),
+                                W, H, step_voxels=1.0) for i in range(args.adj_views)]
+    R = DiffRenderer(ctx, vol, views, W, H * 1, alpha_cutoff=0.99)
+    tf = torch.from_numpy(scene.tf_unorm(lut)).to(dev)
+    g = args.adj_grid
+    grid = torch.ones((g, g, g), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    # one image per view: views render side by side is not needed; each view
+    # writes the same WxH image (the loss gradient is per view)
+    img = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    dimg = torch.full((H, W, 4), 1e-3, dtype=torch.float32, device=dev)
+    dtf = torch.empty_like(tf)
+    dgrid = torch.zeros_like(grid)
+    for _ in range(2):
+        R.forward_raw(tf, grid, out=img)
+        R.adjoint_raw(tf, grid, dimg, dtf, dgrid)
+    torch.cuda.synchronize()
+    free0, total = torch.cuda.mem_get_info()
+    reps = 3
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3 * reps)]
+    for r in range(reps):
+        e[3 * r].record(stream)
+        R.forward_raw(tf, grid, out=img)
+        e[3 * r + 1].record(stream)
+        R.adjoint_raw(tf, grid, dimg, dtf, dgrid)
+        e[3 * r + 2].record(stream)
+    torch.cuda.synchronize()
+    fwd = float(np.mean([e[3 * r].elapsed_time(e[3 * r + 1]) for r in range(reps)]))
+    bwd = float(np.mean([e[3 * r + 1].elapsed_time(e[3 * r + 2]) for r in range(reps)]))
+    return {"views": args.adj_views, "width": W, "height": H, "grid": g, "forward_ms": fwd, "adjoint_ms": bwd,
+            "ratio": (fwd + bwd) / fwd, "hbm_used_GB": (total - free0) / 1e9,
+            "workspace_GB": R.workspace.numel() / 1e9,
+            "grad_finite": bool(torch.isfinite(dtf).all() and torch.isfinite(dgrid).all())}
 
 
 def cpu_baseline(ctx, vol, view, lut, img, args, levels, headers, step_tab, n, W, H):

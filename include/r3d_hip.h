@@ -265,6 +265,38 @@ int r3d_hip_raycast(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_frame
                     uint32_t nviews, const r3d_hip_image *out, const r3d_raycast_opts *opts,
                     r3d_hip_stats *stats, void *stream);
 
+/* ---- differentiable raycast -------------------------------------------------- */
+
+/* No counterpart in the reference (it has no differentiable path). The
+ * compositing integral of raycast.slang:893-1038 with an FP32 transfer table
+ * and a per-voxel density-correction grid g (trilinear, cell-centred over the
+ * unit cube) that scales the extinction: a = 1 - (1 - sat(s.a))^(density *
+ * step * lod_step * g). Output alpha is the accumulated opacity. Empty-space
+ * skipping is off. All views of one call are FULL (shaded) or all FLAT. */
+typedef struct r3d_diff_params {
+  const float *tf;          /* device, 256 x 4 FP32 */
+  const float *grid;        /* device, grid_dims[2] x [1] x [0] FP32 */
+  uint32_t grid_dims[3];    /* x, y, z */
+  float alpha_cutoff;       /* 0 -> 0.98 */
+  void *workspace;          /* device, >= r3d_hip_diff_workspace_bytes(views, nviews) */
+  size_t workspace_bytes;
+} r3d_diff_params;
+
+/* Bytes of per-ray checkpoint state (loop state every 64 iterations). */
+size_t r3d_hip_diff_workspace_bytes(const r3d_frame_params *views, uint32_t nviews);
+/* Forward: renders into out->rgba and writes the checkpoints to the workspace. */
+int r3d_hip_raycast_diff(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_frame_params *views,
+                         uint32_t nviews, const r3d_hip_image *out, const r3d_diff_params *dp,
+                         void *stream);
+/* Adjoint of the preceding r3d_hip_raycast_diff with the same arguments:
+ * dL_drgba is the gradient of the loss with respect to that call's FP32
+ * image (img_w x img_h x 4); writes dL_dtf (256 x 4) and ADDS into dL_dgrid
+ * (grid-shaped; zero it first). Segments of 64 iterations are recomputed from
+ * their checkpoints and run in reverse. */
+int r3d_hip_raycast_adjoint(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_frame_params *views,
+                            uint32_t nviews, const float *dL_drgba, uint32_t img_w, uint32_t img_h,
+                            float *dL_dtf, float *dL_dgrid, const r3d_diff_params *dp, void *stream);
+
 /* ---- synthetic workloads ------------------------------------------------------- */
 
 /* The seeded sheet volume (linear [z][y][x], device dst): concentric wavy

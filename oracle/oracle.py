@@ -63,6 +63,10 @@ def load():
     L.r3do_hash12.restype = C.c_float
     L.r3do_hash12.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
     L.r3do_math_batch.argtypes = [f32p, C.c_uint32, C.c_uint32, f32p]
+    f64p = np.ctypeslib.ndpointer(np.float64, flags="C")
+    L.r3do_raycast_diff.restype = C.c_int
+    L.r3do_raycast_diff.argtypes = [C.POINTER(VolumeC), f64p, f64p, u32p, VP, C.c_uint32, C.c_double, f64p,
+                                    C.c_uint32, C.c_uint32, VP, VP, VP]
     L.r3do_synth_sheets.argtypes = [u8p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
                                     C.c_uint32, C.c_uint32, C.c_int]
     _lib = L
@@ -217,3 +221,27 @@ def math_batch(x, fn):
     out = np.zeros_like(x)
     load().r3do_math_batch(x, x.size, fn, out)
     return out
+
+
+def raycast_diff(ovol, tf, grid, gdims, views, width, height, alpha_cutoff=0.98, dimg=None):
+    """FP64 differentiable forward (and reverse when dimg is given).
+    Returns (img [h,w,4] float64, dtf [256,4] or None, dgrid or None)."""
+    L = load()
+    n = len(views)
+    arr = (type(views[0]) * n)(*views)
+    tf = np.ascontiguousarray(tf, np.float64).reshape(-1)
+    grid = np.ascontiguousarray(grid, np.float64).reshape(-1)
+    gd = np.array(gdims, np.uint32)
+    img = np.zeros(height * width * 4, np.float64)
+    dtf = dgrid = None
+    args = [None, None, None]
+    if dimg is not None:
+        dimg = np.ascontiguousarray(dimg, np.float64).reshape(-1)
+        dtf = np.zeros(1024, np.float64)
+        dgrid = np.zeros(grid.size, np.float64)
+        args = [dimg.ctypes.data, dtf.ctypes.data, dgrid.ctypes.data]
+    rc = L.r3do_raycast_diff(C.byref(ovol.c), tf, grid, gd, C.cast(arr, VP), n, alpha_cutoff, img, width, height,
+                             *args)
+    if rc != 0:
+        raise ValueError("oracle raycast_diff: unsupported view")
+    return img.reshape(height, width, 4), (None if dtf is None else dtf.reshape(256, 4)), dgrid

@@ -385,3 +385,233 @@ void r3do_math_batch(const float *x, uint32_t n, uint32_t fn, float *out) {
     }
   }
 }
+
+/* ---- differentiable raycast (FP64 reference) ------------------------------------
+ * The compositing integral with an FP32 transfer table tf and a density grid g
+ * (see include/r3d_hip.h r3d_hip_raycast_diff): positions, lod, volume samples
+ * and the shading factors follow the pinned FP32 sequence (they do not depend
+ * on the parameters); the TF lookup, the grid sample, the opacity correction,
+ * the 0.003 gate, compositing and the cutoff are evaluated in double. With
+ * dimg != NULL the reverse pass accumulates dL/dtf and dL/dgrid (+=). */
+
+typedef struct {
+  float t, lod, lod_step, v;
+  v3f p;
+  float m, sw;   /* shading: col = s.rgb * m + sw */
+  int cut;
+} diff_sample;
+
+static void shade_f(const r3do_volume *V, v3f p, float lod, float e, v3f lvec, v3f hvec, float *m, float *sw) {
+  const v3f key = mk(0x1.a20bd8p-2f, 0x1.a20bd8p-1f, -0x1.a20bd8p-2f);
+  float gx = vol_sample(V, mk(p.x + e, p.y, p.z), lod) - vol_sample(V, mk(p.x - e, p.y, p.z), lod);
+  float gy = vol_sample(V, mk(p.x, p.y + e, p.z), lod) - vol_sample(V, mk(p.x, p.y - e, p.z), lod);
+  float gz = vol_sample(V, mk(p.x, p.y, p.z + e), lod) - vol_sample(V, mk(p.x, p.y, p.z - e), lod);
+  v3f g = mk(gx, gy, gz);
+  float gm = vlen(g);
+  *m = 1.0f;
+  *sw = 0.0f;
+  if (gm > 1e-4f) {
+    v3f n = mk(g.x / gm, g.y / gm, g.z / gm);
+    float diff = 0.7f * fabsf(vdot(n, lvec)) + 0.3f * fabsf(vdot(n, key));
+    float spec = om_pow24(fabsf(vdot(n, hvec))) * 0.3f;
+    float lit = 0.25f + 0.75f * diff;
+    float w = om_sat(gm * 6.0f);
+    *m = om_lerp(1.0f, lit, w);
+    *sw = spec * w;
+  }
+}
+
+static double grid_d(const double *G, const uint32_t gd[3], v3f p, uint32_t idx[8], double w[8]) {
+  float q[3] = {p.x, p.y, p.z};
+  uint32_t i0[3], i1[3];
+  double f[3];
+  for (int a = 0; a < 3; a++) {
+    float t = om_clamp(q[a] * (float)gd[a] - 0.5f, 0.0f, (float)(gd[a] - 1));
+    i0[a] = (uint32_t)t;
+    i1[a] = i0[a] + 1 < gd[a] ? i0[a] + 1 : gd[a] - 1;
+    f[a] = (double)(t - (float)i0[a]);
+  }
+  double s = 0.0;
+  for (int c = 0; c < 8; c++) {
+    uint32_t x = (c & 1) ? i1[0] : i0[0], y = (c & 2) ? i1[1] : i0[1], z = (c & 4) ? i1[2] : i0[2];
+    idx[c] = (z * gd[1] + y) * gd[0] + x;
+    w[c] = ((c & 1) ? f[0] : 1 - f[0]) * ((c & 2) ? f[1] : 1 - f[1]) * ((c & 4) ? f[2] : 1 - f[2]);
+    s += w[c] * G[idx[c]];
+  }
+  return s;
+}
+
+static double sat_d(double x) { return x < 0 ? 0 : (x > 1 ? 1 : x); }
+
+static void diff_pixel(const r3do_volume *V, const double *tf, const double *G, const uint32_t gd[3],
+                       const r3do_frame_params *pc, double cutoff, uint32_t px, uint32_t py, double out[4],
+                       const double *dout, double *dtf, double *dG, diff_sample *buf) {
+  const v3f fwd = mk(pc->cam_forward[0], pc->cam_forward[1], pc->cam_forward[2]);
+  const v3f right = mk(pc->cam_right[0], pc->cam_right[1], pc->cam_right[2]);
+  const v3f up = mk(pc->cam_up[0], pc->cam_up[1], pc->cam_up[2]);
+  const int ortho = (pc->view_flags & 1u) != 0u;
+  const int shaded = pc->mode == MODE_FULL;
+  float ndx, ndy;
+  pix_ndc(pc, px, py, &ndx, &ndy);
+  v3f dir = ortho ? vnorm(fwd) : vnorm(vadd(vadd(fwd, vscale(right, ndx)), vscale(up, ndy)));
+  uint32_t md = V->dims[0][0];
+  if (V->dims[0][1] > md) md = V->dims[0][1];
+  if (V->dims[0][2] > md) md = V->dims[0][2];
+  const float voxel = 1.0f / (float)md;
+  const float max_lod = om_min(pc->max_mip, (float)(V->nlev - 1));
+  v3f org = mk(pc->cam_origin[0], pc->cam_origin[1], pc->cam_origin[2]);
+  if (ortho) org = vadd(vadd(org, vscale(right, ndx)), vscale(up, ndy));
+  const v3f ro = to_volume_point(pc, org, mk(0.5f, 0.5f, 0.5f));
+  dir = to_volume_dir(pc, dir);
+  v3f inv = mk(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+  v3f t0 = mk((0.0f - ro.x) * inv.x, (0.0f - ro.y) * inv.y, (0.0f - ro.z) * inv.z);
+  v3f t1 = mk((1.0f - ro.x) * inv.x, (1.0f - ro.y) * inv.y, (1.0f - ro.z) * inv.z);
+  float hx = om_max(om_max(om_min(t0.x, t1.x), om_min(t0.y, t1.y)), om_min(t0.z, t1.z));
+  float hy = om_min(om_min(om_max(t0.x, t1.x), om_max(t0.y, t1.y)), om_max(t0.z, t1.z));
+  float tn = om_max(hx, 0.0f), tfar = om_min(hy, 1e30f);
+  out[0] = 0.02; out[1] = 0.02; out[2] = 0.03; out[3] = 0.0;
+  if (tn >= tfar) return;
+  const float cone = 2.0f * vlen(up) / (float)pc->viewport[1];
+  const float dt = pc->step_voxels * voxel;
+  float t = tn + om_hash12(px, py, pc->frame_index) * dt;
+  const float span_step = om_max((tfar - tn) / ((float)MAX_STEPS * dt), 1.0f);
+  const float opac_k = pc->density * pc->step_voxels;
+  const v3f lvec = vscale(dir, -1.0f);
+  const v3f hvec = vnorm(vsub(lvec, dir));
+  double C[3] = {0, 0, 0}, A = 0.0;
+  uint32_t n = 0, steps = 0;
+  while (t < tfar && steps < MAX_STEPS) {
+    steps++;
+    diff_sample S;
+    S.t = t;
+    float foot = ortho ? cone : t * cone;
+    S.lod = om_clamp(om_log2f(om_max(foot / voxel, 1.0f)) + pc->lod_bias, 0.0f, max_lod);
+    S.lod_step = om_max(om_exp2f(S.lod), span_step);
+    S.p = vadd(ro, vscale(dir, t));
+    S.v = vol_sample(V, S.p, S.lod);
+    S.cut = S.v < pc->threshold;
+    S.m = 1.0f;
+    S.sw = 0.0f;
+    int done = 0;
+    if (!S.cut) {
+      float tt = om_clamp(S.v * 256.0f - 0.5f, 0.0f, 255.0f);
+      uint32_t i0 = (uint32_t)tt, i1 = i0 + 1 < 256 ? i0 + 1 : 255;
+      double f = (double)(tt - (float)i0);
+      double s[4];
+      for (int c = 0; c < 4; c++) s[c] = tf[i0 * 4 + c] * (1 - f) + tf[i1 * 4 + c] * f;
+      uint32_t gi[8];
+      double gw[8];
+      double g = grid_d(G, gd, S.p, gi, gw);
+      double k = (double)opac_k * (double)S.lod_step * g;
+      double b = 1.0 - sat_d(s[3]);
+      double a = b <= 0 ? 1.0 : (b >= 1 ? 0.0 : 1.0 - pow(b, k));
+      if (a > 0.003) {
+        if (shaded) shade_f(V, S.p, S.lod, voxel * S.lod_step, lvec, hvec, &S.m, &S.sw);
+        double wa = (1.0 - A) * a;
+        for (int c = 0; c < 3; c++) C[c] += wa * (s[c] * S.m + S.sw);
+        A += wa;
+        buf[n++] = S;
+        if (A >= cutoff) done = 1;
+      }
+    }
+    if (done) break;
+    t += dt * S.lod_step;
+  }
+  out[0] = C[0] + (1 - A) * 0.02;
+  out[1] = C[1] + (1 - A) * 0.02;
+  out[2] = C[2] + (1 - A) * 0.03;
+  out[3] = A;
+  if (!dout) return;
+  /* reverse: recompute T before each contributing sample forward, then sweep back */
+  double *Tb = (double *)malloc(sizeof(double) * (n ? n : 1));
+  double T = 1.0;
+  for (uint32_t i = 0; i < n; i++) {
+    const diff_sample *S = &buf[i];
+    float tt = om_clamp(S->v * 256.0f - 0.5f, 0.0f, 255.0f);
+    uint32_t i0 = (uint32_t)tt, i1 = i0 + 1 < 256 ? i0 + 1 : 255;
+    double f = (double)(tt - (float)i0);
+    double sa = tf[i0 * 4 + 3] * (1 - f) + tf[i1 * 4 + 3] * f;
+    uint32_t gi[8];
+    double gw[8];
+    double g = grid_d(G, gd, S->p, gi, gw);
+    double k = (double)opac_k * (double)S->lod_step * g, b = 1.0 - sat_d(sa);
+    double a = b <= 0 ? 1.0 : (b >= 1 ? 0.0 : 1.0 - pow(b, k));
+    Tb[i] = T;
+    T *= (1 - a);
+  }
+  const double gr = dout[0], gg = dout[1], gb = dout[2], ga = dout[3];
+  double dT = gr * 0.02 + gg * 0.02 + gb * 0.03 - ga;
+  for (int64_t i = (int64_t)n - 1; i >= 0; i--) {
+    const diff_sample *S = &buf[i];
+    float tt = om_clamp(S->v * 256.0f - 0.5f, 0.0f, 255.0f);
+    uint32_t i0 = (uint32_t)tt, i1 = i0 + 1 < 256 ? i0 + 1 : 255;
+    double f = (double)(tt - (float)i0);
+    double s[4];
+    for (int c = 0; c < 4; c++) s[c] = tf[i0 * 4 + c] * (1 - f) + tf[i1 * 4 + c] * f;
+    uint32_t gi[8];
+    double gw[8];
+    double g = grid_d(G, gd, S->p, gi, gw);
+    double kbase = (double)opac_k * (double)S->lod_step;
+    double k = kbase * g, b = 1.0 - sat_d(s[3]);
+    double a = b <= 0 ? 1.0 : (b >= 1 ? 0.0 : 1.0 - pow(b, k));
+    double col[3];
+    for (int c = 0; c < 3; c++) col[c] = s[c] * S->m + S->sw;
+    double Tp = Tb[i];
+    double gc = gr * col[0] + gg * col[1] + gb * col[2];
+    double da = gc * Tp - dT * Tp;
+    dT = dT * (1 - a) + a * gc;
+    double ds[4];
+    ds[0] = gr * Tp * a * S->m;
+    ds[1] = gg * Tp * a * S->m;
+    ds[2] = gb * Tp * a * S->m;
+    ds[3] = (s[3] > 0 && s[3] < 1) ? da * k * (1 - a) / b : 0.0;
+    for (int c = 0; c < 4; c++) {
+      dtf[i0 * 4 + c] += (1 - f) * ds[c];
+      dtf[i1 * 4 + c] += f * ds[c];
+    }
+    if (b > 0 && b < 1) {
+      double dg = da * (-(1 - a) * log(b)) * kbase;
+      for (int c = 0; c < 8; c++) dG[gi[c]] += gw[c] * dg;
+    }
+  }
+  free(Tb);
+}
+
+int r3do_raycast_diff(const r3do_volume *vol, const double *tf, const double *grid, const uint32_t gd[3],
+                      const r3do_frame_params *views, uint32_t nviews, double cutoff, double *img,
+                      uint32_t img_w, uint32_t img_h, const double *dimg, double *dtf, double *dgrid) {
+  for (uint32_t v = 0; v < nviews; v++) {
+    if (!view_supported(&views[v])) return -1;
+    if (views[v].mode != MODE_FULL && views[v].mode != MODE_FLAT) return -1;
+  }
+  diff_sample *buf = (diff_sample *)malloc(sizeof(diff_sample) * MAX_STEPS);
+  if (!buf) return -1;
+  for (uint32_t v = 0; v < nviews; v++) {
+    const r3do_frame_params *pc = &views[v];
+    const int half = (pc->view_flags & 32u) != 0u;
+    const uint32_t ox = pc->view_org & 0xffffu, oy = pc->view_org >> 16;
+    const uint32_t W = pc->viewport[0], H = pc->viewport[1];
+    const uint32_t TW = half ? (W + 1) / 2 : W, TH = half ? (H + 1) / 2 : H;
+    for (uint32_t ty = 0; ty < TH; ty++)
+      for (uint32_t tx = 0; tx < TW; tx++) {
+        uint32_t lx = half ? tx * 2 : tx, ly = half ? ty * 2 : ty;
+        double dout[4] = {0, 0, 0, 0}, c[4];
+        if (dimg)
+          for (uint32_t dy = 0; dy < (half ? 2u : 1u); dy++)
+            for (uint32_t dx = 0; dx < (half ? 2u : 1u); dx++) {
+              uint32_t qx = lx + dx, qy = ly + dy;
+              if (qx >= W || qy >= H || ox + qx >= img_w || oy + qy >= img_h) continue;
+              for (int k = 0; k < 4; k++) dout[k] += dimg[((size_t)(oy + qy) * img_w + ox + qx) * 4 + k];
+            }
+        diff_pixel(vol, tf, grid, gd, pc, cutoff, lx, ly, c, dimg ? dout : NULL, dtf, dgrid, buf);
+        for (uint32_t dy = 0; dy < (half ? 2u : 1u); dy++)
+          for (uint32_t dx = 0; dx < (half ? 2u : 1u); dx++) {
+            uint32_t qx = lx + dx, qy = ly + dy;
+            if (qx >= W || qy >= H || ox + qx >= img_w || oy + qy >= img_h) continue;
+            for (int k = 0; k < 4; k++) img[((size_t)(oy + qy) * img_w + ox + qx) * 4 + k] = c[k];
+          }
+      }
+  }
+  free(buf);
+  return 0;
+}

@@ -138,6 +138,13 @@ int r3do_raycast(const r3do_volume *vol, const float *tf, const r3do_frame_param
                  uint32_t *px_steps, uint32_t row0, uint32_t row_step, int nthreads,
                  r3do_counters *cnt);
 
+/* Differentiable raycast in FP64 (parameter-dependent parts): renders
+ * img (double RGBA, alpha = accumulated opacity) and, with dimg != NULL,
+ * adds dL/dtf (256x4) and dL/dgrid. FULL or FLAT views only. */
+int r3do_raycast_diff(const r3do_volume *vol, const double *tf, const double *grid, const uint32_t gd[3],
+                      const r3do_frame_params *views, uint32_t nviews, double cutoff, double *img,
+                      uint32_t img_w, uint32_t img_h, const double *dimg, double *dtf, double *dgrid);
+
 /* Pinned math, exported so tests can compare against libm and the GPU. */
 float r3do_log2f(float x);
 float r3do_exp2f(float x);

@@ -116,6 +116,11 @@ class RaycastOpts(C.Structure):
                 ("counters", C.c_void_p), ("px_steps", C.c_void_p)]
 
 
+class DiffParams(C.Structure):
+    _fields_ = [("tf", C.c_void_p), ("grid", C.c_void_p), ("grid_dims", C.c_uint32 * 3), ("alpha_cutoff", C.c_float),
+                ("workspace", C.c_void_p), ("workspace_bytes", C.c_size_t)]
+
+
 # name -> (restype, argtypes); also the export list the ABI test checks
 P = C.c_void_p
 SIGNATURES = {
@@ -140,6 +145,11 @@ SIGNATURES = {
     "r3d_hip_set_transfer_f32": (C.c_int, [P, P]),
     "r3d_hip_raycast": (C.c_int, [P, P, C.POINTER(FrameParams), C.c_uint32, C.POINTER(Image),
                                   C.POINTER(RaycastOpts), C.POINTER(Stats), P]),
+    "r3d_hip_diff_workspace_bytes": (C.c_size_t, [C.POINTER(FrameParams), C.c_uint32]),
+    "r3d_hip_raycast_diff": (C.c_int, [P, P, C.POINTER(FrameParams), C.c_uint32, C.POINTER(Image),
+                                       C.POINTER(DiffParams), P]),
+    "r3d_hip_raycast_adjoint": (C.c_int, [P, P, C.POINTER(FrameParams), C.c_uint32, P, C.c_uint32, C.c_uint32,
+                                          P, P, C.POINTER(DiffParams), P]),
     "r3d_hip_synth_sheets": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.c_uint64, P]),
     "r3d_hip_encode_chunks": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P]),

@@ -78,6 +78,7 @@ extern "C" void r3d_hip_destroy(r3d_hip_ctx *c) {
   if (c->d_tf) hipFree(c->d_tf);
   if (c->d_counters) hipFree(c->d_counters);
   if (c->d_partials) hipFree(c->d_partials);
+  if (c->d_gtf) hipFree(c->d_gtf);
   delete c;
 }
 

@@ -52,6 +52,8 @@ struct r3d_hip_ctx {
   unsigned long long *d_counters = nullptr;  // 4 x u64 scratch for stats
   uint32_t *d_partials = nullptr;            // per-tile raycast counter partials
   size_t partials_cap = 0;                   // in tiles
+  float *d_gtf = nullptr;                    // adjoint: per-block TF gradient partials
+  size_t gtf_cap = 0;                        // bytes
   r3d::ViewSlot slots[4];
   int slot_next = 0;
 };
@@ -84,5 +86,8 @@ int r3d_fail(r3d_hip_ctx *ctx, int code, const char *fmt, ...) __attribute__((fo
 namespace r3d {
 // Copies the FP32 DCT basis (dct3d.c:38-49) into the decoder's constant bank.
 int upload_basis(r3d_hip_ctx *ctx);
+// Copies n bytes into the next staging slot (pinned host + device buffer);
+// the caller records the slot's event after the launches that read it.
+int stage(r3d_hip_ctx *ctx, const void *src, size_t n, hipStream_t s, ViewSlot **slot_out);
 r3d::VolDev vol_dev(const r3d_hip_volume *v);
 }  // namespace r3d
