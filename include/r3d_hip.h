@@ -42,6 +42,11 @@ typedef struct r3d_hip_volume r3d_hip_volume;
 
 /* ---- shared POD types (byte-for-byte mirrors of the reference) ------------ */
 
+/* render3d's own src/render/render_types.h (include guard R3D_RENDER_TYPES_H)
+ * defines the same PODs and mode values; when it was included first, its
+ * definitions are used and these are skipped. */
+#ifndef R3D_RENDER_TYPES_H
+
 /* src/render/render_types.h:40-45 */
 typedef struct r3d_volume_desc {
   uint32_t nx, ny, nz;
@@ -93,6 +98,8 @@ enum {
 };
 #define R3D_VIEW_ORTHO 1u
 #define R3D_VIEW_HALF 32u
+
+#endif /* R3D_RENDER_TYPES_H */
 
 /* ---- context ------------------------------------------------------------- */
 
