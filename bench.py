@@ -143,6 +143,9 @@ def main():
     dec_bytes = CHUNK_BYTES * nch
     dec_gbps = dec_bytes / (dec_avg / 1000.0) / 1e9
 
+    variants = decode_variants(ctx, batch, levels, headers, step_tab, flat, stream, dev, max(3, args.steps))
+    ctx.decode(batch)  # leave the exact decode in `flat`
+
     adj = None if args.no_adjoint else adjoint_ratio(ctx, vol, lut, args, W, H, dev)
 
     cpu = None
@@ -171,6 +174,7 @@ def main():
         "decode_ms": dec_avg,
         "pyramid_ms": float(np.mean(pyr_ms)),
         "decode_GBps": dec_gbps,
+        "decode_variants": variants,
         "adjoint_fwd_ratio": None if adj is None else adj["ratio"],
         "adjoint": adj,
         "samples_per_frame": samples,
@@ -186,6 +190,67 @@ def main():
     }
     print(json.dumps(out))
     return 0
+
+
+def fp64_decode(levels, headers, step_tab):
+    """FP64 decode of [k, 4096] int16 levels: dead-zone dequantiser
+    (r = (|l| - 1 + 0.8 + 0.4) step), the orthonormal DCT-II basis in FP64,
+    separable inverse transform, + dcq/16, denormalise by vspan/255 and vmin."""
+    import math
+
+    import torch
+    dev = levels.device
+    k = torch.arange(16, dtype=torch.float64, device=dev)
+    cm = torch.cos(math.pi * (2 * k[None, :] + 1) * k[:, None] / 32)
+    cm = cm * torch.where(k == 0, math.sqrt(1 / 16), math.sqrt(2 / 16))[:, None]
+    f = (k[:, None, None] + k[None, :, None] + k[None, None, :]).long()
+    step = step_tab.double()[f]
+    lv = levels.double().view(-1, 16, 16, 16)
+    r = (lv.abs() - 1) * step + 0.8 * step + 0.4 * step
+    coef = torch.where(lv == 0, torch.zeros_like(lv), torch.sign(lv) * r)
+    t = torch.einsum("nabc,cx->nabx", coef, cm)
+    t = torch.einsum("nabx,by->nayx", t, cm)
+    t = torch.einsum("nayx,az->nzyx", t, cm).reshape(-1, 4096)
+    hf = headers.view(torch.float32)
+    vmin, vspan = hf[:, 0:1].double(), hf[:, 1:2].double()
+    dc = headers[:, 2:3].double() / 16
+    return (t + dc) * (vspan / 255) + vmin
+
+
+def decode_variants(ctx, batch, levels, headers, step_tab, flat, stream, dev, reps):
+    """Each decode variant over the whole batch: time, GB/s, HBM fraction,
+    voxels whose u8 differs from the exact decode, and max/mean |x - FP64| of
+    the value before rounding on a 1024-chunk sample."""
+    import numpy as np
+    import torch
+
+    import render3d_amd as r3d
+    nch = levels.shape[0]
+    exact = flat.clone()
+    idx = torch.arange(0, nch, max(1, nch // 1024), device=dev)[:1024]
+    ls, hs = levels[idx].contiguous(), headers[idx].contiguous()
+    ref = fp64_decode(ls, hs, step_tab)
+    names = {r3d.R3D_DECODE_EXACT_F32: "exact_f32", r3d.R3D_DECODE_MFMA_F32: "mfma_f32",
+             r3d.R3D_DECODE_MFMA_F16: "mfma_f16",
+             r3d.R3D_DECODE_MFMA_F16X3: "mfma_f16x3"}
+    out = {}
+    for v, name in names.items():
+        ctx.decode(batch, variant=v)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+        ev[0].record(stream)
+        for i in range(reps):
+            ctx.decode(batch, variant=v)
+            ev[i + 1].record(stream)
+        torch.cuda.synchronize()
+        ms = float(np.mean([ev[i].elapsed_time(ev[i + 1]) for i in range(reps)]))
+        diff = float((flat != exact).float().mean())
+        f32 = ctx.decode_chunks(headers=hs, step_tables=step_tab, levels=ls, variant=v, f32_out=True)
+        err = (f32.double() - ref).abs()
+        gbps = CHUNK_BYTES * nch / (ms / 1000) / 1e9
+        out[name] = {"ms": ms, "GBps": gbps, "hbm_frac": gbps / HBM_PEAK_GBPS, "u8_differs_from_exact": diff,
+                     "max_abs_err_vs_fp64": float(err.max()), "mean_abs_err_vs_fp64": float(err.mean())}
+    return out
 
 
 def adjoint_ratio(ctx, vol, lut, args, W, H, dev):

@@ -169,9 +169,16 @@ typedef struct r3d_chunk_batch {
 } r3d_chunk_batch;
 
 enum {
-  R3D_DECODE_EXACT_F32 = 0, /* the pinned FP32 sequence: bit-exact with the oracle */
-  R3D_DECODE_MFMA_F16 = 2,  /* FP16 MFMA, FP32 accumulate: approximate */
+  R3D_DECODE_EXACT_F32 = 0,   /* the pinned FP32 sequence: bit-exact with the oracle */
+  R3D_DECODE_MFMA_F32 = 1,    /* FP32 MFMA (16x16x4) for two passes, FP32 third pass: approximate */
+  R3D_DECODE_MFMA_F16 = 2,    /* FP16 MFMA, FP32 accumulate: approximate */
+  R3D_DECODE_MFMA_F16X3 = 3,  /* FP16 MFMA on hi/lo splits (3 products), FP32 accumulate:
+                                 about FP32 accuracy, not bit-exact */
 };
+/* flags */
+#define R3D_DECODE_F32_OUT 1u  /* write the FP32 pre-round values (dct3d.c:590-591) instead
+                                  of u8: out is float[nchunks * 4096], flat (no offsets or
+                                  strides) */
 typedef struct r3d_decode_opts {
   uint32_t variant;
   uint32_t flags;

@@ -30,7 +30,10 @@ R3D_ERR_RANGE = -5
 R3D_COEF_DENSE_I16 = 0
 R3D_COEF_SPARSE = 1
 R3D_DECODE_EXACT_F32 = 0
+R3D_DECODE_MFMA_F32 = 1
 R3D_DECODE_MFMA_F16 = 2
+R3D_DECODE_MFMA_F16X3 = 3
+R3D_DECODE_F32_OUT = 1
 R3D_SRC_HOST = 0
 R3D_SRC_DEVICE = 1
 

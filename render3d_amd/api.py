@@ -113,20 +113,21 @@ class Context:
         b.status = None if status is None else status.data_ptr()
         return b
 
-    def decode(self, batch, variant=_lib.R3D_DECODE_EXACT_F32, stream=None):
-        opts = DecodeOpts(variant, 0)
+    def decode(self, batch, variant=_lib.R3D_DECODE_EXACT_F32, stream=None, flags=0):
+        opts = DecodeOpts(variant, flags)
         self.check(self.lib.r3d_hip_decode_chunks(self.h, C.byref(batch), C.byref(opts), _stream(stream)))
 
     def decode_chunks(self, *, headers, step_tables, levels=None, pairs=None, pair_offsets=None,
                       corr_offsets=None, corr_pos=None, corr_delta=None, status=None,
-                      variant=_lib.R3D_DECODE_EXACT_F32, stream=None):
-        """Decode into a fresh flat [n, 4096] uint8 tensor."""
+                      variant=_lib.R3D_DECODE_EXACT_F32, stream=None, f32_out=False):
+        """Decode into a fresh flat [n, 4096] uint8 tensor, or with f32_out the
+        float32 value before rounding (R3D_DECODE_F32_OUT)."""
         n = int(headers.shape[0])
-        out = torch.empty((n, 4096), dtype=torch.uint8, device=headers.device)
+        out = torch.empty((n, 4096), dtype=torch.float32 if f32_out else torch.uint8, device=headers.device)
         b = self.make_batch(headers=headers, step_tables=step_tables, out=out, levels=levels, pairs=pairs,
                             pair_offsets=pair_offsets, corr_offsets=corr_offsets, corr_pos=corr_pos,
                             corr_delta=corr_delta, status=status)
-        self.decode(b, variant, stream)
+        self.decode(b, variant, stream, _lib.R3D_DECODE_F32_OUT if f32_out else 0)
         return out
 
     # ---- workloads ----------------------------------------------------------

@@ -187,6 +187,7 @@ __device__ __forceinline__ void idct3_tile(float v[16], float *tile, uint32_t t,
   __syncthreads();
 }
 
+template <bool F32OUT>
 __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
   __shared__ float tile[4096];
   __shared__ float steps[48];
@@ -200,7 +201,12 @@ __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
     uint8_t *dst = a.out + (a.out_off ? a.out_off[c] : (uint64_t)c * 4096u) + z * a.sz + y * a.sy;
     if (!ok) {  // dct3d.c:555-556: rejected header; the chunk is zero-filled
       const uint32_t zw[4] = {0, 0, 0, 0};
-      store_row16(dst, zw);
+      if (F32OUT) {
+        float4 *fd = (float4 *)(a.out + ((size_t)c * 4096u + t * 16u) * 4u);
+        for (int i = 0; i < 4; i++) fd[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      } else {
+        store_row16(dst, zw);
+      }
       if (a.status && t == 0) a.status[c] = 0;
       continue;
     }
@@ -248,15 +254,23 @@ __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
       __syncthreads();
     }
     const float inv = h.vspan / 255.0f;
-    uint32_t w[4] = {0, 0, 0, 0};
+    if (F32OUT) {
+      float4 *fd = (float4 *)(a.out + ((size_t)c * 4096u + t * 16u) * 4u);
 #pragma unroll
-    for (int x = 0; x < 16; x++) {
-      const float val = tile[t * 16 + x] * inv + h.vmin;
-      float r = __builtin_roundf(val);
-      r = !(r >= 0.0f) ? 0.0f : (r > 255.0f ? 255.0f : r);
-      w[x >> 2] |= (uint32_t)r << ((x & 3) * 8);
+      for (int x = 0; x < 16; x += 4)
+        fd[x >> 2] = make_float4(tile[t * 16 + x] * inv + h.vmin, tile[t * 16 + x + 1] * inv + h.vmin,
+                                 tile[t * 16 + x + 2] * inv + h.vmin, tile[t * 16 + x + 3] * inv + h.vmin);
+    } else {
+      uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int x = 0; x < 16; x++) {
+        const float val = tile[t * 16 + x] * inv + h.vmin;
+        float r = __builtin_roundf(val);
+        r = !(r >= 0.0f) ? 0.0f : (r > 255.0f ? 255.0f : r);
+        w[x >> 2] |= (uint32_t)r << ((x & 3) * 8);
+      }
+      store_row16(dst, w);
     }
-    store_row16(dst, w);
     if (a.status && t == 0) a.status[c] = 1;
     __syncthreads();
   }
@@ -376,6 +390,365 @@ __global__ __launch_bounds__(256) void k_encode(EncodeArgs a) {
   }
 }
 
+
+// ---- MFMA variants (approximate) --------------------------------------------
+//
+// One wave per chunk, four independent waves per 256-lane workgroup,
+// grid-strided. Lane l = r + 16 g (r = l % 16, g = l / 16). The 16 x 16 MFMA
+// operand layouts used: A: lane l holds row r of A at its k set; B: lane l holds
+// column r of B at its k set; D: lane l holds D[4g + e][r], e = 0..3. For
+// v_mfma_f32_16x16x16_f16 the k set of lane l is 4g .. 4g+3 (slot e);
+// v_mfma_f32_16x16x4_f32 takes one k per lane (k = g), and four instructions
+// e = 0..3 cover contraction index 4g + e: the order in which the k terms are
+// summed is free, so "slot e of every lane" is one instruction's k.
+//
+//   load    lane (r, g) holds X[z = 4g+e][y = r][x] for x = 0..15
+//   pass 1  per x: D1 = X_x (A: row y, k = z) * C (B: k = z, column z')
+//           -> lane (r, g) slot e = D1[y = 4g+e][z' = r]
+//   pass 2  per x: D2 = C^T (A: row y', k = y) * D1 (B, exactly as pass 1 left it)
+//           -> lane (r, g) slot e = D2[y' = 4g+e][z' = r]
+//   pass 3  in registers: lane (r, g) holds all 16 x-frequencies of the four
+//           rows (z' = r, y' = 4g+e) and runs an FP32 16-point inverse DCT
+//           on each, so it ends with four whole 16-byte output rows.
+//
+// No LDS is needed for dense levels. Precision per variant:
+//   PREC_F32   FP32 MFMA (v_mfma_f32_16x16x4_f32), FP32 throughout
+//   PREC_F16   FP16 operands, FP32 accumulate (about 11 significant bits)
+//   PREC_F16X3 each product as hi*hi + lo*hi + hi*lo of FP16 hi/lo halves
+// The FP16 variants scale the chunk by a power of two so that every pass-1
+// input is below 2^13 (and so every pass-2 input, at most 4x larger for an
+// orthonormal basis, below 2^15 < 65504); the scale is undone in the epilogue.
+// Dequantisation uses (|l| + 0.2) * step, equal in exact arithmetic to
+// deq_one's (|l| - 1) * step + 0.8 * step + 0.4 * step.
+
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef short s2 __attribute__((ext_vector_type(2)));
+
+enum : int { PREC_F32 = 1, PREC_F16 = 2, PREC_F16X3 = 3 };
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax_sel(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Orders this wave's LDS accesses (lanes of one wave exchanging data).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// power of two p with bound * p < 2^13; 1 when bound == 0
+__device__ __forceinline__ float pow2_scale(float bound) {
+  if (!(bound > 0.0f)) return 1.0f;
+  const int e = (int)((__float_as_uint(bound) >> 23) & 0xffu) - 127;  // floor(log2 bound), bound normal
+  int k = 12 - e;
+  k = k < -120 ? -120 : (k > 120 ? 120 : k);
+  return __uint_as_float((uint32_t)(k + 127) << 23);
+}
+
+__device__ __forceinline__ h4 to_h4(const float v[4]) {
+  h4 h;
+#pragma unroll
+  for (int e = 0; e < 4; e++) h[e] = (_Float16)v[e];
+  return h;
+}
+
+__device__ __forceinline__ h4 lo_h4(const float v[4], h4 hi) {
+  h4 h;
+#pragma unroll
+  for (int e = 0; e < 4; e++) h[e] = (_Float16)(v[e] - (float)hi[e]);
+  return h;
+}
+
+// D = A * B on the MFMA layouts above; a and b hold this lane's four slots.
+template <int PREC>
+__device__ __forceinline__ f4 mma(const float a[4], const float b[4], h4 ah, h4 al, h4 bh, h4 bl) {
+  f4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (PREC == PREC_F32) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) d = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e], b[e], d, 0, 0, 0);
+  } else {
+    d = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, d, 0, 0, 0);
+    if (PREC == PREC_F16X3) {
+      d = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, d, 0, 0, 0);
+    }
+  }
+  return d;
+}
+
+#define R3D_CM(k, n) (basis_neg(k, n) ? -c_mag[basis_j(k, n)] : c_mag[basis_j(k, n)])
+
+// 16-point inverse DCT (out[n] = sum_k cm[k][n] in[k]) as a partial butterfly
+// with fused multiply-adds: odd k (8 x 8), k = 2 mod 4 (4 x 4), k = 4, 12 and
+// k = 0, 8 (2 x 2 each), then the even/odd recombinations; 116 operations.
+__device__ __forceinline__ void idct16_fma(const float in[16], float out[16]) {
+  float o[8], eo[4], ee[4], e[8];
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    float acc = R3D_CM(1, n) * in[1];
+#pragma unroll
+    for (int k = 3; k < 16; k += 2) acc = __builtin_fmaf(R3D_CM(k, n), in[k], acc);
+    o[n] = acc;
+  }
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    float acc = R3D_CM(2, n) * in[2];
+#pragma unroll
+    for (int k = 6; k < 16; k += 4) acc = __builtin_fmaf(R3D_CM(k, n), in[k], acc);
+    eo[n] = acc;
+  }
+#pragma unroll
+  for (int n = 0; n < 2; n++) {
+    const float eee = __builtin_fmaf(R3D_CM(8, n), in[8], R3D_CM(0, n) * in[0]);
+    const float eeo = __builtin_fmaf(R3D_CM(12, n), in[12], R3D_CM(4, n) * in[4]);
+    ee[n] = eee + eeo;
+    ee[3 - n] = eee - eeo;
+  }
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    e[n] = ee[n] + eo[n];
+    e[7 - n] = ee[n] - eo[n];
+  }
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    out[n] = e[n] + o[n];
+    out[15 - n] = e[n] - o[n];
+  }
+}
+
+// o[idx] += v for a wave-uniform idx, keeping o in registers.
+__device__ __forceinline__ void add_at(float o[16], uint32_t idx, float v) {
+  switch (idx) {
+#define R3D_CASE(i) \
+  case i:           \
+    o[i] += v;      \
+    break;
+#define R3D_CASE8(i) R3D_CASE(i) R3D_CASE(i + 1) R3D_CASE(i + 2) R3D_CASE(i + 3) R3D_CASE(i + 4) \
+    R3D_CASE(i + 5) R3D_CASE(i + 6) R3D_CASE(i + 7)
+    R3D_CASE8(0) R3D_CASE8(8)
+#undef R3D_CASE8
+#undef R3D_CASE
+    default:
+      break;
+  }
+}
+
+__device__ __forceinline__ float lane_level(const uint4 q[4][2], int e, int x) {
+  const uint4 v = q[e][x >> 3];
+  const uint32_t w = ((x & 7) >> 1) == 0 ? v.x : ((x & 7) >> 1) == 1 ? v.y : ((x & 7) >> 1) == 2 ? v.z : v.w;
+  return (float)(int16_t)(w >> ((x & 1) * 16));
+}
+
+// Pass-3 LDS tile for PREC_F16, per wave: FP16 D2 rows (y', z') of 16
+// x-frequencies at dword y' * P3_SY + z' * P3_SZ. With P3_SZ = 10 and
+// P3_SY = 162 (both 2 mod 4 times an odd factor), the 64-bit writes (16 lanes
+// of one y', z' = 0..15) and reads (16 lanes of one z', y' = 0..15) each
+// touch 16 distinct even bank pairs.
+constexpr uint32_t P3_SZ = 10, P3_SY = 162, P3_WAVE = 16 * P3_SY;
+
+template <int PREC, bool SPARSE, bool F32OUT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_decode_mfma(DecodeArgs a) {
+  constexpr bool MFMA3 = PREC == PREC_F16;  // pass 3 on MFMA through LDS
+  __shared__ __attribute__((aligned(16))) int16_t lds_lv[SPARSE ? 4 * 4096 : 8];
+  __shared__ __attribute__((aligned(16))) uint32_t lds_p3[MFMA3 ? 4 * P3_WAVE : 4];
+  const uint32_t lane = threadIdx.x & 63u, r = lane & 15u, g = lane >> 4;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  float cb[4];  // cm[4g + e][r]: B of pass 1, A of passes 2 and 3
+#pragma unroll
+  for (int e = 0; e < 4; e++) cb[e] = c_basis[(4u * g + e) * 16u + r];
+  const h4 bh = to_h4(cb);
+  const h4 bl = lo_h4(cb, bh);
+  // step[z + y + x] = step[(4g + r) + (e + x)], times the scale; reloaded when
+  // a chunk names another table
+  uint32_t cur_table = 0xffffffffu;
+  float st[19];
+  float s = 1.0f;
+  for (uint32_t c = blockIdx.x * 4u + w; c < a.nchunks; c += gridDim.x * 4u) {
+    uint4 q[4][2];
+    if (!SPARSE) {  // issued before the header is examined
+      const int16_t *src = a.levels + (size_t)c * 4096u + r * 16u;
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++) q[e][hh] = *(const uint4 *)(src + (4u * g + e) * 256u + hh * 8u);
+    }
+    const r3d_chunk_header h = a.hdr[c];
+    const float dc = (float)h.dcq / 16.0f;
+    const bool ok = h.table < a.ntables && h.vspan > 0.0f && __builtin_isfinite(h.vmin) && __builtin_isfinite(dc);
+    uint8_t *dst = a.out + (a.out_off ? a.out_off[c] : (uint64_t)c * 4096u);
+    float *fdst = (float *)a.out + (size_t)c * 4096u;
+    if (!ok) {  // rejected header: zero-filled, status 0
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        if (F32OUT) {
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+            ((float4 *)(fdst + r * 256u + (4u * g + e) * 16u))[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        } else {
+          const uint32_t zw[4] = {0, 0, 0, 0};
+          store_row16(dst + r * a.sz + (4u * g + e) * a.sy, zw);
+        }
+      }
+      if (a.status && lane == 0) a.status[c] = 0;
+      continue;
+    }
+    if (SPARSE) {
+      int16_t *t = lds_lv + w * 4096u;
+#pragma unroll
+      for (int i = 0; i < 8; i++) ((uint4 *)t)[lane + 64u * i] = make_uint4(0, 0, 0, 0);
+      wave_sync();
+      const uint64_t p1 = a.pair_off[c + 1] < a.npairs ? a.pair_off[c + 1] : a.npairs;
+      for (uint64_t p = (uint64_t)a.pair_off[c] + lane; p < p1; p += 64u) {
+        const uint32_t pr = a.pairs[p];
+        t[pr & 4095u] = (int16_t)(pr >> 16);
+      }
+      wave_sync();
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++) q[e][hh] = *(const uint4 *)(t + (4u * g + e) * 256u + r * 16u + hh * 8u);
+      wave_sync();
+    }
+    if (h.table != cur_table) {
+      cur_table = h.table;
+      const float *tb = a.tables + (size_t)h.table * 46u + 4u * g + r;
+#pragma unroll
+      for (int o = 0; o < 19; o++) st[o] = tb[o];
+      if (PREC != PREC_F32) {
+        // |level| <= 32768, so |dequantised| <= 32768.2 * max step: the scale
+        // keeps every FP16 input of any chunk on this table below 2^13
+        float ms = st[0];
+#pragma unroll
+        for (int o = 1; o < 19; o++) ms = fmax_sel(ms, st[o]);
+        s = pow2_scale(32768.2f * wave_max(ms));
+#pragma unroll
+        for (int o = 0; o < 19; o++) st[o] *= s;
+      }
+    }
+    // passes 1 and 2, one x-frequency tile at a time
+    f4 d2[16];
+#pragma unroll
+    for (int x = 0; x < 16; x++) {
+      float v[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const float lf = lane_level(q, e, x);
+        v[e] = __builtin_fmaf(fclamp_sel(lf, -1.0f, 1.0f), 0.2f, lf) * st[e + x];
+      }
+      h4 vh = {}, vl = {};
+      if (PREC != PREC_F32) vh = to_h4(v);
+      if (PREC == PREC_F16X3) vl = lo_h4(v, vh);
+      const f4 d1 = mma<PREC>(v, cb, vh, vl, bh, bl);
+      const float d1v[4] = {d1[0], d1[1], d1[2], d1[3]};
+      h4 dh = {}, dl = {};
+      if (PREC != PREC_F32) dh = to_h4(d1v);
+      if (PREC == PREC_F16X3) dl = lo_h4(d1v, dh);
+      d2[x] = mma<PREC>(cb, d1v, bh, bl, dh, dl);
+    }
+    const float inv = h.vspan / 255.0f;
+    const float scale = inv / s;
+    const float bias = __builtin_fmaf(dc, inv, h.vmin);
+    uint64_t c0 = 0, nc = 0;
+    if (a.corr_off) {
+      c0 = a.corr_off[c];
+      const uint64_t c1 = a.corr_off[c + 1] < a.ncorr ? a.corr_off[c + 1] : a.ncorr;
+      nc = c1 > c0 ? (c1 - c0 > 4096u ? 4096u : c1 - c0) : 0;
+    }
+    if (MFMA3) {
+      // pass 3: D3 = C^T (A: row x', k = u) * D2 (B: k = u, column y') per z',
+      // B read back from LDS; lane (r, g) slot e gets voxel (z', y' = r, x' = 4g + e)
+      uint32_t *tw = lds_p3 + w * P3_WAVE;
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          const float hv[4] = {d2[4 * m][e], d2[4 * m + 1][e], d2[4 * m + 2][e], d2[4 * m + 3][e]};
+          *(h4 *)(tw + (4u * g + e) * P3_SY + r * P3_SZ + 2u * m) = to_h4(hv);
+        }
+      wave_sync();
+#pragma unroll 4
+      for (uint32_t zt = 0; zt < 16u; zt++) {
+        const h4 b3 = *(const h4 *)(tw + r * P3_SY + zt * P3_SZ + 2u * g);
+        const f4 d3 = __builtin_amdgcn_mfma_f32_16x16x16f16(bh, b3, (f4){0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+        float o[4] = {d3[0], d3[1], d3[2], d3[3]};
+        for (uint64_t i = 0; i < nc; i++) {  // corrections in stream order
+          const uint32_t pos = __builtin_amdgcn_readfirstlane(a.corr_pos[c0 + i]);
+          if (pos >= 4096u) break;
+          if ((pos >> 8) != zt) continue;
+          const float dv = a.corr_delta[c0 + i] * s;
+          const float add = r == ((pos >> 4) & 15u) && g == ((pos & 15u) >> 2) ? dv : 0.0f;
+          switch (pos & 3u) {
+            case 0: o[0] += add; break;
+            case 1: o[1] += add; break;
+            case 2: o[2] += add; break;
+            default: o[3] += add; break;
+          }
+        }
+        if (F32OUT) {
+          *(float4 *)(fdst + zt * 256u + r * 16u + 4u * g) =
+              make_float4(__builtin_fmaf(o[0], scale, bias), __builtin_fmaf(o[1], scale, bias),
+                          __builtin_fmaf(o[2], scale, bias), __builtin_fmaf(o[3], scale, bias));
+        } else {
+          uint32_t wd = 0;
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+            float t = __builtin_fmaf(o[e], scale, bias + 0.5f);  // round half up, clamp, NaN -> 0
+            t = fclamp_sel(t, 0.0f, 255.0f);
+            wd |= (uint32_t)t << (e * 8);
+          }
+          uint8_t *p = dst + zt * a.sz + r * a.sy + 4u * g;
+          if (((uintptr_t)p & 3u) == 0) {
+            *(uint32_t *)p = wd;
+          } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++) p[e] = (uint8_t)(wd >> (8 * e));
+          }
+        }
+      }
+      wave_sync();
+    } else {
+      // pass 3 in registers, corrections and output, one row group
+      // (z' = r, y' = 4g + e) at a time
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        float in[16], o[16];
+#pragma unroll
+        for (int x = 0; x < 16; x++) in[x] = d2[x][e];
+        idct16_fma(in, o);
+        for (uint64_t i = 0; i < nc; i++) {  // corrections in stream order
+          const uint32_t pos = __builtin_amdgcn_readfirstlane(a.corr_pos[c0 + i]);
+          if (pos >= 4096u) break;
+          const uint32_t y = (pos >> 4) & 15u;
+          if ((y & 3u) != (uint32_t)e) continue;
+          const float dv = a.corr_delta[c0 + i] * s;
+          add_at(o, pos & 15u, r == (pos >> 8) && g == (y >> 2) ? dv : 0.0f);
+        }
+        if (F32OUT) {
+          float4 *fd = (float4 *)(fdst + r * 256u + (4u * g + e) * 16u);
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+            fd[i] = make_float4(__builtin_fmaf(o[4 * i], scale, bias), __builtin_fmaf(o[4 * i + 1], scale, bias),
+                                __builtin_fmaf(o[4 * i + 2], scale, bias), __builtin_fmaf(o[4 * i + 3], scale, bias));
+        } else {
+          uint32_t ww[4] = {0, 0, 0, 0};
+#pragma unroll
+          for (int x = 0; x < 16; x++) {
+            float t = __builtin_fmaf(o[x], scale, bias + 0.5f);  // round half up, clamp, NaN -> 0
+            t = fclamp_sel(t, 0.0f, 255.0f);
+            ww[x >> 2] |= (uint32_t)t << ((x & 3) * 8);
+          }
+          store_row16(dst + r * a.sz + (4u * g + e) * a.sy, ww);
+        }
+      }
+    }
+    if (a.status && lane == 0) a.status[c] = 1;
+  }
+}
+
 static uint32_t grid_for(const r3d_hip_ctx *ctx, uint32_t n, uint32_t per_cu) {
   uint64_t cap = (uint64_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * per_cu;
   return (uint32_t)(n < cap ? n : cap);
@@ -409,8 +782,13 @@ extern "C" int r3d_hip_decode_chunks(r3d_hip_ctx *ctx, const r3d_chunk_batch *b,
   if (sy < 16u || sz < 16u * sy)
     return r3d_fail(ctx, R3D_ERR_INVALID, "decode: strides (%u, %u) overlap rows", sy, sz);
   const uint32_t variant = opts ? opts->variant : R3D_DECODE_EXACT_F32;
-  if (variant != R3D_DECODE_EXACT_F32)
-    return r3d_fail(ctx, R3D_ERR_UNSUPPORTED, "decode: variant %u not built", variant);
+  if (variant > R3D_DECODE_MFMA_F16X3)
+    return r3d_fail(ctx, R3D_ERR_UNSUPPORTED, "decode: unknown variant %u", variant);
+  const bool f32out = opts && (opts->flags & R3D_DECODE_F32_OUT);
+  if (opts && (opts->flags & ~R3D_DECODE_F32_OUT))
+    return r3d_fail(ctx, R3D_ERR_INVALID, "decode: unknown flags 0x%x", opts->flags);
+  if (f32out && (b->out_offsets || b->out_stride_y || b->out_stride_z || ((uintptr_t)b->out & 15u)))
+    return r3d_fail(ctx, R3D_ERR_INVALID, "decode: FP32 output is flat and 16-byte aligned");
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   DecodeArgs a;
   a.nchunks = b->nchunks;
@@ -431,8 +809,25 @@ extern "C" int r3d_hip_decode_chunks(r3d_hip_ctx *ctx, const r3d_chunk_batch *b,
   a.sy = sy;
   a.sz = sz;
   a.status = b->status;
-  const uint32_t grid = grid_for(ctx, b->nchunks, 1u << 16);
-  hipLaunchKernelGGL(k_decode_exact, dim3(grid), dim3(256), 0, (hipStream_t)stream, a);
+  hipStream_t st = (hipStream_t)stream;
+  if (variant == R3D_DECODE_EXACT_F32) {
+    const uint32_t grid = grid_for(ctx, b->nchunks, 1u << 16);
+    if (f32out) hipLaunchKernelGGL(k_decode_exact<true>, dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_decode_exact<false>, dim3(grid), dim3(256), 0, st, a);
+  } else {
+    // four chunks (waves) per workgroup; 3 waves per SIMD are resident
+    const uint32_t grid = grid_for(ctx, (b->nchunks + 3u) / 4u, 8u);
+    const bool sparse = b->coef_format == R3D_COEF_SPARSE;
+    const int k = (int)(variant - 1u) * 4 + (sparse ? 2 : 0) + (f32out ? 1 : 0);
+    static void (*const kernels[12])(DecodeArgs) = {
+        k_decode_mfma<PREC_F32, false, false>,   k_decode_mfma<PREC_F32, false, true>,
+        k_decode_mfma<PREC_F32, true, false>,    k_decode_mfma<PREC_F32, true, true>,
+        k_decode_mfma<PREC_F16, false, false>,   k_decode_mfma<PREC_F16, false, true>,
+        k_decode_mfma<PREC_F16, true, false>,    k_decode_mfma<PREC_F16, true, true>,
+        k_decode_mfma<PREC_F16X3, false, false>, k_decode_mfma<PREC_F16X3, false, true>,
+        k_decode_mfma<PREC_F16X3, true, false>,  k_decode_mfma<PREC_F16X3, true, true>};
+    hipLaunchKernelGGL(kernels[k], dim3(grid), dim3(256), 0, st, a);
+  }
   R3D_HIP_CHECK(ctx, hipGetLastError());
   return R3D_OK;
 }

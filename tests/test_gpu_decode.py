@@ -161,3 +161,82 @@ def test_corrupt_sparse_offsets_stay_in_bounds(ctx):
                             pair_offsets=cuda(offs))
     torch.cuda.synchronize()
     assert out.shape == (2, 4096)
+
+
+def fp64_reference(levels, hdr, step):
+    """FP64 decode of dequantised levels (deq_one in FP32 as dct3d.c:149-154,
+    then an FP64 separable iDCT, + dc, denormalise)."""
+    cm = orc.basis().astype(np.float64).reshape(16, 16)
+    n = levels.shape[0]
+    out = np.zeros((n, 4096))
+    for i in range(n):
+        coef = np.zeros(4096, np.float32)
+        orc.load().r3do_dequant(np.ascontiguousarray(levels[i]), np.ascontiguousarray(step, np.float32), coef)
+        blk = np.einsum("az,by,cx,abc->zyx", cm, cm, cm, coef.astype(np.float64).reshape(16, 16, 16)).reshape(-1)
+        vmin = hdr[i, 0:1].view(np.float32)[0]
+        vspan = hdr[i, 1:2].view(np.float32)[0]
+        out[i] = (blk + hdr[i, 2] / 16.0) * (vspan / 255.0) + vmin
+    return out
+
+
+def decode_f32(ctx, levels, hdr, step, variant):
+    out = ctx.decode_chunks(headers=hdr, step_tables=step, levels=levels, variant=variant, f32_out=True)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def test_exact_f32_output_is_the_pre_round_value(ctx):
+    n = 64
+    vol = ctx.synth_sheets(n, n, n, 3, 71)
+    step = cuda(orc.step_linear())
+    levels, hdr = ctx.encode_chunks(vol, n, n, n, step)
+    got = decode_f32(ctx, levels, hdr, step, 0)
+    lv, hd = levels.cpu().numpy(), hdr.cpu().numpy()
+    for i in range(0, lv.shape[0], 7):
+        ok, _, pre = orc.decode_chunk(lv[i], hd[i, 0:1].view(np.float32)[0], hd[i, 1:2].view(np.float32)[0],
+                                      hd[i, 2], orc.step_linear())
+        assert np.array_equal(got[i].view(np.uint32), pre.view(np.uint32)), i
+
+
+@pytest.mark.parametrize("variant,tol_f32,max_u8,frac_u8", [(1, 2e-3, 1, 2e-3), (3, 2e-3, 1, 2e-3),
+                                                             (2, 0.75, 1, 0.6)])
+def test_mfma_variants_error_against_fp64(ctx, variant, tol_f32, max_u8, frac_u8):
+    n = 64
+    vol = ctx.synth_sheets(n, n, n, 3, 1234)
+    step = cuda(orc.step_linear())
+    levels, hdr = ctx.encode_chunks(vol, n, n, n, step)
+    lv, hd = levels.cpu().numpy(), hdr.cpu().numpy()
+    got = decode_f32(ctx, levels, hdr, step, variant)
+    ref = fp64_reference(lv, hd, orc.step_linear())
+    err = np.abs(got - ref)
+    assert err.max() < tol_f32
+    exact = ctx.decode_chunks(headers=hdr, step_tables=step, levels=levels).cpu().numpy().astype(int)
+    approx = ctx.decode_chunks(headers=hdr, step_tables=step, levels=levels, variant=variant).cpu().numpy().astype(int)
+    d = np.abs(approx - exact)
+    print(f"variant {variant}: max |f32 - fp64| = {err.max():.3g}, mean {err.mean():.3g}; "
+          f"u8 differing from exact: {(d > 0).mean():.3g}")
+    assert d.max() <= max_u8 and (d > 0).mean() <= frac_u8
+
+
+def test_mfma_sparse_matches_dense(ctx):
+    n = 16
+    rng = np.random.default_rng(3)
+    lv = (rng.integers(-40, 40, (n, 4096)) * (rng.random((n, 4096)) < 0.08)).astype(np.int16)
+    hdr = orc.hdr_array(rng.uniform(0, 20, n), rng.uniform(100, 255, n), rng.integers(0, 3000, n))
+    offs, pairs = to_pairs(lv, rng)
+    for variant in (1, 2, 3):
+        a = ctx.decode_chunks(headers=cuda(hdr), step_tables=cuda(orc.step_linear()), levels=cuda(lv),
+                              variant=variant).cpu().numpy()
+        b = ctx.decode_chunks(headers=cuda(hdr), step_tables=cuda(orc.step_linear()), pairs=cuda(pairs),
+                              pair_offsets=cuda(offs), variant=variant).cpu().numpy()
+        assert np.array_equal(a, b), variant
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_mfma_golden_blobs_with_corrections_within_one(ctx, variant):
+    n, hdr, steps, lv = golden_batch()
+    out = ctx.decode_chunks(headers=cuda(hdr), step_tables=cuda(steps), levels=cuda(lv),
+                            corr_offsets=cuda(GOLD["blob_corr_off"]), corr_pos=cuda(GOLD["blob_corr_pos"]),
+                            corr_delta=cuda(GOLD["blob_corr_delta"]), variant=variant)
+    d = np.abs(out.cpu().numpy().astype(int) - GOLD["blob_u8"].astype(int))
+    assert d.max() <= 1
