@@ -279,6 +279,43 @@ int r3d_hip_raycast(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_frame
                     uint32_t nviews, const r3d_hip_image *out, const r3d_raycast_opts *opts,
                     r3d_hip_stats *stats, void *stream);
 
+/* Launch shape of the raycast kernel: the workgroup renders tile_w x
+ * tile_h-pixel tiles (8x8, 16x4, 16x8 or 16x16); blocks_per_cu > 0 makes
+ * the launch persistent, with that many workgroups per CU taking tiles from a
+ * queue (at most 32); 0 launches one workgroup per tile. ms is filled by the
+ * tuner. The reference dispatches 16x8 workgroups, one per tile
+ * (src/vk/vkbackend.c:7121-7173). */
+typedef struct r3d_raycast_tuning {
+  uint32_t tile_w, tile_h;
+  uint32_t blocks_per_cu;
+  float ms;
+} r3d_raycast_tuning;
+
+/* Launch shape used by r3d_hip_raycast (default 8x8, one workgroup per tile). */
+int r3d_hip_set_raycast_tuning(r3d_hip_ctx *ctx, const r3d_raycast_tuning *t);
+
+/* A raycast plan holds a fixed view set and output on the device, so that a
+ * launch is kernel launches only (no copies, allocations or host waits) and
+ * can be captured into a hipGraph. No counterpart in the reference (its
+ * Vulkan dispatch records a command buffer per frame). The plan keeps
+ * pointers to vol, the image buffers, opts->tf, opts->counters (stats
+ * accumulate there on every launch) and opts->px_steps; they must stay valid
+ * while the plan is used. */
+typedef struct r3d_hip_raycast_plan r3d_hip_raycast_plan;
+int r3d_hip_raycast_plan_create(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_frame_params *views,
+                                uint32_t nviews, const r3d_hip_image *out, const r3d_raycast_opts *opts,
+                                r3d_hip_raycast_plan **plan);
+int r3d_hip_raycast_plan_launch(r3d_hip_raycast_plan *plan, void *stream);
+/* Autotuner: times each tile shape at one workgroup per tile and at 2, 4, 8
+ * and 16 persistent workgroups per CU, keeps the fastest in the plan and
+ * writes up to max_results measurements. Synchronises `stream`: call it
+ * before capturing the plan, not during. */
+int r3d_hip_raycast_plan_tune(r3d_hip_raycast_plan *plan, void *stream, r3d_raycast_tuning *results,
+                              uint32_t max_results, uint32_t *nresults);
+int r3d_hip_raycast_plan_set_tuning(r3d_hip_raycast_plan *plan, const r3d_raycast_tuning *t);
+int r3d_hip_raycast_plan_get_tuning(const r3d_hip_raycast_plan *plan, r3d_raycast_tuning *t);
+void r3d_hip_raycast_plan_destroy(r3d_hip_raycast_plan *plan);
+
 /* ---- differentiable raycast -------------------------------------------------- */
 
 /* No counterpart in the reference (it has no differentiable path). The

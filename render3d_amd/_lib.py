@@ -119,6 +119,10 @@ class RaycastOpts(C.Structure):
                 ("counters", C.c_void_p), ("px_steps", C.c_void_p)]
 
 
+class RaycastTuning(C.Structure):
+    _fields_ = [("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("ms", C.c_float)]
+
+
 class DiffParams(C.Structure):
     _fields_ = [("tf", C.c_void_p), ("grid", C.c_void_p), ("grid_dims", C.c_uint32 * 3), ("alpha_cutoff", C.c_float),
                 ("workspace", C.c_void_p), ("workspace_bytes", C.c_size_t)]
@@ -148,6 +152,14 @@ SIGNATURES = {
     "r3d_hip_set_transfer_f32": (C.c_int, [P, P]),
     "r3d_hip_raycast": (C.c_int, [P, P, C.POINTER(FrameParams), C.c_uint32, C.POINTER(Image),
                                   C.POINTER(RaycastOpts), C.POINTER(Stats), P]),
+    "r3d_hip_set_raycast_tuning": (C.c_int, [P, C.POINTER(RaycastTuning)]),
+    "r3d_hip_raycast_plan_create": (C.c_int, [P, P, C.POINTER(FrameParams), C.c_uint32, C.POINTER(Image),
+                                              C.POINTER(RaycastOpts), C.POINTER(P)]),
+    "r3d_hip_raycast_plan_launch": (C.c_int, [P, P]),
+    "r3d_hip_raycast_plan_tune": (C.c_int, [P, P, C.POINTER(RaycastTuning), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "r3d_hip_raycast_plan_set_tuning": (C.c_int, [P, C.POINTER(RaycastTuning)]),
+    "r3d_hip_raycast_plan_get_tuning": (C.c_int, [P, C.POINTER(RaycastTuning)]),
+    "r3d_hip_raycast_plan_destroy": (None, [P]),
     "r3d_hip_diff_workspace_bytes": (C.c_size_t, [C.POINTER(FrameParams), C.c_uint32]),
     "r3d_hip_raycast_diff": (C.c_int, [P, P, C.POINTER(FrameParams), C.c_uint32, C.POINTER(Image),
                                        C.POINTER(DiffParams), P]),

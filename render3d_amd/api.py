@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (ChunkBatch, DecodeOpts, FrameParams, Image, R3DError, RaycastOpts, Stats,
+from ._lib import (ChunkBatch, DecodeOpts, FrameParams, Image, R3DError, RaycastOpts, RaycastTuning, Stats,
                    VolumeDesc)
 
 try:
@@ -179,6 +179,75 @@ class Context:
         if stats:
             res["stats"] = {"samples": st.samples, "skips": st.skips, "shaded": st.shaded, "rays": st.rays}
         return res
+
+    def set_raycast_tuning(self, tile_w, tile_h, blocks_per_cu=0):
+        """Launch shape of raycast() (r3d_hip_set_raycast_tuning)."""
+        self.check(self.lib.r3d_hip_set_raycast_tuning(self.h, C.byref(RaycastTuning(tile_w, tile_h,
+                                                                                     blocks_per_cu, 0.0))))
+
+    def raycast_plan(self, vol, views, width, height, *, alpha_cutoff=0.98, rgba=True, rgba8=False, tf=None,
+                     counters=None, px_steps=False):
+        """A capture-safe RaycastPlan over fixed views; owns its output tensors."""
+        return RaycastPlan(self, vol, views, width, height, alpha_cutoff=alpha_cutoff, rgba=rgba, rgba8=rgba8,
+                           tf=tf, counters=counters, px_steps=px_steps)
+
+
+class RaycastPlan:
+    """r3d_hip_raycast_plan: views and output fixed at creation; launch() is
+    kernel launches only, so it can be captured into a HIP graph."""
+
+    def __init__(self, ctx, vol, views, width, height, *, alpha_cutoff, rgba, rgba8, tf, counters, px_steps):
+        if isinstance(views, FrameParams):
+            views = [views]
+        self.ctx = ctx
+        self.vol = vol
+        dev = f"cuda:{ctx.device}"
+        self.rgba = torch.zeros((height, width, 4), dtype=torch.float32, device=dev) if rgba else None
+        self.rgba8 = torch.zeros((height, width, 4), dtype=torch.uint8, device=dev) if rgba8 else None
+        self.px_steps = torch.zeros((height, width), dtype=torch.int32, device=dev) if px_steps else None
+        self.tf = tf
+        self.counters = counters
+        arr = (FrameParams * len(views))(*views)
+        im = Image(None if self.rgba is None else self.rgba.data_ptr(),
+                   None if self.rgba8 is None else self.rgba8.data_ptr(), width, height)
+        opts = RaycastOpts(alpha_cutoff, 0, None if tf is None else tf.data_ptr(),
+                           None if counters is None else counters.data_ptr(),
+                           None if self.px_steps is None else self.px_steps.data_ptr())
+        h = C.c_void_p()
+        ctx.check(ctx.lib.r3d_hip_raycast_plan_create(ctx.h, vol.h, arr, len(views), C.byref(im), C.byref(opts),
+                                                      C.byref(h)))
+        self.h = h
+
+    def launch(self, stream=None):
+        self.ctx.check(self.ctx.lib.r3d_hip_raycast_plan_launch(self.h, _stream(stream)))
+
+    def tune(self, stream=None):
+        """Run the autotuner; returns every measured shape as a dict (the fastest is kept)."""
+        res = (RaycastTuning * 64)()
+        n = C.c_uint32()
+        self.ctx.check(self.ctx.lib.r3d_hip_raycast_plan_tune(self.h, _stream(stream), res, 64, C.byref(n)))
+        return [{"tile": (r.tile_w, r.tile_h), "blocks_per_cu": r.blocks_per_cu, "ms": r.ms} for r in res[:n.value]]
+
+    @property
+    def tuning(self):
+        t = RaycastTuning()
+        self.ctx.check(self.ctx.lib.r3d_hip_raycast_plan_get_tuning(self.h, C.byref(t)))
+        return {"tile": (t.tile_w, t.tile_h), "blocks_per_cu": t.blocks_per_cu, "ms": t.ms}
+
+    def set_tuning(self, tile_w, tile_h, blocks_per_cu=0):
+        self.ctx.check(self.ctx.lib.r3d_hip_raycast_plan_set_tuning(
+            self.h, C.byref(RaycastTuning(tile_w, tile_h, blocks_per_cu, 0.0))))
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.r3d_hip_raycast_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Volume:

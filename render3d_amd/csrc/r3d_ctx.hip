@@ -44,7 +44,8 @@ extern "C" int r3d_hip_init(r3d_hip_ctx **out, int device) {
   build_basis(c->basis);
   int rc = R3D_OK;
   if (hipMalloc(&c->d_tf, 256 * 4 * sizeof(float)) != hipSuccess ||
-      hipMalloc(&c->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc(&c->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&c->d_queue, 64 * sizeof(uint32_t)) != hipSuccess) {
     rc = R3D_ERR_NOMEM;
   }
   if (rc == R3D_OK) rc = r3d::upload_basis(c);
@@ -77,6 +78,7 @@ extern "C" void r3d_hip_destroy(r3d_hip_ctx *c) {
   }
   if (c->d_tf) hipFree(c->d_tf);
   if (c->d_counters) hipFree(c->d_counters);
+  if (c->d_queue) hipFree(c->d_queue);
   if (c->d_partials) hipFree(c->d_partials);
   if (c->d_gtf) hipFree(c->d_gtf);
   delete c;

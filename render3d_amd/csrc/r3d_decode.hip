@@ -553,8 +553,10 @@ constexpr uint32_t P3_SZ = 10, P3_SY = 162, P3_WAVE = 16 * P3_SY;
 template <int PREC, bool SPARSE, bool F32OUT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_decode_mfma(DecodeArgs a) {
   constexpr bool MFMA3 = PREC == PREC_F16;  // pass 3 on MFMA through LDS
-  __shared__ __attribute__((aligned(16))) int16_t lds_lv[SPARSE ? 4 * 4096 : 8];
-  __shared__ __attribute__((aligned(16))) uint32_t lds_p3[MFMA3 ? 4 * P3_WAVE : 4];
+  // per wave: the sparse level tile (2048 words, used while loading) and the
+  // pass-3 tile (used after the levels are in registers) share one region
+  constexpr uint32_t WAVE_WORDS = (MFMA3 && P3_WAVE > 2048u) ? P3_WAVE : (SPARSE || MFMA3 ? 2048u : 4u);
+  __shared__ __attribute__((aligned(16))) uint32_t lds_w[4 * WAVE_WORDS];
   const uint32_t lane = threadIdx.x & 63u, r = lane & 15u, g = lane >> 4;
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   float cb[4];  // cm[4g + e][r]: B of pass 1, A of passes 2 and 3
@@ -597,7 +599,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       continue;
     }
     if (SPARSE) {
-      int16_t *t = lds_lv + w * 4096u;
+      int16_t *t = (int16_t *)(lds_w + w * WAVE_WORDS);
 #pragma unroll
       for (int i = 0; i < 8; i++) ((uint4 *)t)[lane + 64u * i] = make_uint4(0, 0, 0, 0);
       wave_sync();
@@ -661,7 +663,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     if (MFMA3) {
       // pass 3: D3 = C^T (A: row x', k = u) * D2 (B: k = u, column y') per z',
       // B read back from LDS; lane (r, g) slot e gets voxel (z', y' = r, x' = 4g + e)
-      uint32_t *tw = lds_p3 + w * P3_WAVE;
+      uint32_t *tw = lds_w + w * WAVE_WORDS;
 #pragma unroll
       for (int e = 0; e < 4; e++)
 #pragma unroll

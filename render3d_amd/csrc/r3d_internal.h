@@ -56,6 +56,8 @@ struct r3d_hip_ctx {
   size_t gtf_cap = 0;                        // bytes
   r3d::ViewSlot slots[4];
   int slot_next = 0;
+  r3d_raycast_tuning tuning{8u, 8u, 0u, 0.0f};  // r3d_hip_raycast's launch shape
+  uint32_t *d_queue = nullptr;                   // tile queues of r3d_hip_raycast, one per kind
 };
 
 struct r3d_hip_volume {

@@ -1,8 +1,10 @@
 // r3d_raycast.hip -- cube-mode raycaster for gfx950.
 //
-// One lane per ray, 8x8-pixel tiles as 64-lane workgroups (one wave), every
-// view of a call in one launch per mode class (tile -> view through a prefix
-// table). Per lane the march is src/shaders/raycast.slang main() with
+// One lane per ray; a workgroup renders 8x8, 16x4, 16x8 or 16x16-pixel
+// tiles, either one tile per workgroup or persistently from a tile queue (the
+// shape and the persistence are chosen per plan by r3d_hip_raycast_plan_tune).
+// Every view of a call renders in one launch per mode class (tile -> view
+// through a prefix table). Per lane the march is src/shaders/raycast.slang main() with
 // R3D_MODE 0 (lines 505-1039): ray setup, ray-cone LOD, trilinear + linear-mip
 // sample, low cut, dilated-occupancy skip, TF lookup, opacity correction,
 // 6-tap gradient Blinn-Phong, front-to-back compositing, early termination.
@@ -20,7 +22,10 @@
 // trilinear footprint are fixed offsets from one brick address, so a sample
 // costs one address computation and 8 byte loads; the TF table and the level
 // table sit in LDS.
+#include <stdlib.h>
 #include <string.h>
+
+#include <new>
 
 #include "r3d_ray.h"
 
@@ -36,8 +41,10 @@ struct RayArgs {
   uint8_t *img8;
   uint32_t img_w, img_h;
   float cutoff;
-  uint32_t *partials;  // [tile][4] counter sums of this launch's tiles, or NULL
+  uint32_t *partials;  // [workgroup][4] counter sums of this launch, or NULL
   uint32_t *px_steps;
+  uint32_t ntiles;     // tiles of this launch
+  uint32_t *queue;     // tile counter of a persistent launch, or NULL
 };
 
 struct RayOut {
@@ -205,69 +212,118 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
-template <int KIND>
-__global__ __launch_bounds__(64) void k_raycast(RayArgs a) {
+// Workgroup tile shapes in pixels, one lane per pixel.
+constexpr int NSHAPES = 4;
+constexpr uint32_t SHAPE_W[NSHAPES] = {8u, 16u, 16u, 16u};
+constexpr uint32_t SHAPE_H[NSHAPES] = {8u, 4u, 8u, 16u};
+constexpr uint32_t MAX_BLOCKS_PER_CU = 32u;
+
+static int shape_index(uint32_t w, uint32_t h) {
+  for (int i = 0; i < NSHAPES; i++)
+    if (SHAPE_W[i] == w && SHAPE_H[i] == h) return i;
+  return -1;
+}
+
+// One workgroup renders TW x TH-pixel tiles of the launch's views. With
+// a.queue == NULL the workgroup renders tile blockIdx.x; otherwise it is
+// persistent and takes tiles from the queue counter until they run out (the
+// TF and level tables are then staged once per workgroup, not once per tile).
+// Counter partials: one uint4 per workgroup.
+template <int KIND, int TW, int TH>
+__global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
+  constexpr uint32_t NT = TW * TH, NW = NT / 64u;
   __shared__ float4 s_tf[256];
   __shared__ LevelDev s_lev[R3D_MAX_LEVELS];
+  __shared__ uint32_t s_ts[R3D_MAX_VIEWS_PER_CALL + 1];
+  __shared__ uint32_t s_tile;
+  __shared__ uint4 s_red[NW];
   const uint32_t lane = threadIdx.x;
-  for (uint32_t i = lane; i < 256u; i += 64u)
+  for (uint32_t i = lane; i < 256u; i += NT)
     s_tf[i] = make_float4(a.tf[i * 4 + 0], a.tf[i * 4 + 1], a.tf[i * 4 + 2], a.tf[i * 4 + 3]);
   if (lane < a.vol.nlev) s_lev[lane] = a.vol.levels[lane];
+  for (uint32_t i = lane; i <= a.nviews; i += NT) s_ts[i] = a.tile_start[i];
   __syncthreads();
-  // tile -> view
-  const uint32_t tile = blockIdx.x;
-  uint32_t v = 0;
-  while (v + 1u < a.nviews && a.tile_start[v + 1] <= tile) v++;
-  const r3d_frame_params &pc = a.views[v];
-  const bool half = (pc.view_flags & 32u) != 0u;
-  const uint32_t W = pc.viewport[0], H = pc.viewport[1];
-  const uint32_t TW = half ? (W + 1u) / 2u : W, TH = half ? (H + 1u) / 2u : H;
-  const uint32_t tiles_x = (TW + TILE - 1u) / TILE;
-  const uint32_t lt = tile - a.tile_start[v];
-  const uint32_t tx = (lt % tiles_x) * TILE + (lane % TILE);
-  const uint32_t ty = (lt / tiles_x) * TILE + (lane / TILE);
+  const Sampler S{a.vol.base, s_lev, a.vol.nlev};
   uint32_t st = 0, sk = 0, sh = 0, nr = 0;
-  if (tx < TW && ty < TH) {
-    const uint32_t lx = half ? tx * 2u : tx, ly = half ? ty * 2u : ty;
-    Sampler S{a.vol.base, s_lev, a.vol.nlev};
-    const RayOut R = march<KIND>(S, a.vol, s_tf, pc, a.cutoff, lx, ly);
-    st = R.steps;
-    sk = R.skips;
-    sh = R.shaded;
-    nr = 1;
-    const uint32_t ox = pc.view_org & 0xffffu, oy = pc.view_org >> 16;
-    const uint32_t rep = half ? 2u : 1u;
-    for (uint32_t dy = 0; dy < rep; dy++)
-      for (uint32_t dx = 0; dx < rep; dx++) {
-        const uint32_t qx = lx + dx, qy = ly + dy;
-        if (qx >= W || qy >= H) continue;
-        const uint32_t gx = ox + qx, gy = oy + qy;
-        if (gx >= a.img_w || gy >= a.img_h) continue;
-        const size_t pi = (size_t)gy * a.img_w + gx;
-        if (a.img) ((float4 *)a.img)[pi] = make_float4(R.c[0], R.c[1], R.c[2], R.c[3]);
-        if (a.img8) {
-          uint32_t w = 0;
+  uint32_t tile = blockIdx.x;
+  for (;;) {
+    if (a.queue) {
+      if (lane == 0) s_tile = atomicAdd(a.queue, 1u);
+      __syncthreads();
+      tile = __builtin_amdgcn_readfirstlane(s_tile);
+      __syncthreads();
+    }
+    if (tile >= a.ntiles) break;
+    // tile -> view: the last v with tile_start[v] <= tile
+    uint32_t lo = 0, hi = a.nviews;
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_ts[mid] <= tile) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t v = __builtin_amdgcn_readfirstlane(lo);
+    const r3d_frame_params &pc = a.views[v];
+    const bool half = (pc.view_flags & 32u) != 0u;
+    const uint32_t W = pc.viewport[0], H = pc.viewport[1];
+    const uint32_t RW = half ? (W + 1u) / 2u : W, RH = half ? (H + 1u) / 2u : H;
+    const uint32_t tiles_x = (RW + TW - 1u) / TW;
+    const uint32_t lt = tile - s_ts[v];
+    const uint32_t tx = (lt % tiles_x) * TW + (lane % TW);
+    const uint32_t ty = (lt / tiles_x) * TH + (lane / TW);
+    if (tx < RW && ty < RH) {
+      const uint32_t lx = half ? tx * 2u : tx, ly = half ? ty * 2u : ty;
+      const RayOut R = march<KIND>(S, a.vol, s_tf, pc, a.cutoff, lx, ly);
+      st += R.steps;
+      sk += R.skips;
+      sh += R.shaded;
+      nr += 1u;
+      const uint32_t ox = pc.view_org & 0xffffu, oy = pc.view_org >> 16;
+      const uint32_t rep = half ? 2u : 1u;
+      for (uint32_t dy = 0; dy < rep; dy++)
+        for (uint32_t dx = 0; dx < rep; dx++) {
+          const uint32_t qx = lx + dx, qy = ly + dy;
+          if (qx >= W || qy >= H) continue;
+          const uint32_t gx = ox + qx, gy = oy + qy;
+          if (gx >= a.img_w || gy >= a.img_h) continue;
+          const size_t pi = (size_t)gy * a.img_w + gx;
+          if (a.img) ((float4 *)a.img)[pi] = make_float4(R.c[0], R.c[1], R.c[2], R.c[3]);
+          if (a.img8) {
+            uint32_t w = 0;
 #pragma unroll
-          for (int c = 0; c < 4; c++) w |= (uint32_t)(fsat(R.c[c]) * 255.0f + 0.5f) << (8 * c);
-          ((uint32_t *)a.img8)[pi] = w;
+            for (int c = 0; c < 4; c++) w |= (uint32_t)(fsat(R.c[c]) * 255.0f + 0.5f) << (8 * c);
+            ((uint32_t *)a.img8)[pi] = w;
+          }
+          if (a.px_steps) a.px_steps[pi] = R.steps;
         }
-        if (a.px_steps) a.px_steps[pi] = R.steps;
-      }
+    }
+    if (!a.queue) break;
   }
-  if (a.partials) {  // one plain store per tile; k_reduce_counters sums them
+  if (a.partials) {  // one plain store per workgroup; k_reduce_counters sums them
     st = wave_sum(st);
     sk = wave_sum(sk);
     sh = wave_sum(sh);
     nr = wave_sum(nr);
-    if (lane == 0) ((uint4 *)a.partials)[tile] = make_uint4(st, sk, sh, nr);
+    if ((lane & 63u) == 0) s_red[lane >> 6] = make_uint4(st, sk, sh, nr);
+    __syncthreads();
+    if (lane == 0) {
+      uint4 t = s_red[0];
+#pragma unroll
+      for (uint32_t w = 1; w < NW; w++) {
+        t.x += s_red[w].x;
+        t.y += s_red[w].y;
+        t.z += s_red[w].z;
+        t.w += s_red[w].w;
+      }
+      ((uint4 *)a.partials)[blockIdx.x] = t;
+    }
   }
 }
 
-__global__ __launch_bounds__(256) void k_reduce_counters(const uint32_t *partials, uint32_t ntiles,
+__global__ __launch_bounds__(256) void k_reduce_counters(const uint32_t *partials, uint32_t n,
                                                          unsigned long long *counters) {
   __shared__ unsigned long long red[4][256];
   unsigned long long acc[4] = {0, 0, 0, 0};
-  for (uint32_t i = threadIdx.x; i < ntiles; i += 256u) {
+  for (uint32_t i = threadIdx.x; i < n; i += 256u) {
     const uint4 p = ((const uint4 *)partials)[i];
     acc[0] += p.x;
     acc[1] += p.y;
@@ -288,6 +344,17 @@ __global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
   if (threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+  if (threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+typedef void (*RayKernel)(RayArgs);
+#define R3D_RAY_SHAPES(K)                                                                        \
+  { k_raycast<K, 8, 8>, k_raycast<K, 16, 4>, k_raycast<K, 16, 8>, k_raycast<K, 16, 16> }
+static const RayKernel ray_kernels[NKINDS][NSHAPES] = {R3D_RAY_SHAPES(KIND_FULL), R3D_RAY_SHAPES(KIND_MIP),
+                                                       R3D_RAY_SHAPES(KIND_FLAT), R3D_RAY_SHAPES(KIND_OTHER)};
+#undef R3D_RAY_SHAPES
+
 static int kind_of(uint32_t mode) {
   switch (mode) {
   case MODE_FULL: return KIND_FULL;
@@ -295,6 +362,143 @@ static int kind_of(uint32_t mode) {
   case MODE_FLAT: return KIND_FLAT;
   default: return KIND_OTHER;
   }
+}
+
+// Views grouped by kernel kind, with one tile prefix table per shape.
+// Device layout: [views by kind][NSHAPES x (nviews + NKINDS) tile starts].
+struct ViewGroups {
+  uint32_t nviews = 0;
+  uint32_t gstart[NKINDS + 1] = {};
+  uint32_t tpos[NKINDS] = {};             // offset of kind k's table inside a shape's table
+  uint32_t gtiles[NSHAPES][NKINDS] = {};  // tiles of kind k at shape s
+  size_t view_bytes() const { return sizeof(r3d_frame_params) * nviews; }
+  size_t table_words() const { return (size_t)nviews + NKINDS; }
+  size_t bytes() const { return view_bytes() + sizeof(uint32_t) * table_words() * NSHAPES; }
+};
+
+static void group_views(const r3d_frame_params *views, uint32_t nviews, uint8_t *buf, ViewGroups *g) {
+  g->nviews = nviews;
+  r3d_frame_params *pv = (r3d_frame_params *)buf;
+  uint32_t *tab = (uint32_t *)(buf + g->view_bytes());
+  uint32_t n = 0, tcur = 0;
+  uint32_t tiles[NSHAPES];
+  for (int k = 0; k < NKINDS; k++) {
+    g->gstart[k] = n;
+    g->tpos[k] = tcur;
+    for (int s = 0; s < NSHAPES; s++) tiles[s] = 0;
+    for (uint32_t i = 0; i < nviews; i++) {
+      if (kind_of(views[i].mode) != k) continue;
+      pv[n] = views[i];
+      const bool half = (views[i].view_flags & 32u) != 0u;
+      const uint32_t RW = half ? (views[i].viewport[0] + 1u) / 2u : views[i].viewport[0];
+      const uint32_t RH = half ? (views[i].viewport[1] + 1u) / 2u : views[i].viewport[1];
+      for (int s = 0; s < NSHAPES; s++) {
+        tab[s * g->table_words() + tcur] = tiles[s];
+        tiles[s] += ((RW + SHAPE_W[s] - 1u) / SHAPE_W[s]) * ((RH + SHAPE_H[s] - 1u) / SHAPE_H[s]);
+      }
+      tcur++;
+      n++;
+    }
+    for (int s = 0; s < NSHAPES; s++) {
+      tab[s * g->table_words() + tcur] = tiles[s];
+      g->gtiles[s][k] = tiles[s];
+    }
+    tcur++;
+  }
+  g->gstart[NKINDS] = n;
+}
+
+// Counter partial slots one launch of `g` at tuning `t` writes.
+static uint32_t partial_slots(const r3d_hip_ctx *ctx, const ViewGroups &g, const r3d_raycast_tuning &t) {
+  const int s = shape_index(t.tile_w, t.tile_h);
+  uint32_t n = 0;
+  for (int k = 0; k < NKINDS; k++) {
+    const uint32_t tiles = g.gtiles[s][k];
+    const uint32_t cap = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * t.blocks_per_cu;
+    n += t.blocks_per_cu && cap < tiles ? cap : tiles;
+  }
+  return n;
+}
+
+static int check_tuning(r3d_hip_ctx *ctx, const r3d_raycast_tuning *t) {
+  if (!t) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast tuning is NULL");
+  if (shape_index(t->tile_w, t->tile_h) < 0)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "raycast tuning: tile %ux%u is not 8x8, 16x4, 16x8 or 16x16",
+                    t->tile_w, t->tile_h);
+  if (t->blocks_per_cu > MAX_BLOCKS_PER_CU)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "raycast tuning: blocks_per_cu %u above %u", t->blocks_per_cu,
+                    MAX_BLOCKS_PER_CU);
+  return R3D_OK;
+}
+
+// Kernel launches only (capture-safe): queue reset, one launch per kind,
+// counter reduction. `partials` holds partial_slots() uint4 entries.
+static int launch_groups(r3d_hip_ctx *ctx, RayArgs a, const r3d_frame_params *dviews, const uint32_t *dtables,
+                         const ViewGroups &g, const r3d_raycast_tuning &t, uint32_t *partials, uint32_t *queue,
+                         unsigned long long *counters, hipStream_t s) {
+  const int sh = shape_index(t.tile_w, t.tile_h);
+  const uint32_t nt = SHAPE_W[sh] * SHAPE_H[sh];
+  if (t.blocks_per_cu) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, queue, (uint32_t)NKINDS);
+  uint32_t pbase = 0;
+  for (int k = 0; k < NKINDS; k++) {
+    const uint32_t nv = g.gstart[k + 1] - g.gstart[k];
+    const uint32_t tiles = g.gtiles[sh][k];
+    if (nv == 0 || tiles == 0) continue;
+    uint32_t grid = tiles;
+    if (t.blocks_per_cu) {
+      const uint32_t cap = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * t.blocks_per_cu;
+      if (cap < grid) grid = cap;
+    }
+    a.partials = counters ? partials + (size_t)pbase * 4u : nullptr;
+    pbase += grid;
+    a.views = dviews + g.gstart[k];
+    a.tile_start = dtables + (size_t)sh * g.table_words() + g.tpos[k];
+    a.nviews = nv;
+    a.ntiles = tiles;
+    a.queue = t.blocks_per_cu ? queue + k : nullptr;
+    hipLaunchKernelGGL(ray_kernels[k][sh], dim3(grid), dim3(nt), 0, s, a);
+    R3D_HIP_CHECK(ctx, hipGetLastError());
+  }
+  if (counters && pbase) {
+    hipLaunchKernelGGL(k_reduce_counters, dim3(1), dim3(256), 0, s, (const uint32_t *)partials, pbase, counters);
+    R3D_HIP_CHECK(ctx, hipGetLastError());
+  }
+  return R3D_OK;
+}
+
+static int validate_views(r3d_hip_ctx *ctx, const r3d_frame_params *views, uint32_t nviews,
+                          const r3d_hip_image *out) {
+  if (!views || nviews == 0 || nviews > R3D_MAX_VIEWS_PER_CALL)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: nviews %u outside 1..%u", nviews, R3D_MAX_VIEWS_PER_CALL);
+  if (!out || (!out->rgba && !out->rgba8) || out->width == 0 || out->height == 0)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: no output image");
+  if (out->width > 65535u || out->height > 65535u)
+    return r3d_fail(ctx, R3D_ERR_RANGE, "raycast: image above 65535 pixels per side");
+  for (uint32_t i = 0; i < nviews; i++) {
+    const r3d_frame_params &p = views[i];
+    if (p.slab_grid || p.clip_valid || p.brick_mode || (p.view_flags & 2u))
+      return r3d_fail(ctx, R3D_ERR_UNSUPPORTED, "raycast: view %u is not a cube-mode view", i);
+    if (p.mode > MODE_FLAT) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: view %u mode %u", i, p.mode);
+    if (p.viewport[0] == 0 || p.viewport[1] == 0 || p.viewport[0] > 65535u || p.viewport[1] > 65535u)
+      return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: view %u viewport %ux%u", i, p.viewport[0], p.viewport[1]);
+    if (!(p.step_voxels > 0.0f)) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: view %u step_voxels", i);
+    if (!(p.max_mip >= 0.0f)) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: view %u max_mip", i);
+  }
+  return R3D_OK;
+}
+
+static RayArgs base_args(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_hip_image *out,
+                         const r3d_raycast_opts *opts) {
+  RayArgs a{};
+  a.vol = vol_dev(vol);
+  a.tf = opts && opts->tf ? opts->tf : ctx->d_tf;
+  a.img = out->rgba;
+  a.img8 = out->rgba8;
+  a.img_w = out->width;
+  a.img_h = out->height;
+  a.cutoff = opts && opts->alpha_cutoff > 0.0f ? opts->alpha_cutoff : 0.98f;
+  a.px_steps = opts ? opts->px_steps : nullptr;
+  return a;
 }
 
 // Copies `n` bytes into the next staging slot (pinned host + device buffer);
@@ -327,112 +531,59 @@ int stage(r3d_hip_ctx *ctx, const void *src, size_t n, hipStream_t s, ViewSlot *
 
 using namespace r3d;
 
+struct r3d_hip_raycast_plan {
+  r3d_hip_ctx *ctx = nullptr;
+  const r3d_hip_volume *vol = nullptr;
+  ViewGroups g;
+  void *d_mem = nullptr;  // views, tile tables, queue counters, counter partials
+  r3d_frame_params *d_views = nullptr;
+  uint32_t *d_tables = nullptr;
+  uint32_t *d_queue = nullptr;
+  uint32_t *d_partials = nullptr;
+  r3d_hip_image out{};
+  r3d_raycast_opts opts{};
+  r3d_raycast_tuning tuning{8u, 8u, 0u, 0.0f};
+};
+
 extern "C" int r3d_hip_raycast(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_frame_params *views,
                                uint32_t nviews, const r3d_hip_image *out, const r3d_raycast_opts *opts,
                                r3d_hip_stats *stats, void *stream) {
   if (!ctx) return R3D_ERR_INVALID;
   if (!vol || vol->ctx != ctx) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: bad volume");
-  if (!views || nviews == 0 || nviews > R3D_MAX_VIEWS_PER_CALL)
-    return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: nviews %u outside 1..%u", nviews, R3D_MAX_VIEWS_PER_CALL);
-  if (!out || (!out->rgba && !out->rgba8) || out->width == 0 || out->height == 0)
-    return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: no output image");
-  if (out->width > 65535u || out->height > 65535u)
-    return r3d_fail(ctx, R3D_ERR_RANGE, "raycast: image above 65535 pixels per side");
-  const float cutoff = opts && opts->alpha_cutoff > 0.0f ? opts->alpha_cutoff : 0.98f;
-  for (uint32_t i = 0; i < nviews; i++) {
-    const r3d_frame_params &p = views[i];
-    if (p.slab_grid || p.clip_valid || p.brick_mode || (p.view_flags & 2u))
-      return r3d_fail(ctx, R3D_ERR_UNSUPPORTED, "raycast: view %u is not a cube-mode view", i);
-    if (p.mode > MODE_FLAT) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: view %u mode %u", i, p.mode);
-    if (p.viewport[0] == 0 || p.viewport[1] == 0 || p.viewport[0] > 65535u || p.viewport[1] > 65535u)
-      return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: view %u viewport %ux%u", i, p.viewport[0], p.viewport[1]);
-    if (!(p.step_voxels > 0.0f)) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: view %u step_voxels", i);
-    if (!(p.max_mip >= 0.0f)) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: view %u max_mip", i);
-  }
+  int rc = validate_views(ctx, views, nviews, out);
+  if (rc != R3D_OK) return rc;
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   hipStream_t s = (hipStream_t)stream;
-
-  // group views by kernel kind; staging layout: [views by group][tile starts]
-  const size_t pbytes = sizeof(r3d_frame_params) * nviews;
-  const size_t tbytes = sizeof(uint32_t) * (nviews + NKINDS);
-  uint8_t buf[sizeof(r3d_frame_params) * R3D_MAX_VIEWS_PER_CALL + sizeof(uint32_t) * (R3D_MAX_VIEWS_PER_CALL + NKINDS)];
-  r3d_frame_params *pv = (r3d_frame_params *)buf;
-  uint32_t *ts = (uint32_t *)(buf + pbytes);
-  uint32_t gstart[NKINDS + 1], gtiles[NKINDS], tpos[NKINDS];
-  uint32_t n = 0, tcur = 0;
-  for (int k = 0; k < NKINDS; k++) {
-    gstart[k] = n;
-    tpos[k] = tcur;
-    uint32_t tiles = 0;
-    for (uint32_t i = 0; i < nviews; i++) {
-      if (kind_of(views[i].mode) != k) continue;
-      pv[n] = views[i];
-      ts[tcur++] = tiles;
-      const bool half = (views[i].view_flags & 32u) != 0u;
-      const uint32_t TW = half ? (views[i].viewport[0] + 1u) / 2u : views[i].viewport[0];
-      const uint32_t TH = half ? (views[i].viewport[1] + 1u) / 2u : views[i].viewport[1];
-      tiles += ((TW + TILE - 1u) / TILE) * ((TH + TILE - 1u) / TILE);
-      n++;
-    }
-    ts[tcur++] = tiles;
-    gtiles[k] = tiles;
-  }
-  gstart[NKINDS] = n;
+  uint8_t buf[sizeof(r3d_frame_params) * R3D_MAX_VIEWS_PER_CALL +
+              sizeof(uint32_t) * (R3D_MAX_VIEWS_PER_CALL + NKINDS) * NSHAPES];
+  ViewGroups g;
+  group_views(views, nviews, buf, &g);
   ViewSlot *slot = nullptr;
-  int rc = stage(ctx, buf, pbytes + tbytes, s, &slot);
+  rc = stage(ctx, buf, g.bytes(), s, &slot);
   if (rc != R3D_OK) return rc;
   unsigned long long *counters = opts ? (unsigned long long *)opts->counters : nullptr;
   if (stats) {
     counters = ctx->d_counters;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, counters, 4u);
   }
-  const uint32_t total_tiles = gtiles[0] + gtiles[1] + gtiles[2] + gtiles[3];
-  if (counters && ctx->partials_cap < total_tiles) {
+  const uint32_t slots = partial_slots(ctx, g, ctx->tuning);
+  if (counters && ctx->partials_cap < slots) {
     if (ctx->d_partials) {
       R3D_HIP_CHECK(ctx, hipStreamSynchronize(s));
       R3D_HIP_CHECK(ctx, hipFree(ctx->d_partials));
       ctx->d_partials = nullptr;
       ctx->partials_cap = 0;
     }
-    const size_t cap = total_tiles < 65536u ? 65536u : total_tiles;
+    const size_t cap = slots < 65536u ? 65536u : slots;
     if (hipMalloc(&ctx->d_partials, cap * 16u) != hipSuccess)
-      return r3d_fail(ctx, R3D_ERR_NOMEM, "raycast: counter partials for %zu tiles", cap);
+      return r3d_fail(ctx, R3D_ERR_NOMEM, "raycast: counter partials for %zu workgroups", cap);
     ctx->partials_cap = cap;
   }
-  RayArgs a;
-  a.vol = vol_dev(vol);
-  a.tf = opts && opts->tf ? opts->tf : ctx->d_tf;
-  a.img = out->rgba;
-  a.img8 = out->rgba8;
-  a.img_w = out->width;
-  a.img_h = out->height;
-  a.cutoff = cutoff;
-  a.px_steps = opts ? opts->px_steps : nullptr;
   const r3d_frame_params *dviews = (const r3d_frame_params *)slot->dev;
-  const uint32_t *dts = (const uint32_t *)((const uint8_t *)slot->dev + pbytes);
-  uint32_t tile_base = 0;
-  for (int k = 0; k < NKINDS; k++) {
-    const uint32_t nv = gstart[k + 1] - gstart[k];
-    if (nv == 0 || gtiles[k] == 0) continue;
-    a.partials = counters ? ctx->d_partials + (size_t)tile_base * 4u : nullptr;
-    tile_base += gtiles[k];
-    a.views = dviews + gstart[k];
-    a.tile_start = dts + tpos[k];
-    a.nviews = nv;
-    const dim3 grid(gtiles[k]), block(64);
-    switch (k) {
-    case KIND_FULL: hipLaunchKernelGGL(k_raycast<KIND_FULL>, grid, block, 0, s, a); break;
-    case KIND_MIP: hipLaunchKernelGGL(k_raycast<KIND_MIP>, grid, block, 0, s, a); break;
-    case KIND_FLAT: hipLaunchKernelGGL(k_raycast<KIND_FLAT>, grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL(k_raycast<KIND_OTHER>, grid, block, 0, s, a); break;
-    }
-    R3D_HIP_CHECK(ctx, hipGetLastError());
-  }
-  if (counters && total_tiles) {
-    hipLaunchKernelGGL(k_reduce_counters, dim3(1), dim3(256), 0, s, (const uint32_t *)ctx->d_partials,
-                       total_tiles, counters);
-    R3D_HIP_CHECK(ctx, hipGetLastError());
-  }
+  const uint32_t *dtables = (const uint32_t *)((const uint8_t *)slot->dev + g.view_bytes());
+  rc = launch_groups(ctx, base_args(ctx, vol, out, opts), dviews, dtables, g, ctx->tuning, ctx->d_partials,
+                     ctx->d_queue, counters, s);
+  if (rc != R3D_OK) return rc;
   R3D_HIP_CHECK(ctx, hipEventRecord(slot->ev, s));
   slot->recorded = true;
   if (stats) {
@@ -445,4 +596,149 @@ extern "C" int r3d_hip_raycast(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, cons
     stats->rays = h[3];
   }
   return R3D_OK;
+}
+
+extern "C" int r3d_hip_set_raycast_tuning(r3d_hip_ctx *ctx, const r3d_raycast_tuning *t) {
+  if (!ctx) return R3D_ERR_INVALID;
+  const int rc = check_tuning(ctx, t);
+  if (rc != R3D_OK) return rc;
+  ctx->tuning = *t;
+  return R3D_OK;
+}
+
+extern "C" int r3d_hip_raycast_plan_create(r3d_hip_ctx *ctx, const r3d_hip_volume *vol,
+                                           const r3d_frame_params *views, uint32_t nviews,
+                                           const r3d_hip_image *out, const r3d_raycast_opts *opts,
+                                           r3d_hip_raycast_plan **plan) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!plan) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast plan: NULL output");
+  *plan = nullptr;
+  if (!vol || vol->ctx != ctx) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast plan: bad volume");
+  int rc = validate_views(ctx, views, nviews, out);
+  if (rc != R3D_OK) return rc;
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  r3d_hip_raycast_plan *p = new (std::nothrow) r3d_hip_raycast_plan;
+  if (!p) return r3d_fail(ctx, R3D_ERR_NOMEM, "raycast plan");
+  p->ctx = ctx;
+  p->vol = vol;
+  p->out = *out;
+  if (opts) p->opts = *opts;
+  p->tuning = ctx->tuning;
+  uint8_t *buf = (uint8_t *)malloc(sizeof(r3d_frame_params) * nviews +
+                                   sizeof(uint32_t) * ((size_t)nviews + NKINDS) * NSHAPES);
+  if (!buf) {
+    delete p;
+    return r3d_fail(ctx, R3D_ERR_NOMEM, "raycast plan: host tables");
+  }
+  group_views(views, nviews, buf, &p->g);
+  // partial slots for the largest launch any tuning can make
+  uint32_t slots = 0;
+  for (int s = 0; s < NSHAPES; s++) {
+    const r3d_raycast_tuning t{SHAPE_W[s], SHAPE_H[s], 0u, 0.0f};
+    const uint32_t n = partial_slots(ctx, p->g, t);
+    slots = n > slots ? n : slots;
+  }
+  const uint32_t pers = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * MAX_BLOCKS_PER_CU * NKINDS;
+  slots = pers > slots ? pers : slots;
+  const size_t off_tab = (p->g.view_bytes() + 255u) & ~(size_t)255u;
+  const size_t off_q = (off_tab + sizeof(uint32_t) * p->g.table_words() * NSHAPES + 255u) & ~(size_t)255u;
+  const size_t off_part = off_q + 256u;
+  const size_t total = off_part + (size_t)slots * 16u;
+  if (hipMalloc(&p->d_mem, total) != hipSuccess) {
+    free(buf);
+    delete p;
+    return r3d_fail(ctx, R3D_ERR_NOMEM, "raycast plan: %zu device bytes", total);
+  }
+  p->d_views = (r3d_frame_params *)p->d_mem;
+  p->d_tables = (uint32_t *)((uint8_t *)p->d_mem + off_tab);
+  p->d_queue = (uint32_t *)((uint8_t *)p->d_mem + off_q);
+  p->d_partials = (uint32_t *)((uint8_t *)p->d_mem + off_part);
+  hipError_t e = hipMemcpy(p->d_views, buf, p->g.view_bytes(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(p->d_tables, buf + p->g.view_bytes(), sizeof(uint32_t) * p->g.table_words() * NSHAPES,
+                  hipMemcpyHostToDevice);
+  free(buf);
+  if (e != hipSuccess) {
+    hipFree(p->d_mem);
+    delete p;
+    return r3d_fail(ctx, R3D_ERR_DEVICE, "raycast plan: upload: %s", hipGetErrorString(e));
+  }
+  *plan = p;
+  return R3D_OK;
+}
+
+extern "C" int r3d_hip_raycast_plan_launch(r3d_hip_raycast_plan *p, void *stream) {
+  if (!p) return R3D_ERR_INVALID;
+  r3d_hip_ctx *ctx = p->ctx;
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  return launch_groups(ctx, base_args(ctx, p->vol, &p->out, &p->opts), p->d_views, p->d_tables, p->g, p->tuning,
+                       p->d_partials, p->d_queue, (unsigned long long *)p->opts.counters, (hipStream_t)stream);
+}
+
+extern "C" int r3d_hip_raycast_plan_set_tuning(r3d_hip_raycast_plan *p, const r3d_raycast_tuning *t) {
+  if (!p) return R3D_ERR_INVALID;
+  const int rc = check_tuning(p->ctx, t);
+  if (rc != R3D_OK) return rc;
+  p->tuning = *t;
+  return R3D_OK;
+}
+
+extern "C" int r3d_hip_raycast_plan_get_tuning(const r3d_hip_raycast_plan *p, r3d_raycast_tuning *t) {
+  if (!p || !t) return R3D_ERR_INVALID;
+  *t = p->tuning;
+  return R3D_OK;
+}
+
+// Times every tile shape at one workgroup per tile and at 2, 4, 8 and 16
+// persistent workgroups per CU (one warm-up launch, then the mean of three),
+// keeps the fastest in the plan and returns the measurements.
+extern "C" int r3d_hip_raycast_plan_tune(r3d_hip_raycast_plan *p, void *stream, r3d_raycast_tuning *results,
+                                         uint32_t max_results, uint32_t *nresults) {
+  if (!p) return R3D_ERR_INVALID;
+  r3d_hip_ctx *ctx = p->ctx;
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  static const uint32_t bpc[5] = {0u, 2u, 4u, 8u, 16u};
+  hipEvent_t e0, e1;
+  R3D_HIP_CHECK(ctx, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return r3d_fail(ctx, R3D_ERR_DEVICE, "raycast tune: event");
+  }
+  const RayArgs a = base_args(ctx, p->vol, &p->out, &p->opts);
+  r3d_raycast_tuning best{8u, 8u, 0u, 3.0e38f};
+  uint32_t n = 0;
+  int rc = R3D_OK;
+  for (int sh = 0; sh < NSHAPES && rc == R3D_OK; sh++)
+    for (int b = 0; b < 5 && rc == R3D_OK; b++) {
+      r3d_raycast_tuning t{SHAPE_W[sh], SHAPE_H[sh], bpc[b], 0.0f};
+      rc = launch_groups(ctx, a, p->d_views, p->d_tables, p->g, t, p->d_partials, p->d_queue, nullptr, s);
+      if (rc != R3D_OK) break;
+      if (hipEventRecord(e0, s) != hipSuccess) rc = R3D_ERR_DEVICE;
+      for (int r = 0; r < 3 && rc == R3D_OK; r++)
+        rc = launch_groups(ctx, a, p->d_views, p->d_tables, p->g, t, p->d_partials, p->d_queue, nullptr, s);
+      float ms = 0.0f;
+      if (rc == R3D_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                           hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = r3d_fail(ctx, R3D_ERR_DEVICE, "raycast tune: timing");
+      if (rc != R3D_OK) break;
+      t.ms = ms / 3.0f;
+      if (results && n < max_results) results[n] = t;
+      n++;
+      if (t.ms < best.ms) best = t;
+    }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc != R3D_OK) return rc;
+  p->tuning = best;
+  if (nresults) *nresults = n;
+  return R3D_OK;
+}
+
+extern "C" void r3d_hip_raycast_plan_destroy(r3d_hip_raycast_plan *p) {
+  if (!p) return;
+  hipSetDevice(p->ctx->device);
+  hipDeviceSynchronize();
+  hipFree(p->d_mem);
+  delete p;
 }
