@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adj-views", type=int, default=8)
     ap.add_argument("--adj-grid", type=int, default=128)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched (graph) workload")
+    ap.add_argument("--batched-chunks", type=int, default=32768)
+    ap.add_argument("--no-cfg0", action="store_true", help="skip the 256^3 / 512^2 workload")
     ap.add_argument("--profile-decode", action="store_true",
                     help="for rocprofv3 --pmc: one 1 GiB device copy (calibration) then one decode launch")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "decode_exact_1024.md"),
@@ -147,11 +150,17 @@ def main():
     ctx.decode(batch)  # leave the exact decode in `flat`
 
     adj = None if args.no_adjoint else adjoint_ratio(ctx, vol, lut, args, W, H, dev)
+    bat = None if args.no_batched else batched(ctx, vol, lut, args, dev, stream)
+    c0 = None if args.no_cfg0 else cfg0(ctx, args, dev, stream)
 
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
         cpu, parity = cpu_baseline(ctx, vol, view, lut, img, args, levels, headers, step_tab, n, W, H)
+        if c0 is not None:
+            c0["cpu_baseline"] = cpu_cfg0(c0.pop("_state"), args)
+    if c0 is not None:
+        c0.pop("_state", None)
 
     out = {
         "metric": "ray samples/s + fps @1024³ 1080p; chunk decode GB/s; adjoint/fwd time ratio",
@@ -177,6 +186,8 @@ def main():
         "decode_variants": variants,
         "adjoint_fwd_ratio": None if adj is None else adj["ratio"],
         "adjoint": adj,
+        "batched": bat,
+        "cfg0": c0,
         "samples_per_frame": samples,
         "skips_per_frame": skips,
         "shaded_per_frame": shaded,
@@ -250,6 +261,161 @@ def decode_variants(ctx, batch, levels, headers, step_tab, flat, stream, dev, re
         gbps = CHUNK_BYTES * nch / (ms / 1000) / 1e9
         out[name] = {"ms": ms, "GBps": gbps, "hbm_frac": gbps / HBM_PEAK_GBPS, "u8_differs_from_exact": diff,
                      "max_abs_err_vs_fp64": float(err.max()), "mean_abs_err_vs_fp64": float(err.mean())}
+    return out
+
+
+def batched(ctx, vol, lut, args, dev, stream):
+    """The batched workload over the resident volume: `batched_chunks` sparse
+    chunks (Laplacian nonzero counts in 5..400, Laplacian levels of scale 12)
+    decoded, and 64 views of mixed resolution (16 each at 256^2, 512^2,
+    1280x720, 1920x1080; 53,166,080 rays) rendered into one RGBA8 atlas. The
+    render's launch shape comes from the autotuner; decode and render are
+    captured into one HIP graph, whose replays are timed."""
+    import numpy as np
+    import torch
+
+    import render3d_amd as r3d
+    from render3d_amd import scene, workloads
+
+    n = args.batched_chunks
+    t0 = time.perf_counter()
+    offs, pairs, hdr = workloads.sparse_laplacian_batch(n, 20261203)
+    gen_s = time.perf_counter() - t0
+    nnz = int(offs[-1])
+    step = torch.from_numpy(workloads.step_table_linear()).to(dev)
+    d_offs, d_pairs, d_hdr = (torch.from_numpy(x).to(dev) for x in (offs, pairs, hdr))
+    out = torch.empty((n, 4096), dtype=torch.uint8, device=dev)
+    batch = ctx.make_batch(headers=d_hdr, step_tables=step, out=out, pairs=d_pairs, pair_offsets=d_offs)
+    views, aw, ah = workloads.mixed_views(gate=scene.skip_gate(lut))
+    counters = torch.zeros(4, dtype=torch.int64, device=dev)
+    plan = ctx.raycast_plan(vol, views, aw, ah, rgba=False, rgba8=True, counters=counters)
+    t0 = time.perf_counter()
+    tuning = plan.tune()
+    tune_s = time.perf_counter() - t0
+    chosen = plan.tuning
+    reps = max(3, args.steps)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+        ev[0].record(stream)
+        for i in range(reps):
+            fn()
+            ev[i + 1].record(stream)
+        torch.cuda.synchronize()
+        return float(np.mean([ev[i].elapsed_time(ev[i + 1]) for i in range(reps)]))
+
+    in_bytes = 4 * nnz + 4 * (n + 1) + 16 * n
+    variants = {}
+    exact = None
+    k = min(1024, n)
+    idx = np.linspace(0, n - 1, k).astype(np.int64)
+    sub_off = np.zeros(k + 1, np.uint32)
+    cnt = np.diff(offs.astype(np.int64))[idx]
+    sub_off[1:] = np.cumsum(cnt)
+    sub_pairs = np.concatenate([pairs[offs[i]:offs[i + 1]] for i in idx])
+    ref = fp64_decode(torch.from_numpy(workloads.densify(sub_off, sub_pairs)).to(dev),
+                      torch.from_numpy(hdr[idx]).to(dev), step)
+    s_off, s_pairs, s_hdr = (torch.from_numpy(x).to(dev) for x in (sub_off, sub_pairs, np.ascontiguousarray(hdr[idx])))
+    for v, name in ((r3d.R3D_DECODE_EXACT_F32, "exact_f32"), (r3d.R3D_DECODE_MFMA_F32, "mfma_f32"),
+                    (r3d.R3D_DECODE_MFMA_F16, "mfma_f16"), (r3d.R3D_DECODE_MFMA_F16X3, "mfma_f16x3")):
+        ms = timed(lambda: ctx.decode(batch, variant=v))
+        if exact is None:
+            exact = out.clone()
+        f32 = ctx.decode_chunks(headers=s_hdr, step_tables=step, pairs=s_pairs, pair_offsets=s_off, variant=v,
+                                f32_out=True)
+        err = (f32.double() - ref).abs()
+        variants[name] = {"ms": ms, "GBps": (in_bytes + 4096 * n) / (ms / 1e3) / 1e9,
+                          "u8_differs_from_exact": float((out != exact).float().mean()),
+                          "max_abs_err_vs_fp64": float(err.max()), "mean_abs_err_vs_fp64": float(err.mean())}
+    ray_ms = timed(plan.launch)
+    # decode (exact) + render captured into one graph
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(stream)
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g, stream=side):
+            ctx.decode(batch)
+            plan.launch()
+    stream.wait_stream(side)
+    torch.cuda.synchronize()
+    graph_ms = timed(g.replay)
+    counters.zero_()
+    g.replay()
+    torch.cuda.synchronize()
+    samples, skips, shaded, rays = (int(x) for x in counters.cpu().numpy())
+    ok = bool(torch.equal(out, exact))
+    plan.close()
+    return {"chunks": n, "nnz": nnz, "views": len(views), "atlas": [aw, ah], "rays": rays,
+            "samples": samples, "shaded": shaded, "skips": skips,
+            "graph_ms": graph_ms, "decode_ms": variants["exact_f32"]["ms"], "raycast_ms": ray_ms,
+            "samples_per_s": samples / (ray_ms / 1e3), "graph_decode_matches_eager": ok,
+            "tuning": {"chosen": chosen, "candidates": tuning, "tune_s": tune_s},
+            "decode_variants": variants, "generate_s": gen_s}
+
+
+def cfg0(ctx, args, dev, stream):
+    """The 256^3 workload: 12-sheet volume with its full pyramid, 512x512,
+    step 0.5, cutoff 0.99, FULL and MIP."""
+    import numpy as np
+    import torch
+
+    import render3d_amd as r3d
+    from render3d_amd import scene
+
+    n = 256
+    vol = ctx.volume(n, n, n)
+    vol.upload(ctx.synth_sheets(n, n, n, 12, 20261203))
+    vol.build()
+    lut = scene.tf_ramp_40_200()
+    ctx.set_transfer(lut)
+    cam = scene.orbit_camera(45.0, 0.0, 2.0)
+    res, state = {}, {"vol": vol, "lut": lut, "views": {}, "images": {}}
+    reps = max(5, args.steps)
+    for name, mode in (("full", r3d.MODE_FULL), ("mip", r3d.MODE_MIP)):
+        view = scene.frame_params(cam, 512, 512, mode=mode, step_voxels=0.5, gate=scene.skip_gate(lut))
+        img = torch.empty((512, 512, 4), dtype=torch.float32, device=dev)
+        counters = torch.zeros(4, dtype=torch.int64, device=dev)
+        ctx.raycast(vol, [view], 512, 512, alpha_cutoff=0.99, out=img)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+        ev[0].record(stream)
+        for i in range(reps):
+            ctx.raycast(vol, [view], 512, 512, alpha_cutoff=0.99, out=img, counters=counters)
+            ev[i + 1].record(stream)
+        torch.cuda.synchronize()
+        ms = float(np.mean([ev[i].elapsed_time(ev[i + 1]) for i in range(reps)]))
+        samples = int(counters[0]) // reps
+        res[name] = {"ms": ms, "samples": samples, "samples_per_s": samples / (ms / 1e3)}
+        state["views"][name] = view
+        state["images"][name] = img.cpu().numpy()
+    ctx.set_transfer(scene.tf_ramp_40_200())
+    res["config"] = "256^3 12-sheet volume, full pyramid, 512x512, step 0.5, cutoff 0.99"
+    res["_state"] = state
+    return res
+
+
+def cpu_cfg0(state, args):
+    """The oracle raycaster on the 256^3 frames (same volume read back from
+    HBM), on the host's cores; GPU/oracle max-abs difference."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    from render3d_amd import scene
+
+    vol = state["vol"]
+    cores = min(16, os.cpu_count() or 1)
+    lv = [((d), vol.read_level(l)) for l, d in enumerate(vol.level_dims)]
+    ov = orc.OracleVolume(None, 256, 256, 256, levels=lv, occ=(vol.odim, vol.read_occupancy()))
+    tf = scene.tf_unorm(state["lut"])
+    out = {"cores": cores}
+    for name, view in state["views"].items():
+        t = time.perf_counter()
+        ref, cnt, _ = orc.raycast(ov, tf, [view], 512, 512, alpha_cutoff=0.99, nthreads=cores)
+        dt = time.perf_counter() - t
+        out[name] = {"samples_per_s": cnt["samples"] / dt, "s": dt,
+                     "gpu_max_abs": float(np.max(np.abs(state["images"][name] - ref)))}
     return out
 
 

@@ -415,9 +415,12 @@ __global__ __launch_bounds__(256) void k_encode(EncodeArgs a) {
 //   PREC_F32   FP32 MFMA (v_mfma_f32_16x16x4_f32), FP32 throughout
 //   PREC_F16   FP16 operands, FP32 accumulate (about 11 significant bits)
 //   PREC_F16X3 each product as hi*hi + lo*hi + hi*lo of FP16 hi/lo halves
-// The FP16 variants scale the chunk by a power of two so that every pass-1
-// input is below 2^13 (and so every pass-2 input, at most 4x larger for an
-// orthonormal basis, below 2^15 < 65504); the scale is undone in the epilogue.
+//              (hi rounded toward zero, lo the FP32 remainder; about 21 bits)
+// The FP16 variants scale the chunk by a power of two chosen from its largest
+// level and the table's largest step, so that every FP16 operand stays below
+// 2^15 < 65504 (each pass of an orthonormal basis grows magnitudes at most
+// 4x); the scale is undone in the epilogue. PREC_F16X3 carries the low halves
+// times 2^10 so that they stay out of FP16's subnormal range.
 // Dequantisation uses (|l| + 0.2) * step, equal in exact arithmetic to
 // deq_one's (|l| - 1) * step + 0.8 * step + 0.4 * step.
 
@@ -440,11 +443,11 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// power of two p with bound * p < 2^13; 1 when bound == 0
-__device__ __forceinline__ float pow2_scale(float bound) {
+// power of two p with bound * p < 2^top; 1 when bound == 0
+__device__ __forceinline__ float pow2_scale(float bound, int top) {
   if (!(bound > 0.0f)) return 1.0f;
   const int e = (int)((__float_as_uint(bound) >> 23) & 0xffu) - 127;  // floor(log2 bound), bound normal
-  int k = 12 - e;
+  int k = top - 1 - e;
   k = k < -120 ? -120 : (k > 120 ? 120 : k);
   return __uint_as_float((uint32_t)(k + 127) << 23);
 }
@@ -456,11 +459,19 @@ __device__ __forceinline__ h4 to_h4(const float v[4]) {
   return h;
 }
 
-__device__ __forceinline__ h4 lo_h4(const float v[4], h4 hi) {
-  h4 h;
+// hi/lo split for PREC_F16X3: hi = v rounded toward zero to FP16 by one
+// intrinsic (so the value the matrix core sees is the value lo is taken
+// from), lo = (v - hi) * LO_SCALE. |v - hi| < 2^-10 |hi|, so the scaled low
+// half stays in FP16's normal range wherever hi does (the matrix cores flush
+// FP16 subnormal operands, which would drop an unscaled low half) and below
+// 2^15 for |hi| < 2^15.
+constexpr float LO_SCALE = 1024.0f, LO_UNSCALE = 1.0f / 1024.0f;
+__device__ __forceinline__ void split_h4(const float v[4], h4 *hi, h4 *lo) {
+  const uint2 u = make_uint2(__builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(v[0], v[1])),
+                             __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(v[2], v[3])));
+  *hi = __builtin_bit_cast(h4, u);
 #pragma unroll
-  for (int e = 0; e < 4; e++) h[e] = (_Float16)(v[e] - (float)hi[e]);
-  return h;
+  for (int e = 0; e < 4; e++) (*lo)[e] = (_Float16)((v[e] - (float)(*hi)[e]) * LO_SCALE);
 }
 
 // D = A * B on the MFMA layouts above; a and b hold this lane's four slots.
@@ -472,9 +483,12 @@ __device__ __forceinline__ f4 mma(const float a[4], const float b[4], h4 ah, h4 
     for (int e = 0; e < 4; e++) d = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e], b[e], d, 0, 0, 0);
   } else {
     d = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, d, 0, 0, 0);
-    if (PREC == PREC_F16X3) {
-      d = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, d, 0, 0, 0);
+    if (PREC == PREC_F16X3) {  // hi*hi + 2^-10 (lo'*hi + hi*lo')
+      f4 x = {0.0f, 0.0f, 0.0f, 0.0f};
+      x = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, x, 0, 0, 0);
+      x = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, x, 0, 0, 0);
+#pragma unroll
+      for (int e = 0; e < 4; e++) d[e] = __builtin_fmaf(x[e], LO_UNSCALE, d[e]);
     }
   }
   return d;
@@ -551,8 +565,12 @@ __device__ __forceinline__ float lane_level(const uint4 q[4][2], int e, int x) {
 constexpr uint32_t P3_SZ = 10, P3_SY = 162, P3_WAVE = 16 * P3_SY;
 
 template <int PREC, bool SPARSE, bool F32OUT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_decode_mfma(DecodeArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREC == PREC_F16X3 ? 2 : 3))) void k_decode_mfma(
+    DecodeArgs a) {
   constexpr bool MFMA3 = PREC == PREC_F16;  // pass 3 on MFMA through LDS
+  // FP16 operand range: pass-1 inputs below 2^SCALE_TOP, pass-1 outputs at
+  // most 4x that, pass-2 outputs 16x (converted to FP16 only for MFMA3)
+  constexpr int SCALE_TOP = MFMA3 ? 11 : 13;
   // per wave: the sparse level tile (2048 words, used while loading) and the
   // pass-3 tile (used after the levels are in registers) share one region
   constexpr uint32_t WAVE_WORDS = (MFMA3 && P3_WAVE > 2048u) ? P3_WAVE : (SPARSE || MFMA3 ? 2048u : 4u);
@@ -562,13 +580,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   float cb[4];  // cm[4g + e][r]: B of pass 1, A of passes 2 and 3
 #pragma unroll
   for (int e = 0; e < 4; e++) cb[e] = c_basis[(4u * g + e) * 16u + r];
-  const h4 bh = to_h4(cb);
-  const h4 bl = lo_h4(cb, bh);
-  // step[z + y + x] = step[(4g + r) + (e + x)], times the scale; reloaded when
-  // a chunk names another table
+  h4 bh, bl = {};
+  if (PREC == PREC_F16X3) split_h4(cb, &bh, &bl);
+  else bh = to_h4(cb);
+  // step[z + y + x] = step[(4g + r) + (e + x)]; reloaded when a chunk names
+  // another table, with the table's largest step
   uint32_t cur_table = 0xffffffffu;
-  float st[19];
-  float s = 1.0f;
+  float st_raw[19];
+  float max_step = 0.0f;
   for (uint32_t c = blockIdx.x * 4u + w; c < a.nchunks; c += gridDim.x * 4u) {
     uint4 q[4][2];
     if (!SPARSE) {  // issued before the header is examined
@@ -619,18 +638,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       cur_table = h.table;
       const float *tb = a.tables + (size_t)h.table * 46u + 4u * g + r;
 #pragma unroll
-      for (int o = 0; o < 19; o++) st[o] = tb[o];
+      for (int o = 0; o < 19; o++) st_raw[o] = tb[o];
       if (PREC != PREC_F32) {
-        // |level| <= 32768, so |dequantised| <= 32768.2 * max step: the scale
-        // keeps every FP16 input of any chunk on this table below 2^13
-        float ms = st[0];
+        float ms = st_raw[0];
 #pragma unroll
-        for (int o = 1; o < 19; o++) ms = fmax_sel(ms, st[o]);
-        s = pow2_scale(32768.2f * wave_max(ms));
-#pragma unroll
-        for (int o = 0; o < 19; o++) st[o] *= s;
+        for (int o = 1; o < 19; o++) ms = fmax_sel(ms, st_raw[o]);
+        max_step = wave_max(ms);
       }
     }
+    // FP16: scale the chunk so |dequantised| <= (max|level| + 0.2) * max step
+    // maps below 2^13; small levels stay far from the FP16 subnormal range
+    float s = 1.0f;
+    if (PREC != PREC_F32) {
+      s2 mx = {0, 0}, mn = {0, 0};
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++) {
+          const uint32_t ww[4] = {q[e][hh].x, q[e][hh].y, q[e][hh].z, q[e][hh].w};
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const s2 v = __builtin_bit_cast(s2, ww[i]);
+            mx = __builtin_elementwise_max(mx, v);
+            mn = __builtin_elementwise_min(mn, v);
+          }
+        }
+      const float ml = fmax_sel(fmax_sel((float)mx[0], (float)mx[1]), fmax_sel(-(float)mn[0], -(float)mn[1]));
+      s = pow2_scale((wave_max(ml) + 0.2f) * max_step, SCALE_TOP);
+    }
+    float st[19];
+#pragma unroll
+    for (int o = 0; o < 19; o++) st[o] = st_raw[o] * s;
     // passes 1 and 2, one x-frequency tile at a time
     f4 d2[16];
 #pragma unroll
@@ -642,13 +680,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         v[e] = __builtin_fmaf(fclamp_sel(lf, -1.0f, 1.0f), 0.2f, lf) * st[e + x];
       }
       h4 vh = {}, vl = {};
-      if (PREC != PREC_F32) vh = to_h4(v);
-      if (PREC == PREC_F16X3) vl = lo_h4(v, vh);
+      if (PREC == PREC_F16) vh = to_h4(v);
+      if (PREC == PREC_F16X3) split_h4(v, &vh, &vl);
       const f4 d1 = mma<PREC>(v, cb, vh, vl, bh, bl);
       const float d1v[4] = {d1[0], d1[1], d1[2], d1[3]};
       h4 dh = {}, dl = {};
-      if (PREC != PREC_F32) dh = to_h4(d1v);
-      if (PREC == PREC_F16X3) dl = lo_h4(d1v, dh);
+      if (PREC == PREC_F16) dh = to_h4(d1v);
+      if (PREC == PREC_F16X3) split_h4(d1v, &dh, &dl);
       d2[x] = mma<PREC>(cb, d1v, bh, bl, dh, dl);
     }
     const float inv = h.vspan / 255.0f;

@@ -319,11 +319,13 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
   }
 }
 
+// Sums `n` uint4 partials into counters[0..3]: each workgroup reduces a
+// strided share and adds it with 64-bit integer atomics (exact in any order).
 __global__ __launch_bounds__(256) void k_reduce_counters(const uint32_t *partials, uint32_t n,
                                                          unsigned long long *counters) {
   __shared__ unsigned long long red[4][256];
   unsigned long long acc[4] = {0, 0, 0, 0};
-  for (uint32_t i = threadIdx.x; i < n; i += 256u) {
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
     const uint4 p = ((const uint4 *)partials)[i];
     acc[0] += p.x;
     acc[1] += p.y;
@@ -337,7 +339,12 @@ __global__ __launch_bounds__(256) void k_reduce_counters(const uint32_t *partial
       for (int c = 0; c < 4; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x < 4) counters[threadIdx.x] += red[threadIdx.x][0];
+  if (threadIdx.x < 4 && red[threadIdx.x][0]) atomicAdd(&counters[threadIdx.x], red[threadIdx.x][0]);
+}
+
+static uint32_t reduce_grid(uint32_t n) {
+  const uint32_t g = (n + 2047u) / 2048u;
+  return g < 1u ? 1u : (g > 512u ? 512u : g);
 }
 
 __global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
@@ -460,7 +467,8 @@ static int launch_groups(r3d_hip_ctx *ctx, RayArgs a, const r3d_frame_params *dv
     R3D_HIP_CHECK(ctx, hipGetLastError());
   }
   if (counters && pbase) {
-    hipLaunchKernelGGL(k_reduce_counters, dim3(1), dim3(256), 0, s, (const uint32_t *)partials, pbase, counters);
+    hipLaunchKernelGGL(k_reduce_counters, dim3(reduce_grid(pbase)), dim3(256), 0, s, (const uint32_t *)partials,
+                       pbase, counters);
     R3D_HIP_CHECK(ctx, hipGetLastError());
   }
   return R3D_OK;

@@ -59,6 +59,16 @@ def sparse_laplacian_batch(nchunks, seed, count_scale=100.0, value_scale=12.0, m
     return offs.astype(np.uint32), pairs.astype(np.uint32), hdr
 
 
+def select(offsets, pairs, headers, idx):
+    """The chunks `idx` of a sparse batch as a batch of their own."""
+    idx = np.asarray(idx, np.int64)
+    cnt = np.diff(offsets.astype(np.int64))[idx]
+    offs = np.zeros(idx.size + 1, np.uint32)
+    offs[1:] = np.cumsum(cnt)
+    sub = np.concatenate([pairs[offsets[i]:offsets[i + 1]] for i in idx]) if idx.size else np.zeros(0, np.uint32)
+    return offs, sub.astype(np.uint32), np.ascontiguousarray(headers[idx])
+
+
 def densify(offsets, pairs):
     """Sparse (offsets, pairs) -> dense int16 [n, 4096] levels."""
     n = offsets.size - 1

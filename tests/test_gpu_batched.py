@@ -138,3 +138,19 @@ def test_graph_of_sparse_decode_and_plan_replays(ctx, small_scene):
         assert np.array_equal(counters.cpu().numpy(), (rep + 1) * c_one), rep
         del junk
     plan.close()
+
+
+@pytest.mark.parametrize("bpc", [0, 16])
+def test_counters_reduce_over_many_workgroups(ctx, small_scene, bpc):
+    vol = small_scene[0]
+    W, H = 640, 480  # 4,800 8x8 tiles: the counter reduction runs on several workgroups
+    view = scene.frame_params(scene.orbit_camera(30.0, 10.0, 1.8), W, H, step_voxels=0.5)
+    try:
+        ctx.set_raycast_tuning(8, 8, bpc)
+        res = ctx.raycast(vol, [view], W, H, px_steps=True, stats=True)
+    finally:
+        ctx.set_raycast_tuning(8, 8, 0)
+    st = res["stats"]
+    steps = res["px_steps"].cpu().numpy().astype(np.int64)
+    assert st["rays"] == W * H
+    assert st["samples"] == int(steps.sum())

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
+from render3d_amd import workloads
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -240,3 +241,66 @@ def test_mfma_golden_blobs_with_corrections_within_one(ctx, variant):
                             corr_delta=cuda(GOLD["blob_corr_delta"]), variant=variant)
     d = np.abs(out.cpu().numpy().astype(int) - GOLD["blob_u8"].astype(int))
     assert d.max() <= 1
+
+
+def abs_transform_bound(levels, hdr, step):
+    """Per voxel (|C| x |C| x |C|) |coef| * vspan / 255: the magnitude every
+    rounding error of the separable transform is measured against."""
+    cm = np.abs(orc.basis().astype(np.float64).reshape(16, 16))
+    out = np.zeros((levels.shape[0], 4096))
+    for i in range(levels.shape[0]):
+        coef = np.zeros(4096, np.float32)
+        orc.load().r3do_dequant(np.ascontiguousarray(levels[i]), np.ascontiguousarray(step, np.float32), coef)
+        a = np.einsum("az,by,cx,abc->zyx", cm, cm, cm, np.abs(coef.astype(np.float64)).reshape(16, 16, 16))
+        out[i] = a.reshape(-1) * hdr[i, 1:2].view(np.float32)[0] / 255.0
+    return out
+
+
+# Each FP16 rounding (pass-1 input, pass-2 input, pass-3 input, and the basis
+# in each pass) contributes at most u16 = 2^-11 times the absolute transform;
+# the FP32-level variants get a generous 96 * 2^-24.
+BOUND = {1: 96 * 2.0 ** -24, 3: 96 * 2.0 ** -24, 2: 6 * 2.0 ** -11}
+
+
+@pytest.fixture(scope="module")
+def sparse_cases():
+    # the batched workload's chunks with the largest absolute transforms, plus a spread
+    offs, pairs, hdr = workloads.sparse_laplacian_batch(32768, 20261203)
+    idx = [15182, 29724, 11338, 10570, 22645, 25560] + list(np.linspace(0, 32767, 122).astype(np.int64))
+    return {"seed17": workloads.sparse_laplacian_batch(192, 17), "batched": workloads.select(offs, pairs, hdr, idx)}
+
+
+@pytest.mark.parametrize("case", ["seed17", "batched"])
+@pytest.mark.parametrize("variant", [1, 3, 2])
+def test_mfma_variants_on_small_sparse_levels(ctx, sparse_cases, case, variant):
+    # Laplacian levels of scale 12 over a wide range of magnitudes in one chunk
+    offs, pairs, hdr = sparse_cases[case]
+    step = workloads.step_table_linear()
+    got = ctx.decode_chunks(headers=cuda(hdr), step_tables=cuda(step), pairs=cuda(pairs), pair_offsets=cuda(offs),
+                            variant=variant, f32_out=True).cpu().numpy()
+    lv = workloads.densify(offs, pairs)
+    ref = fp64_reference(lv, hdr, step)
+    err = np.abs(got - ref)
+    bound = BOUND[variant] * abs_transform_bound(lv, hdr, step) + 1e-5
+    print(f"variant {variant} sparse: max |f32 - fp64| = {err.max():.3g}, mean {err.mean():.3g}, "
+          f"max err / bound {np.max(err / bound):.3g}")
+    assert np.all(err <= bound)
+
+
+@pytest.mark.parametrize("variant", [1, 3])
+def test_split_is_consistent_at_fp16_ties(ctx, variant):
+    # level 86 at L1 = 12 (step 25): (86 + 0.2) * 25 rounds to 2155.0 in FP32, an
+    # exact tie between the FP16 values 2154 and 2156; single and double rounding
+    # of the product pick different sides, and the hi/lo split must still agree
+    lv = np.zeros((4, 4096), np.int16)
+    lv[0, 2 * 256 + 0 * 16 + 10] = 86
+    lv[1, 2 * 256 + 0 * 16 + 10] = -86
+    lv[2, 5 * 256 + 3 * 16 + 4] = 86  # L1 = 12 again, another position
+    lv[3, ::97] = 86
+    hdr = orc.hdr_array([0, 0, 0, 0], [255, 255, 255, 255], [0, 0, 0, 0])
+    step = orc.step_linear()
+    got = ctx.decode_chunks(headers=cuda(hdr), step_tables=cuda(step), levels=cuda(lv), variant=variant,
+                            f32_out=True).cpu().numpy()
+    ref = fp64_reference(lv, hdr, step)
+    bound = BOUND[variant] * abs_transform_bound(lv, hdr, step) + 1e-5
+    assert np.all(np.abs(got - ref) <= bound), np.max(np.abs(got - ref))
