@@ -49,6 +49,10 @@ def parse():
     ap.add_argument("--no-cfg0", action="store_true", help="skip the 256^3 / 512^2 workload")
     ap.add_argument("--profile-decode", action="store_true",
                     help="for rocprofv3 --pmc: one 1 GiB device copy (calibration) then one decode launch")
+    ap.add_argument("--profile-variant", type=int, default=0,
+                    help="decode variant the --profile-decode launch uses (0 exact, 1 MFMA FP32, 2 MFMA FP16, 3 split FP16)")
+    ap.add_argument("--profile-raycast", action="store_true",
+                    help="for rocprofv3 --pmc: build the volume, one 1 GiB device copy, then one raycast of the frame")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "decode_exact_1024.md"),
                     help="rocprofv3 summary holding the decode kernel's measured HBM bytes per launch")
     return ap.parse_args()
@@ -110,9 +114,21 @@ def main():
         b = torch.empty_like(a)
         b.copy_(a)  # a plain copy of known size: 1 GiB read + 1 GiB written
         torch.cuda.synchronize()
-        ctx.decode(batch)
+        ctx.decode(batch, variant=args.profile_variant)
         torch.cuda.synchronize()
-        print(json.dumps({"profile_decode": True, "chunks": nch, "copy_bytes": 1 << 30}))
+        print(json.dumps({"profile_decode": True, "variant": args.profile_variant, "chunks": nch,
+                          "copy_bytes": 1 << 30}))
+        return 0
+    if args.profile_raycast:
+        ctx.decode(batch)
+        vol.build_from_chunks(flat)
+        a = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        ctx.raycast(vol, [view], W, H, alpha_cutoff=0.99, out=img, counters=counters)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_raycast": True, "samples": int(counters[0]), "copy_bytes": 1 << 30}))
         return 0
 
     def step(timed):
@@ -261,6 +277,9 @@ def decode_variants(ctx, batch, levels, headers, step_tab, flat, stream, dev, re
         gbps = CHUNK_BYTES * nch / (ms / 1000) / 1e9
         out[name] = {"ms": ms, "GBps": gbps, "hbm_frac": gbps / HBM_PEAK_GBPS, "u8_differs_from_exact": diff,
                      "max_abs_err_vs_fp64": float(err.max()), "mean_abs_err_vs_fp64": float(err.mean())}
+    # HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated)
+    for name, doc in (("exact_f32", "decode_exact_1024.md"), ("mfma_f16", "decode_mfma_f16_1024.md")):
+        out[name]["traffic"] = read_traffic(os.path.join(ROOT, "profiles", doc))
     return out
 
 
