@@ -67,12 +67,20 @@ struct DecodeArgs {
   int32_t *status;
 };
 
-// dct3d.c:149-154
-__device__ __forceinline__ float deq_one(float lv, float step) {
+// deq_one (dct3d.c:149-154), r = (|l| - 1) * step + 0.80f * step + 0.40f * step
+// with 0.80f * step and 0.40f * step taken from a table (the same rounded
+// products, computed once per table entry).
+__device__ __forceinline__ float deq_tab(float lv, float step, float s8, float s4) {
   if (lv == 0.0f) return 0.0f;
-  float a = __builtin_fabsf(lv);
-  float r = (a - 1.0f) * step + 0.80f * step + 0.40f * step;
+  const float r = (__builtin_fabsf(lv) - 1.0f) * step + s8 + s4;
   return lv < 0.0f ? -r : r;
+}
+
+// dct3d.c:625-635 for the u8 output: clamp(roundf(v), 0, 255), NaN -> 0.
+// For 0.5 <= v <= 255, v + 0.5 is exact, so roundf(v) = trunc(v + 0.5);
+// v < 0.5 and NaN give 0, v > 255 gives 255.
+__device__ __forceinline__ uint32_t round_u8(float v) {
+  return v >= 0.5f ? (uint32_t)(fmin_sel(v, 255.0f) + 0.5f) : 0u;
 }
 
 // The basis has 15 distinct magnitudes: cm[k][n] = +-M[j] with
@@ -94,14 +102,19 @@ constexpr bool basis_neg(int k, int n) {
 // skips zero inputs (the only possible difference is the sign of an exact
 // zero, which the + dc that follows removes).
 __device__ __forceinline__ void idct16(const float in[16], float out[16]) {
+  // Each sum starts from its first term instead of 0 + term: that differs from
+  // the reference (which starts at +0 and skips zero inputs) only in the sign
+  // of an exact zero, which the + dc after the last pass removes.
   float e[8], o[8];
 #pragma unroll
   for (int n = 0; n < 8; n++) {
-    e[n] = 0.0f;
-    o[n] = 0.0f;
+    const float p = c_mag[basis_j(0, n)] * in[0];
+    e[n] = basis_neg(0, n) ? -p : p;
+    const float q = c_mag[basis_j(1, n)] * in[1];
+    o[n] = basis_neg(1, n) ? -q : q;
   }
 #pragma unroll
-  for (int k = 0; k < 16; k += 2) {
+  for (int k = 2; k < 16; k += 2) {
     const float v = in[k];
 #pragma unroll
     for (int n = 0; n < 8; n++) {
@@ -110,7 +123,7 @@ __device__ __forceinline__ void idct16(const float in[16], float out[16]) {
     }
   }
 #pragma unroll
-  for (int k = 1; k < 16; k += 2) {
+  for (int k = 3; k < 16; k += 2) {
     const float v = in[k];
 #pragma unroll
     for (int n = 0; n < 8; n++) {
@@ -190,7 +203,7 @@ __device__ __forceinline__ void idct3_tile(float v[16], float *tile, uint32_t t,
 template <bool F32OUT>
 __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
   __shared__ float tile[4096];
-  __shared__ float steps[48];
+  __shared__ float steps[3][48];  // step, 0.80f * step, 0.40f * step
   const uint32_t t = threadIdx.x;
   const uint32_t z = t >> 4, y = t & 15u;
   for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
@@ -210,7 +223,12 @@ __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
       if (a.status && t == 0) a.status[c] = 0;
       continue;
     }
-    if (t < 46) steps[t] = a.tables[(size_t)h.table * 46u + t];
+    if (t < 46) {
+      const float st = a.tables[(size_t)h.table * 46u + t];
+      steps[0][t] = st;
+      steps[1][t] = 0.80f * st;
+      steps[2][t] = 0.40f * st;
+    }
     float v[16];
     if (a.fmt == R3D_COEF_DENSE_I16) {
       const int4 *src = (const int4 *)(a.levels + (size_t)c * 4096u + t * 16u);
@@ -220,7 +238,8 @@ __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
 #pragma unroll
       for (int k = 0; k < 16; k++) {
         const int16_t l = (int16_t)((uint32_t)w[k >> 1] >> ((k & 1) * 16));
-        v[k] = deq_one((float)l, steps[z + y + k]);
+        const uint32_t f = z + y + k;
+        v[k] = deq_tab((float)l, steps[0][f], steps[1][f], steps[2][f]);
       }
     } else {
 #pragma unroll
@@ -232,7 +251,8 @@ __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
         const uint32_t pr = a.pairs[p];
         const uint32_t pos = pr & 4095u;
         const float l = (float)(int16_t)(pr >> 16);
-        tile[pos] = deq_one(l, steps[(pos >> 8) + ((pos >> 4) & 15u) + (pos & 15u)]);
+        const uint32_t f = (pos >> 8) + ((pos >> 4) & 15u) + (pos & 15u);
+        tile[pos] = deq_tab(l, steps[0][f], steps[1][f], steps[2][f]);
       }
       __syncthreads();
 #pragma unroll
@@ -263,12 +283,7 @@ __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
     } else {
       uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
-      for (int x = 0; x < 16; x++) {
-        const float val = tile[t * 16 + x] * inv + h.vmin;
-        float r = __builtin_roundf(val);
-        r = !(r >= 0.0f) ? 0.0f : (r > 255.0f ? 255.0f : r);
-        w[x >> 2] |= (uint32_t)r << ((x & 3) * 8);
-      }
+      for (int x = 0; x < 16; x++) w[x >> 2] |= round_u8(tile[t * 16 + x] * inv + h.vmin) << ((x & 3) * 8);
       store_row16(dst, w);
     }
     if (a.status && t == 0) a.status[c] = 1;
