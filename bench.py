@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (graph) workload")
     ap.add_argument("--batched-chunks", type=int, default=32768)
     ap.add_argument("--no-cfg0", action="store_true", help="skip the 256^3 / 512^2 workload")
+    ap.add_argument("--no-fit", action="store_true", help="skip the differentiable 256^3 fit (200 Adam steps)")
     ap.add_argument("--profile-decode", action="store_true",
                     help="for rocprofv3 --pmc: one 1 GiB device copy (calibration) then one decode launch")
     ap.add_argument("--profile-variant", type=int, default=0,
@@ -168,6 +169,7 @@ def main():
     adj = None if args.no_adjoint else adjoint_ratio(ctx, vol, lut, args, W, H, dev)
     bat = None if args.no_batched else batched(ctx, vol, lut, args, dev, stream)
     c0 = None if args.no_cfg0 else cfg0(ctx, args, dev, stream)
+    fit = None if args.no_fit else fit_256(ctx, dev)
 
     cpu = None
     parity = None
@@ -204,6 +206,7 @@ def main():
         "adjoint": adj,
         "batched": bat,
         "cfg0": c0,
+        "fit": fit,
         "samples_per_frame": samples,
         "skips_per_frame": skips,
         "shaded_per_frame": shaded,
@@ -413,6 +416,55 @@ def cfg0(ctx, args, dev, stream):
     res["config"] = "256^3 12-sheet volume, full pyramid, 512x512, step 0.5, cutoff 0.99"
     res["_state"] = state
     return res
+
+
+def fit_256(ctx, dev, steps=200):
+    """Differentiable workload: the 256^3 sheet volume, a 64^3 FP32 density
+    grid (init 1.0) and an FP32 TF; 16 views at 512^2 in one 2048^2 image;
+    targets rendered with a ground-truth TF of Gaussian alpha bumps at 90 and
+    170 (sigma 15); L2 loss over RGBA; Adam on TF and grid from the 40->200
+    ramp TF. Reports the loss curve and time per forward + adjoint step."""
+    import numpy as np
+    import torch
+
+    from render3d_amd import scene
+    from render3d_amd.autograd import DiffRenderer
+
+    n = 256
+    vol = ctx.volume(n, n, n)
+    vol.upload(ctx.synth_sheets(n, n, n, 12, 20261203))
+    vol.build()
+    views = [scene.frame_params(scene.orbit_camera(22.5 * i, 20.0 * (i % 3 - 1), 2.0), 512, 512, step_voxels=0.5,
+                                view_org=((i % 4) * 512, (i // 4) * 512)) for i in range(16)]
+    R = DiffRenderer(ctx, vol, views, 2048, 2048, alpha_cutoff=0.99)
+    x = np.arange(256, dtype=np.float64)
+    gt = np.zeros((256, 4))
+    gt[:, 0], gt[:, 1], gt[:, 2] = np.clip(x / 255 * 1.1, 0, 1), np.clip(x / 255 * 0.9, 0, 1), np.clip(x / 255 * 0.7, 0, 1)
+    gt[:, 3] = np.clip(0.8 * (np.exp(-0.5 * ((x - 90) / 15) ** 2) + np.exp(-0.5 * ((x - 170) / 15) ** 2)), 0, 0.95)
+    ones = torch.ones((64, 64, 64), dtype=torch.float32, device=dev)
+    target = R.forward_raw(torch.from_numpy(gt.astype(np.float32)).to(dev), ones).clone()
+    tf = torch.from_numpy(scene.tf_unorm(scene.tf_ramp_40_200())).to(dev).requires_grad_(True)
+    grid = ones.clone().requires_grad_(True)
+    opt = torch.optim.Adam([tf, grid], lr=2e-2)
+    curve = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for it in range(steps):
+        opt.zero_grad()
+        loss = ((R(tf, grid) - target) ** 2).mean()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            tf.clamp_(0.0, 1.0)
+            grid.clamp_(min=0.0)
+        if it % 20 == 0 or it == steps - 1:
+            curve.append(float(loss))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    err = float((tf.detach() - torch.from_numpy(gt.astype(np.float32)).to(dev)).abs().mean())
+    return {"views": 16, "size": [512, 512], "grid": 64, "steps": steps, "loss_first": curve[0],
+            "loss_last": curve[-1], "loss_curve_every_20": curve, "ms_per_step": dt / steps * 1e3,
+            "tf_mean_abs_err_vs_truth": err}
 
 
 def cpu_cfg0(state, args):

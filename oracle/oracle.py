@@ -67,6 +67,8 @@ def load():
     L.r3do_raycast_diff.restype = C.c_int
     L.r3do_raycast_diff.argtypes = [C.POINTER(VolumeC), f64p, f64p, u32p, VP, C.c_uint32, C.c_double, f64p,
                                     C.c_uint32, C.c_uint32, VP, VP, VP]
+    L.r3do_raycast_diff_window.restype = C.c_int
+    L.r3do_raycast_diff_window.argtypes = L.r3do_raycast_diff.argtypes + [C.c_uint32] * 4
     L.r3do_synth_sheets.argtypes = [u8p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
                                     C.c_uint32, C.c_uint32, C.c_int]
     _lib = L
@@ -223,8 +225,9 @@ def math_batch(x, fn):
     return out
 
 
-def raycast_diff(ovol, tf, grid, gdims, views, width, height, alpha_cutoff=0.98, dimg=None):
+def raycast_diff(ovol, tf, grid, gdims, views, width, height, alpha_cutoff=0.98, dimg=None, window=None):
     """FP64 differentiable forward (and reverse when dimg is given).
+    window = (x0, y0, x1, y1) limits both to that ray window of each view.
     Returns (img [h,w,4] float64, dtf [256,4] or None, dgrid or None)."""
     L = load()
     n = len(views)
@@ -240,8 +243,12 @@ def raycast_diff(ovol, tf, grid, gdims, views, width, height, alpha_cutoff=0.98,
         dtf = np.zeros(1024, np.float64)
         dgrid = np.zeros(grid.size, np.float64)
         args = [dimg.ctypes.data, dtf.ctypes.data, dgrid.ctypes.data]
-    rc = L.r3do_raycast_diff(C.byref(ovol.c), tf, grid, gd, C.cast(arr, VP), n, alpha_cutoff, img, width, height,
-                             *args)
+    if window is None:
+        rc = L.r3do_raycast_diff(C.byref(ovol.c), tf, grid, gd, C.cast(arr, VP), n, alpha_cutoff, img, width, height,
+                                 *args)
+    else:
+        rc = L.r3do_raycast_diff_window(C.byref(ovol.c), tf, grid, gd, C.cast(arr, VP), n, alpha_cutoff, img, width,
+                                        height, *args, *[int(x) for x in window])
     if rc != 0:
         raise ValueError("oracle raycast_diff: unsupported view")
     return img.reshape(height, width, 4), (None if dtf is None else dtf.reshape(256, 4)), dgrid

@@ -577,9 +577,10 @@ static void diff_pixel(const r3do_volume *V, const double *tf, const double *G, 
   free(Tb);
 }
 
-int r3do_raycast_diff(const r3do_volume *vol, const double *tf, const double *grid, const uint32_t gd[3],
-                      const r3do_frame_params *views, uint32_t nviews, double cutoff, double *img,
-                      uint32_t img_w, uint32_t img_h, const double *dimg, double *dtf, double *dgrid) {
+int r3do_raycast_diff_window(const r3do_volume *vol, const double *tf, const double *grid, const uint32_t gd[3],
+                             const r3do_frame_params *views, uint32_t nviews, double cutoff, double *img,
+                             uint32_t img_w, uint32_t img_h, const double *dimg, double *dtf, double *dgrid,
+                             uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1) {
   for (uint32_t v = 0; v < nviews; v++) {
     if (!view_supported(&views[v])) return -1;
     if (views[v].mode != MODE_FULL && views[v].mode != MODE_FLAT) return -1;
@@ -592,8 +593,8 @@ int r3do_raycast_diff(const r3do_volume *vol, const double *tf, const double *gr
     const uint32_t ox = pc->view_org & 0xffffu, oy = pc->view_org >> 16;
     const uint32_t W = pc->viewport[0], H = pc->viewport[1];
     const uint32_t TW = half ? (W + 1) / 2 : W, TH = half ? (H + 1) / 2 : H;
-    for (uint32_t ty = 0; ty < TH; ty++)
-      for (uint32_t tx = 0; tx < TW; tx++) {
+    for (uint32_t ty = y0; ty < TH && ty < y1; ty++)
+      for (uint32_t tx = x0; tx < TW && tx < x1; tx++) {
         uint32_t lx = half ? tx * 2 : tx, ly = half ? ty * 2 : ty;
         double dout[4] = {0, 0, 0, 0}, c[4];
         if (dimg)
@@ -614,4 +615,11 @@ int r3do_raycast_diff(const r3do_volume *vol, const double *tf, const double *gr
   }
   free(buf);
   return 0;
+}
+
+int r3do_raycast_diff(const r3do_volume *vol, const double *tf, const double *grid, const uint32_t gd[3],
+                      const r3do_frame_params *views, uint32_t nviews, double cutoff, double *img,
+                      uint32_t img_w, uint32_t img_h, const double *dimg, double *dtf, double *dgrid) {
+  return r3do_raycast_diff_window(vol, tf, grid, gd, views, nviews, cutoff, img, img_w, img_h, dimg, dtf, dgrid, 0,
+                                  0, 0xffffffffu, 0xffffffffu);
 }

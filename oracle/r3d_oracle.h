@@ -144,6 +144,11 @@ int r3do_raycast(const r3do_volume *vol, const float *tf, const r3do_frame_param
 int r3do_raycast_diff(const r3do_volume *vol, const double *tf, const double *grid, const uint32_t gd[3],
                       const r3do_frame_params *views, uint32_t nviews, double cutoff, double *img,
                       uint32_t img_w, uint32_t img_h, const double *dimg, double *dtf, double *dgrid);
+/* The same, over the ray grid window [x0, x1) x [y0, y1) of every view only. */
+int r3do_raycast_diff_window(const r3do_volume *vol, const double *tf, const double *grid, const uint32_t gd[3],
+                             const r3do_frame_params *views, uint32_t nviews, double cutoff, double *img,
+                             uint32_t img_w, uint32_t img_h, const double *dimg, double *dtf, double *dgrid,
+                             uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1);
 
 /* Pinned math, exported so tests can compare against libm and the GPU. */
 float r3do_log2f(float x);

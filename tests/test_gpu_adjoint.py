@@ -125,3 +125,80 @@ def test_adam_fit_lowers_the_loss(ctx):
         opt.step()
         losses.append(float(loss))
     assert losses[-1] < 0.5 * losses[0], losses[::8]
+
+
+def test_full_scale_gradient_in_a_window(ctx):
+    """1024^3 volume decoded from its chunks (4 levels), a 128^3 grid and one
+    1920x1080 view: the GPU adjoint of a linear loss on a 16x16 ray window
+    against the oracle's FP64 adjoint of the same rays and FP64 central
+    differences (h = 1e-3) on 32 entries."""
+    from render3d_amd.autograd import DiffRenderer
+    from test_oracle_adjoint import gaussian_tf
+
+    n, W, H = 1024, 1920, 1080
+    step = torch.arange(46, dtype=torch.float32, device="cuda") * 2 + 1
+    lin = ctx.synth_sheets(n, n, n, 48, 20261203)
+    levels, headers = ctx.encode_chunks(lin, n, n, n, step)
+    del lin
+    flat = torch.empty((levels.shape[0], 4096), dtype=torch.uint8, device="cuda")
+    ctx.decode(ctx.make_batch(headers=headers, step_tables=step, out=flat, levels=levels))
+    vol = ctx.volume(n, n, n, 4)
+    vol.build_from_chunks(flat)
+    del levels, headers, flat
+    torch.cuda.synchronize()
+    lv = [(d, vol.read_level(l)) for l, d in enumerate(vol.level_dims)]
+    ov = orc.OracleVolume(None, n, n, n, levels=lv, occ=(vol.odim, vol.read_occupancy()))
+    del lv
+    g = 128
+    zz, yy, xx = np.meshgrid(*(np.arange(g),) * 3, indexing="ij")
+    grid = (1.0 + 0.2 * np.sin(xx / 9.0) * np.cos(yy / 7.0 + zz / 11.0)).astype(np.float32)
+    gd = (g, g, g)
+    tf = gaussian_tf().astype(np.float32)
+    view = scene.frame_params(scene.orbit_camera(45.0, 10.0, 2.0), W, H, step_voxels=1.0)
+    win = (952, 532, 968, 548)
+    rng = np.random.default_rng(4)
+    dimg = np.zeros((H, W, 4), np.float32)
+    dimg[win[1]:win[3], win[0]:win[2]] = rng.standard_normal((win[3] - win[1], win[2] - win[0], 4))
+    R = DiffRenderer(ctx, vol, [view], W, H, alpha_cutoff=0.98)
+    t_tf, t_g = torch.from_numpy(tf).cuda(), torch.from_numpy(grid).cuda()
+    img = R.forward_raw(t_tf, t_g)
+    a, b = R.adjoint_raw(t_tf, t_g, torch.from_numpy(dimg).cuda())
+    torch.cuda.synchronize()
+    gtf = a.cpu().numpy().astype(np.float64)
+    gg = b.cpu().numpy().reshape(-1).astype(np.float64)
+    tfd, gdd = tf.astype(np.float64), grid.astype(np.float64).reshape(-1)
+    ref, rtf, rg = orc.raycast_diff(ov, tfd, gdd, gd, [view], W, H, 0.98, dimg=dimg.astype(np.float64), window=win)
+    sub = (slice(win[1], win[3]), slice(win[0], win[2]))
+    err = np.abs(img.cpu().numpy()[sub] - ref[sub])
+    assert np.mean(err < 1e-4) > 0.99 and err.max() < 1e-2, (np.mean(err < 1e-4), err.max())
+    assert rel(gtf, rtf) <= 1e-3, rel(gtf, rtf)
+    assert rel(gg, rg) <= 1e-3, rel(gg, rg)
+
+    def L():
+        im, _, _ = orc.raycast_diff(ov, tfd, gdd, gd, [view], W, H, 0.98, window=win)
+        return float(np.sum(im[sub] * dimg[sub]))
+
+    cand_tf = np.argwhere(np.abs(gtf) > 1e-6)
+    cand_g = np.nonzero(np.abs(gg) > 1e-6)[0]
+    picks = [("tf", tuple(c)) for c in cand_tf[rng.choice(len(cand_tf), 16, replace=False)]]
+    picks += [("g", int(i)) for i in rng.choice(cand_g, 16, replace=False)]
+    scale = max(np.abs(gtf).max(), np.abs(gg).max())
+    checked = 0
+    for kind, idx in picks:
+        arr = tfd if kind == "tf" else gdd
+        base = arr[idx]
+        vals = []
+        for h in (1e-3, 5e-4):
+            arr[idx] = base + h
+            lp = L()
+            arr[idx] = base - h
+            lm = L()
+            arr[idx] = base
+            vals.append((lp - lm) / (2 * h))
+        if abs(vals[0] - vals[1]) > 1e-5 * max(scale, 1.0) + 1e-3 * abs(vals[0]):
+            continue  # a gate, the cutoff or a clamp within h of this entry
+        got = gtf[idx] if kind == "tf" else gg[idx]
+        checked += 1
+        assert abs(got - vals[0]) <= 1e-3 * max(abs(vals[0]), abs(got)) + 1e-5 * scale, (kind, idx, got, vals[0])
+    print(f"full-scale window: {checked} of 32 entries checked against FP64 central differences")
+    assert checked >= 20
