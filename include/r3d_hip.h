@@ -166,6 +166,12 @@ typedef struct r3d_chunk_batch {
    * zero-filled, as r3d_shard_chunk_decode does for a corrupt chunk,
    * src/shard/shardio.c:170-189). */
   int32_t *status;
+  /* optional per-chunk counts for capacity layouts (r3d_hip_parse_blobs):
+   * chunk c's pairs are pairs[pair_offsets[c] .. + pair_counts[c]) and its
+   * corrections corr_pos/corr_delta[corr_offsets[c] .. + corr_counts[c]),
+   * both still clamped to the next offset and to npairs / ncorr. */
+  const uint32_t *pair_counts;
+  const uint32_t *corr_counts;
 } r3d_chunk_batch;
 
 enum {
@@ -362,6 +368,59 @@ int r3d_hip_synth_sheets(r3d_hip_ctx *ctx, uint8_t *dst, uint32_t nx, uint32_t n
 int r3d_hip_encode_chunks(r3d_hip_ctx *ctx, const uint8_t *vol, uint32_t nx, uint32_t ny,
                           uint32_t nz, const float *step_table, int16_t *levels,
                           r3d_chunk_header *headers, void *stream);
+
+/* ---- 3ddct blobs: the entropy stage ------------------------------------------ */
+
+/* A 3ddct blob (tools/3ddct/dct3d.c:414-433): the preamble [0xD3][dtype], then
+ * one range-coded stream with vmin, vspan, qstep, dcq, the corrections flag,
+ * the levels and the corrections. dtype ids as dct3d.c:430. */
+enum { R3D_DT_U8 = 1, R3D_DT_U16 = 2, R3D_DT_U32 = 3, R3D_DT_S8 = 4, R3D_DT_S16 = 5, R3D_DT_S32 = 6,
+       R3D_DT_F32 = 7 };
+
+typedef struct r3d_blob_batch {
+  uint32_t nchunks;
+  uint32_t dtype;               /* the dtype every blob must carry (R3D_DT_U8 for u8 output) */
+  uint32_t max_pairs;           /* per-chunk capacity of levels and of corrections, 1..4096 */
+  uint32_t reserved0;
+  const uint8_t *bytes;         /* device: the blobs */
+  const uint64_t *offsets;      /* device [nchunks]: first byte of each blob */
+  const uint32_t *lengths;      /* device [nchunks]: length of each blob */
+} r3d_blob_batch;
+
+/* Device outputs of r3d_hip_parse_blobs, laid out as a sparse
+ * r3d_chunk_batch with per-chunk counts: chunk c's data start at
+ * c * max_pairs. */
+typedef struct r3d_blob_parse_out {
+  r3d_chunk_header *headers;    /* [nchunks]: table = c; vspan = 0 for a rejected blob */
+  float *step_tables;           /* [nchunks * 46]: qstep * powf(1 + f, 0.65f) (dct3d.c:131-143) */
+  uint32_t *pair_offsets;       /* [nchunks + 1] */
+  uint32_t *pair_counts;        /* [nchunks] */
+  uint32_t *pairs;              /* [nchunks * max_pairs]: R3D_PAIR(pos, level) */
+  uint32_t *corr_offsets;       /* [nchunks + 1] */
+  uint32_t *corr_counts;        /* [nchunks] */
+  uint16_t *corr_pos;           /* [nchunks * max_pairs] */
+  float *corr_delta;            /* [nchunks * max_pairs]: +-mag * cq (dct3d.c:583-585) */
+  int32_t *status;              /* [nchunks] or NULL: 1 parsed; 0 malformed (decode_float would
+                                   return 0); -1 a level beyond int16; -2 more levels or
+                                   corrections than max_pairs. A chunk with status != 1 decodes
+                                   to zeros with decode status 0. */
+} r3d_blob_parse_out;
+
+/* Parses each blob (the entropy part of decode_float, dct3d.c:542-587), one
+ * GPU lane per blob. The result feeds r3d_hip_decode_chunks as a sparse batch
+ * (R3D_COEF_SPARSE, ntables = nchunks, with pair_counts / corr_counts), which
+ * together give dct3d_decode_u8's output. Replaces the per-chunk CPU loop of
+ * r3d_shard_decode_region (src/shard/shardio.c:269-306). */
+int r3d_hip_parse_blobs(r3d_hip_ctx *ctx, const r3d_blob_batch *in, const r3d_blob_parse_out *out, void *stream);
+
+/* Workload encoder: the entropy-coding back half of encode_float
+ * (dct3d.c:499-537) without corrections, one lane per chunk. Blob c is written
+ * at out + c * slot_bytes and lengths[c] is its length (longer than
+ * slot_bytes if it did not fit; the excess is dropped). headers give vmin,
+ * vspan and dcq; qsteps[c] is coded as the blob's qstep. */
+int r3d_hip_encode_blobs(r3d_hip_ctx *ctx, uint32_t nchunks, const int16_t *levels,
+                         const r3d_chunk_header *headers, const float *qsteps, uint32_t dtype, uint8_t *out,
+                         uint64_t slot_bytes, uint32_t *lengths, void *stream);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,11 @@ def load():
     L.r3do_raycast_diff_window.argtypes = L.r3do_raycast_diff.argtypes + [C.c_uint32] * 4
     L.r3do_synth_sheets.argtypes = [u8p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
                                     C.c_uint32, C.c_uint32, C.c_int]
+    L.r3do_parse_blob.restype = C.c_int
+    L.r3do_parse_blob.argtypes = [u8p, C.c_size_t, C.c_uint32, C.POINTER(BlobHdr), i16p, C.POINTER(C.c_uint32),
+                                  np.ctypeslib.ndpointer(np.uint16, flags="C"), f32p]
+    L.r3do_encode_blob.restype = C.c_size_t
+    L.r3do_encode_blob.argtypes = [C.POINTER(BlobHdr), i16p, C.c_uint32, u8p, C.c_size_t]
     _lib = L
     return L
 
@@ -223,6 +228,33 @@ def math_batch(x, fn):
     out = np.zeros_like(x)
     load().r3do_math_batch(x, x.size, fn, out)
     return out
+
+
+class BlobHdr(C.Structure):
+    _fields_ = [("vmin", C.c_float), ("vspan", C.c_float), ("qstep", C.c_float), ("dcq", C.c_int32)]
+
+
+def parse_blob(blob, dtype=1):
+    """r3do_parse_blob -> (status, (vmin, vspan, qstep, dcq), levels int16[4096],
+    corr_pos uint16[n], corr_delta float32[n])."""
+    blob = np.ascontiguousarray(blob, np.uint8)
+    h = BlobHdr()
+    lvl = np.zeros(4096, np.int16)
+    nc = C.c_uint32()
+    cp = np.zeros(4096, np.uint16)
+    cd = np.zeros(4096, np.float32)
+    st = load().r3do_parse_blob(blob if blob.size else np.zeros(1, np.uint8), blob.size, dtype, C.byref(h), lvl,
+                                C.byref(nc), cp, cd)
+    return st, (h.vmin, h.vspan, h.qstep, h.dcq), lvl, cp[:nc.value].copy(), cd[:nc.value].copy()
+
+
+def encode_blob(vmin, vspan, qstep, dcq, levels, dtype=1, cap=4096 * 8 + 256):
+    """r3do_encode_blob (no corrections) -> bytes as uint8 array."""
+    h = BlobHdr(vmin, vspan, qstep, dcq)
+    out = np.zeros(cap, np.uint8)
+    n = load().r3do_encode_blob(C.byref(h), np.ascontiguousarray(levels, np.int16), dtype, out, cap)
+    assert n <= cap
+    return out[:n].copy()
 
 
 def raycast_diff(ovol, tf, grid, gdims, views, width, height, alpha_cutoff=0.98, dimg=None, window=None):

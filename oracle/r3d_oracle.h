@@ -167,6 +167,15 @@ void r3do_math_batch(const float *x, uint32_t n, uint32_t fn, float *out);
 void r3do_synth_sheets(uint8_t *vol, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t nsheets,
                        uint64_t seed, uint32_t z0, uint32_t z1, int nthreads);
 
+/* ---- 3ddct entropy stage (oracle_entropy.c) ---- */
+typedef struct r3do_blob_hdr {
+  float vmin, vspan, qstep;
+  int32_t dcq;
+} r3do_blob_hdr;
+int r3do_parse_blob(const uint8_t *blob, size_t len, uint32_t want_dtype, r3do_blob_hdr *h, int16_t lvl[4096],
+                    uint32_t *ncorr, uint16_t *corr_pos, float *corr_delta);
+size_t r3do_encode_blob(const r3do_blob_hdr *h, const int16_t lvl[4096], uint32_t dtype, uint8_t *out, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,7 @@ class ChunkBatch(C.Structure):
         ("out", C.c_void_p), ("out_offsets", C.c_void_p),
         ("out_stride_y", C.c_uint32), ("out_stride_z", C.c_uint32),
         ("status", C.c_void_p),
+        ("pair_counts", C.c_void_p), ("corr_counts", C.c_void_p),
     ]
 
 
@@ -117,6 +118,21 @@ class Stats(C.Structure):
 class RaycastOpts(C.Structure):
     _fields_ = [("alpha_cutoff", C.c_float), ("flags", C.c_uint32), ("tf", C.c_void_p),
                 ("counters", C.c_void_p), ("px_steps", C.c_void_p)]
+
+
+class BlobBatch(C.Structure):
+    _fields_ = [("nchunks", C.c_uint32), ("dtype", C.c_uint32), ("max_pairs", C.c_uint32), ("reserved0", C.c_uint32),
+                ("bytes", C.c_void_p), ("offsets", C.c_void_p), ("lengths", C.c_void_p)]
+
+
+class BlobParseOut(C.Structure):
+    _fields_ = [("headers", C.c_void_p), ("step_tables", C.c_void_p), ("pair_offsets", C.c_void_p),
+                ("pair_counts", C.c_void_p), ("pairs", C.c_void_p), ("corr_offsets", C.c_void_p),
+                ("corr_counts", C.c_void_p), ("corr_pos", C.c_void_p), ("corr_delta", C.c_void_p),
+                ("status", C.c_void_p)]
+
+
+R3D_DT_U8 = 1
 
 
 class RaycastTuning(C.Structure):
@@ -168,6 +184,8 @@ SIGNATURES = {
     "r3d_hip_synth_sheets": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.c_uint64, P]),
     "r3d_hip_encode_chunks": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P]),
+    "r3d_hip_parse_blobs": (C.c_int, [P, C.POINTER(BlobBatch), C.POINTER(BlobParseOut), P]),
+    "r3d_hip_encode_blobs": (C.c_int, [P, C.c_uint32, P, P, P, C.c_uint32, P, C.c_uint64, P, P]),
 }
 
 _lib = None

@@ -10,7 +10,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (ChunkBatch, DecodeOpts, FrameParams, Image, R3DError, RaycastOpts, RaycastTuning, Stats,
+from ._lib import (BlobBatch, BlobParseOut, ChunkBatch, DecodeOpts, FrameParams, Image, R3DError, RaycastOpts,
+                   RaycastTuning, Stats,
                    VolumeDesc)
 
 try:
@@ -87,7 +88,7 @@ class Context:
     # ---- decode -----------------------------------------------------------
     def make_batch(self, *, headers, step_tables, out, levels=None, pairs=None, pair_offsets=None,
                    out_offsets=None, stride_y=0, stride_z=0, corr_offsets=None, corr_pos=None,
-                   corr_delta=None, status=None, nchunks=None):
+                   corr_delta=None, status=None, nchunks=None, pair_counts=None, corr_counts=None):
         b = ChunkBatch()
         b.nchunks = int(headers.shape[0]) if nchunks is None else int(nchunks)
         if levels is not None:
@@ -111,6 +112,8 @@ class Context:
         b.out_stride_y = stride_y
         b.out_stride_z = stride_z
         b.status = None if status is None else status.data_ptr()
+        b.pair_counts = None if pair_counts is None else pair_counts.data_ptr()
+        b.corr_counts = None if corr_counts is None else corr_counts.data_ptr()
         return b
 
     def decode(self, batch, variant=_lib.R3D_DECODE_EXACT_F32, stream=None, flags=0):
@@ -129,6 +132,62 @@ class Context:
                             corr_delta=corr_delta, status=status)
         self.decode(b, variant, stream, _lib.R3D_DECODE_F32_OUT if f32_out else 0)
         return out
+
+    # ---- 3ddct blobs -------------------------------------------------------
+    def encode_blobs(self, levels, headers, qstep, *, dtype=_lib.R3D_DT_U8, slot_bytes=4096 * 8 + 256, stream=None):
+        """Entropy-code dense levels [n, 4096] int16 with headers [n, 4] (vmin,
+        vspan, dcq) and qstep (a float or a [n] float32 tensor) into 3ddct
+        blobs. Returns (bytes uint8 [n * slot_bytes], offsets int64 [n],
+        lengths int32 [n]); blob c starts at c * slot_bytes."""
+        n = int(levels.shape[0])
+        dev = levels.device
+        qs = qstep if isinstance(qstep, torch.Tensor) else torch.full((n,), float(qstep), dtype=torch.float32,
+                                                                        device=dev)
+        out = torch.empty(n * slot_bytes, dtype=torch.uint8, device=dev)
+        lengths = torch.empty(n, dtype=torch.int32, device=dev)
+        self.check(self.lib.r3d_hip_encode_blobs(self.h, n, _ptr(levels), _ptr(headers), _ptr(qs), dtype, _ptr(out),
+                                                 slot_bytes, _ptr(lengths), _stream(stream)))
+        offsets = torch.arange(n, dtype=torch.int64, device=dev) * slot_bytes
+        return out, offsets, lengths
+
+    def parse_blobs(self, data, offsets, lengths, *, max_pairs=4096, dtype=_lib.R3D_DT_U8, stream=None):
+        """r3d_hip_parse_blobs: blobs -> the arrays of a sparse decode batch
+        (dict of tensors, plus 'status')."""
+        n = int(offsets.shape[0])
+        dev = data.device
+        o = {"headers": torch.empty((n, 4), dtype=torch.int32, device=dev),
+             "step_tables": torch.empty(n * 46, dtype=torch.float32, device=dev),
+             "pair_offsets": torch.empty(n + 1, dtype=torch.int32, device=dev),
+             "pair_counts": torch.empty(n, dtype=torch.int32, device=dev),
+             "pairs": torch.empty(n * max_pairs, dtype=torch.int32, device=dev),
+             "corr_offsets": torch.empty(n + 1, dtype=torch.int32, device=dev),
+             "corr_counts": torch.empty(n, dtype=torch.int32, device=dev),
+             "corr_pos": torch.empty(n * max_pairs, dtype=torch.int16, device=dev),
+             "corr_delta": torch.empty(n * max_pairs, dtype=torch.float32, device=dev),
+             "status": torch.empty(n, dtype=torch.int32, device=dev)}
+        bb = BlobBatch(n, dtype, max_pairs, 0, _ptr(data), _ptr(offsets), _ptr(lengths))
+        po = BlobParseOut(*[_ptr(o[k]) for k in ("headers", "step_tables", "pair_offsets", "pair_counts", "pairs",
+                                                  "corr_offsets", "corr_counts", "corr_pos", "corr_delta",
+                                                  "status")])
+        self.check(self.lib.r3d_hip_parse_blobs(self.h, C.byref(bb), C.byref(po), _stream(stream)))
+        return o
+
+    def parsed_batch(self, parsed, out, **kw):
+        """A sparse r3d_chunk_batch over parse_blobs' output."""
+        return self.make_batch(headers=parsed["headers"], step_tables=parsed["step_tables"], out=out,
+                               pairs=parsed["pairs"], pair_offsets=parsed["pair_offsets"],
+                               pair_counts=parsed["pair_counts"], corr_offsets=parsed["corr_offsets"],
+                               corr_pos=parsed["corr_pos"], corr_delta=parsed["corr_delta"],
+                               corr_counts=parsed["corr_counts"], **kw)
+
+    def decode_blobs(self, data, offsets, lengths, *, max_pairs=4096, variant=_lib.R3D_DECODE_EXACT_F32,
+                     stream=None):
+        """3ddct blobs -> u8 [n, 4096] (parse, then decode) and the parse status."""
+        parsed = self.parse_blobs(data, offsets, lengths, max_pairs=max_pairs, stream=stream)
+        n = int(offsets.shape[0])
+        out = torch.empty((n, 4096), dtype=torch.uint8, device=data.device)
+        self.decode(self.parsed_batch(parsed, out), variant, stream)
+        return out, parsed["status"]
 
     # ---- workloads ----------------------------------------------------------
     def synth_sheets(self, nx, ny, nz, nsheets, seed, stream=None):

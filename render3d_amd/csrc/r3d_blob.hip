@@ -1,0 +1,454 @@
+// r3d_blob.hip -- the 3ddct entropy stage on gfx950: blobs -> sparse levels,
+// and the matching encoder for workloads.
+//
+// One lane per blob. The stage is the CABAC-style binary range coder of
+// tools/3ddct/dct3d.c:156-405 with the framing of dct3d.c:414-433, 542-587:
+// a 2-byte preamble (0xD3, dtype), then one range-coded stream holding vmin,
+// vspan, qstep (float fields), dcq (signed Exp-Golomb), the corrections flag,
+// the levels (EOB, banded significance contexts conditioned on the density of
+// the last 16 decisions, an 11-context unary magnitude ladder with an
+// Exp-Golomb escape, bypass signs) and the corrections. Every step is integer
+// arithmetic, so the parse is exact by construction.
+//
+// The adaptive contexts (59 per lane, 16-bit) live in LDS, one row per
+// context, so lanes touching the same context hit consecutive addresses; the
+// ascending-L1 scan table is staged in LDS once per workgroup.
+#include "r3d_internal.h"
+
+namespace r3d {
+
+constexpr uint32_t BLOB_NT = 256;  // lanes (blobs) per workgroup
+constexpr uint32_t RC_TOP = 1u << 24;
+enum : uint32_t { CTX_SIG = 0, CTX_MAG = 32, CTX_EOB = 44, CTX_FLAG = 58, NCTX = 59 };
+
+__constant__ uint16_t c_scan[4096];  // ascending L1 frequency, ties by index (dct3d.c:336-351)
+__constant__ float c_hfw[46];        // powf(1 + f, 0.65f), computed by the host's libm (dct3d.c:131-137)
+// Trained priors P(bit == 0) in 1/4096 (dct3d.c:290-299), in context order.
+__constant__ uint16_t c_prior[NCTX] = {
+    1980, 575,  846,  743,  3916, 3707, 3515, 1463, 3969, 3753, 3527, 2313, 4002, 3826, 3635, 2819,
+    4039, 3851, 3613, 2974, 3985, 3840, 3703, 3475, 4021, 3991, 3951, 3969, 4065, 4024, 2048, 2048,
+    1893, 1217, 908,  728,  610,  527,  465,  416,  377,  346,  320,  297,  74,   1,    1,    1,
+    1,    5,    109,  504,  772,  1063, 1458, 2326, 4095, 2048, 2048};
+
+__device__ __forceinline__ uint32_t band_of(uint32_t idx) {
+  const uint32_t b = ((idx >> 8) + ((idx >> 4) & 15u) + (idx & 15u)) * 8u / 48u;
+  return b > 7u ? 7u : b;
+}
+
+// ---- decode side -----------------------------------------------------------
+struct RDec {
+  const uint8_t *p;
+  uint32_t n, at;
+  uint32_t code, range;
+  uint16_t *ctx;  // this lane's contexts: ctx[i * BLOB_NT]
+
+  __device__ __forceinline__ uint32_t next() { return at < n ? p[at++] : 0u; }
+  __device__ __forceinline__ void init(const uint8_t *b, uint32_t len) {
+    p = b;
+    n = len;
+    at = 0;
+    code = 0;
+    range = 0xFFFFFFFFu;
+    for (int i = 0; i < 5; i++) code = (code << 8) | next();
+  }
+  __device__ __forceinline__ void norm() {
+    while (range < RC_TOP) {
+      code = (code << 8) | next();
+      range <<= 8;
+    }
+  }
+  __device__ __forceinline__ uint32_t bit(uint32_t ci) {
+    uint16_t *pc = ctx + ci * BLOB_NT;
+    const uint32_t p0 = *pc;
+    const uint32_t r0 = (range >> 12) * p0;
+    uint32_t b;
+    if (code < r0) {
+      range = r0;
+      *pc = (uint16_t)(p0 + ((4096u - p0) >> 4));
+      b = 0;
+    } else {
+      code -= r0;
+      range -= r0;
+      *pc = (uint16_t)(p0 - (p0 >> 4));
+      b = 1;
+    }
+    norm();
+    return b;
+  }
+  __device__ __forceinline__ uint32_t bypass() {
+    range >>= 1;
+    const uint32_t b = code >= range ? 1u : 0u;
+    if (b) code -= range;
+    norm();
+    return b;
+  }
+  __device__ __forceinline__ uint32_t bits(int k) {  // k raw bits, most significant first
+    uint32_t v = 0;
+    while (k > 0) {
+      const int s = k > 16 ? 16 : k;
+      range >>= s;
+      uint32_t q = code / range;
+      const uint32_t m = (1u << s) - 1u;
+      if (q > m) q = m;
+      code -= q * range;
+      norm();
+      v = (v << s) | q;
+      k -= s;
+    }
+    return v;
+  }
+  __device__ __forceinline__ uint32_t eg() {
+    uint32_t nb = 0;
+    while (bypass())
+      if (++nb > 31u) return 0;
+    return nb ? ((1u << nb) | bits((int)nb)) - 1u : 0u;
+  }
+  __device__ __forceinline__ int32_t seg() {
+    const uint32_t z = eg();
+    return (z & 1u) ? -(int32_t)((z + 1u) >> 1) : (int32_t)(z >> 1);
+  }
+  __device__ __forceinline__ float fval() {
+    if (!bypass()) return (float)seg();
+    return __uint_as_float(bits(32));
+  }
+};
+
+struct ParseArgs {
+  uint32_t n, dtype, cap;
+  const uint8_t *bytes;
+  const uint64_t *offsets;
+  const uint32_t *lengths;
+  r3d_chunk_header *hdr;
+  float *steps;
+  uint32_t *pair_off, *pair_cnt, *pairs;
+  uint32_t *corr_off, *corr_cnt;
+  uint16_t *corr_pos;
+  float *corr_delta;
+  int32_t *status;
+};
+
+__global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
+  __shared__ uint16_t s_scan[4096];
+  __shared__ uint16_t s_ctx[NCTX * BLOB_NT];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < 4096u; i += BLOB_NT) s_scan[i] = c_scan[i];
+  __syncthreads();
+  const uint32_t c = blockIdx.x * BLOB_NT + t;
+  if (c >= a.n) return;
+  RDec d;
+  d.ctx = s_ctx + t;
+  const uint64_t base = (uint64_t)c * a.cap;
+  const uint8_t *b = a.bytes + a.offsets[c];
+  const uint32_t len = a.lengths[c];
+  int32_t st = 1;
+  uint32_t np = 0, nc = 0;
+  float vmin = 0.0f, vspan = 0.0f, qstep = 0.0f;
+  int32_t dcq = 0;
+  if (len < 2u || b[0] != 0xD3u || b[1] != a.dtype) {
+    st = 0;
+  } else {
+    d.init(b + 2, len - 2u);
+    vmin = d.fval();
+    vspan = d.fval();
+    qstep = d.fval();
+    dcq = d.seg();
+    const float dc = (float)dcq / 16.0f;
+    if (!(vspan > 0.0f) || !(qstep > 0.0f) || !__builtin_isfinite(vmin) || !__builtin_isfinite(dc)) st = 0;
+  }
+  if (st == 1) {
+    for (uint32_t i = 0; i < NCTX; i++) d.ctx[i * BLOB_NT] = c_prior[i];
+    const uint32_t has_corr = d.bit(CTX_FLAG);
+    // EOB (dct3d.c:365-371): up to 13 context bits, then k - 1 raw bits
+    uint32_t k = 0;
+    while (k < 13u && d.bit(CTX_EOB + k)) k++;
+    uint32_t eob = k <= 1u ? k : ((1u << (k - 1u)) | d.bits((int)k - 1));
+    if (eob > 4096u) eob = 4096u;
+    uint32_t recent = 0;
+    for (uint32_t p = 0; p < eob; p++) {
+      const uint32_t idx = s_scan[p];
+      uint32_t dens = __builtin_popcount(recent & 0xFFFFu);
+      dens = dens < 3u ? dens : 3u;
+      const uint32_t sig = p + 1u == eob ? 1u : d.bit(CTX_SIG + band_of(idx) * 4u + dens);
+      recent = (recent << 1) | sig;
+      if (!sig) continue;
+      // magnitude (dct3d.c:322-329)
+      uint32_t m = 0;
+      while (m < 11u && d.bit(CTX_MAG + m)) m++;
+      if (m < 11u) {
+        m += 1u;
+      } else if (!d.bit(CTX_MAG + 11u)) {
+        m = 12u;
+      } else {
+        uint32_t nb = 0;
+        while (d.bypass())
+          if (++nb > 31u) break;
+        // nb can reach 32 on a corrupt stream; the shift count is taken mod
+        // 32, as the x86 and gfx9 shift instructions do for dct3d.c:326
+        m = 12u + (nb ? ((1u << (nb & 31u)) | d.bits((int)nb)) - 1u : 0u);
+      }
+      const uint32_t neg = d.bypass();
+      if (m > (neg ? 32768u : 32767u)) st = -1;  // beyond the int16 level formats
+      if (np >= a.cap) {
+        st = st == 1 ? -2 : st;  // more nonzero levels than the capacity
+      } else {
+        const uint32_t lv = (uint32_t)(uint16_t)(int16_t)(neg ? -(int32_t)m : (int32_t)m);
+        a.pairs[base + np] = idx | (lv << 16);
+        np++;
+      }
+    }
+    if (has_corr) {  // dct3d.c:572-587
+      uint32_t cnt = d.eg() + 1u, pos = 0;
+      if (cnt > 4096u) cnt = 4096u;
+      float cq = d.fval();
+      if (!(cq > 0.0f) || !__builtin_isfinite(cq)) cq = 1.0f;
+      for (uint32_t i = 0; i < cnt; i++) {
+        pos += d.eg();
+        const uint32_t neg = d.bypass();
+        const uint32_t mag = d.eg() + 1u;
+        if (pos >= 4096u) break;
+        const float delta = (float)mag * cq;
+        if (nc >= a.cap) {
+          st = st == 1 ? -2 : st;
+          break;
+        }
+        a.corr_pos[base + nc] = (uint16_t)pos;
+        a.corr_delta[base + nc] = neg ? -delta : delta;
+        nc++;
+      }
+    }
+  }
+  r3d_chunk_header h;
+  h.vmin = vmin;
+  h.vspan = st == 1 ? vspan : 0.0f;  // vspan 0 makes the decoder zero-fill the chunk (status 0)
+  h.dcq = dcq;
+  h.table = c;
+  a.hdr[c] = h;
+  for (uint32_t f = 0; f < 46u; f++) a.steps[(size_t)c * 46u + f] = st == 1 ? qstep * c_hfw[f] : 0.0f;
+  a.pair_off[c] = (uint32_t)base;
+  a.corr_off[c] = (uint32_t)base;
+  a.pair_cnt[c] = st == 1 ? np : 0u;
+  a.corr_cnt[c] = st == 1 ? nc : 0u;
+  if (c + 1u == a.n) {
+    a.pair_off[a.n] = (uint32_t)((uint64_t)a.n * a.cap);
+    a.corr_off[a.n] = (uint32_t)((uint64_t)a.n * a.cap);
+  }
+  if (a.status) a.status[c] = st;
+}
+
+// ---- encode side (workloads) -------------------------------------------------
+struct REnc {
+  uint8_t *out;
+  uint64_t cap, n;
+  uint64_t low;
+  uint32_t range;
+  uint32_t cache;
+  uint64_t pending;
+  uint16_t *ctx;
+
+  __device__ __forceinline__ void put(uint32_t b) {
+    if (n < cap) out[n] = (uint8_t)b;
+    n++;
+  }
+  __device__ __forceinline__ void shift() {  // dct3d.c:188-195
+    if ((uint32_t)(low >> 32) != 0u || low < 0xFF000000ull) {
+      const uint32_t carry = (uint32_t)(low >> 32);
+      do {
+        put((cache + carry) & 0xFFu);
+        cache = 0xFFu;
+      } while (--pending);
+      cache = (uint32_t)(low >> 24) & 0xFFu;
+    }
+    pending++;
+    low = (low << 8) & 0xFFFFFFFFull;
+  }
+  __device__ __forceinline__ void norm() {
+    while (range < RC_TOP) {
+      shift();
+      range <<= 8;
+    }
+  }
+  __device__ __forceinline__ void bit(uint32_t ci, uint32_t b) {
+    uint16_t *pc = ctx + ci * BLOB_NT;
+    const uint32_t p0 = *pc;
+    const uint32_t r0 = (range >> 12) * p0;
+    if (!b) {
+      range = r0;
+      *pc = (uint16_t)(p0 + ((4096u - p0) >> 4));
+    } else {
+      low += r0;
+      range -= r0;
+      *pc = (uint16_t)(p0 - (p0 >> 4));
+    }
+    norm();
+  }
+  __device__ __forceinline__ void bypass(uint32_t b) {
+    range >>= 1;
+    if (b) low += range;
+    norm();
+  }
+  __device__ __forceinline__ void bits(uint32_t v, int k) {
+    while (k > 0) {
+      const int s = k > 16 ? 16 : k;
+      range >>= s;
+      low += (uint64_t)((v >> (k - s)) & ((1u << s) - 1u)) * range;
+      norm();
+      k -= s;
+    }
+  }
+  __device__ __forceinline__ void eg(uint32_t v) {
+    uint32_t nb = 0, t = v + 1u;
+    while (t > 1u) {
+      t >>= 1;
+      nb++;
+    }
+    for (uint32_t i = 0; i < nb; i++) bypass(1);
+    bypass(0);
+    if (nb) bits((v + 1u) & ((1u << nb) - 1u), (int)nb);
+  }
+  __device__ __forceinline__ void seg(int32_t v) { eg(v < 0 ? (uint32_t)(-2 * v - 1) : (uint32_t)(2 * v)); }
+  __device__ __forceinline__ void fval(float f) {  // dct3d.c:266-271
+    const float r = __builtin_truncf(f);
+    if (r == f && __builtin_fabsf(f) < 16777216.0f) {
+      bypass(0);
+      seg((int32_t)r);
+    } else {
+      bypass(1);
+      bits(__float_as_uint(f), 32);
+    }
+  }
+};
+
+struct EncodeBlobArgs {
+  uint32_t n, dtype;
+  const int16_t *levels;
+  const r3d_chunk_header *hdr;
+  const float *qsteps;
+  uint8_t *out;
+  uint64_t slot;
+  uint32_t *lengths;
+};
+
+__global__ __launch_bounds__(BLOB_NT) void k_encode_blobs(EncodeBlobArgs a) {
+  __shared__ uint16_t s_scan[4096];
+  __shared__ uint16_t s_ctx[NCTX * BLOB_NT];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < 4096u; i += BLOB_NT) s_scan[i] = c_scan[i];
+  __syncthreads();
+  const uint32_t c = blockIdx.x * BLOB_NT + t;
+  if (c >= a.n) return;
+  const int16_t *lv = a.levels + (size_t)c * 4096u;
+  uint8_t *o = a.out + (uint64_t)c * a.slot;
+  if (a.slot >= 1u) o[0] = 0xD3u;
+  if (a.slot >= 2u) o[1] = (uint8_t)a.dtype;
+  REnc e;
+  e.out = o + 2;
+  e.cap = a.slot >= 2u ? a.slot - 2u : 0u;
+  e.n = 0;
+  e.low = 0;
+  e.range = 0xFFFFFFFFu;
+  e.cache = 0;
+  e.pending = 1;
+  e.ctx = s_ctx + t;
+  for (uint32_t i = 0; i < NCTX; i++) e.ctx[i * BLOB_NT] = c_prior[i];
+  const r3d_chunk_header h = a.hdr[c];
+  e.fval(h.vmin);
+  e.fval(h.vspan);
+  e.fval(a.qsteps[c]);
+  e.seg(h.dcq);
+  e.bit(CTX_FLAG, 0);  // no corrections
+  uint32_t eob = 0;
+  for (uint32_t p = 4096u; p-- > 0;)
+    if (lv[s_scan[p]] != 0) {
+      eob = p + 1u;
+      break;
+    }
+  uint32_t k = 0;  // dct3d.c:357-363
+  while ((1u << k) <= eob) k++;
+  for (uint32_t i = 0; i < k; i++) e.bit(CTX_EOB + i, 1);
+  if (k < 13u) e.bit(CTX_EOB + k, 0);
+  if (k > 1u) e.bits(eob & ((1u << (k - 1u)) - 1u), (int)k - 1);
+  uint32_t recent = 0;
+  for (uint32_t p = 0; p < eob; p++) {
+    const uint32_t idx = s_scan[p];
+    const int32_t v = lv[idx];
+    uint32_t dens = __builtin_popcount(recent & 0xFFFFu);
+    dens = dens < 3u ? dens : 3u;
+    if (p + 1u != eob) e.bit(CTX_SIG + band_of(idx) * 4u + dens, v != 0);
+    recent = (recent << 1) | (v != 0 ? 1u : 0u);
+    if (!v) continue;
+    uint32_t rest = (uint32_t)(v < 0 ? -v : v) - 1u, j = 0;  // dct3d.c:310-319
+    while (j < 11u && rest > 0u) {
+      e.bit(CTX_MAG + j, 1);
+      rest--;
+      j++;
+    }
+    if (rest == 0u) {
+      e.bit(CTX_MAG + j, 0);
+    } else {
+      e.bit(CTX_MAG + 11u, 1);
+      e.eg(rest);
+    }
+    e.bypass(v < 0 ? 1u : 0u);
+  }
+  for (int i = 0; i < 5; i++) e.shift();
+  a.lengths[c] = (uint32_t)(e.n + 2u);
+}
+
+int upload_blob_tables(r3d_hip_ctx *ctx) {
+  uint16_t scan[4096];
+  int start[47] = {0};
+  for (int i = 0; i < 4096; i++) start[(i >> 8) + ((i >> 4) & 15) + (i & 15) + 1]++;
+  for (int f = 1; f < 47; f++) start[f] += start[f - 1];
+  for (int i = 0; i < 4096; i++) scan[start[(i >> 8) + ((i >> 4) & 15) + (i & 15)]++] = (uint16_t)i;
+  float hfw[46];
+  for (int f = 0; f < 46; f++) hfw[f] = powf(1.0f + (float)f, 0.65f);  // the host libm, as the CPU decoder
+  R3D_HIP_CHECK(ctx, hipMemcpyToSymbol(HIP_SYMBOL(c_scan), scan, sizeof scan, 0, hipMemcpyHostToDevice));
+  R3D_HIP_CHECK(ctx, hipMemcpyToSymbol(HIP_SYMBOL(c_hfw), hfw, sizeof hfw, 0, hipMemcpyHostToDevice));
+  return R3D_OK;
+}
+
+}  // namespace r3d
+
+using namespace r3d;
+
+extern "C" int r3d_hip_parse_blobs(r3d_hip_ctx *ctx, const r3d_blob_batch *in, const r3d_blob_parse_out *out,
+                                   void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!in || !out) return r3d_fail(ctx, R3D_ERR_INVALID, "parse_blobs: NULL batch or output");
+  if (in->nchunks == 0) return R3D_OK;
+  if (!in->bytes || !in->offsets || !in->lengths)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "parse_blobs: bytes, offsets and lengths are required");
+  if (in->max_pairs == 0 || in->max_pairs > 4096u)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "parse_blobs: max_pairs %u outside 1..4096", in->max_pairs);
+  if ((uint64_t)in->nchunks * in->max_pairs > 0xFFFFFFFFull)
+    return r3d_fail(ctx, R3D_ERR_RANGE, "parse_blobs: nchunks * max_pairs exceeds 32-bit offsets");
+  if (!out->headers || !out->step_tables || !out->pair_offsets || !out->pair_counts || !out->pairs ||
+      !out->corr_offsets || !out->corr_counts || !out->corr_pos || !out->corr_delta)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "parse_blobs: every output array is required");
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  ParseArgs a{in->nchunks,       in->dtype,        in->max_pairs,     in->bytes,         in->offsets,
+              in->lengths,       out->headers,     out->step_tables,  out->pair_offsets, out->pair_counts,
+              out->pairs,        out->corr_offsets, out->corr_counts, out->corr_pos,     out->corr_delta,
+              out->status};
+  hipLaunchKernelGGL(k_parse_blobs, dim3((in->nchunks + BLOB_NT - 1) / BLOB_NT), dim3(BLOB_NT), 0,
+                     (hipStream_t)stream, a);
+  R3D_HIP_CHECK(ctx, hipGetLastError());
+  return R3D_OK;
+}
+
+extern "C" int r3d_hip_encode_blobs(r3d_hip_ctx *ctx, uint32_t nchunks, const int16_t *levels,
+                                    const r3d_chunk_header *headers, const float *qsteps, uint32_t dtype,
+                                    uint8_t *out, uint64_t slot_bytes, uint32_t *lengths, void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (nchunks == 0) return R3D_OK;
+  if (!levels || !headers || !qsteps || !out || !lengths)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "encode_blobs: NULL argument");
+  if (slot_bytes < 2u) return r3d_fail(ctx, R3D_ERR_INVALID, "encode_blobs: slot_bytes below the 2-byte preamble");
+  if (dtype == 0u || dtype > 255u) return r3d_fail(ctx, R3D_ERR_INVALID, "encode_blobs: dtype %u", dtype);
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  EncodeBlobArgs a{nchunks, dtype, levels, headers, qsteps, out, slot_bytes, lengths};
+  hipLaunchKernelGGL(k_encode_blobs, dim3((nchunks + BLOB_NT - 1) / BLOB_NT), dim3(BLOB_NT), 0,
+                     (hipStream_t)stream, a);
+  R3D_HIP_CHECK(ctx, hipGetLastError());
+  return R3D_OK;
+}

@@ -65,7 +65,28 @@ struct DecodeArgs {
   const uint64_t *out_off;
   uint32_t sy, sz;
   int32_t *status;
+  const uint32_t *pair_cnt;
+  const uint32_t *corr_cnt;
 };
+
+// [begin, end) of chunk c's pairs: offsets are caller data, so the range is
+// clamped to the next offset, to the count when given and to npairs.
+__device__ __forceinline__ void pair_range(const DecodeArgs &a, uint32_t c, uint64_t *b, uint64_t *e) {
+  uint64_t p0 = a.pair_off[c], p1 = a.pair_off[c + 1];
+  if (a.pair_cnt && p0 + a.pair_cnt[c] < p1) p1 = p0 + a.pair_cnt[c];
+  if (p1 > a.npairs) p1 = a.npairs;
+  *b = p0;
+  *e = p1;
+}
+
+// Chunk c's corrections, clamped the same way and to 4096.
+__device__ __forceinline__ void corr_range(const DecodeArgs &a, uint32_t c, uint64_t *b, uint64_t *n) {
+  const uint64_t c0 = a.corr_off[c];
+  uint64_t c1 = a.corr_off[c + 1] < a.ncorr ? a.corr_off[c + 1] : a.ncorr;
+  if (a.corr_cnt && c0 + a.corr_cnt[c] < c1) c1 = c0 + a.corr_cnt[c];
+  *b = c0;
+  *n = c1 > c0 ? (c1 - c0 > 4096u ? 4096u : c1 - c0) : 0;
+}
 
 // deq_one (dct3d.c:149-154), r = (|l| - 1) * step + 0.80f * step + 0.40f * step
 // with 0.80f * step and 0.40f * step taken from a table (the same rounded
@@ -245,9 +266,9 @@ __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
 #pragma unroll
       for (int k = 0; k < 16; k++) tile[k * 256 + t] = 0.0f;
       __syncthreads();
-      // offsets are caller data: clamp so a corrupt batch cannot read past pairs
-      const uint64_t p1 = a.pair_off[c + 1] < a.npairs ? a.pair_off[c + 1] : a.npairs;
-      for (uint64_t p = (uint64_t)a.pair_off[c] + t; p < p1; p += 256u) {
+      uint64_t p0, p1;
+      pair_range(a, c, &p0, &p1);
+      for (uint64_t p = p0 + t; p < p1; p += 256u) {
         const uint32_t pr = a.pairs[p];
         const uint32_t pos = pr & 4095u;
         const float l = (float)(int16_t)(pr >> 16);
@@ -262,9 +283,8 @@ __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
     idct3_tile(v, tile, t, dc);
     if (a.corr_off) {  // dct3d.c:572-587: serial, in stream order
       if (t == 0) {
-        const uint64_t c0 = a.corr_off[c];
-        const uint64_t c1 = a.corr_off[c + 1] < a.ncorr ? a.corr_off[c + 1] : a.ncorr;
-        const uint64_t n = c1 > c0 ? (c1 - c0 > 4096u ? 4096u : c1 - c0) : 0;
+        uint64_t c0, n;
+        corr_range(a, c, &c0, &n);
         for (uint64_t i = 0; i < n; i++) {
           const uint32_t pos = a.corr_pos[c0 + i];
           if (pos >= 4096u) break;
@@ -637,8 +657,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREC == PRE
 #pragma unroll
       for (int i = 0; i < 8; i++) ((uint4 *)t)[lane + 64u * i] = make_uint4(0, 0, 0, 0);
       wave_sync();
-      const uint64_t p1 = a.pair_off[c + 1] < a.npairs ? a.pair_off[c + 1] : a.npairs;
-      for (uint64_t p = (uint64_t)a.pair_off[c] + lane; p < p1; p += 64u) {
+      uint64_t p0, p1;
+      pair_range(a, c, &p0, &p1);
+      for (uint64_t p = p0 + lane; p < p1; p += 64u) {
         const uint32_t pr = a.pairs[p];
         t[pr & 4095u] = (int16_t)(pr >> 16);
       }
@@ -708,11 +729,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREC == PRE
     const float scale = inv / s;
     const float bias = __builtin_fmaf(dc, inv, h.vmin);
     uint64_t c0 = 0, nc = 0;
-    if (a.corr_off) {
-      c0 = a.corr_off[c];
-      const uint64_t c1 = a.corr_off[c + 1] < a.ncorr ? a.corr_off[c + 1] : a.ncorr;
-      nc = c1 > c0 ? (c1 - c0 > 4096u ? 4096u : c1 - c0) : 0;
-    }
+    if (a.corr_off) corr_range(a, c, &c0, &nc);
     if (MFMA3) {
       // pass 3: D3 = C^T (A: row x', k = u) * D2 (B: k = u, column y') per z',
       // B read back from LDS; lane (r, g) slot e gets voxel (z', y' = r, x' = 4g + e)
@@ -864,6 +881,8 @@ extern "C" int r3d_hip_decode_chunks(r3d_hip_ctx *ctx, const r3d_chunk_batch *b,
   a.sy = sy;
   a.sz = sz;
   a.status = b->status;
+  a.pair_cnt = b->pair_counts;
+  a.corr_cnt = b->corr_counts;
   hipStream_t st = (hipStream_t)stream;
   if (variant == R3D_DECODE_EXACT_F32) {
     const uint32_t grid = grid_for(ctx, b->nchunks, 1u << 16);

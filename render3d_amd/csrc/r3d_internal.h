@@ -88,6 +88,9 @@ int r3d_fail(r3d_hip_ctx *ctx, int code, const char *fmt, ...) __attribute__((fo
 namespace r3d {
 // Copies the FP32 DCT basis (dct3d.c:38-49) into the decoder's constant bank.
 int upload_basis(r3d_hip_ctx *ctx);
+// Uploads the 3ddct scan order and the powf(1 + f, 0.65f) weights (computed by
+// the host's libm) for the blob parser and encoder.
+int upload_blob_tables(r3d_hip_ctx *ctx);
 // Copies n bytes into the next staging slot (pinned host + device buffer);
 // the caller records the slot's event after the launches that read it.
 int stage(r3d_hip_ctx *ctx, const void *src, size_t n, hipStream_t s, ViewSlot **slot_out);

@@ -1,0 +1,142 @@
+"""3ddct blobs on the GPU through the C ABI: the reference's own blobs parsed
+and decoded end to end (bit-exact with dct3d_decode_u8), corrupted blobs
+treated as the reference parser treats them, and the workload encoder
+reproducing the reference encoder's bytes."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from render3d_amd import workloads
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(__file__)
+GOLD = np.load(os.path.join(HERE, "golden", "dct3d_golden.npz"))
+CORRUPT = np.load(os.path.join(HERE, "golden", "dct3d_corrupt.npz"))
+
+
+def cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def upload(byts, offsets):
+    n = offsets.size - 1
+    lengths = np.diff(offsets).astype(np.int32)
+    return cuda(byts), cuda(offsets[:n].astype(np.int64)), cuda(lengths)
+
+
+def levels_of(parsed, cap):
+    pairs = parsed["pairs"].cpu().numpy().view(np.uint32)
+    cnt = parsed["pair_counts"].cpu().numpy()
+    out = np.zeros((cnt.size, 4096), np.int16)
+    for c in range(cnt.size):
+        p = pairs[c * cap:c * cap + cnt[c]]
+        out[c, p & 4095] = (p >> 16).astype(np.uint16).view(np.int16)
+    return out
+
+
+def test_golden_blobs_parse_and_decode_bit_exact(ctx):
+    n = len(GOLD["blob_names"])
+    data, offs, lens = upload(GOLD["blob_bytes"], GOLD["blob_offsets"])
+    parsed = ctx.parse_blobs(data, offs, lens)
+    torch.cuda.synchronize()
+    assert np.all(parsed["status"].cpu().numpy() == 1)
+    hdr = parsed["headers"].cpu().numpy()
+    assert np.array_equal(hdr[:, 0].view(np.float32), GOLD["blob_hdr"][:, 0])
+    assert np.array_equal(hdr[:, 1].view(np.float32), GOLD["blob_hdr"][:, 1])
+    assert np.array_equal(hdr[:, 2], GOLD["blob_dcq"]) and np.array_equal(hdr[:, 3], np.arange(n))
+    steps = parsed["step_tables"].cpu().numpy().reshape(n, 46)
+    assert np.array_equal(steps.view(np.uint32), GOLD["blob_steps"].view(np.uint32))  # qstep * powf(1+f, 0.65)
+    assert np.array_equal(levels_of(parsed, 4096), GOLD["blob_levels"])
+    cc = parsed["corr_counts"].cpu().numpy()
+    cp = parsed["corr_pos"].cpu().numpy().view(np.uint16)
+    cd = parsed["corr_delta"].cpu().numpy()
+    for i in range(n):
+        c0, c1 = GOLD["blob_corr_off"][i], GOLD["blob_corr_off"][i + 1]
+        assert cc[i] == c1 - c0
+        assert np.array_equal(cp[i * 4096:i * 4096 + cc[i]], GOLD["blob_corr_pos"][c0:c1])
+        assert np.array_equal(cd[i * 4096:i * 4096 + cc[i]].view(np.uint32),
+                              GOLD["blob_corr_delta"][c0:c1].view(np.uint32))
+    u8, st = ctx.decode_blobs(data, offs, lens)
+    torch.cuda.synchronize()
+    bad = [GOLD["blob_names"][i] for i in range(n) if not np.array_equal(u8[i].cpu().numpy(), GOLD["blob_u8"][i])]
+    assert not bad, f"differing from dct3d_decode_u8: {bad}"
+
+
+def test_corrupt_blobs_follow_the_reference_parser(ctx):
+    n = CORRUPT["ok"].size
+    data, offs, lens = upload(CORRUPT["bytes"], CORRUPT["offsets"])
+    parsed = ctx.parse_blobs(data, offs, lens)
+    u8, _ = ctx.decode_blobs(data, offs, lens)
+    torch.cuda.synchronize()
+    st = parsed["status"].cpu().numpy()
+    lv = levels_of(parsed, 4096)
+    u8 = u8.cpu().numpy()
+    for i in range(n):
+        if not CORRUPT["ok"][i]:
+            assert st[i] == 0, i
+            assert not u8[i].any(), i  # rejected: zero-filled
+            continue
+        assert st[i] == 1, i
+        assert np.array_equal(lv[i].astype(np.float32), CORRUPT["levels"][i]), i
+        hdr = parsed["headers"][i].cpu().numpy()
+        assert hdr[0:2].view(np.float32).tolist() == CORRUPT["hdr"][i, 0:2].tolist() and hdr[2] == CORRUPT["dcq"][i]
+
+
+def test_encoder_reproduces_reference_blobs(ctx):
+    idx = [i for i in range(len(GOLD["blob_names"])) if GOLD["blob_corr_off"][i + 1] == GOLD["blob_corr_off"][i]]
+    hdr = np.zeros((len(idx), 4), np.int32)
+    hdr[:, 0] = GOLD["blob_hdr"][idx, 0].view(np.int32)
+    hdr[:, 1] = GOLD["blob_hdr"][idx, 1].view(np.int32)
+    hdr[:, 2] = GOLD["blob_dcq"][idx]
+    data, offs, lens = ctx.encode_blobs(cuda(GOLD["blob_levels"][idx]), cuda(hdr), cuda(GOLD["blob_hdr"][idx, 2]))
+    torch.cuda.synchronize()
+    data, offs, lens = data.cpu().numpy(), offs.cpu().numpy(), lens.cpu().numpy()
+    for j, i in enumerate(idx):
+        want = GOLD["blob_bytes"][GOLD["blob_offsets"][i]:GOLD["blob_offsets"][i + 1]]
+        assert lens[j] == want.size, GOLD["blob_names"][i]
+        assert np.array_equal(data[offs[j]:offs[j] + lens[j]], want), GOLD["blob_names"][i]
+
+
+def test_volume_round_trip_and_oracle_encoder_agreement(ctx):
+    n = 64
+    vol = ctx.synth_sheets(n, n, n, 3, 77)
+    qstep = np.float32(2.0)
+    step = cuda(orc.step_qstep(qstep))  # qstep * powf(1 + f, 0.65f), the table the blob's qstep implies
+    levels, headers = ctx.encode_chunks(vol, n, n, n, step)
+    data, offs, lens = ctx.encode_blobs(levels, headers, float(qstep))
+    u8, st = ctx.decode_blobs(data, offs, lens)
+    direct = ctx.decode_chunks(headers=headers, step_tables=step, levels=levels)
+    torch.cuda.synchronize()
+    assert np.all(st.cpu().numpy() == 1)
+    assert torch.equal(u8, direct)
+    lv, hd = levels.cpu().numpy(), headers.cpu().numpy()
+    d, o, ln = data.cpu().numpy(), offs.cpu().numpy(), lens.cpu().numpy()
+    for c in range(0, lv.shape[0], 5):
+        want = orc.encode_blob(hd[c, 0:1].view(np.float32)[0], hd[c, 1:2].view(np.float32)[0], qstep, int(hd[c, 2]),
+                               lv[c])
+        assert np.array_equal(d[o[c]:o[c] + ln[c]], want), c
+
+
+def test_capacity_dtype_and_empty_blobs(ctx):
+    offs_np, pairs, hdr = workloads.sparse_laplacian_batch(8, 3)
+    lv = workloads.densify(offs_np, pairs)
+    counts = np.diff(offs_np.astype(np.int64))
+    data, offs, lens = ctx.encode_blobs(cuda(lv), cuda(hdr), 1.0)
+    cap = int(np.sort(counts)[3])  # the larger half overflows
+    parsed = ctx.parse_blobs(data, offs, lens, max_pairs=cap)
+    u8, _ = ctx.decode_blobs(data, offs, lens, max_pairs=cap)
+    torch.cuda.synchronize()
+    st = parsed["status"].cpu().numpy()
+    assert np.array_equal(st == -2, counts > cap) and np.all(st[counts <= cap] == 1)
+    assert not u8[torch.from_numpy(counts > cap).cuda()].any()
+    parsed = ctx.parse_blobs(data, offs, lens, dtype=2)  # blobs carry dtype 1
+    torch.cuda.synchronize()
+    assert np.all(parsed["status"].cpu().numpy() == 0)
+    short = torch.zeros(8, dtype=torch.int32, device="cuda") + 1  # one byte: shorter than the preamble
+    parsed = ctx.parse_blobs(data, offs, short)
+    torch.cuda.synchronize()
+    assert np.all(parsed["status"].cpu().numpy() == 0)
