@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+VALU_PEAK_G = 1228.8  # wave64 VALU instructions/s: the FP32 peak 157.3 TFLOP/s / (2 x 64)
 CHUNK_BYTES = 8192 + 4096
 
 
@@ -47,11 +48,14 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (graph) workload")
     ap.add_argument("--batched-chunks", type=int, default=32768)
     ap.add_argument("--no-cfg0", action="store_true", help="skip the 256^3 / 512^2 workload")
+    ap.add_argument("--no-blobs", action="store_true", help="skip the 3ddct blob (entropy stage) workload")
     ap.add_argument("--no-fit", action="store_true", help="skip the differentiable 256^3 fit (200 Adam steps)")
     ap.add_argument("--profile-decode", action="store_true",
                     help="for rocprofv3 --pmc: one 1 GiB device copy (calibration) then one decode launch")
     ap.add_argument("--profile-variant", type=int, default=0,
                     help="decode variant the --profile-decode launch uses (0 exact, 1 MFMA FP32, 2 MFMA FP16, 3 split FP16)")
+    ap.add_argument("--profile-blobs", action="store_true",
+                    help="build the blob workload, run one 1 GiB copy and one blob parse, then exit (for rocprofv3)")
     ap.add_argument("--profile-raycast", action="store_true",
                     help="for rocprofv3 --pmc: build the volume, one 1 GiB device copy, then one raycast of the frame")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "decode_exact_1024.md"),
@@ -59,12 +63,13 @@ def parse():
     return ap.parse_args()
 
 
-def read_traffic(path):
-    """`traffic_bytes_per_launch: N` from a committed rocprof summary, or None."""
+def read_traffic(path, key="traffic_bytes_per_launch"):
+    """`<key>: N` (HBM bytes or instructions per launch) from a committed
+    rocprof summary, or None."""
     try:
         with open(path) as f:
             for line in f:
-                if line.strip().startswith("traffic_bytes_per_launch:"):
+                if line.strip().startswith(key + ":"):
                     return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -120,6 +125,19 @@ def main():
         print(json.dumps({"profile_decode": True, "variant": args.profile_variant, "chunks": nch,
                           "copy_bytes": 1 << 30}))
         return 0
+    if args.profile_blobs:
+        del levels, headers, batch, flat
+        _, lv, hd, data, offs, lens = make_blobs(ctx, n, nsheets, seed, dev)
+        del lv, hd
+        a = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        parsed = ctx.parse_blobs(data, offs, lens)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_blobs": True, "blobs": int(offs.shape[0]), "blob_bytes": int(lens.long().sum()),
+                          "nnz": int(parsed["pair_counts"].long().sum()), "copy_bytes": 1 << 30}))
+        return 0
     if args.profile_raycast:
         ctx.decode(batch)
         vol.build_from_chunks(flat)
@@ -169,6 +187,7 @@ def main():
     adj = None if args.no_adjoint else adjoint_ratio(ctx, vol, lut, args, W, H, dev)
     bat = None if args.no_batched else batched(ctx, vol, lut, args, dev, stream)
     c0 = None if args.no_cfg0 else cfg0(ctx, args, dev, stream)
+    blob = None if args.no_blobs else blob_stage(ctx, args, dev, stream, n, nsheets, seed)
     fit = None if args.no_fit else fit_256(ctx, dev)
 
     cpu = None
@@ -177,8 +196,12 @@ def main():
         cpu, parity = cpu_baseline(ctx, vol, view, lut, img, args, levels, headers, step_tab, n, W, H)
         if c0 is not None:
             c0["cpu_baseline"] = cpu_cfg0(c0.pop("_state"), args)
+        if blob is not None:
+            blob["cpu_baseline"], blob["parity_vs_oracle"] = cpu_blobs(blob["_state"], args)
     if c0 is not None:
         c0.pop("_state", None)
+    if blob is not None:
+        blob.pop("_state", None)
 
     out = {
         "metric": "ray samples/s + fps @1024³ 1080p; chunk decode GB/s; adjoint/fwd time ratio",
@@ -207,6 +230,7 @@ def main():
         "batched": bat,
         "cfg0": c0,
         "fit": fit,
+        "blobs": blob,
         "samples_per_frame": samples,
         "skips_per_frame": skips,
         "shaded_per_frame": shaded,
@@ -375,6 +399,138 @@ def batched(ctx, vol, lut, args, dev, stream):
             "samples_per_s": samples / (ray_ms / 1e3), "graph_decode_matches_eager": ok,
             "tuning": {"chosen": chosen, "candidates": tuning, "tune_s": tune_s},
             "decode_variants": variants, "generate_s": gen_s}
+
+
+BLOB_QSTEP = 8.0  # lodpack's level-0 quality (tools/lodpack/main.c:392)
+
+
+def make_blobs(ctx, n, nsheets, seed, dev):
+    """The n^3 sheet volume quantised with the qstep table and entropy-coded on
+    the GPU into one 3ddct blob per chunk, packed back to back as in a shard.
+    Returns (step table, levels, headers, bytes, offsets int64, lengths int32)."""
+    import torch
+
+    from render3d_amd import workloads
+
+    step = torch.from_numpy(workloads.step_table_qstep(BLOB_QSTEP)).to(dev)
+    vol = ctx.synth_sheets(n, n, n, nsheets, seed)
+    levels, headers = ctx.encode_chunks(vol, n, n, n, step)
+    del vol
+    slots, slot_offs, lens = ctx.encode_blobs(levels, headers, BLOB_QSTEP)
+    l64 = lens.long()
+    offs = torch.cumsum(l64, 0) - l64
+    total = int(l64.sum())
+    within = torch.arange(total, device=dev) - torch.repeat_interleave(offs, l64)
+    data = slots[torch.repeat_interleave(slot_offs, l64) + within].contiguous()
+    return step, levels, headers, data, offs, lens
+
+
+def blob_stage(ctx, args, dev, stream, n, nsheets, seed):
+    """The 3ddct blob path at n^3: the sheet volume quantised with the qstep
+    table (qstep 8, lodpack's level-0 quality, tools/lodpack/main.c:392) and
+    entropy-coded on the GPU into one blob per chunk, packed back to back as in
+    a shard. Timed: the parse (blobs -> sparse levels and per-chunk tables),
+    the exact decode of the parsed batch, and both captured as one HIP graph.
+    The graph's output must equal the decode of the quantised levels."""
+    import numpy as np
+    import torch
+
+    qstep = BLOB_QSTEP
+    step, levels, headers, data, offs, lens = make_blobs(ctx, n, nsheets, seed, dev)
+    nch = int(levels.shape[0])
+    reps = max(3, args.steps)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+        ev[0].record(stream)
+        for i in range(reps):
+            fn()
+            ev[i + 1].record(stream)
+        torch.cuda.synchronize()
+        return float(np.mean([ev[i].elapsed_time(ev[i + 1]) for i in range(reps)]))
+
+    enc_ms = timed(lambda: ctx.encode_blobs(levels, headers, qstep))
+    total = int(lens.long().sum())
+    direct = ctx.decode_chunks(headers=headers, step_tables=step, levels=levels)
+    del levels
+    parse_ms = timed(lambda: ctx.parse_blobs(data, offs, lens))
+    parsed = ctx.parse_blobs(data, offs, lens)
+    out = torch.empty((nch, 4096), dtype=torch.uint8, device=dev)
+    batch = ctx.parsed_batch(parsed, out)
+    decode_ms = timed(lambda: ctx.decode(batch))
+    nnz = int(parsed["pair_counts"].long().sum())
+    status_ok = bool((parsed["status"] == 1).all())
+    del parsed, batch
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(stream)
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g, stream=side):
+            gp = ctx.parse_blobs(data, offs, lens)
+            ctx.decode(ctx.parsed_batch(gp, out))
+    stream.wait_stream(side)
+    torch.cuda.synchronize()
+    graph_ms = timed(g.replay)
+    same = bool(torch.equal(out, direct))
+    del direct
+    # parse kernel: bound by VALU issue (a serial chain of binary decisions per
+    # lane); its instruction count per launch is a property of the kernel and
+    # this workload, read from the committed counter summary. Algorithmic bytes
+    # = blob bytes read + what it writes per chunk (16 B header, 46 x 4 B step
+    # table, 2 x 4 B offsets, 2 x 4 B counts, 4 B status) + 4 B per nonzero level
+    prof = os.path.join(ROOT, "profiles", "blob_parse_1024.md")
+    valu = read_traffic(prof, "valu_insts_per_launch") if n == 1024 else None
+    parse_bytes = total + nch * (16 + 184 + 16 + 4) + 4 * nnz
+    parse_gbps = parse_bytes / (parse_ms / 1e3) / 1e9
+    valu_rate = None if valu is None else valu / (parse_ms / 1e3) / 1e9
+    res = {"chunks": nch, "qstep": qstep, "blob_bytes": total, "bytes_per_blob": total / nch, "nnz": nnz,
+           "ratio": nch * 4096 / total, "all_parsed": status_ok, "graph_matches_level_decode": same,
+           "encode_ms": enc_ms, "parse_ms": parse_ms, "decode_ms": decode_ms, "graph_ms": graph_ms,
+           "blobs_per_s": nch / (graph_ms / 1e3), "compressed_GBps": total / (graph_ms / 1e3) / 1e9,
+           "voxel_GBps": nch * 4096 / (graph_ms / 1e3) / 1e9,
+           "parse_roofline": {"kernel": "k_parse_blobs", "bound": "valu", "achieved": valu_rate,
+                              "peak": VALU_PEAK_G, "unit": "G wave64 instr/s",
+                              "frac": None if valu_rate is None else valu_rate / VALU_PEAK_G,
+                              "algorithmic_GBps": parse_gbps,
+                              "traffic": read_traffic(prof) if n == 1024 else None}}
+    k = min(4096, nch)
+    idx = torch.linspace(0, nch - 1, k, device=dev).long()
+    res["_state"] = {"data": data, "offs": offs, "lens": lens, "idx": idx, "out": out[idx].cpu().numpy()}
+    return res
+
+
+def cpu_blobs(state, args):
+    """The oracle's blob decode (parse, qstep table, decode: dct3d_decode_u8
+    per chunk) on a 4096-blob sample of the same blobs, on the host's cores,
+    repeated for at least half of `cpu_seconds`; and its output against the
+    GPU's for that sample."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    idx = state["idx"].cpu().numpy()
+    offs = state["offs"].cpu().numpy()[idx]
+    lens = state["lens"].cpu().numpy()[idx]
+    data = state["data"].cpu().numpy()
+    sub = np.concatenate([data[o:o + ln] for o, ln in zip(offs, lens)])
+    sub_off = np.concatenate([[0], np.cumsum(lens.astype(np.int64))[:-1]])
+    cores = min(16, os.cpu_count() or 1)
+    t = time.perf_counter()
+    reps = 0
+    while True:
+        u8, st = orc.decode_blobs(sub, sub_off, lens, nthreads=cores)
+        reps += 1
+        if time.perf_counter() - t >= args.cpu_seconds / 2:
+            break
+    dt = time.perf_counter() - t
+    cpu = {"value": reps * idx.size / dt, "unit": "blobs/s", "cores": cores, "kind": "port",
+           "voxel_GBps": reps * idx.size * CHUNK_BYTES / dt / 1e9,
+           "sample": f"{reps} x {idx.size} blobs ({int(lens.sum())} bytes), {dt:.1f} s"}
+    parity = {"blobs_checked": int(idx.size), "u8_equal": bool(np.array_equal(u8, state["out"])),
+              "status_ok": bool(np.all(st == 1))}
+    return cpu, parity
 
 
 def cfg0(ctx, args, dev, stream):

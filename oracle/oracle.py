@@ -76,6 +76,7 @@ def load():
                                   np.ctypeslib.ndpointer(np.uint16, flags="C"), f32p]
     L.r3do_encode_blob.restype = C.c_size_t
     L.r3do_encode_blob.argtypes = [C.POINTER(BlobHdr), i16p, C.c_uint32, u8p, C.c_size_t]
+    L.r3do_decode_blobs.argtypes = [C.c_uint32, u8p, VP, VP, C.c_uint32, u8p, VP, C.c_int]
     _lib = L
     return L
 
@@ -255,6 +256,19 @@ def encode_blob(vmin, vspan, qstep, dcq, levels, dtype=1, cap=4096 * 8 + 256):
     n = load().r3do_encode_blob(C.byref(h), np.ascontiguousarray(levels, np.int16), dtype, out, cap)
     assert n <= cap
     return out[:n].copy()
+
+
+def decode_blobs(data, offsets, lengths, dtype=1, nthreads=8):
+    """r3do_decode_blobs: blobs (one byte array, per-blob offsets and lengths)
+    -> (u8 [n, 4096], status [n])."""
+    data = np.ascontiguousarray(data, np.uint8)
+    offs = np.ascontiguousarray(offsets, np.uint64)
+    lens = np.ascontiguousarray(lengths, np.uint32)
+    n = offs.size
+    out = np.zeros(n * 4096, np.uint8)
+    status = np.zeros(n, np.int32)
+    load().r3do_decode_blobs(n, data, offs.ctypes.data, lens.ctypes.data, dtype, out, status.ctypes.data, nthreads)
+    return out.reshape(n, 4096), status
 
 
 def raycast_diff(ovol, tf, grid, gdims, views, width, height, alpha_cutoff=0.98, dimg=None, window=None):

@@ -367,3 +367,31 @@ size_t r3do_encode_blob(const r3do_blob_hdr *h, const int16_t lvl[4096], uint32_
   for (int i = 0; i < 5; i++) renc_shift(&e);
   return e.n + 2;
 }
+
+/* Blob batch decode, the CPU shape of r3d_shard_decode_region
+ * (src/shard/shardio.c:269-306 calling dct3d_decode_u8 per chunk): blob i is
+ * bytes[offsets[i] .. + lengths[i]], parsed, dequantised with its own
+ * qstep table and decoded into out + i*4096. status[i] as r3do_parse_blob;
+ * a chunk with status != 1 is zero-filled. Runs on nthreads threads. */
+void r3do_decode_blobs(uint32_t n, const uint8_t *bytes, const uint64_t *offsets, const uint32_t *lengths,
+                       uint32_t dtype, uint8_t *out, int32_t *status, int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+  tables_init();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+  for (uint32_t i = 0; i < n; i++) {
+    r3do_blob_hdr h;
+    int16_t lvl[N3];
+    uint16_t cp[N3];
+    float cd[N3], step[46];
+    uint32_t nc = 0;
+    uint8_t *o = out + (size_t)i * N3;
+    int st = r3do_parse_blob(bytes + offsets[i], lengths[i], dtype, &h, lvl, &nc, cp, cd);
+    if (st == 1) {
+      r3do_step_table_qstep(h.qstep, step);
+      const r3do_chunk_hdr ch = {h.vmin, h.vspan, h.dcq, 0};
+      if (!r3do_decode_chunk(lvl, &ch, step, cp, cd, nc, o, NULL)) st = 0;
+    }
+    if (st != 1) memset(o, 0, N3);
+    status[i] = st;
+  }
+}

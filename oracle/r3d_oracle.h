@@ -175,6 +175,10 @@ typedef struct r3do_blob_hdr {
 int r3do_parse_blob(const uint8_t *blob, size_t len, uint32_t want_dtype, r3do_blob_hdr *h, int16_t lvl[4096],
                     uint32_t *ncorr, uint16_t *corr_pos, float *corr_delta);
 size_t r3do_encode_blob(const r3do_blob_hdr *h, const int16_t lvl[4096], uint32_t dtype, uint8_t *out, size_t cap);
+/* Blob batch decode (parse, qstep table, decode) as r3d_shard_decode_region
+ * does per chunk; status[i] as r3do_parse_blob, zero-filled unless 1. */
+void r3do_decode_blobs(uint32_t n, const uint8_t *bytes, const uint64_t *offsets, const uint32_t *lengths,
+                       uint32_t dtype, uint8_t *out, int32_t *status, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,13 @@
 // Exp-Golomb escape, bypass signs) and the corrections. Every step is integer
 // arithmetic, so the parse is exact by construction.
 //
+// The parse is bound by VALU issue, not by HBM: a blob is a serial chain of
+// binary decisions (about 6,000 for a 16^3 chunk at qstep 8), so the level
+// loop is a per-lane state machine making one decision per iteration (the
+// lanes of a wave stay converged), renormalisation is branch-free over a
+// register byte window, and the window's loads are issued 8 iterations
+// before their bytes are needed.
+//
 // The adaptive contexts (59 per lane, 16-bit) live in LDS, one row per
 // context, so lanes touching the same context hit consecutive addresses; the
 // ascending-L1 scan table is staged in LDS once per workgroup.
@@ -21,7 +28,10 @@ constexpr uint32_t BLOB_NT = 256;  // lanes (blobs) per workgroup
 constexpr uint32_t RC_TOP = 1u << 24;
 enum : uint32_t { CTX_SIG = 0, CTX_MAG = 32, CTX_EOB = 44, CTX_FLAG = 58, NCTX = 59 };
 
-__constant__ uint16_t c_scan[4096];  // ascending L1 frequency, ties by index (dct3d.c:336-351)
+// Ascending L1 frequency, ties by index (dct3d.c:336-351); each entry packs
+// the coefficient index (bits 0-11) and its significance band (bits 12-14,
+// dct3d.c:352-355).
+__constant__ uint16_t c_scan[4096];
 __constant__ float c_hfw[46];        // powf(1 + f, 0.65f), computed by the host's libm (dct3d.c:131-137)
 // Trained priors P(bit == 0) in 1/4096 (dct3d.c:290-299), in context order.
 __constant__ uint16_t c_prior[NCTX] = {
@@ -30,32 +40,96 @@ __constant__ uint16_t c_prior[NCTX] = {
     1893, 1217, 908,  728,  610,  527,  465,  416,  377,  346,  320,  297,  74,   1,    1,    1,
     1,    5,    109,  504,  772,  1063, 1458, 2326, 4095, 2048, 2048};
 
-__device__ __forceinline__ uint32_t band_of(uint32_t idx) {
-  const uint32_t b = ((idx >> 8) + ((idx >> 4) & 15u) + (idx & 15u)) * 8u / 48u;
-  return b > 7u ? 7u : b;
-}
 
 // ---- decode side -----------------------------------------------------------
+// The stream is read through a per-lane window of up to 8 bytes held in a
+// register pair (next byte in the top bits), refilled one aligned dword at a
+// time. refill() issues the load for the next dword and a later merge()
+// appends it, so a caller that calls them a few iterations apart hides the
+// load latency; a lane that runs short in between fills synchronously.
+// Aligned dword loads never cross a page the blob does not touch, and bytes
+// past the end of the blob read as zero (dct3d.c:218-223).
 struct RDec {
-  const uint8_t *p;
-  uint32_t n, at;
+  const uint8_t *p;  // stream start
+  uint32_t n;        // stream length
+  uint32_t at;       // next byte to consume
+  uint64_t w;        // bytes [at, at + wn), stream order from bit 63 down
+  uint32_t wn;
+  uint32_t stg, stg_skip, stg_cnt;  // a staged dword, pending while stg_cnt > 0
   uint32_t code, range;
   uint16_t *ctx;  // this lane's contexts: ctx[i * BLOB_NT]
 
-  __device__ __forceinline__ uint32_t next() { return at < n ? p[at++] : 0u; }
+  // The aligned dword holding stream byte q (q < n): skip leading bytes, cnt
+  // bytes from q on that lie inside the stream.
+  __device__ __forceinline__ uint32_t load_at(uint32_t q, uint32_t &skip, uint32_t &cnt) const {
+    const uintptr_t a = (uintptr_t)(p + q);
+    skip = (uint32_t)(a & 3u);
+    const uint32_t left = n - q;
+    cnt = 4u - skip < left ? 4u - skip : left;
+    return *(const uint32_t *)(a & ~(uintptr_t)3u);
+  }
+  __device__ __forceinline__ void append(uint32_t d, uint32_t skip, uint32_t cnt) {  // needs wn <= 4
+    uint32_t be = __builtin_bswap32(d) << (8u * skip);
+    be &= ~(0xFFFFFFFFu >> (8u * cnt - 1u) >> 1);  // keep the first cnt bytes (cnt in 1..4)
+    w |= (uint64_t)be << (32u - 8u * wn);
+    wn += cnt;
+  }
+  __device__ __forceinline__ void merge() {
+    if (stg_cnt) {
+      append(stg, stg_skip, stg_cnt);
+      stg_cnt = 0;
+    }
+  }
+  // Issue the load of the dword after the window (when it has room for it).
+  __device__ __forceinline__ void refill() {
+    if (stg_cnt == 0u && wn <= 4u) {
+      const uint32_t q = at + wn;
+      if (q >= n)
+        wn = 8;  // the rest of the window is the stream's zero tail
+      else
+        stg = load_at(q, stg_skip, stg_cnt);
+    }
+  }
+  __device__ __forceinline__ void fill_sync(uint32_t need) {
+    merge();
+    while (wn < need) {
+      const uint32_t q = at + wn;
+      if (q >= n) {
+        wn = 8;
+        break;
+      }
+      uint32_t sk, cn;
+      const uint32_t d = load_at(q, sk, cn);
+      append(d, sk, cn);
+    }
+  }
+  // Shift k <= 2 bytes from the window into code.
+  __device__ __forceinline__ void take(uint32_t k) {
+    if (wn < k) fill_sync(k);
+    code = (uint32_t)((((uint64_t)code << 32) | (w >> 32)) >> (32u - 8u * k));
+    w <<= 8u * k;
+    wn -= k;
+    at += k;
+  }
   __device__ __forceinline__ void init(const uint8_t *b, uint32_t len) {
     p = b;
     n = len;
     at = 0;
+    w = 0;
+    wn = 0;
+    stg_cnt = 0;
     code = 0;
+    take(2);
+    take(2);
+    take(1);  // the first byte is shifted out of code, as in dct3d.c:227
     range = 0xFFFFFFFFu;
-    for (int i = 0; i < 5; i++) code = (code << 8) | next();
   }
+  // Renormalise: range >= 2^8 on entry (every caller guarantees it), so at
+  // most two bytes.
   __device__ __forceinline__ void norm() {
-    while (range < RC_TOP) {
-      code = (code << 8) | next();
-      range <<= 8;
-    }
+    const uint32_t k = (range < RC_TOP ? 1u : 0u) + (range < (1u << 16) ? 1u : 0u);
+    take(k);
+    range <<= 8u * k;
   }
   __device__ __forceinline__ uint32_t bit(uint32_t ci) {
     uint16_t *pc = ctx + ci * BLOB_NT;
@@ -74,6 +148,25 @@ struct RDec {
     }
     norm();
     return b;
+  }
+  // One decision of either kind with the same instruction stream, free of
+  // branches: an adaptive context bit (ci) or, when byp, an equiprobable
+  // bypass bit (ci is then any context, written back unchanged).
+  __device__ __forceinline__ uint32_t decide(uint32_t ci, bool byp) {
+    uint16_t *pc = ctx + ci * BLOB_NT;
+    const uint32_t p0 = *pc;
+    // every arm computed first, so each choice is a select, not a branch
+    const uint32_t r_ctx = (range >> 12) * p0, r_byp = range >> 1;
+    const uint32_t r0 = __builtin_unpredictable(byp) ? r_byp : r_ctx;
+    const bool b = code >= r0;
+    const uint32_t code1 = code - r0, range1 = range - r0;
+    const uint32_t p_one = p0 - (p0 >> 4), p_zero = p0 + ((4096u - p0) >> 4);
+    code = __builtin_unpredictable(b) ? code1 : code;
+    range = __builtin_unpredictable(b && !byp) ? range1 : r0;
+    const uint32_t up = __builtin_unpredictable(b) ? p_one : p_zero;
+    *pc = (uint16_t)(__builtin_unpredictable(byp) ? p0 : up);
+    norm();
+    return b ? 1u : 0u;
   }
   __device__ __forceinline__ uint32_t bypass() {
     range >>= 1;
@@ -163,37 +256,69 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
     while (k < 13u && d.bit(CTX_EOB + k)) k++;
     uint32_t eob = k <= 1u ? k : ((1u << (k - 1u)) | d.bits((int)k - 1));
     if (eob > 4096u) eob = 4096u;
-    uint32_t recent = 0;
-    for (uint32_t p = 0; p < eob; p++) {
-      const uint32_t idx = s_scan[p];
+    // Levels (dct3d.c:390-405) as a per-lane state machine that makes exactly
+    // one binary decision per iteration: significance in context (band,
+    // min(3, popcount of the last 16 decisions)), the unary magnitude ladder
+    // (contexts 0-10, escape flag 11), the Exp-Golomb escape prefix (bypass),
+    // the sign (bypass). Lanes of a wave stay converged whatever each blob's
+    // mix of zero runs and ladders; the last position's significance is
+    // implied.
+    // The window's refills are issued every 8 iterations and merged 8
+    // iterations later.
+    enum : uint32_t { PH_SIG, PH_MAG, PH_EGP, PH_SIGN, PH_DONE };
+    uint32_t p = 0, recent = 0, m = 0, nb = 0;
+    uint32_t e = s_scan[0];
+    uint32_t ph = eob == 0u ? PH_DONE : eob == 1u ? PH_MAG : PH_SIG;
+    if (eob == 1u) recent = 1u;
+    d.refill();
+    for (uint32_t it = 1; ph != PH_DONE; it++) {
       uint32_t dens = __builtin_popcount(recent & 0xFFFFu);
       dens = dens < 3u ? dens : 3u;
-      const uint32_t sig = p + 1u == eob ? 1u : d.bit(CTX_SIG + band_of(idx) * 4u + dens);
-      recent = (recent << 1) | sig;
-      if (!sig) continue;
-      // magnitude (dct3d.c:322-329)
-      uint32_t m = 0;
-      while (m < 11u && d.bit(CTX_MAG + m)) m++;
-      if (m < 11u) {
-        m += 1u;
-      } else if (!d.bit(CTX_MAG + 11u)) {
-        m = 12u;
-      } else {
-        uint32_t nb = 0;
-        while (d.bypass())
-          if (++nb > 31u) break;
-        // nb can reach 32 on a corrupt stream; the shift count is taken mod
-        // 32, as the x86 and gfx9 shift instructions do for dct3d.c:326
-        m = 12u + (nb ? ((1u << (nb & 31u)) | d.bits((int)nb)) - 1u : 0u);
+      const bool sig = ph == PH_SIG, mag = ph == PH_MAG, egp = ph == PH_EGP, sgn = ph == PH_SIGN;
+      const uint32_t ci_sig = CTX_SIG + (e >> 12) * 4u + dens, ci_mag = CTX_MAG + m;
+      const uint32_t ci_oth = __builtin_unpredictable(mag) ? ci_mag : (uint32_t)CTX_FLAG;
+      const uint32_t ci = __builtin_unpredictable(sig) ? ci_sig : ci_oth;
+      const uint32_t b = d.decide(ci, egp || sgn);
+      if (egp) {
+        nb += b;
+        if (!b || nb > 31u) {
+          // nb can reach 32 on a corrupt stream; the shift count is taken mod
+          // 32, as the x86 and gfx9 shift instructions do for dct3d.c:326
+          m = 12u + (nb ? ((1u << (nb & 31u)) | d.bits((int)nb)) - 1u : 0u);
+          ph = PH_SIGN;
+        }
       }
-      const uint32_t neg = d.bypass();
-      if (m > (neg ? 32768u : 32767u)) st = -1;  // beyond the int16 level formats
-      if (np >= a.cap) {
-        st = st == 1 ? -2 : st;  // more nonzero levels than the capacity
-      } else {
-        const uint32_t lv = (uint32_t)(uint16_t)(int16_t)(neg ? -(int32_t)m : (int32_t)m);
-        a.pairs[base + np] = idx | (lv << 16);
-        np++;
+      if (sgn) {
+        if (m > (b ? 32768u : 32767u)) st = -1;  // beyond the int16 level formats
+        if (np >= a.cap) {
+          st = st == 1 ? -2 : st;  // more nonzero levels than the capacity
+        } else {
+          const uint32_t lv = (uint32_t)(uint16_t)(int16_t)(b ? -(int32_t)m : (int32_t)m);
+          a.pairs[base + np] = (e & 4095u) | (lv << 16);
+          np++;
+        }
+      }
+      // next state: SIG -> MAG (1) or next position (0); MAG -> SIGN with
+      // magnitude m + 1 (0), m + 1 (1, m < 11) or EGP (1, m == 11); SIGN ->
+      // next position. The last position's significance is implied.
+      const bool adv = (sig && !b) || sgn;
+      const uint32_t pn = p + (adv ? 1u : 0u);
+      const uint32_t en = s_scan[pn < 4095u ? pn : 4095u];
+      recent = __builtin_unpredictable(sig) ? (recent << 1) | b : recent;
+      const bool implied = adv && pn + 1u == eob;
+      recent = __builtin_unpredictable(implied) ? (recent << 1) | 1u : recent;
+      const uint32_t m1 = m + 1u;
+      m = __builtin_unpredictable(sig || implied) ? 0u : __builtin_unpredictable(mag) ? m1 : m;
+      nb = __builtin_unpredictable(mag) ? 0u : nb;
+      const uint32_t from_mag = __builtin_unpredictable(!b) ? PH_SIGN : m1 < 12u ? PH_MAG : PH_EGP;
+      const uint32_t at_adv = __builtin_unpredictable(pn >= eob) ? PH_DONE : implied ? PH_MAG : PH_SIG;
+      const uint32_t at_stay = __builtin_unpredictable(sig) ? PH_MAG : __builtin_unpredictable(mag) ? from_mag : ph;
+      ph = __builtin_unpredictable(adv) ? at_adv : at_stay;
+      p = pn;
+      e = en;
+      if ((it & 7u) == 0u) {
+        d.merge();
+        d.refill();
       }
     }
     if (has_corr) {  // dct3d.c:572-587
@@ -358,7 +483,7 @@ __global__ __launch_bounds__(BLOB_NT) void k_encode_blobs(EncodeBlobArgs a) {
   e.bit(CTX_FLAG, 0);  // no corrections
   uint32_t eob = 0;
   for (uint32_t p = 4096u; p-- > 0;)
-    if (lv[s_scan[p]] != 0) {
+    if (lv[s_scan[p] & 4095u] != 0) {
       eob = p + 1u;
       break;
     }
@@ -369,11 +494,11 @@ __global__ __launch_bounds__(BLOB_NT) void k_encode_blobs(EncodeBlobArgs a) {
   if (k > 1u) e.bits(eob & ((1u << (k - 1u)) - 1u), (int)k - 1);
   uint32_t recent = 0;
   for (uint32_t p = 0; p < eob; p++) {
-    const uint32_t idx = s_scan[p];
-    const int32_t v = lv[idx];
+    const uint32_t se = s_scan[p];
+    const int32_t v = lv[se & 4095u];
     uint32_t dens = __builtin_popcount(recent & 0xFFFFu);
     dens = dens < 3u ? dens : 3u;
-    if (p + 1u != eob) e.bit(CTX_SIG + band_of(idx) * 4u + dens, v != 0);
+    if (p + 1u != eob) e.bit(CTX_SIG + (se >> 12) * 4u + dens, v != 0);
     recent = (recent << 1) | (v != 0 ? 1u : 0u);
     if (!v) continue;
     uint32_t rest = (uint32_t)(v < 0 ? -v : v) - 1u, j = 0;  // dct3d.c:310-319
@@ -399,7 +524,11 @@ int upload_blob_tables(r3d_hip_ctx *ctx) {
   int start[47] = {0};
   for (int i = 0; i < 4096; i++) start[(i >> 8) + ((i >> 4) & 15) + (i & 15) + 1]++;
   for (int f = 1; f < 47; f++) start[f] += start[f - 1];
-  for (int i = 0; i < 4096; i++) scan[start[(i >> 8) + ((i >> 4) & 15) + (i & 15)]++] = (uint16_t)i;
+  for (int i = 0; i < 4096; i++) {
+    const int f = (i >> 8) + ((i >> 4) & 15) + (i & 15);
+    const int band = f * 8 / 48;  // dct3d.c:352-355
+    scan[start[f]++] = (uint16_t)(i | (band > 7 ? 7 : band) << 12);
+  }
   float hfw[46];
   for (int f = 0; f < 46; f++) hfw[f] = powf(1.0f + (float)f, 0.65f);  // the host libm, as the CPU decoder
   R3D_HIP_CHECK(ctx, hipMemcpyToSymbol(HIP_SYMBOL(c_scan), scan, sizeof scan, 0, hipMemcpyHostToDevice));

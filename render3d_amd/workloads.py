@@ -24,6 +24,19 @@ def step_table_linear():
     return (1.0 + 2.0 * np.arange(46)).astype(np.float32)
 
 
+def step_table_qstep(qstep):
+    """step[f] = qstep * powf(1 + f, 0.65f), f = 0..45: the table a 3ddct blob's
+    qstep implies (tools/3ddct/dct3d.c:131-143). powf is the host libm's, the
+    same values the blob parser's table holds."""
+    import ctypes
+    import ctypes.util
+    libm = ctypes.CDLL(ctypes.util.find_library("m"))
+    libm.powf.restype = ctypes.c_float
+    libm.powf.argtypes = [ctypes.c_float, ctypes.c_float]
+    q = np.float32(qstep)
+    return np.array([q * np.float32(libm.powf(1.0 + f, 0.65)) for f in range(46)], np.float32)
+
+
 def sparse_laplacian_batch(nchunks, seed, count_scale=100.0, value_scale=12.0, min_count=5, max_count=400):
     """A sparse chunk batch with uneven nonzero counts.
 

@@ -5,7 +5,9 @@ Run where ``make -C oracle ref`` has built ``oracle/_ref/libdct3d_ref.so``
 (the reference ``tools/3ddct/dct3d.c`` plus the probe ``oracle/ref_probe.c``).
 Reads the reference-encoded blobs of ``dct3d_golden.npz`` and writes
 ``dct3d_corrupt.npz``: 160 damaged variants (single bit flips, byte
-overwrites, truncations) and, for each, what the reference parser
+overwrites, truncations), plus 8 blobs the reference encoder writes at tiny
+quality (0.003-0.1) whose levels exceed int16, and, for each, what the
+reference parser
 (``ref_parse_u8``: the entropy part of decode_float, dct3d.c:542-587)
 returns: the accept flag, the header, the levels (as the reference's floats)
 and the corrections. Data only.
@@ -15,7 +17,7 @@ import sys
 
 import numpy as np
 
-from make_golden import HERE, REF, u16p  # noqa: F401  (REF carries the argtypes)
+from make_golden import HERE, REF, encode, u16p  # noqa: F401  (REF carries the argtypes)
 import ctypes as C
 
 
@@ -64,13 +66,35 @@ def main():
         cds.append(cd)
         coffs.append(coffs[-1] + cp.size)
         src.append(i)
+    # valid blobs whose levels exceed int16: the reference keeps levels as
+    # floats; the int16 level formats must report them rather than wrap
+    zz, yy, xx = np.meshgrid(np.arange(16), np.arange(16), np.arange(16), indexing="ij")
+    chk = (((zz + yy + xx) & 1) * 255).astype(np.uint8).reshape(-1)
+    ramp = (xx * 17).astype(np.uint8).reshape(-1)
+    for q in (0.003, 0.01, 0.03, 0.1):
+        for blk in (chk, ramp):
+            b = encode(blk, q)
+            ok, (vmin, vspan, qstep, dcq), lvl, cp, cd = parse(b)
+            assert ok == 1
+            blobs.append(b)
+            offs.append(offs[-1] + b.size)
+            oks.append(ok)
+            hdrs.append((vmin, vspan, qstep))
+            dcqs.append(dcq)
+            lvls.append(lvl)
+            cps.append(cp)
+            cds.append(cd)
+            coffs.append(coffs[-1] + cp.size)
+            src.append(-1)
     path = os.path.join(HERE, "dct3d_corrupt.npz")
     np.savez_compressed(path, bytes=np.concatenate(blobs), offsets=np.array(offs, np.int64),
                         ok=np.array(oks, np.int32), hdr=np.array(hdrs, np.float32), dcq=np.array(dcqs, np.int32),
                         levels=np.stack(lvls), corr_off=np.array(coffs, np.uint32),
                         corr_pos=np.concatenate(cps).astype(np.uint16),
                         corr_delta=np.concatenate(cds).astype(np.float32), source=np.array(src, np.int32))
-    print("wrote", path, os.path.getsize(path), "bytes;", int(np.sum(oks)), "of 160 accepted")
+    wide = int(sum(np.any(np.abs(v) > 32767) for v in lvls))
+    print("wrote", path, os.path.getsize(path), "bytes;", int(np.sum(oks)), "of", len(oks), "accepted;", wide,
+          "with levels beyond int16")
 
 
 if __name__ == "__main__":

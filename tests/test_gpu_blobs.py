@@ -75,15 +75,23 @@ def test_corrupt_blobs_follow_the_reference_parser(ctx):
     st = parsed["status"].cpu().numpy()
     lv = levels_of(parsed, 4096)
     u8 = u8.cpu().numpy()
+    wide = 0
     for i in range(n):
         if not CORRUPT["ok"][i]:
             assert st[i] == 0, i
             assert not u8[i].any(), i  # rejected: zero-filled
             continue
+        ref = CORRUPT["levels"][i]
+        if np.any(ref > 32767) or np.any(ref < -32768):
+            assert st[i] == -1, i  # valid, but beyond the int16 level formats: reported, zero-filled
+            assert not u8[i].any(), i
+            wide += 1
+            continue
         assert st[i] == 1, i
         assert np.array_equal(lv[i].astype(np.float32), CORRUPT["levels"][i]), i
         hdr = parsed["headers"][i].cpu().numpy()
         assert hdr[0:2].view(np.float32).tolist() == CORRUPT["hdr"][i, 0:2].tolist() and hdr[2] == CORRUPT["dcq"][i]
+    assert wide >= 4
 
 
 def test_encoder_reproduces_reference_blobs(ctx):
@@ -140,3 +148,31 @@ def test_capacity_dtype_and_empty_blobs(ctx):
     parsed = ctx.parse_blobs(data, offs, short)
     torch.cuda.synchronize()
     assert np.all(parsed["status"].cpu().numpy() == 0)
+
+
+def test_parse_is_independent_of_blob_alignment(ctx):
+    """The parser reads each blob through aligned dwords: the same blobs at
+    every byte alignment, the last one ending at the buffer's end, parse alike."""
+    byts, offs = GOLD["blob_bytes"], GOLD["blob_offsets"].astype(np.int64)
+    n = offs.size - 1
+    ref = None
+    for shift in range(4):
+        order = np.roll(np.arange(n), shift)  # a different blob ends the buffer each time
+        parts = [np.zeros(shift, np.uint8)] + [byts[offs[i]:offs[i + 1]] for i in order]
+        data = np.concatenate(parts)
+        lens = np.diff(offs)[order]
+        starts = shift + np.concatenate([[0], np.cumsum(lens)[:-1]])
+        assert starts[-1] + lens[-1] == data.size
+        parsed = ctx.parse_blobs(cuda(data), cuda(starts.astype(np.int64)), cuda(lens.astype(np.int32)))
+        torch.cuda.synchronize()
+        got = {k: parsed[k].cpu().numpy() for k in ("status", "headers", "pair_counts", "corr_counts")}
+        lv = levels_of(parsed, 4096)
+        inv = np.argsort(order)
+        got = {k: v[inv] if k != "headers" else v[inv, :3] for k, v in got.items()}
+        got["levels"] = lv[inv]
+        assert np.all(got["status"] == 1)
+        if ref is None:
+            ref = got
+            assert np.array_equal(got["levels"], GOLD["blob_levels"])
+        for k in ref:
+            assert np.array_equal(got[k], ref[k]), (shift, k)

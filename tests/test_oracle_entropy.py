@@ -100,3 +100,14 @@ def test_corrupt_blobs_parse_as_the_reference_parses_them():
         assert np.array_equal(cp, CORRUPT["corr_pos"][c0:c1]), i
         assert np.array_equal(cd.view(np.uint32), CORRUPT["corr_delta"][c0:c1].view(np.uint32)), i
     print(f"{int(CORRUPT['ok'].sum())} accepted by the reference, {wide} with levels beyond int16")
+
+
+def test_blob_batch_decode_is_dct3d_decode_u8():
+    offs = GOLD["blob_offsets"].astype(np.int64)
+    u8, st = orc.decode_blobs(GOLD["blob_bytes"], offs[:-1], np.diff(offs), nthreads=4)
+    assert np.all(st == 1)
+    assert np.array_equal(u8, GOLD["blob_u8"])  # the reference decoder's output for the same blobs
+    offs = CORRUPT["offsets"].astype(np.int64)
+    u8, st = orc.decode_blobs(CORRUPT["bytes"], offs[:-1], np.diff(offs), nthreads=4)
+    assert np.array_equal(st != 0, CORRUPT["ok"].astype(bool))
+    assert not u8[st != 1].any()

@@ -46,3 +46,11 @@ def test_mixed_views_tile_the_atlas_without_overlap():
         assert x + v.viewport[0] <= w and y + v.viewport[1] <= h
         cover[y:y + v.viewport[1], x:x + v.viewport[0]] += 1
     assert cover.max() == 1
+
+
+def test_qstep_table_matches_the_reference_blobs():
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "dct3d_golden.npz"))
+    for i in range(g["blob_hdr"].shape[0]):  # step tables the reference decoder built for its own blobs
+        t = workloads.step_table_qstep(g["blob_hdr"][i, 2])
+        assert np.array_equal(t.view(np.uint32), g["blob_steps"][i].view(np.uint32)), i
