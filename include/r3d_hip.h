@@ -422,6 +422,62 @@ int r3d_hip_encode_blobs(r3d_hip_ctx *ctx, uint32_t nchunks, const int16_t *leve
                          const r3d_chunk_header *headers, const float *qsteps, uint32_t dtype, uint8_t *out,
                          uint64_t slot_bytes, uint32_t *lengths, void *stream);
 
+/* ---- zarr v3 shards of 3ddct chunks -------------------------------------
+ * A shard is one 1024^3 region of the volume: its 16^3 chunks' blobs back to
+ * back, then an index of one (u64le offset, u64le nbytes) pair per chunk in
+ * z-major order, (~0, ~0) for a missing chunk (zeros), then the crc32c of the
+ * index (tools/3ddct/tools/export/shard.h:1-18; src/shard/shardio.h:1-5).
+ * Replaces the reader src/shard/shardio.c for region decodes. */
+#define R3D_HIP_SHARD_GRID 64u
+#define R3D_HIP_SHARD_INDEX_BYTES (64u * 64u * 64u * 16u + 4u)
+
+/* One present shard, resident in device memory: the whole file. */
+typedef struct r3d_shard_src {
+  const uint8_t *bytes;         /* device */
+  uint64_t nbytes;
+  uint32_t sz, sy, sx;          /* shard coordinates, as in <sz>_<sy>_<sx>.shard */
+  uint32_t reserved0;
+} r3d_shard_src;
+
+/* The store's volume dims (r3d_shard_store, shardio.h:16-20) and a region of
+ * it (the arguments of r3d_shard_decode_region, shardio.h:59-63). */
+typedef struct r3d_shard_region {
+  uint64_t nz, ny, nx;
+  uint64_t z0, y0, x0;
+  uint32_t dz, dy, dx;
+  uint32_t reserved0;
+} r3d_shard_region;
+
+/* Chunks whose voxels the region decode writes: the chunks that intersect
+ * the region clamped to the volume, as a box (x fastest). The length of
+ * chunk_status below; 0 for an empty region or one outside the volume. */
+uint64_t r3d_hip_shard_region_chunks(const r3d_shard_region *region);
+
+/* r3d_shard_decode_region (shardio.c:269-306) on the GPU. dst (device,
+ * dz * dy * dx bytes, x fastest) receives the region; voxels outside the
+ * volume, in absent shards (the shards not listed) or in missing chunks are
+ * 0. Each listed shard is checked on the device (long enough for its index,
+ * index crc32c) and every chunk's index entry is checked against the payload
+ * (shardio.c:157-172). chunk_status (device, optional,
+ * r3d_hip_shard_region_chunks entries in chunk-box order): 1 decoded, 0
+ * missing or absent, -1 corrupt (a bad shard, index entry or blob: zeros),
+ * -2 levels beyond int16 or over the 4096-entry capacity (zeros; the
+ * reference decodes these: decode them with dct3d_decode_u8 on the host).
+ * counts (device, optional, 4 x u32) accumulates decoded, missing, corrupt
+ * and -2 chunks; the reference returns -1 exactly when counts[2] grows.
+ * Decoded voxels are bit-identical to dct3d_decode_u8. The context keeps a
+ * workspace of about 41 KB per chunk, up to 262,144 chunks at a time.
+ * Returns 0, or a negative R3D_ERR_* for invalid arguments (a zero size, a
+ * size that overflows) before any launch. */
+int r3d_hip_shard_decode_region(r3d_hip_ctx *ctx, const r3d_shard_src *shards, uint32_t nshards,
+                                const r3d_shard_region *region, uint8_t *dst, int32_t *chunk_status,
+                                uint32_t *counts, void *stream);
+
+/* CRC-32C (reflected Castagnoli polynomial, init and final xor 0xFFFFFFFF:
+ * the zarr index trailer, shardio.c:39-52) of n device bytes into *crc
+ * (device). */
+int r3d_hip_crc32c(r3d_hip_ctx *ctx, const uint8_t *data, uint64_t n, uint32_t *crc, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

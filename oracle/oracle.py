@@ -77,6 +77,13 @@ def load():
     L.r3do_encode_blob.restype = C.c_size_t
     L.r3do_encode_blob.argtypes = [C.POINTER(BlobHdr), i16p, C.c_uint32, u8p, C.c_size_t]
     L.r3do_decode_blobs.argtypes = [C.c_uint32, u8p, VP, VP, C.c_uint32, u8p, VP, C.c_int]
+    L.r3do_crc32c.restype = C.c_uint32
+    L.r3do_crc32c.argtypes = [u8p, C.c_size_t]
+    L.r3do_shard_valid.restype = C.c_int
+    L.r3do_shard_valid.argtypes = [u8p, C.c_uint64]
+    L.r3do_shard_decode_region.restype = C.c_int
+    L.r3do_shard_decode_region.argtypes = [VP, C.c_uint32] + [C.c_uint64] * 6 + [C.c_uint32] * 3 + [u8p, VP, VP,
+                                                                                                  C.c_int]
     _lib = L
     return L
 
@@ -269,6 +276,65 @@ def decode_blobs(data, offsets, lengths, dtype=1, nthreads=8):
     status = np.zeros(n, np.int32)
     load().r3do_decode_blobs(n, data, offs.ctypes.data, lens.ctypes.data, dtype, out, status.ctypes.data, nthreads)
     return out.reshape(n, 4096), status
+
+
+class ShardC(C.Structure):
+    _fields_ = [("bytes", C.c_void_p), ("n", C.c_uint64), ("sz", C.c_uint32), ("sy", C.c_uint32),
+                ("sx", C.c_uint32), ("reserved0", C.c_uint32)]
+
+
+SHARD_GRID = 64
+SHARD_INDEX_BYTES = SHARD_GRID ** 3 * 16 + 4
+
+
+def crc32c(data):
+    data = np.ascontiguousarray(data, np.uint8)
+    return int(load().r3do_crc32c(data if data.size else np.zeros(1, np.uint8), data.size))
+
+
+def shard_valid(shard):
+    shard = np.ascontiguousarray(shard, np.uint8)
+    return bool(load().r3do_shard_valid(shard if shard.size else np.zeros(1, np.uint8), shard.size))
+
+
+def shard_decode_region(shards, dims, region, nthreads=8):
+    """r3do_shard_decode_region. shards: list of (bytes uint8, sz, sy, sx);
+    dims (nz, ny, nx); region (z0, y0, x0, dz, dy, dx). Returns (rc, dst
+    uint8 [dz, dy, dx], counts [decoded, missing, corrupt, beyond int16],
+    per-chunk status in the clamped chunk box)."""
+    keep = [np.ascontiguousarray(b, np.uint8) for b, *_ in shards]
+    arr = (ShardC * max(1, len(shards)))()
+    for i, (b, (_, sz, sy, sx)) in enumerate(zip(keep, shards)):
+        arr[i] = ShardC(b.ctypes.data if b.size else None, b.size, sz, sy, sx, 0)
+    z0, y0, x0, dz, dy, dx = region
+    nz, ny, nx = dims
+    dst = np.zeros(int(dz) * int(dy) * int(dx), np.uint8)
+    counts = np.zeros(4, np.uint32)
+    nbox = 1
+    if z0 < nz and y0 < ny and x0 < nx:
+        for a0, d, n in ((z0, dz, nz), (y0, dy, ny), (x0, dx, nx)):
+            a1 = min(a0 + d, n)
+            nbox *= (a1 - 1) // 16 - a0 // 16 + 1
+    status = np.zeros(nbox, np.int32)
+    rc = load().r3do_shard_decode_region(C.addressof(arr), len(shards), nz, ny, nx, z0, y0, x0, dz, dy, dx, dst,
+                                         counts.ctypes.data, status.ctypes.data, nthreads)
+    return rc, dst.reshape(dz, dy, dx), counts, status
+
+
+def shard_pack(blobs):
+    """A shard in the layout of tools/3ddct/tools/export/shard.c:55-97: blobs
+    (dict: chunk index (cz*64 + cy)*64 + cx -> uint8 array) back to back in
+    chunk order, then the index ((~0, ~0) for a missing chunk) and its crc32c."""
+    idx = np.full((SHARD_GRID ** 3, 2), np.iinfo(np.uint64).max, np.uint64)
+    parts, off = [], 0
+    for ci in sorted(blobs):
+        b = np.ascontiguousarray(blobs[ci], np.uint8)
+        idx[ci] = (off, b.size)
+        parts.append(b)
+        off += b.size
+    ib = idx.astype("<u8").view(np.uint8).reshape(-1)
+    crc = np.array([crc32c(ib)], "<u4").view(np.uint8)
+    return np.concatenate(parts + [ib, crc]) if parts else np.concatenate([ib, crc])
 
 
 def raycast_diff(ovol, tf, grid, gdims, views, width, height, alpha_cutoff=0.98, dimg=None, window=None):

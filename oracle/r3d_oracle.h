@@ -180,6 +180,28 @@ size_t r3do_encode_blob(const r3do_blob_hdr *h, const int16_t lvl[4096], uint32_
 void r3do_decode_blobs(uint32_t n, const uint8_t *bytes, const uint64_t *offsets, const uint32_t *lengths,
                        uint32_t dtype, uint8_t *out, int32_t *status, int nthreads);
 
+/* ---- zarr v3 shard container (src/shard/shardio.c) ------------------- */
+
+/* A present shard: the whole file in memory and its shard coordinates. */
+typedef struct r3do_shard {
+  const uint8_t *bytes;
+  uint64_t n;
+  uint32_t sz, sy, sx;
+  uint32_t reserved0;
+} r3do_shard;
+
+/* CRC-32C (reflected 0x82F63B78, init and xorout 0xFFFFFFFF). */
+uint32_t r3do_crc32c(const uint8_t *p, size_t n);
+/* 1 when the shard is long enough for its index and the index crc matches. */
+int r3do_shard_valid(const uint8_t *shard, uint64_t n);
+/* 0 present, 1 missing, -1 invalid (shardio.c:157-172). */
+int r3do_shard_chunk_location(const uint8_t *shard, uint64_t n, uint32_t cz, uint32_t cy, uint32_t cx,
+                              uint64_t *off, uint64_t *len);
+/* r3d_shard_decode_region over the listed shards; see oracle_shard.c. */
+int r3do_shard_decode_region(const r3do_shard *shards, uint32_t nshards, uint64_t nz, uint64_t ny, uint64_t nx,
+                             uint64_t z0, uint64_t y0, uint64_t x0, uint32_t dz, uint32_t dy, uint32_t dx,
+                             uint8_t *dst, uint32_t counts[4], int32_t *chunk_status, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -135,6 +135,21 @@ class BlobParseOut(C.Structure):
 R3D_DT_U8 = 1
 
 
+class ShardSrc(C.Structure):
+    _fields_ = [("bytes", C.c_void_p), ("nbytes", C.c_uint64), ("sz", C.c_uint32), ("sy", C.c_uint32),
+                ("sx", C.c_uint32), ("reserved0", C.c_uint32)]
+
+
+class ShardRegion(C.Structure):
+    _fields_ = [("nz", C.c_uint64), ("ny", C.c_uint64), ("nx", C.c_uint64), ("z0", C.c_uint64), ("y0", C.c_uint64),
+                ("x0", C.c_uint64), ("dz", C.c_uint32), ("dy", C.c_uint32), ("dx", C.c_uint32),
+                ("reserved0", C.c_uint32)]
+
+
+SHARD_GRID = 64
+SHARD_INDEX_BYTES = SHARD_GRID ** 3 * 16 + 4
+
+
 class RaycastTuning(C.Structure):
     _fields_ = [("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("ms", C.c_float)]
 
@@ -186,6 +201,9 @@ SIGNATURES = {
     "r3d_hip_encode_chunks": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P]),
     "r3d_hip_parse_blobs": (C.c_int, [P, C.POINTER(BlobBatch), C.POINTER(BlobParseOut), P]),
     "r3d_hip_encode_blobs": (C.c_int, [P, C.c_uint32, P, P, P, C.c_uint32, P, C.c_uint64, P, P]),
+    "r3d_hip_shard_region_chunks": (C.c_uint64, [C.POINTER(ShardRegion)]),
+    "r3d_hip_shard_decode_region": (C.c_int, [P, P, C.c_uint32, C.POINTER(ShardRegion), P, P, P, P]),
+    "r3d_hip_crc32c": (C.c_int, [P, P, C.c_uint64, P, P]),
 }
 
 _lib = None

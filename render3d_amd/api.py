@@ -189,6 +189,46 @@ class Context:
         self.decode(self.parsed_batch(parsed, out), variant, stream)
         return out, parsed["status"]
 
+    # ---- zarr v3 shards ---------------------------------------------------------
+    def crc32c(self, data, stream=None):
+        """r3d_hip_crc32c of a uint8 device tensor -> int (synchronises)."""
+        out = torch.zeros(1, dtype=torch.int32, device=data.device)
+        self.check(self.lib.r3d_hip_crc32c(self.h, _ptr(data), int(data.numel()), _ptr(out), _stream(stream)))
+        return int(out.cpu().numpy().view(np.uint32)[0])
+
+    @staticmethod
+    def _region(dims, region):
+        nz, ny, nx = (int(v) for v in dims)
+        z0, y0, x0, dz, dy, dx = (int(v) for v in region)
+        return _lib.ShardRegion(nz, ny, nx, z0, y0, x0, dz, dy, dx, 0)
+
+    def shard_region_chunks(self, dims, region):
+        r = self._region(dims, region)
+        return int(self.lib.r3d_hip_shard_region_chunks(C.byref(r)))
+
+    def shard_decode_region(self, shards, dims, region, *, out=None, counts=None, status=True, stream=None):
+        """r3d_hip_shard_decode_region. shards: [(uint8 device tensor: the
+        whole shard file, sz, sy, sx)] (unlisted shards are absent); dims
+        (nz, ny, nx); region (z0, y0, x0, dz, dy, dx). Returns (dst uint8
+        [dz, dy, dx], counts int32 [4]: decoded, missing, corrupt, beyond
+        int16, per-chunk status int32 or None); nothing is synchronised."""
+        r = self._region(dims, region)
+        dev = f"cuda:{self.device}"
+        if out is None:
+            out = torch.empty((r.dz, r.dy, r.dx), dtype=torch.uint8, device=dev)
+        if counts is None:
+            counts = torch.zeros(4, dtype=torch.int32, device=dev)
+        st = None
+        if status:
+            st = torch.empty(max(1, int(self.lib.r3d_hip_shard_region_chunks(C.byref(r)))), dtype=torch.int32,
+                             device=dev)
+        arr = (_lib.ShardSrc * max(1, len(shards)))()
+        for i, (t, sz, sy, sx) in enumerate(shards):
+            arr[i] = _lib.ShardSrc(t.data_ptr(), int(t.numel()), int(sz), int(sy), int(sx), 0)
+        self.check(self.lib.r3d_hip_shard_decode_region(self.h, C.cast(arr, C.c_void_p), len(shards), C.byref(r),
+                                                        _ptr(out), _ptr(st), _ptr(counts), _stream(stream)))
+        return out, counts, st
+
     # ---- workloads ----------------------------------------------------------
     def synth_sheets(self, nx, ny, nz, nsheets, seed, stream=None):
         out = torch.empty(nx * ny * nz, dtype=torch.uint8, device=f"cuda:{self.device}")
@@ -387,3 +427,44 @@ class Volume:
                                                                   out.ctypes.data_as(C.c_void_p), _lib.R3D_SRC_HOST,
                                                                   _stream(stream)))
         return out
+
+
+class ShardStore:
+    """r3d_shard_store (src/shard/shardio.h:16-20) on the GPU: a directory of
+    shard files <dir>/<Z>_<Y>_<X>.shard for a volume of nz x ny x nx voxels.
+    A shard is read into HBM the first time a region needs it and kept."""
+
+    def __init__(self, ctx, path, nz, ny, nx):
+        self.ctx, self.path, self.dims = ctx, path, (int(nz), int(ny), int(nx))
+        self.cache = {}
+
+    def shard(self, sz, sy, sx):
+        """The shard's bytes on the device, or None when its file is absent."""
+        import os
+        key = (int(sz), int(sy), int(sx))
+        if key not in self.cache:
+            f = os.path.join(self.path, f"{key[0]}_{key[1]}_{key[2]}.shard")
+            self.cache[key] = (torch.from_numpy(np.fromfile(f, np.uint8)).to(f"cuda:{self.ctx.device}")
+                               if os.path.exists(f) else None)
+        return self.cache[key]
+
+    def decode_region(self, z0, y0, x0, dz, dy, dx, stream=None):
+        """r3d_shard_decode_region (shardio.c:269-306): returns (rc, dst uint8
+        [dz, dy, dx] on the device). rc is 0, -1 when a shard or chunk was
+        corrupt (those chunks are zero, as in the reference), or -2 when a
+        chunk's levels exceed int16 (zero here; the reference decodes it)."""
+        region = (z0, y0, x0, dz, dy, dx)
+        nz, ny, nx = self.dims
+        shards = []
+        if z0 < nz and y0 < ny and x0 < nx:
+            g = _lib.SHARD_GRID * 16
+            ends = [min(a + d, n) - 1 for a, d, n in ((z0, dz, nz), (y0, dy, ny), (x0, dx, nx))]
+            for sz in range(z0 // g, ends[0] // g + 1):
+                for sy in range(y0 // g, ends[1] // g + 1):
+                    for sx in range(x0 // g, ends[2] // g + 1):
+                        t = self.shard(sz, sy, sx)
+                        if t is not None:
+                            shards.append((t, sz, sy, sx))
+        dst, counts, _ = self.ctx.shard_decode_region(shards, self.dims, region, status=False, stream=stream)
+        c = counts.cpu().numpy()
+        return (-1 if c[2] else -2 if c[3] else 0), dst

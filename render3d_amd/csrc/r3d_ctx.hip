@@ -50,6 +50,7 @@ extern "C" int r3d_hip_init(r3d_hip_ctx **out, int device) {
   }
   if (rc == R3D_OK) rc = r3d::upload_basis(c);
   if (rc == R3D_OK) rc = r3d::upload_blob_tables(c);
+  if (rc == R3D_OK) rc = r3d::upload_crc_tables(c);
   if (rc == R3D_OK) {
     // default transfer function: the grayscale identity ramp (r3d_tf_preset 0,
     // src/core/transfer.c:28-32), as the reference uploads before any TF is set
@@ -82,6 +83,7 @@ extern "C" void r3d_hip_destroy(r3d_hip_ctx *c) {
   if (c->d_queue) hipFree(c->d_queue);
   if (c->d_partials) hipFree(c->d_partials);
   if (c->d_gtf) hipFree(c->d_gtf);
+  if (c->d_ws) hipFree(c->d_ws);
   delete c;
 }
 

@@ -67,6 +67,14 @@ struct DecodeArgs {
   int32_t *status;
   const uint32_t *pair_cnt;
   const uint32_t *corr_cnt;
+  // Region destination (shard region decodes, exact kernel only): chunk c is
+  // cell rfirst + c of a chunk box (x fastest, ncy x ncx per slice) whose first chunk
+  // starts at (rz, ry, rx) relative to the region origin. Voxels outside
+  // [0, hz) x [0, hy) x [0, hx) of the region are not written; the region is
+  // x fastest with row stride rsy and slice stride rsz.
+  uint32_t rgn, ncy, ncx, rfirst;  // rfirst: the box cell of chunk 0
+  int64_t rz, ry, rx, hz, hy, hx;
+  uint64_t rsy, rsz;
 };
 
 // [begin, end) of chunk c's pairs: offsets are caller data, so the range is
@@ -183,6 +191,45 @@ __device__ __forceinline__ void fdct16(const float in[16], float out[16]) {
   }
 }
 
+// Where row (z, y) of chunk c goes, and which of its 16 voxels are stored.
+struct RowDst {
+  uint8_t *p;
+  uint32_t lx, hx;
+  bool on;
+};
+__device__ __forceinline__ RowDst row_dst(const DecodeArgs &a, uint32_t c, uint32_t z, uint32_t y) {
+  RowDst d;
+  if (!a.rgn) {
+    d.p = a.out + (a.out_off ? a.out_off[c] : (uint64_t)c * 4096u) + z * a.sz + y * a.sy;
+    d.lx = 0;
+    d.hx = 16;
+    d.on = true;
+    return d;
+  }
+  const uint32_t cl = a.rfirst + c;
+  const uint32_t ix = cl % a.ncx, r = cl / a.ncx, iy = r % a.ncy, iz = r / a.ncy;
+  const int64_t gz = a.rz + 16 * (int64_t)iz + z, gy = a.ry + 16 * (int64_t)iy + y;
+  const int64_t ox = a.rx + 16 * (int64_t)ix;
+  d.on = gz >= 0 && gz < a.hz && gy >= 0 && gy < a.hy;
+  const int64_t lx = ox < 0 ? -ox : 0, hx = a.hx - ox < 16 ? a.hx - ox : 16;
+  d.lx = (uint32_t)lx;
+  d.hx = hx > lx ? (uint32_t)hx : (uint32_t)lx;
+  // byte address of the row's x = 0, which may lie before the region when
+  // ox < 0; only x in [lx, hx) is dereferenced
+  d.p = (uint8_t *)((uintptr_t)a.out + (uint64_t)(gz * (int64_t)a.rsz + gy * (int64_t)a.rsy + ox));
+  return d;
+}
+
+__device__ __forceinline__ void store_row16(uint8_t *dst, const uint32_t w[4]);
+__device__ __forceinline__ void store_row(const RowDst &d, const uint32_t w[4]) {
+  if (!d.on) return;
+  if (d.lx == 0 && d.hx == 16) {
+    store_row16(d.p, w);
+    return;
+  }
+  for (uint32_t x = d.lx; x < d.hx; x++) d.p[x] = (uint8_t)(w[x >> 2] >> ((x & 3) * 8));
+}
+
 __device__ __forceinline__ void store_row16(uint8_t *dst, const uint32_t w[4]) {
   const uintptr_t a = (uintptr_t)dst;
   if ((a & 15u) == 0) {
@@ -232,14 +279,14 @@ __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
     const float dc = (float)h.dcq / 16.0f;
     const bool ok = h.table < a.ntables && h.vspan > 0.0f && __builtin_isfinite(h.vmin) &&
                     __builtin_isfinite(dc);
-    uint8_t *dst = a.out + (a.out_off ? a.out_off[c] : (uint64_t)c * 4096u) + z * a.sz + y * a.sy;
+    const RowDst dst = row_dst(a, c, z, y);
     if (!ok) {  // dct3d.c:555-556: rejected header; the chunk is zero-filled
       const uint32_t zw[4] = {0, 0, 0, 0};
       if (F32OUT) {
         float4 *fd = (float4 *)(a.out + ((size_t)c * 4096u + t * 16u) * 4u);
         for (int i = 0; i < 4; i++) fd[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       } else {
-        store_row16(dst, zw);
+        store_row(dst, zw);
       }
       if (a.status && t == 0) a.status[c] = 0;
       continue;
@@ -304,7 +351,7 @@ __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
       uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
       for (int x = 0; x < 16; x++) w[x >> 2] |= round_u8(tile[t * 16 + x] * inv + h.vmin) << ((x & 3) * 8);
-      store_row16(dst, w);
+      store_row(dst, w);
     }
     if (a.status && t == 0) a.status[c] = 1;
     __syncthreads();
@@ -830,8 +877,8 @@ static uint32_t grid_for(const r3d_hip_ctx *ctx, uint32_t n, uint32_t per_cu) {
 
 using namespace r3d;
 
-extern "C" int r3d_hip_decode_chunks(r3d_hip_ctx *ctx, const r3d_chunk_batch *b,
-                                     const r3d_decode_opts *opts, void *stream) {
+static int decode_impl(r3d_hip_ctx *ctx, const r3d_chunk_batch *b, const r3d_decode_opts *opts, void *stream,
+                       const RegionDst *rg) {
   if (!ctx) return R3D_ERR_INVALID;
   if (!b) return r3d_fail(ctx, R3D_ERR_INVALID, "decode: batch is NULL");
   if (b->nchunks == 0) return R3D_OK;
@@ -861,8 +908,10 @@ extern "C" int r3d_hip_decode_chunks(r3d_hip_ctx *ctx, const r3d_chunk_batch *b,
     return r3d_fail(ctx, R3D_ERR_INVALID, "decode: unknown flags 0x%x", opts->flags);
   if (f32out && (b->out_offsets || b->out_stride_y || b->out_stride_z || ((uintptr_t)b->out & 15u)))
     return r3d_fail(ctx, R3D_ERR_INVALID, "decode: FP32 output is flat and 16-byte aligned");
+  if (rg && (variant != R3D_DECODE_EXACT_F32 || f32out || b->out_offsets))
+    return r3d_fail(ctx, R3D_ERR_UNSUPPORTED, "decode: a region destination takes the exact u8 decode");
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  DecodeArgs a;
+  DecodeArgs a{};
   a.nchunks = b->nchunks;
   a.fmt = b->coef_format;
   a.levels = b->levels;
@@ -883,6 +932,20 @@ extern "C" int r3d_hip_decode_chunks(r3d_hip_ctx *ctx, const r3d_chunk_batch *b,
   a.status = b->status;
   a.pair_cnt = b->pair_counts;
   a.corr_cnt = b->corr_counts;
+  if (rg) {
+    a.rgn = 1;
+    a.ncy = rg->ncy;
+    a.ncx = rg->ncx;
+    a.rfirst = rg->first;
+    a.rz = rg->rz;
+    a.ry = rg->ry;
+    a.rx = rg->rx;
+    a.hz = rg->hz;
+    a.hy = rg->hy;
+    a.hx = rg->hx;
+    a.rsy = rg->sy;
+    a.rsz = rg->sz;
+  }
   hipStream_t st = (hipStream_t)stream;
   if (variant == R3D_DECODE_EXACT_F32) {
     const uint32_t grid = grid_for(ctx, b->nchunks, 1u << 16);
@@ -904,6 +967,15 @@ extern "C" int r3d_hip_decode_chunks(r3d_hip_ctx *ctx, const r3d_chunk_batch *b,
   }
   R3D_HIP_CHECK(ctx, hipGetLastError());
   return R3D_OK;
+}
+
+extern "C" int r3d_hip_decode_chunks(r3d_hip_ctx *ctx, const r3d_chunk_batch *b,
+                                     const r3d_decode_opts *opts, void *stream) {
+  return decode_impl(ctx, b, opts, stream, nullptr);
+}
+
+int r3d::decode_region(r3d_hip_ctx *ctx, const r3d_chunk_batch *b, const RegionDst &rg, hipStream_t stream) {
+  return decode_impl(ctx, b, nullptr, (void *)stream, &rg);
 }
 
 extern "C" int r3d_hip_encode_chunks(r3d_hip_ctx *ctx, const uint8_t *vol, uint32_t nx, uint32_t ny,

@@ -58,6 +58,8 @@ struct r3d_hip_ctx {
   int slot_next = 0;
   r3d_raycast_tuning tuning{8u, 8u, 0u, 0.0f};  // r3d_hip_raycast's launch shape
   uint32_t *d_queue = nullptr;                   // tile queues of r3d_hip_raycast, one per kind
+  void *d_ws = nullptr;                          // workspace of r3d_hip_shard_decode_region
+  size_t ws_cap = 0;
 };
 
 struct r3d_hip_volume {
@@ -91,8 +93,25 @@ int upload_basis(r3d_hip_ctx *ctx);
 // Uploads the 3ddct scan order and the powf(1 + f, 0.65f) weights (computed by
 // the host's libm) for the blob parser and encoder.
 int upload_blob_tables(r3d_hip_ctx *ctx);
+// Uploads the CRC-32C byte table and the x^(2^k) mod P powers used to fold
+// piecewise CRCs.
+int upload_crc_tables(r3d_hip_ctx *ctx);
 // Copies n bytes into the next staging slot (pinned host + device buffer);
 // the caller records the slot's event after the launches that read it.
 int stage(r3d_hip_ctx *ctx, const void *src, size_t n, hipStream_t s, ViewSlot **slot_out);
 r3d::VolDev vol_dev(const r3d_hip_volume *v);
+
+// A region destination for the exact decode (shard region decodes): chunk c
+// of the batch is cell first + c of an ncz x ncy x ncx chunk box (x fastest) whose
+// first chunk starts at (rz, ry, rx) relative to the region origin; only
+// voxels in [0, hz) x [0, hy) x [0, hx) of the region are written, at row
+// stride sy and slice stride sz.
+struct RegionDst {
+  uint32_t ncz, ncy, ncx, first;  // first: the box cell of the batch's chunk 0
+  int64_t rz, ry, rx, hz, hy, hx;
+  uint64_t sy, sz;
+};
+int decode_region(r3d_hip_ctx *ctx, const r3d_chunk_batch *b, const RegionDst &rg, hipStream_t stream);
+// Grow-only device workspace of the context (shard region decodes).
+int workspace(r3d_hip_ctx *ctx, size_t bytes, void **out);
 }  // namespace r3d

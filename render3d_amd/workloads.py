@@ -37,6 +37,30 @@ def step_table_qstep(qstep):
     return np.array([q * np.float32(libm.powf(1.0 + f, 0.65)) for f in range(46)], np.float32)
 
 
+def pack_shard(ctx, data, offsets, lengths, present=None):
+    """A zarr v3 shard (the layout of tools/3ddct/tools/export/shard.c:55-97)
+    from 262,144 blobs on the device (uint8 bytes, int64 offsets, int32
+    lengths, in z-major chunk order): present chunks' blobs back to back, the
+    index ((~0, ~0) where present is False) and its crc32c, as one uint8
+    device tensor."""
+    import torch
+    n = 64 ** 3
+    assert offsets.numel() == n and lengths.numel() == n
+    dev = data.device
+    keep = torch.ones(n, dtype=torch.bool, device=dev) if present is None else present.to(dev)
+    l64 = torch.where(keep, lengths.long(), torch.zeros_like(offsets))
+    out_off = torch.cumsum(l64, 0) - l64
+    total = int(l64.sum())
+    within = torch.arange(total, device=dev) - torch.repeat_interleave(out_off, l64)
+    payload = data[torch.repeat_interleave(offsets, l64) + within]
+    idx = torch.stack([out_off, l64], 1)
+    idx[~keep] = -1  # (~0, ~0)
+    ib = idx.contiguous().view(torch.uint8).reshape(-1)  # little-endian int64 pairs
+    crc = ctx.crc32c(ib)
+    tail = torch.tensor(np.array([crc], "<u4").view(np.uint8), device=dev)
+    return torch.cat([payload, ib, tail])
+
+
 def sparse_laplacian_batch(nchunks, seed, count_scale=100.0, value_scale=12.0, min_count=5, max_count=400):
     """A sparse chunk batch with uneven nonzero counts.
 
