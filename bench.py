@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (graph) workload")
     ap.add_argument("--batched-chunks", type=int, default=32768)
     ap.add_argument("--no-cfg0", action="store_true", help="skip the 256^3 / 512^2 workload")
+    ap.add_argument("--no-shard", action="store_true", help="skip the zarr shard region-decode workload")
     ap.add_argument("--no-blobs", action="store_true", help="skip the 3ddct blob (entropy stage) workload")
     ap.add_argument("--no-fit", action="store_true", help="skip the differentiable 256^3 fit (200 Adam steps)")
     ap.add_argument("--profile-decode", action="store_true",
@@ -188,6 +189,7 @@ def main():
     bat = None if args.no_batched else batched(ctx, vol, lut, args, dev, stream)
     c0 = None if args.no_cfg0 else cfg0(ctx, args, dev, stream)
     blob = None if args.no_blobs else blob_stage(ctx, args, dev, stream, n, nsheets, seed)
+    shd = None if args.no_shard else shard_stage(ctx, args, dev, stream, n, nsheets, seed)
     fit = None if args.no_fit else fit_256(ctx, dev)
 
     cpu = None
@@ -198,10 +200,14 @@ def main():
             c0["cpu_baseline"] = cpu_cfg0(c0.pop("_state"), args)
         if blob is not None:
             blob["cpu_baseline"], blob["parity_vs_oracle"] = cpu_blobs(blob["_state"], args)
+        if shd is not None:
+            shd["cpu_baseline"], shd["parity_vs_reference"] = cpu_shard(shd["_state"], args)
     if c0 is not None:
         c0.pop("_state", None)
     if blob is not None:
         blob.pop("_state", None)
+    if shd is not None:
+        shd.pop("_state", None)
 
     out = {
         "metric": "ray samples/s + fps @1024³ 1080p; chunk decode GB/s; adjoint/fwd time ratio",
@@ -231,6 +237,7 @@ def main():
         "cfg0": c0,
         "fit": fit,
         "blobs": blob,
+        "shard": shd,
         "samples_per_frame": samples,
         "skips_per_frame": skips,
         "shaded_per_frame": shaded,
@@ -499,6 +506,117 @@ def blob_stage(ctx, args, dev, stream, n, nsheets, seed):
     idx = torch.linspace(0, nch - 1, k, device=dev).long()
     res["_state"] = {"data": data, "offs": offs, "lens": lens, "idx": idx, "out": out[idx].cpu().numpy()}
     return res
+
+
+def shard_stage(ctx, args, dev, stream, n, nsheets, seed):
+    """The zarr shard path at 1024^3 (one shard): the blob workload packed as
+    a shard (blobs, index, crc32c) in HBM, and r3d_hip_shard_decode_region of
+    the whole shard timed: index crc, index lookups, parse and the exact
+    decode straight into the region. Its output must equal the decode of the
+    levels."""
+    import numpy as np
+    import torch
+
+    from render3d_amd import workloads
+
+    if n != 1024:
+        return None
+    step, levels, headers, data, offs, lens = make_blobs(ctx, n, nsheets, seed, dev)
+    shard = workloads.pack_shard(ctx, data, offs, lens)
+    del data
+    flat = ctx.decode_chunks(headers=headers, step_tables=step, levels=levels)
+    del levels
+    want = flat.view(64, 64, 64, 16, 16, 16).permute(0, 3, 1, 4, 2, 5).reshape(n, n, n)
+    del flat
+    dims, region = (n, n, n), (0, 0, 0, n, n, n)
+    dst = torch.empty((n, n, n), dtype=torch.uint8, device=dev)
+    counts = torch.zeros(4, dtype=torch.int32, device=dev)
+    reps = max(3, args.steps)
+
+    def run():
+        ctx.shard_decode_region([(shard, 0, 0, 0)], dims, region, out=dst, counts=counts, status=False)
+
+    run()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    ev[0].record(stream)
+    for i in range(reps):
+        run()
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.mean([ev[i].elapsed_time(ev[i + 1]) for i in range(reps)]))
+    same = bool(torch.equal(dst, want))
+    del want
+    c = (counts.cpu().numpy() // (reps + 1)).tolist()
+    nbytes = int(shard.numel())
+    res = {"shard_bytes": nbytes, "chunks": 64 ** 3, "ms": ms, "chunks_per_s": 64 ** 3 / (ms / 1e3),
+           "compressed_GBps": nbytes / (ms / 1e3) / 1e9, "voxel_GBps": n ** 3 / (ms / 1e3) / 1e9,
+           "counts_decoded_missing_corrupt_wide": c, "matches_level_decode": same}
+    res["_state"] = {"shard": shard, "dst": dst}
+    return res
+
+
+def cpu_shard(state, args):
+    """The reference reader r3d_shard_decode_region (src/shard/shardio.c, built
+    from the reference's sources in oracle/_ref) on a 64 x 1024 x 1024 slab
+    of the same shard (16,384 chunks), on the host's cores (its own pthread
+    pool, at most 16), repeated for at least half of `cpu_seconds`; its
+    output against the GPU's slab. The oracle's restatement stands in when
+    the reference build is absent."""
+    import ctypes as C
+    import tempfile
+
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    shard = state["shard"].cpu().numpy()
+    gpu = state["dst"][:64].cpu().numpy()
+    cores = min(16, os.cpu_count() or 1)
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libshard_ref.so")
+    region = (0, 0, 0, 64, 1024, 1024)
+    out = np.zeros(64 * 1024 * 1024, np.uint8)
+    if os.path.exists(ref_path):
+        ref = C.CDLL(ref_path)
+
+        class Store(C.Structure):
+            _fields_ = [("dir", C.c_char * 512), ("nz", C.c_uint64), ("ny", C.c_uint64), ("nx", C.c_uint64)]
+
+        ref.r3d_shard_store_init.argtypes = [C.POINTER(Store), C.c_char_p, C.c_uint64, C.c_uint64, C.c_uint64]
+        ref.r3d_shard_decode_region.restype = C.c_int
+        ref.r3d_shard_decode_region.argtypes = ([C.POINTER(Store)] + [C.c_uint64] * 3 + [C.c_uint32] * 3 +
+                                                [C.c_void_p, C.c_int])
+        tmp = tempfile.mkdtemp(prefix="r3d_shard_")
+        path = os.path.join(tmp, "0_0_0.shard")
+        shard.tofile(path)
+        st = Store()
+        ref.r3d_shard_store_init(C.byref(st), tmp.encode(), 1024, 1024, 1024)
+
+        def run():
+            return ref.r3d_shard_decode_region(C.byref(st), *region, out.ctypes.data, cores)
+        kind = "reference"
+    else:
+        def run():
+            rc, d, _, _ = orc.shard_decode_region([(shard, 0, 0, 0)], (1024, 1024, 1024), region, nthreads=cores)
+            out[:] = d.reshape(-1)
+            return rc
+        kind = "port"
+    t = time.perf_counter()
+    reps = 0
+    while True:
+        rc = run()
+        reps += 1
+        if time.perf_counter() - t >= args.cpu_seconds / 2:
+            break
+    dt = time.perf_counter() - t
+    if kind == "reference":
+        os.remove(path)
+        os.rmdir(tmp)
+    cpu = {"value": reps * 16384 / dt, "unit": "chunks/s", "cores": cores, "kind": kind,
+           "voxel_GBps": reps * 64 * 1024 * 1024 / dt / 1e9,
+           "sample": f"{reps} x r3d_shard_decode_region of a 64x1024x1024 slab (16,384 chunks), {dt:.1f} s"}
+    parity = {"slab_equal": bool(np.array_equal(out.reshape(64, 1024, 1024), gpu)), "rc": int(rc)}
+    return cpu, parity
 
 
 def cpu_blobs(state, args):
