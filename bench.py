@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (graph) workload")
     ap.add_argument("--batched-chunks", type=int, default=32768)
     ap.add_argument("--no-cfg0", action="store_true", help="skip the 256^3 / 512^2 workload")
+    ap.add_argument("--no-deblock", action="store_true", help="skip the deblocking workload")
     ap.add_argument("--no-shard", action="store_true", help="skip the zarr shard region-decode workload")
     ap.add_argument("--no-blobs", action="store_true", help="skip the 3ddct blob (entropy stage) workload")
     ap.add_argument("--no-fit", action="store_true", help="skip the differentiable 256^3 fit (200 Adam steps)")
@@ -190,6 +191,9 @@ def main():
     c0 = None if args.no_cfg0 else cfg0(ctx, args, dev, stream)
     blob = None if args.no_blobs else blob_stage(ctx, args, dev, stream, n, nsheets, seed)
     shd = None if args.no_shard else shard_stage(ctx, args, dev, stream, n, nsheets, seed)
+    deb = None
+    if shd is not None and not args.no_deblock:
+        deb = deblock_stage(ctx, args, dev, stream, shd["_state"]["dst"], BLOB_QSTEP)
     fit = None if args.no_fit else fit_256(ctx, dev)
 
     cpu = None
@@ -202,12 +206,16 @@ def main():
             blob["cpu_baseline"], blob["parity_vs_oracle"] = cpu_blobs(blob["_state"], args)
         if shd is not None:
             shd["cpu_baseline"], shd["parity_vs_reference"] = cpu_shard(shd["_state"], args)
+        if deb is not None:
+            deb["cpu_baseline"], deb["parity_vs_reference"] = cpu_deblock(deb["_state"], args)
     if c0 is not None:
         c0.pop("_state", None)
     if blob is not None:
         blob.pop("_state", None)
     if shd is not None:
         shd.pop("_state", None)
+    if deb is not None:
+        deb.pop("_state", None)
 
     out = {
         "metric": "ray samples/s + fps @1024³ 1080p; chunk decode GB/s; adjoint/fwd time ratio",
@@ -238,6 +246,7 @@ def main():
         "fit": fit,
         "blobs": blob,
         "shard": shd,
+        "deblock": deb,
         "samples_per_frame": samples,
         "skips_per_frame": skips,
         "shaded_per_frame": shaded,
@@ -617,6 +626,84 @@ def cpu_shard(state, args):
            "sample": f"{reps} x r3d_shard_decode_region of a 64x1024x1024 slab (16,384 chunks), {dt:.1f} s"}
     parity = {"slab_equal": bool(np.array_equal(out.reshape(64, 1024, 1024), gpu)), "rc": int(rc)}
     return cpu, parity
+
+
+def deblock_stage(ctx, args, dev, stream, vol, qstep):
+    """Decode-side deblocking of a decoded 1024^3 region (the shard workload's
+    output, encoded at qstep): r3d_hip_deblock_u8 in place with step = qstep,
+    timed per call on a fresh copy."""
+    import numpy as np
+    import torch
+
+    reps = max(3, args.steps)
+    work = torch.empty_like(vol)
+    ms = []
+    for i in range(reps + 1):
+        work.copy_(vol)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ctx.deblock(work, qstep, 1.0)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if i:
+            ms.append(e0.elapsed_time(e1))
+    t = float(np.mean(ms))
+    changed = int((work != vol).sum())
+    nz, ny, nx = vol.shape
+    samples = nz * ny * ((nx - 2) // 16) + nz * ((ny - 2) // 16) * nx + ((nz - 2) // 16) * ny * nx
+    # algorithmic bytes: each seam sample reads its four voxels; the writes
+    # are the changed voxels
+    alg = 4 * samples + changed
+    res = {"ms": t, "voxels_per_s": vol.numel() / (t / 1e3), "seam_samples": samples, "voxels_changed": changed,
+           "roofline": {"kernel": "k_db_pass<0..2>", "bound": "hbm", "achieved": alg / (t / 1e3) / 1e9,
+                        "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": alg / (t / 1e3) / 1e9 / HBM_PEAK_GBPS,
+                        "traffic": None}}
+    sub = vol[:256, :256, :256].contiguous()
+    g = sub.clone()
+    ctx.deblock(g, qstep, 1.0)
+    res["_state"] = {"vol": sub.cpu().numpy(), "gpu": g.cpu().numpy(), "qstep": qstep}
+    return res
+
+
+def cpu_deblock(state, args):
+    """The reference dct3d_deblock_u8 (tools/3ddct/deblock.c, built from its
+    source in oracle/_ref; single-threaded, as the reference is) on a 256^3
+    sub-volume, repeated for at least a quarter of `cpu_seconds`; and the GPU
+    filter of the same sub-volume against it (the oracle stands in when the
+    reference build is absent)."""
+    import ctypes as C
+
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    sub, q = state["vol"], state["qstep"]
+    path = os.path.join(ROOT, "oracle", "_ref", "libdeblock_ref.so")
+    if os.path.exists(path):
+        ref = C.CDLL(path)
+        ref.dct3d_deblock_u8.argtypes = [C.c_void_p, C.c_void_p, C.c_float, C.c_float]
+        dims = (C.c_size_t * 3)(*sub.shape)
+
+        def run():
+            v = sub.copy()
+            ref.dct3d_deblock_u8(v.ctypes.data, dims, q, 1.0)
+            return v
+        kind = "reference"
+    else:
+        def run():
+            return orc.deblock(sub, q, 1.0)
+        kind = "port"
+    t = time.perf_counter()
+    reps = 0
+    while True:
+        want = run()
+        reps += 1
+        if time.perf_counter() - t >= args.cpu_seconds / 4:
+            break
+    dt = time.perf_counter() - t
+    cpu = {"value": reps * sub.size / dt, "unit": "voxels/s", "cores": 1, "kind": kind,
+           "sample": f"{reps} x dct3d_deblock_u8 of a 256^3 sub-volume, {dt:.1f} s"}
+    return cpu, {"subvolume_equal": bool(np.array_equal(state["gpu"], want))}
 
 
 def cpu_blobs(state, args):

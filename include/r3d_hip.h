@@ -478,6 +478,25 @@ int r3d_hip_shard_decode_region(r3d_hip_ctx *ctx, const r3d_shard_src *shards, u
  * (device). */
 int r3d_hip_crc32c(r3d_hip_ctx *ctx, const uint8_t *data, uint64_t n, uint32_t *crc, void *stream);
 
+/* ---- decode-side deblocking (tools/3ddct/deblock.h), u8 ---------------- */
+
+/* dct3d_deblock_u8 (deblock.c:132-203): in place over a dense device volume
+ * (dims z, y, x; x fastest), every internal plane at a multiple of 16 voxels,
+ * x then y then z seams. step is the encode quality (thresholds 2.0 / 1.2 x
+ * step x strength); step <= 0 derives them from the data range over a coarse
+ * subsample (deblock.c:144-152). strength <= 0 is a no-op. Bit-identical to
+ * the reference. */
+int r3d_hip_deblock_u8(r3d_hip_ctx *ctx, uint8_t *vol, const uint64_t dims[3], float step, float strength,
+                       void *stream);
+
+/* dct3d_deblock_chunk_u8 (deblock.c:69-130) for a batch: chunk c (device,
+ * 4096 bytes each, z-major, 16-byte aligned) is filtered against its
+ * pristine face neighbours neighbors[c * 6 + f] (device; f = xlo, xhi, ylo,
+ * yhi, zlo, zhi; indices into chunks, negative or >= nchunks for none), faces
+ * in that order, into out (must not overlap chunks). */
+int r3d_hip_deblock_chunks_u8(r3d_hip_ctx *ctx, uint32_t nchunks, const uint8_t *chunks, const int32_t *neighbors,
+                              float step, float strength, uint8_t *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

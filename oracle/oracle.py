@@ -81,6 +81,8 @@ def load():
     L.r3do_crc32c.argtypes = [u8p, C.c_size_t]
     L.r3do_shard_valid.restype = C.c_int
     L.r3do_shard_valid.argtypes = [u8p, C.c_uint64]
+    L.r3do_deblock_chunk_u8.argtypes = [u8p, VP, C.c_float, C.c_float, u8p]
+    L.r3do_deblock_u8.argtypes = [u8p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_float, C.c_float]
     L.r3do_shard_decode_region.restype = C.c_int
     L.r3do_shard_decode_region.argtypes = [VP, C.c_uint32] + [C.c_uint64] * 6 + [C.c_uint32] * 3 + [u8p, VP, VP,
                                                                                                   C.c_int]
@@ -335,6 +337,25 @@ def shard_pack(blobs):
     ib = idx.astype("<u8").view(np.uint8).reshape(-1)
     crc = np.array([crc32c(ib)], "<u4").view(np.uint8)
     return np.concatenate(parts + [ib, crc]) if parts else np.concatenate([ib, crc])
+
+
+def deblock(vol, step, strength=1.0):
+    """r3do_deblock_u8 on a copy of vol (uint8 [nz, ny, nx])."""
+    v = np.ascontiguousarray(vol, np.uint8).copy()
+    nz, ny, nx = v.shape
+    load().r3do_deblock_u8(v.reshape(-1), nz, ny, nx, step, strength)
+    return v
+
+
+def deblock_chunk(chunk, nbrs, step, strength=1.0):
+    """r3do_deblock_chunk_u8: nbrs = [xlo, xhi, ylo, yhi, zlo, zhi], each a
+    uint8[4096] or None."""
+    keep = [None if b is None else np.ascontiguousarray(b, np.uint8).reshape(-1) for b in nbrs]
+    arr = (C.c_void_p * 6)(*[None if b is None else b.ctypes.data for b in keep])
+    out = np.zeros(4096, np.uint8)
+    load().r3do_deblock_chunk_u8(np.ascontiguousarray(chunk, np.uint8).reshape(-1), C.cast(arr, C.c_void_p), step,
+                                 strength, out)
+    return out
 
 
 def raycast_diff(ovol, tf, grid, gdims, views, width, height, alpha_cutoff=0.98, dimg=None, window=None):

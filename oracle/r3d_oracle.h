@@ -202,6 +202,14 @@ int r3do_shard_decode_region(const r3do_shard *shards, uint32_t nshards, uint64_
                              uint64_t z0, uint64_t y0, uint64_t x0, uint32_t dz, uint32_t dy, uint32_t dx,
                              uint8_t *dst, uint32_t counts[4], int32_t *chunk_status, int nthreads);
 
+/* ---- decode-side deblocking, u8 (tools/3ddct/deblock.c) --------------- */
+
+/* dct3d_deblock_chunk_u8: nbr = xlo, xhi, ylo, yhi, zlo, zhi (NULL: none). */
+void r3do_deblock_chunk_u8(const uint8_t *chunk, const uint8_t *const nbr[6], float step, float strength,
+                           uint8_t *out);
+/* dct3d_deblock_u8 in place; dims z, y, x. */
+void r3do_deblock_u8(uint8_t *vol, uint64_t nz, uint64_t ny, uint64_t nx, float step, float strength);
+
 #ifdef __cplusplus
 }
 #endif

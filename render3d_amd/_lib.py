@@ -204,6 +204,8 @@ SIGNATURES = {
     "r3d_hip_shard_region_chunks": (C.c_uint64, [C.POINTER(ShardRegion)]),
     "r3d_hip_shard_decode_region": (C.c_int, [P, P, C.c_uint32, C.POINTER(ShardRegion), P, P, P, P]),
     "r3d_hip_crc32c": (C.c_int, [P, P, C.c_uint64, P, P]),
+    "r3d_hip_deblock_u8": (C.c_int, [P, P, P, C.c_float, C.c_float, P]),
+    "r3d_hip_deblock_chunks_u8": (C.c_int, [P, C.c_uint32, P, P, C.c_float, C.c_float, P, P]),
 }
 
 _lib = None

@@ -229,6 +229,25 @@ class Context:
                                                         _ptr(out), _ptr(st), _ptr(counts), _stream(stream)))
         return out, counts, st
 
+    # ---- deblocking ----------------------------------------------------------
+    def deblock(self, vol, step, strength=1.0, stream=None):
+        """r3d_hip_deblock_u8 in place on a uint8 device tensor [nz, ny, nx]."""
+        assert vol.dtype == torch.uint8 and vol.dim() == 3 and vol.is_contiguous()
+        dims = (C.c_uint64 * 3)(*vol.shape)
+        self.check(self.lib.r3d_hip_deblock_u8(self.h, _ptr(vol), dims, float(step), float(strength),
+                                               _stream(stream)))
+        return vol
+
+    def deblock_chunks(self, chunks, neighbors, step, strength=1.0, out=None, stream=None):
+        """r3d_hip_deblock_chunks_u8: chunks uint8 [n, 4096], neighbors int32
+        [n, 6] (xlo, xhi, ylo, yhi, zlo, zhi; -1 for none) -> uint8 [n, 4096]."""
+        n = int(chunks.shape[0])
+        if out is None:
+            out = torch.empty_like(chunks)
+        self.check(self.lib.r3d_hip_deblock_chunks_u8(self.h, n, _ptr(chunks), _ptr(neighbors), float(step),
+                                                      float(strength), _ptr(out), _stream(stream)))
+        return out
+
     # ---- workloads ----------------------------------------------------------
     def synth_sheets(self, nx, ny, nz, nsheets, seed, stream=None):
         out = torch.empty(nx * ny * nz, dtype=torch.uint8, device=f"cuda:{self.device}")

@@ -1,0 +1,275 @@
+// r3d_deblock.hip -- 3ddct's decode-side deblocking post-filter on gfx950
+// (tools/3ddct/deblock.c), u8.
+//
+// Volume form: the reference filters every internal 16-aligned plane in
+// place, x seams, then y seams, then z seams. Within one pass the seams are 16
+// voxels apart and each touches only the two layers on either side, so every
+// seam sample of a pass is independent: one lane per sample, one kernel per
+// pass (the passes stay in order). Chunk form: one workgroup per chunk
+// filters its six faces in order against pristine neighbours, with the
+// chunk's running values in LDS. The arithmetic is the reference's FP32
+// sequence (unfused, correctly rounded division, roundf, clamp), so the output
+// is bit-identical. The volume passes move about a quarter of the volume's
+// bytes each (the layers at the seams): HBM-bound byte work.
+#include "r3d_internal.h"
+
+namespace r3d {
+
+constexpr uint32_t DB_N = 16u;
+
+// deblock.c:26-46: the nudge of the p0|q0 step, 0 for a real edge.
+__device__ __forceinline__ float db_delta(float p1, float p0, float q0, float q1, float alpha, float beta, float *d1) {
+  *d1 = 0.0f;
+  const float d = q0 - p0, ad = __builtin_fabsf(d);
+  if (ad >= alpha || ad == 0.0f) return 0.0f;
+  if (__builtin_fabsf(p1 - p0) >= beta) return 0.0f;
+  if (__builtin_fabsf(q1 - q0) >= beta) return 0.0f;
+  float d0 = 0.40f * d;
+  const float w = 1.0f - ad / alpha;
+  d0 *= w;
+  *d1 = 0.4f * d0;
+  return d0;
+}
+
+__device__ __forceinline__ uint8_t db_u8(float v) {
+  return (uint8_t)roundf(v < 0.0f ? 0.0f : v > 255.0f ? 255.0f : v);
+}
+
+// deblock.c:48-67
+__host__ __device__ __forceinline__ void db_limits(float step, float range, float strength, float *alpha,
+                                                   float *beta) {
+  if (step > 0.0f) {
+    *alpha = 2.0f * step * strength;
+    *beta = 1.2f * step * strength;
+  } else {
+    *alpha = 0.06f * range * strength;
+    *beta = 0.04f * range * strength;
+  }
+}
+
+struct DbLim {
+  float alpha, beta;
+  uint32_t on;
+};
+
+// deblock.c:144-152: the data range over the samples i = 0, s, 2s, ... with
+// s = max(1, total / 4096), then the fallback thresholds.
+__global__ __launch_bounds__(1024) void k_db_range(const uint8_t *vol, uint64_t total, float strength, DbLim *lim) {
+  __shared__ uint32_t s_mn[16], s_mx[16];
+  const uint64_t stride = total / 4096u ? total / 4096u : 1u;
+  const uint64_t ns = (total + stride - 1u) / stride;
+  uint32_t mn = 255u, mx = 0u;
+  for (uint64_t k = threadIdx.x; k < ns; k += 1024u) {
+    const uint32_t v = vol[k * stride];
+    mn = v < mn ? v : mn;
+    mx = v > mx ? v : mx;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t a = __shfl_xor(mn, o), b = __shfl_xor(mx, o);
+    mn = a < mn ? a : mn;
+    mx = b > mx ? b : mx;
+  }
+  if ((threadIdx.x & 63u) == 0u) {
+    s_mn[threadIdx.x >> 6] = mn;
+    s_mx[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 16; w++) {
+      mn = s_mn[w] < mn ? s_mn[w] : mn;
+      mx = s_mx[w] > mx ? s_mx[w] : mx;
+    }
+    const float range = (float)mx - (float)mn;
+    DbLim l;
+    l.on = range > 0.0f ? 1u : 0u;
+    db_limits(0.0f, range, strength, &l.alpha, &l.beta);
+    *lim = l;
+  }
+}
+
+// One seam sample: v points at q0, s is the stride across the seam.
+__device__ __forceinline__ void db_filter4(uint8_t *v, int64_t s, float alpha, float beta) {
+  const float p1 = v[-2 * s], p0 = v[-s], q0 = v[0], q1 = v[s];
+  float d1;
+  const float d0 = db_delta(p1, p0, q0, q1, alpha, beta, &d1);
+  if (d0 == 0.0f) return;
+  v[-s] = db_u8(p0 + d0);
+  v[0] = db_u8(q0 - d0);
+  v[-2 * s] = db_u8(p1 + d1);
+  v[s] = db_u8(q1 - d1);
+}
+
+struct DbPass {
+  uint8_t *vol;
+  uint64_t nz, ny, nx;
+  uint64_t nseam;   // seams along the pass's axis (planes 16, 32, ... with plane + 1 < extent)
+  uint64_t total;   // seam samples in the pass
+  DbLim lim;        // used when flim is null
+  const DbLim *flim;
+};
+
+// AXIS 0: x seams, sample i = (row, j); 1: y seams, i = ((z, j), x); 2: z seams,
+// i = ((j, y), x). Consecutive lanes take consecutive x where they can.
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_db_pass(DbPass a) {
+  const DbLim l = a.flim ? *a.flim : a.lim;
+  if (!l.on) return;
+  const uint64_t step = (uint64_t)gridDim.x * 256u;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < a.total; i += step) {
+    if (AXIS == 0) {
+      const uint64_t row = i / a.nseam, j = i % a.nseam;
+      db_filter4(a.vol + row * a.nx + (j + 1u) * DB_N, 1, l.alpha, l.beta);
+    } else if (AXIS == 1) {
+      const uint64_t x = i % a.nx, r = i / a.nx, j = r % a.nseam, z = r / a.nseam;
+      db_filter4(a.vol + (z * a.ny + (j + 1u) * DB_N) * a.nx + x, (int64_t)a.nx, l.alpha, l.beta);
+    } else {
+      const uint64_t x = i % a.nx, r = i / a.nx, y = r % a.ny, j = r / a.ny;
+      db_filter4(a.vol + (((j + 1u) * DB_N) * a.ny + y) * a.nx + x, (int64_t)(a.ny * a.nx), l.alpha, l.beta);
+    }
+  }
+}
+
+__device__ __forceinline__ void block_minmax(const uint8_t *c, uint32_t t, uint32_t *s_red, float *mn, float *mx) {
+  uint32_t lo = 255u, hi = 0u;
+  const uint4 q = ((const uint4 *)c)[t];
+  const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const uint32_t v = (w[k >> 2] >> ((k & 3) * 8)) & 255u;
+    lo = v < lo ? v : lo;
+    hi = v > hi ? v : hi;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  if ((t & 63u) == 0u) s_red[t >> 6] = lo | (hi << 8);
+  __syncthreads();
+  lo = 255u;
+  hi = 0u;
+  for (int k = 0; k < 4; k++) {
+    lo = (s_red[k] & 255u) < lo ? (s_red[k] & 255u) : lo;
+    hi = (s_red[k] >> 8) > hi ? (s_red[k] >> 8) : hi;
+  }
+  __syncthreads();
+  *mn = (float)lo;
+  *mx = (float)hi;
+}
+
+// deblock.c:69-130, one workgroup per chunk. nbr[c * 6 + f] indexes chunks
+// (faces xlo, xhi, ylo, yhi, zlo, zhi); out of range means no neighbour.
+__global__ __launch_bounds__(256) void k_db_chunks(const uint8_t *chunks, const int32_t *nbr, uint32_t n, float step,
+                                                   float strength, uint8_t *out) {
+  __shared__ __align__(16) uint8_t s[4096];
+  __shared__ uint32_t s_red[4];
+  const uint32_t t = threadIdx.x;
+  const uint32_t a = t >> 4, b = t & 15u;
+  for (uint32_t c = blockIdx.x; c < n; c += gridDim.x) {
+    const uint8_t *ctr = chunks + (size_t)c * 4096u;
+    ((uint4 *)s)[t] = ((const uint4 *)ctr)[t];
+    __syncthreads();
+    if (strength > 0.0f) {
+      float cmn = 0.0f, cmx = 0.0f;
+      if (!(step > 0.0f)) block_minmax(ctr, t, s_red, &cmn, &cmx);
+      for (uint32_t f = 0; f < 6; f++) {
+        const int32_t j = nbr[(size_t)c * 6u + f];
+        if (j < 0 || (uint32_t)j >= n) continue;  // the whole workgroup agrees
+        const uint8_t *nb = chunks + (size_t)j * 4096u;
+        float range = 1.0f;
+        if (!(step > 0.0f)) {
+          float nmn, nmx;
+          block_minmax(nb, t, s_red, &nmn, &nmx);
+          range = (cmx > nmx ? cmx : nmx) - (cmn < nmn ? cmn : nmn);
+        }
+        if (range > 0.0f) {
+          float alpha, beta;
+          db_limits(step, range, strength, &alpha, &beta);
+          const bool lo = (f & 1u) == 0u;
+          const uint32_t axis = f >> 1;
+          const uint32_t c0d = lo ? 0u : 15u, c1d = lo ? 1u : 14u, n0d = lo ? 15u : 0u, n1d = lo ? 14u : 1u;
+          auto at = [&](uint32_t d) {  // voxel at depth d from the face, in-plane (a, b)
+            return axis == 0 ? (a * 16u + b) * 16u + d : axis == 1 ? (a * 16u + d) * 16u + b : (d * 16u + a) * 16u + b;
+          };
+          const uint32_t ci0 = at(c0d), ci1 = at(c1d);
+          const float n0 = nb[at(n0d)], n1 = nb[at(n1d)], c0 = s[ci0], c1 = s[ci1];
+          const float p1 = lo ? n1 : c1, p0 = lo ? n0 : c0, q0 = lo ? c0 : n0, q1 = lo ? c1 : n1;
+          float d1;
+          const float d0 = db_delta(p1, p0, q0, q1, alpha, beta, &d1);
+          if (d0 != 0.0f) {
+            s[ci0] = db_u8(lo ? c0 - d0 : c0 + d0);
+            s[ci1] = db_u8(lo ? c1 - d1 : c1 + d1);
+          }
+        }
+        __syncthreads();
+      }
+    }
+    ((uint4 *)(out + (size_t)c * 4096u))[t] = ((const uint4 *)s)[t];
+    __syncthreads();
+  }
+}
+
+}  // namespace r3d
+
+using namespace r3d;
+
+extern "C" int r3d_hip_deblock_u8(r3d_hip_ctx *ctx, uint8_t *vol, const uint64_t dims[3], float step, float strength,
+                                  void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!dims) return r3d_fail(ctx, R3D_ERR_INVALID, "deblock: NULL dims");
+  const uint64_t nz = dims[0], ny = dims[1], nx = dims[2];
+  if (nz == 0 || ny == 0 || nx == 0 || !(strength > 0.0f)) return R3D_OK;  // deblock.c:137-139: no-op
+  if (!vol) return r3d_fail(ctx, R3D_ERR_INVALID, "deblock: NULL volume");
+  if (ny > (1ull << 40) / nx || nz > (1ull << 40) / (ny * nx))
+    return r3d_fail(ctx, R3D_ERR_RANGE, "deblock: volume too large");
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = (hipStream_t)stream;
+  DbPass p{};
+  p.vol = vol;
+  p.nz = nz;
+  p.ny = ny;
+  p.nx = nx;
+  if (step > 0.0f) {
+    db_limits(step, 0.0f, strength, &p.lim.alpha, &p.lim.beta);
+    p.lim.on = 1u;
+    p.flim = nullptr;
+  } else {
+    void *ws = nullptr;
+    int rc = workspace(ctx, sizeof(DbLim), &ws);
+    if (rc) return rc;
+    p.flim = (const DbLim *)ws;
+    hipLaunchKernelGGL(k_db_range, dim3(1), dim3(1024), 0, st, vol, nz * ny * nx, strength, (DbLim *)ws);
+  }
+  const uint64_t seams[3] = {nx >= 2u ? (nx - 2u) / DB_N : 0u, ny >= 2u ? (ny - 2u) / DB_N : 0u,
+                             nz >= 2u ? (nz - 2u) / DB_N : 0u};
+  const uint64_t totals[3] = {nz * ny * seams[0], nz * seams[1] * nx, seams[2] * ny * nx};
+  void (*const kern[3])(DbPass) = {k_db_pass<0>, k_db_pass<1>, k_db_pass<2>};
+  for (int ax = 0; ax < 3; ax++) {
+    if (!totals[ax]) continue;
+    p.nseam = seams[ax];
+    p.total = totals[ax];
+    const uint64_t blocks = (p.total + 255u) / 256u;
+    const uint32_t grid = (uint32_t)(blocks < (uint64_t)ctx->num_cus * 32u ? blocks : (uint64_t)ctx->num_cus * 32u);
+    hipLaunchKernelGGL(kern[ax], dim3(grid ? grid : 1u), dim3(256), 0, st, p);
+  }
+  R3D_HIP_CHECK(ctx, hipGetLastError());
+  return R3D_OK;
+}
+
+extern "C" int r3d_hip_deblock_chunks_u8(r3d_hip_ctx *ctx, uint32_t nchunks, const uint8_t *chunks,
+                                         const int32_t *neighbors, float step, float strength, uint8_t *out,
+                                         void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (nchunks == 0) return R3D_OK;
+  if (!chunks || !neighbors || !out) return r3d_fail(ctx, R3D_ERR_INVALID, "deblock_chunks: NULL argument");
+  if (((uintptr_t)chunks & 15u) || ((uintptr_t)out & 15u))
+    return r3d_fail(ctx, R3D_ERR_INVALID, "deblock_chunks: chunks and out must be 16-byte aligned");
+  if ((const uint8_t *)out < chunks + (size_t)nchunks * 4096u && chunks < (const uint8_t *)out + (size_t)nchunks * 4096u)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "deblock_chunks: out overlaps chunks");
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const uint32_t cap = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 16u;
+  hipLaunchKernelGGL(k_db_chunks, dim3(nchunks < cap ? nchunks : cap), dim3(256), 0, (hipStream_t)stream, chunks,
+                     neighbors, nchunks, step, strength, out);
+  R3D_HIP_CHECK(ctx, hipGetLastError());
+  return R3D_OK;
+}
