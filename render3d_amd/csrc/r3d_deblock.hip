@@ -11,6 +11,8 @@
 // sequence (unfused, correctly rounded division, roundf, clamp), so the output
 // is bit-identical. The volume passes move about a quarter of the volume's
 // bytes each (the layers at the seams): HBM-bound byte work.
+#include <type_traits>
+
 #include "r3d_internal.h"
 
 namespace r3d {
@@ -87,45 +89,110 @@ __global__ __launch_bounds__(1024) void k_db_range(const uint8_t *vol, uint64_t 
   }
 }
 
-// One seam sample: v points at q0, s is the stride across the seam.
-__device__ __forceinline__ void db_filter4(uint8_t *v, int64_t s, float alpha, float beta) {
-  const float p1 = v[-2 * s], p0 = v[-s], q0 = v[0], q1 = v[s];
-  float d1;
-  const float d0 = db_delta(p1, p0, q0, q1, alpha, beta, &d1);
-  if (d0 == 0.0f) return;
-  v[-s] = db_u8(p0 + d0);
-  v[0] = db_u8(q0 - d0);
-  v[-2 * s] = db_u8(p1 + d1);
-  v[s] = db_u8(q1 - d1);
-}
-
 struct DbPass {
   uint8_t *vol;
   uint64_t nz, ny, nx;
   uint64_t nseam;   // seams along the pass's axis (planes 16, 32, ... with plane + 1 < extent)
-  uint64_t total;   // seam samples in the pass
+  uint64_t total;   // work items of the pass (seam samples, or groups of W of them)
   DbLim lim;        // used when flim is null
   const DbLim *flim;
 };
 
-// AXIS 0: x seams, sample i = (row, j); 1: y seams, i = ((z, j), x); 2: z seams,
-// i = ((j, y), x). Consecutive lanes take consecutive x where they can.
-template <int AXIS>
+// W consecutive seam samples along x: the four layers p1, p0 | q0, q1 as W-byte
+// vectors (v points at q0, s is the stride across the seam). A layer is
+// stored back only if one of its bytes changed, and no other lane of the pass
+// touches its bytes.
+template <int W>
+__device__ __forceinline__ void db_filter_vec(uint8_t *v, int64_t s, float alpha, float beta) {
+  using V = typename std::conditional<W == 16, uint4, typename std::conditional<W == 4, uint32_t, uint8_t>::type>::type;
+  V r[4];
+  r[0] = *(const V *)(v - 2 * s);
+  r[1] = *(const V *)(v - s);
+  r[2] = *(const V *)v;
+  r[3] = *(const V *)(v + s);
+  uint8_t b[4][W];
+#pragma unroll
+  for (int k = 0; k < 4; k++) __builtin_memcpy(b[k], &r[k], W);
+  bool ch[4] = {false, false, false, false};
+#pragma unroll
+  for (int x = 0; x < W; x++) {
+    const float p1 = b[0][x], p0 = b[1][x], q0 = b[2][x], q1 = b[3][x];
+    float d1;
+    const float d0 = db_delta(p1, p0, q0, q1, alpha, beta, &d1);
+    if (d0 != 0.0f) {
+      b[1][x] = db_u8(p0 + d0);
+      b[2][x] = db_u8(q0 - d0);
+      b[0][x] = db_u8(p1 + d1);
+      b[3][x] = db_u8(q1 - d1);
+      ch[0] = ch[1] = ch[2] = ch[3] = true;
+    }
+  }
+  if (ch[0]) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) __builtin_memcpy(&r[k], b[k], W);
+    *(V *)(v - 2 * s) = r[0];
+    *(V *)(v - s) = r[1];
+    *(V *)v = r[2];
+    *(V *)(v + s) = r[3];
+  }
+}
+
+// x seams, rows 4-byte aligned: bytes x-4..x-1 and x..x+3 of the row as two
+// aligned dwords (x = 16 (j + 1)).
+__device__ __forceinline__ void db_filter_x4(uint8_t *q, float alpha, float beta) {
+  const uint32_t lo = *(const uint32_t *)(q - 4), hi = *(const uint32_t *)q;
+  const float p1 = (float)((lo >> 16) & 255u), p0 = (float)(lo >> 24), q0 = (float)(hi & 255u),
+              q1 = (float)((hi >> 8) & 255u);
+  float d1;
+  const float d0 = db_delta(p1, p0, q0, q1, alpha, beta, &d1);
+  if (d0 == 0.0f) return;
+  const uint32_t np1 = db_u8(p1 + d1), np0 = db_u8(p0 + d0), nq0 = db_u8(q0 - d0), nq1 = db_u8(q1 - d1);
+  *(uint32_t *)(q - 4) = (lo & 0x0000FFFFu) | (np1 << 16) | (np0 << 24);
+  *(uint32_t *)q = (hi & 0xFFFF0000u) | nq0 | (nq1 << 8);
+}
+
+// AXIS 0: x seams, item i = (row, j); W = 4 uses aligned dwords, W = 1 bytes.
+// AXIS 1: y seams, item i = ((z, j), x / W). AXIS 2: z seams, i = ((j, y), x / W).
+// I is the index type: 32-bit divisions when the pass has fewer than 2^32
+// items (a 64-bit division by a runtime value costs far more than the filter).
+template <int AXIS, int W, typename I>
+__device__ __forceinline__ void db_item(const DbPass &a, I i, const DbLim &l) {
+  const I nseam = (I)a.nseam, nx = (I)a.nx, ny = (I)a.ny;
+  if (AXIS == 0) {
+    const I row = i / nseam, j = i - row * nseam;
+    uint8_t *q = a.vol + (uint64_t)row * a.nx + (uint64_t)(j + 1u) * DB_N;
+    if (W == 4)
+      db_filter_x4(q, l.alpha, l.beta);
+    else
+      db_filter_vec<1>(q, 1, l.alpha, l.beta);
+  } else {
+    const I nxw = nx / (I)W;
+    const I r = i / nxw, xw = i - r * nxw;
+    if (AXIS == 1) {
+      const I z = r / nseam, j = r - z * nseam;
+      db_filter_vec<W>(a.vol + ((uint64_t)z * a.ny + (uint64_t)(j + 1u) * DB_N) * a.nx + (uint64_t)xw * W,
+                       (int64_t)a.nx, l.alpha, l.beta);
+    } else {
+      const I j = r / ny, y = r - j * ny;
+      db_filter_vec<W>(a.vol + ((uint64_t)(j + 1u) * DB_N * a.ny + y) * a.nx + (uint64_t)xw * W,
+                       (int64_t)(a.ny * a.nx), l.alpha, l.beta);
+    }
+  }
+}
+
+template <int AXIS, int W>
 __global__ __launch_bounds__(256) void k_db_pass(DbPass a) {
   const DbLim l = a.flim ? *a.flim : a.lim;
   if (!l.on) return;
-  const uint64_t step = (uint64_t)gridDim.x * 256u;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < a.total; i += step) {
-    if (AXIS == 0) {
-      const uint64_t row = i / a.nseam, j = i % a.nseam;
-      db_filter4(a.vol + row * a.nx + (j + 1u) * DB_N, 1, l.alpha, l.beta);
-    } else if (AXIS == 1) {
-      const uint64_t x = i % a.nx, r = i / a.nx, j = r % a.nseam, z = r / a.nseam;
-      db_filter4(a.vol + (z * a.ny + (j + 1u) * DB_N) * a.nx + x, (int64_t)a.nx, l.alpha, l.beta);
-    } else {
-      const uint64_t x = i % a.nx, r = i / a.nx, y = r % a.ny, j = r / a.ny;
-      db_filter4(a.vol + (((j + 1u) * DB_N) * a.ny + y) * a.nx + x, (int64_t)(a.ny * a.nx), l.alpha, l.beta);
+  if (a.total <= 0xFFFFFFFFull && a.nx <= 0xFFFFFFFFull && a.ny <= 0xFFFFFFFFull) {
+    const uint32_t step = gridDim.x * 256u;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < (uint32_t)a.total; i += step) {
+      db_item<AXIS, W, uint32_t>(a, i, l);
+      if (i > 0xFFFFFFFFu - step) break;  // the next index would wrap
     }
+  } else {
+    const uint64_t step = (uint64_t)gridDim.x * 256u;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < a.total; i += step) db_item<AXIS, W, uint64_t>(a, i, l);
   }
 }
 
@@ -242,8 +309,15 @@ extern "C" int r3d_hip_deblock_u8(r3d_hip_ctx *ctx, uint8_t *vol, const uint64_t
   }
   const uint64_t seams[3] = {nx >= 2u ? (nx - 2u) / DB_N : 0u, ny >= 2u ? (ny - 2u) / DB_N : 0u,
                              nz >= 2u ? (nz - 2u) / DB_N : 0u};
-  const uint64_t totals[3] = {nz * ny * seams[0], nz * seams[1] * nx, seams[2] * ny * nx};
-  void (*const kern[3])(DbPass) = {k_db_pass<0>, k_db_pass<1>, k_db_pass<2>};
+  // vector width: rows aligned to 16 (or 4) bytes take 16 (or 4) samples per lane
+  const uintptr_t base = (uintptr_t)vol;
+  const int w = (nx % 16u == 0u && base % 16u == 0u) ? 16 : (nx % 4u == 0u && base % 4u == 0u) ? 4 : 1;
+  const uint64_t totals[3] = {nz * ny * seams[0], nz * seams[1] * (nx / (uint64_t)w),
+                              seams[2] * ny * (nx / (uint64_t)w)};
+  void (*kern[3])(DbPass);
+  kern[0] = w >= 4 ? k_db_pass<0, 4> : k_db_pass<0, 1>;
+  kern[1] = w == 16 ? k_db_pass<1, 16> : w == 4 ? k_db_pass<1, 4> : k_db_pass<1, 1>;
+  kern[2] = w == 16 ? k_db_pass<2, 16> : w == 4 ? k_db_pass<2, 4> : k_db_pass<2, 1>;
   for (int ax = 0; ax < 3; ax++) {
     if (!totals[ax]) continue;
     p.nseam = seams[ax];
