@@ -497,6 +497,44 @@ int r3d_hip_deblock_u8(r3d_hip_ctx *ctx, uint8_t *vol, const uint64_t dims[3], f
 int r3d_hip_deblock_chunks_u8(r3d_hip_ctx *ctx, uint32_t nchunks, const uint8_t *chunks, const int32_t *neighbors,
                               float step, float strength, uint8_t *out, void *stream);
 
+/* ---- bricks-LOD volumes (raycast.slang R3D_MODE 3 with BR_LOD_FLAG) ------
+ * A volume as a pyramid of 128^3 bricks, each resident in a slot of an atlas
+ * or absent, found through the reference's page table (vkbackend.c:4185-4205):
+ * page[0] = nlev, page[1..3] = base shape (x, y, z), per level l at 4 + 4l
+ * {offset of its entries, bx | by << 10 | bz << 20, scale = 2^l}; an entry is
+ * slot | max << 24 (max = the brick's largest voxel), 0xFFFFFFFF when absent.
+ * Slots are 128^3 u8, z-major, slot s at atlas + s * 128^3. A sample resolves
+ * the desired level, then +1, +2 and the coarsest (raycast.slang:198-279).
+ * Replaces the bricks-LOD dispatch of vkbackend.c for the render. */
+typedef struct r3d_hip_bricks r3d_hip_bricks;
+
+typedef struct r3d_bricks_desc {
+  uint32_t nx, ny, nz;          /* base shape in voxels */
+  uint32_t nlev;                /* levels, 1..15: level l has dims max(1, dim >> l) */
+  uint32_t abpa;                /* atlas slots per axis (1..255): abpa^3 slots */
+  uint32_t reserved0;
+} r3d_bricks_desc;
+
+int r3d_hip_bricks_create(r3d_hip_ctx *ctx, const r3d_bricks_desc *desc, r3d_hip_bricks **out);
+void r3d_hip_bricks_destroy(r3d_hip_bricks *b);
+/* Builds the pyramid of vol (device, nz x ny x nx u8, x fastest) with the
+ * rounded 2x box (tools/lodpack/main.c:216-227) and makes resident the bricks
+ * whose resident[] flag is set (host, one byte per page entry in page order
+ * after the 64-word header; NULL = all), in slots taken in page order; the
+ * others read 0xFFFFFFFF. Voxels of a slot past its level's edge are 0. */
+int r3d_hip_bricks_build(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const uint8_t *vol, const uint8_t *resident,
+                         void *stream);
+/* The device page table and atlas, for a streamer that manages residency
+ * itself (write slots, then entries, between frames). */
+int r3d_hip_bricks_tables(const r3d_hip_bricks *b, uint32_t **page, uint32_t *page_words, uint8_t **atlas,
+                          uint32_t *nslots);
+/* r3d_frame_views for bricks-LOD views (brick_mode = abpa | 0x20000; zoom
+ * crop, axis and oblique slabs as the reference; overlays off): as
+ * r3d_hip_raycast otherwise. */
+int r3d_hip_bricks_raycast(r3d_hip_ctx *ctx, const r3d_hip_bricks *b, const r3d_frame_params *views, uint32_t nviews,
+                           const r3d_hip_image *out, const r3d_raycast_opts *opts, r3d_hip_stats *stats,
+                           void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,10 @@ class Counters(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("skips", C.c_uint64), ("shaded", C.c_uint64), ("rays", C.c_uint64)]
 
 
+class BricksC(C.Structure):
+    _fields_ = [("page", C.c_void_p), ("npage", C.c_uint32), ("atlas", C.c_void_p), ("abpa", C.c_uint32)]
+
+
 class VolumeC(C.Structure):
     _fields_ = [("nlev", C.c_uint32), ("dims", (C.c_uint32 * 3) * 16), ("lev", C.c_void_p * 16),
                 ("odim", C.c_uint32 * 3), ("occ", C.c_void_p)]
@@ -55,6 +59,11 @@ def load():
     L.r3do_raycast.restype = C.c_int
     L.r3do_raycast.argtypes = [C.POINTER(VolumeC), f32p, VP, C.c_uint32, C.c_float, f32p, C.c_uint32,
                                C.c_uint32, VP, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Counters)]
+    L.r3do_raycast_bricks.restype = C.c_int
+    L.r3do_raycast_bricks.argtypes = [C.POINTER(BricksC), f32p, VP, C.c_uint32, C.c_float, f32p, C.c_uint32,
+                                      C.c_uint32, VP, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Counters)]
+    L.r3do_bricks_build.restype = C.c_uint32
+    L.r3do_bricks_build.argtypes = [C.POINTER(VolumeC), VP, u32p, C.c_uint32, u8p, C.c_uint32]
     for fn in ("r3do_log2f", "r3do_exp2f"):
         getattr(L, fn).restype = C.c_float
         getattr(L, fn).argtypes = [C.c_float]
@@ -228,6 +237,59 @@ def raycast(ovol, tf_f32, views, width, height, alpha_cutoff=0.98, row0=0, row_s
                         row_step, nthreads, C.byref(cnt))
     if rc != 0:
         raise ValueError("oracle raycast: unsupported view")
+    return (img.reshape(height, width, 4),
+            {"samples": cnt.samples, "skips": cnt.skips, "shaded": cnt.shaded, "rays": cnt.rays},
+            None if steps is None else steps.reshape(height, width))
+
+
+BRICK = 128
+SLOT_VOX = BRICK ** 3
+
+
+def brick_page_words(level_dims):
+    """Words of a page table: the 64-word header and one entry per 128^3 brick of every level."""
+    return 64 + sum(-(-x // BRICK) * -(-y // BRICK) * -(-z // BRICK) for x, y, z in level_dims)
+
+
+class OracleBricks:
+    """A page table + slot atlas (r3do_bricks), kept alive for the C struct's pointers."""
+
+    def __init__(self, page, atlas, abpa):
+        self.page = np.ascontiguousarray(page, np.uint32)
+        self.atlas = np.ascontiguousarray(atlas, np.uint8).reshape(-1)
+        self.abpa = abpa
+        assert self.atlas.size >= abpa ** 3 * SLOT_VOX
+        self.c = BricksC(self.page.ctypes.data, self.page.size, self.atlas.ctypes.data, abpa)
+
+
+def bricks_build(ovol, abpa, resident=None):
+    """r3do_bricks_build over an OracleVolume's levels -> OracleBricks (slots
+    taken in page order by the resident bricks)."""
+    words = brick_page_words([d for d, _ in ovol.levels])
+    page = np.zeros(words, np.uint32)
+    atlas = np.zeros(abpa ** 3 * SLOT_VOX, np.uint8)
+    res = None if resident is None else np.ascontiguousarray(resident, np.uint8)
+    n = load().r3do_bricks_build(C.byref(ovol.c), None if res is None else res.ctypes.data, page, words, atlas,
+                                 abpa ** 3)
+    if n != words:
+        raise ValueError("oracle bricks_build: more resident bricks than slots")
+    return OracleBricks(page, atlas, abpa)
+
+
+def raycast_bricks(ob, tf_f32, views, width, height, alpha_cutoff=0.98, row0=0, row_step=1, nthreads=8,
+                   px_steps=False):
+    """r3do_raycast_bricks -> (rgba float32 [h,w,4], counters dict, px_steps or None)."""
+    n = len(views)
+    arr = (type(views[0]) * n)(*views)
+    img = np.zeros(height * width * 4, np.float32)
+    steps = np.zeros(height * width, np.uint32) if px_steps else None
+    cnt = Counters()
+    rc = load().r3do_raycast_bricks(C.byref(ob.c), np.ascontiguousarray(tf_f32, np.float32).reshape(-1),
+                                    C.cast(arr, VP), n, alpha_cutoff, img, width, height,
+                                    None if steps is None else steps.ctypes.data, row0, row_step, nthreads,
+                                    C.byref(cnt))
+    if rc != 0:
+        raise ValueError("oracle raycast_bricks: unsupported view")
     return (img.reshape(height, width, 4),
             {"samples": cnt.samples, "skips": cnt.skips, "shaded": cnt.shaded, "rays": cnt.rays},
             None if steps is None else steps.reshape(height, width))

@@ -307,6 +307,394 @@ int r3do_raycast(const r3do_volume *vol, const float *tf, const r3do_frame_param
   return 0;
 }
 
+/* ---- bricks-LOD mode ----------------------------------------------------
+ * raycast.slang main() with R3D_MODE 3 and BR_LOD_FLAG (lines 142-279 and the
+ * bricks branches of 640-1039), overlays off. The page table is the
+ * reference's (vkbackend.c:4185-4205): page[0] = nlev, page[1..3] = base shape
+ * (x, y, z), per level at 4 + 4l {offset of its entries, packed brick dims
+ * bx | by << 10 | bz << 20, scale}; an entry is slot | max << 24, ~0 when the
+ * brick is absent. Slots are 128^3 u8, slot s at atlas + s * 2 MiB. The
+ * texture fetch is the pinned trilinear of the cube path inside the slot:
+ *   t = clamp(bl, 0.5, 127.5) - 0.5 per axis, i0 = (int)t, i1 = min(i0+1, 127),
+ *   lerp over x, then y, then z, times K255 (bricks_fetch, atlas lod 0). */
+#define BR_SLOT 128.0f
+#define BR_SLOT_VOX (128u * 128u * 128u)
+
+typedef struct {
+  int ok;
+  float maxv;
+  v3f bl;
+  uint32_t slot;
+  float grad_vox, world_brick;
+} bctx;
+
+typedef struct {
+  int valid, desired;
+  uint32_t scale, bc[3];
+  bctx c;
+} lodcache;
+
+static float brick_fetch(const r3do_bricks *B, uint32_t slot, v3f bl) {
+  const uint8_t *S = B->atlas + (size_t)slot * BR_SLOT_VOX;
+  const float tx = om_clamp(bl.x, 0.5f, 127.5f) - 0.5f;
+  const float ty = om_clamp(bl.y, 0.5f, 127.5f) - 0.5f;
+  const float tz = om_clamp(bl.z, 0.5f, 127.5f) - 0.5f;
+  const uint32_t x0 = (uint32_t)tx, y0 = (uint32_t)ty, z0 = (uint32_t)tz;
+  const uint32_t x1 = x0 + 1 < 128 ? x0 + 1 : 127, y1 = y0 + 1 < 128 ? y0 + 1 : 127, z1 = z0 + 1 < 128 ? z0 + 1 : 127;
+  const float fx = tx - (float)x0, fy = ty - (float)y0, fz = tz - (float)z0;
+#define BV(Z, Y, X) ((float)S[((size_t)(Z) * 128u + (Y)) * 128u + (X)])
+  const float c00 = om_lerp(BV(z0, y0, x0), BV(z0, y0, x1), fx);
+  const float c10 = om_lerp(BV(z0, y1, x0), BV(z0, y1, x1), fx);
+  const float c01 = om_lerp(BV(z1, y0, x0), BV(z1, y0, x1), fx);
+  const float c11 = om_lerp(BV(z1, y1, x0), BV(z1, y1, x1), fx);
+#undef BV
+  return om_lerp(om_lerp(c00, c10, fy), om_lerp(c01, c11, fy), fz) * K255;
+}
+
+/* shared by the locate and its cache: the sample position in base voxels */
+static v3f base_voxel(const uint32_t *page, v3f p, float *maxdim) {
+  const float sx = (float)page[1], sy = (float)page[2], sz = (float)page[3];
+  const float md = om_max(sx, om_max(sy, sz));
+  *maxdim = md;
+  return mk(om_clamp(p.x * md, 0.0f, om_max(sx - 0.001f, 0.0f)), om_clamp(p.y * md, 0.0f, om_max(sy - 0.001f, 0.0f)),
+            om_clamp(p.z * md, 0.0f, om_max(sz - 0.001f, 0.0f)));
+}
+
+/* bricks_lod_locate_ex (raycast.slang:198-247): desired level, +1, +2, then the coarsest */
+static bctx locate_ex(const r3do_bricks *B, v3f p, float lod, uint32_t *scale_out, uint32_t bc_out[3]) {
+  const uint32_t *page = B->page;
+  bctx c;
+  memset(&c, 0, sizeof c);
+  *scale_out = 1;
+  bc_out[0] = bc_out[1] = bc_out[2] = 0;
+  c.grad_vox = 1.0f;
+  const uint32_t nlev = page[0];
+  float maxdim;
+  const v3f bv = base_voxel(page, p, &maxdim);
+  c.world_brick = BR_SLOT * (float)(1u << (nlev - 1u)) / maxdim;
+  const int desired = (int)om_clamp(floorf(lod), 0.0f, (float)(nlev - 1u));
+  const int last = (int)nlev - 1;
+  for (uint32_t step = 0; step < 4u; step++) {
+    const int li = step < 3u ? desired + (int)step : last;
+    if (li < 0 || li > last || (step == 3u && li <= desired + 2)) continue;
+    const uint32_t h = 4u + (uint32_t)li * 4u;
+    const uint32_t off = page[h], packed = page[h + 1u], scale = page[h + 2u];
+    const uint32_t bdx = packed & 1023u, bdy = (packed >> 10) & 1023u, bdz = packed >> 20;
+    const v3f lv = mk(bv.x / (float)scale, bv.y / (float)scale, bv.z / (float)scale);
+    uint32_t bcx = (uint32_t)lv.x >> 7, bcy = (uint32_t)lv.y >> 7, bcz = (uint32_t)lv.z >> 7;
+    if (bcx > bdx - 1u) bcx = bdx - 1u;
+    if (bcy > bdy - 1u) bcy = bdy - 1u;
+    if (bcz > bdz - 1u) bcz = bdz - 1u;
+    const uint32_t e = page[off + (bcz * bdy + bcy) * bdx + bcx];
+    if (e == 0xFFFFFFFFu) continue;
+    c.ok = 1;
+    c.maxv = (float)(e >> 24) / 255.0f;
+    c.slot = e & 0xFFFFFFu;
+    c.bl = mk(lv.x - (float)(bcx << 7), lv.y - (float)(bcy << 7), lv.z - (float)(bcz << 7));
+    c.grad_vox = om_exp2f(lod) / (float)scale;
+    c.world_brick = BR_SLOT * (float)scale / maxdim;
+    *scale_out = scale;
+    bc_out[0] = bcx;
+    bc_out[1] = bcy;
+    bc_out[2] = bcz;
+    return c;
+  }
+  return c;
+}
+
+/* bricks_lod_locate_cached (raycast.slang:255-279) */
+static bctx locate_cached(const r3do_bricks *B, v3f p, float lod, lodcache *lc) {
+  const uint32_t nlev = B->page[0];
+  const int desired = (int)om_clamp(floorf(lod), 0.0f, (float)(nlev - 1u));
+  if (lc->valid && lc->desired == desired) {
+    float maxdim;
+    const v3f bv = base_voxel(B->page, p, &maxdim);
+    const float sc = (float)lc->scale;
+    const v3f lv = mk(bv.x / sc, bv.y / sc, bv.z / sc);
+    const uint32_t bcx = (uint32_t)lv.x >> 7, bcy = (uint32_t)lv.y >> 7, bcz = (uint32_t)lv.z >> 7;
+    if (bcx == lc->bc[0] && bcy == lc->bc[1] && bcz == lc->bc[2]) {
+      bctx c = lc->c;
+      c.bl = mk(lv.x - (float)(bcx << 7), lv.y - (float)(bcy << 7), lv.z - (float)(bcz << 7));
+      return c;
+    }
+  }
+  uint32_t scale, bc[3];
+  bctx c = locate_ex(B, p, lod, &scale, bc);
+  lc->valid = c.ok;
+  lc->desired = desired;
+  lc->scale = scale;
+  lc->bc[0] = bc[0];
+  lc->bc[1] = bc[1];
+  lc->bc[2] = bc[2];
+  lc->c = c;
+  return c;
+}
+
+static void march_bricks(const r3do_bricks *B, const float *tf, const r3do_frame_params *pc, float cutoff,
+                         uint32_t px, uint32_t py, float out[4], uint32_t *nsteps, uint64_t *nskip,
+                         uint64_t *nshade) {
+  const uint32_t *page = B->page;
+  const v3f fwd = mk(pc->cam_forward[0], pc->cam_forward[1], pc->cam_forward[2]);
+  const v3f right = mk(pc->cam_right[0], pc->cam_right[1], pc->cam_right[2]);
+  const v3f up = mk(pc->cam_up[0], pc->cam_up[1], pc->cam_up[2]);
+  const int ortho = (pc->view_flags & 1u) != 0u;
+  float ndx, ndy;
+  pix_ndc(pc, px, py, &ndx, &ndy);
+  v3f dir = ortho ? vnorm(fwd) : vnorm(vadd(vadd(fwd, vscale(right, ndx)), vscale(up, ndy)));
+  *nsteps = 0;
+  if (pc->mode == MODE_RAYDIR) {
+    out[0] = dir.x * 0.5f + 0.5f; out[1] = dir.y * 0.5f + 0.5f; out[2] = dir.z * 0.5f + 0.5f;
+    out[3] = 1.0f;
+    return;
+  }
+  /* raycast.slang:655-660, 305-309 */
+  const float sx = (float)page[1], sy = (float)page[2], sz = (float)page[3];
+  const float md = (float)(page[1] > page[2] ? (page[1] > page[3] ? page[1] : page[3]) : (page[2] > page[3] ? page[2] : page[3]));
+  const float voxel = 1.0f / md;
+  const float max_lod = (float)(page[0] - 1u);
+  const float emax = om_max(sx, om_max(sy, sz));
+  const v3f ext = mk(sx / emax, sy / emax, sz / emax);
+  v3f org = mk(pc->cam_origin[0], pc->cam_origin[1], pc->cam_origin[2]);
+  if (ortho) org = vadd(vadd(org, vscale(right, ndx)), vscale(up, ndy));
+  const v3f ro = to_volume_point(pc, org, vscale(ext, 0.5f));
+  dir = to_volume_dir(pc, dir);
+  /* raycast.slang:690-718: zoom-crop box or slab range */
+  v3f lo = mk(0.0f, 0.0f, 0.0f), hi = ext;
+  float t_lo = 0.0f, t_hi = 1e30f;
+  if ((pc->view_flags & 16u) != 0u) {
+    const v3f clo = mk(pc->slab_x0 / md, pc->slab_y0 / md, pc->slab_z0 / md);
+    const v3f chi = vadd(clo, mk(pc->slab_px / md, pc->slab_py / md, pc->slab_nx / md));
+    lo = mk(om_clamp(clo.x, 0.0f, ext.x), om_clamp(clo.y, 0.0f, ext.y), om_clamp(clo.z, 0.0f, ext.z));
+    hi = mk(om_clamp(chi.x, lo.x, ext.x), om_clamp(chi.y, lo.y, ext.y), om_clamp(chi.z, lo.z, ext.z));
+  } else if (pc->slab_depth > 0u) {
+    if ((pc->view_flags & 4u) != 0u) {
+      t_lo = pc->slab_z0 / md;
+      t_hi = (pc->slab_z0 + (float)pc->slab_depth) / md;
+    } else {
+      const uint32_t axc = (pc->view_flags >> 8) & 3u;
+      const int a = axc == 1u ? 0 : (axc == 2u ? 1 : 2);
+      float *plo = a == 0 ? &lo.x : a == 1 ? &lo.y : &lo.z, *phi = a == 0 ? &hi.x : a == 1 ? &hi.y : &hi.z;
+      const float ea = a == 0 ? ext.x : a == 1 ? ext.y : ext.z;
+      *plo = om_clamp(pc->slab_z0 / md, 0.0f, ea);
+      *phi = om_clamp((pc->slab_z0 + (float)pc->slab_depth) / md, *plo, ea);
+    }
+  }
+  v3f inv = mk(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+  v3f t0 = mk((lo.x - ro.x) * inv.x, (lo.y - ro.y) * inv.y, (lo.z - ro.z) * inv.z);
+  v3f t1 = mk((hi.x - ro.x) * inv.x, (hi.y - ro.y) * inv.y, (hi.z - ro.z) * inv.z);
+  float hx = om_max(om_max(om_min(t0.x, t1.x), om_min(t0.y, t1.y)), om_min(t0.z, t1.z));
+  float hy = om_min(om_min(om_max(t0.x, t1.x), om_max(t0.y, t1.y)), om_max(t0.z, t1.z));
+  float tn = om_max(hx, t_lo), tfar = om_min(hy, t_hi);
+  if (tn >= tfar) {
+    out[0] = 0.02f; out[1] = 0.02f; out[2] = 0.03f; out[3] = 1.0f;
+    return;
+  }
+  const float cone = 2.0f * vlen(up) / (float)pc->viewport[1];
+  const float dt = pc->step_voxels * voxel;
+  float t = tn + om_hash12(px, py, pc->frame_index) * dt;
+  float mip_max = 0.0f, first_hit = -1.0f;
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  uint32_t steps = 0;
+  const int composite = pc->mode == MODE_FULL || pc->mode == MODE_FLAT;
+  const int shaded = pc->mode == MODE_FULL;
+  const uint32_t budget = shaded ? 192u : 256u; /* raycast.slang:743 */
+  const float span_step = om_max((tfar - tn) / ((float)budget * dt), 1.0f);
+  const float occ_gate = om_max(pc->skip_gate, 0.5f / 255.0f);
+  const float bgate = om_max(occ_gate, 4.5f / 255.0f);
+  const float opac_k = pc->density * pc->step_voxels;
+  const v3f lvec = vscale(dir, -1.0f);
+  const v3f hvec = vnorm(vsub(lvec, dir));
+  const v3f key = mk(0x1.a20bd8p-2f, 0x1.a20bd8p-1f, -0x1.a20bd8p-2f);
+  const v3f sgn = mk(dir.x > 0.0f ? 1.0f : 0.0f, dir.y > 0.0f ? 1.0f : 0.0f, dir.z > 0.0f ? 1.0f : 0.0f);
+  lodcache lc;
+  memset(&lc, 0, sizeof lc);
+  lc.desired = -1;
+  lc.scale = 1u;
+  while (t < tfar && steps < budget) {
+    steps++;
+    const float foot = ortho ? cone : t * cone;
+    const float lod = om_clamp(om_log2f(om_max(foot / voxel, 1.0f)) + pc->lod_bias, 0.0f, max_lod);
+    const float lod_step = om_max(om_exp2f(lod), span_step);
+    const v3f p = vadd(ro, vscale(dir, t));
+    const bctx bc = locate_cached(B, p, lod, &lc);
+    float v = bc.ok ? brick_fetch(B, bc.slot, bc.bl) : 0.0f;
+    const int cut = v < pc->threshold;
+    if (cut) v = 0.0f;
+    /* raycast.slang:834-868: whole-brick skips */
+    float bsz = 0.0f;
+    if (pc->mode == MODE_MIP && bc.ok && bc.maxv <= mip_max)
+      bsz = bc.world_brick;
+    else if (v < bgate && bc.maxv < bgate)
+      bsz = bc.world_brick;
+    if (bsz > 0.0f) {
+      v3f nextp = mk((floorf(p.x / bsz) + sgn.x) * bsz, (floorf(p.y / bsz) + sgn.y) * bsz,
+                     (floorf(p.z / bsz) + sgn.z) * bsz);
+      float tx = fabsf(dir.x) > 1e-8f ? (nextp.x - ro.x) / dir.x : 1e30f;
+      float ty = fabsf(dir.y) > 1e-8f ? (nextp.y - ro.y) / dir.y : 1e30f;
+      float tz = fabsf(dir.z) > 1e-8f ? (nextp.z - ro.z) / dir.z : 1e30f;
+      t = om_max(om_min(om_min(tx, ty), tz), t) + bsz * 0.01f;
+      (*nskip)++;
+      continue;
+    }
+    if (v > mip_max) mip_max = v;
+    if (pc->mode == MODE_MIP && mip_max >= 0.999f) break;
+    if (first_hit < 0.0f && v > 0.05f) first_hit = t;
+    if (composite && !cut) {
+      float s[4];
+      tf_sample(tf, v, s);
+      float a = 1.0f - om_pow_unit(1.0f - s[3], opac_k * lod_step);
+      if (a > 0.003f) {
+        float col[3] = {s[0], s[1], s[2]};
+        if (shaded) {
+          (*nshade)++;
+          const float ev = bc.grad_vox; /* taps reuse the located brick (raycast.slang:964-973) */
+          const v3f b = bc.bl;
+          float gx = brick_fetch(B, bc.slot, mk(b.x + ev, b.y, b.z)) - brick_fetch(B, bc.slot, mk(b.x + -ev, b.y, b.z));
+          float gy = brick_fetch(B, bc.slot, mk(b.x, b.y + ev, b.z)) - brick_fetch(B, bc.slot, mk(b.x, b.y + -ev, b.z));
+          float gz = brick_fetch(B, bc.slot, mk(b.x, b.y, b.z + ev)) - brick_fetch(B, bc.slot, mk(b.x, b.y, b.z + -ev));
+          v3f g = mk(gx, gy, gz);
+          float gm = vlen(g);
+          if (gm > 1e-4f) {
+            v3f n = mk(g.x / gm, g.y / gm, g.z / gm);
+            float diff = 0.7f * fabsf(vdot(n, lvec)) + 0.3f * fabsf(vdot(n, key));
+            float spec = om_pow24(fabsf(vdot(n, hvec))) * 0.3f;
+            float lit = 0.25f + 0.75f * diff;
+            float w = om_sat(gm * 6.0f);
+            float m = om_lerp(1.0f, lit, w);
+            float sw = spec * w;
+            col[0] = col[0] * m + sw;
+            col[1] = col[1] * m + sw;
+            col[2] = col[2] * m + sw;
+          }
+        }
+        float wa = (1.0f - acc[3]) * a;
+        acc[0] = acc[0] + wa * col[0];
+        acc[1] = acc[1] + wa * col[1];
+        acc[2] = acc[2] + wa * col[2];
+        acc[3] = acc[3] + wa;
+        if (acc[3] >= cutoff) break;
+      }
+    }
+    t += dt * lod_step;
+  }
+  *nsteps = steps;
+  if (pc->mode == MODE_HEATMAP) {
+    float x = (float)steps / 1024.0f;
+    out[0] = x; out[1] = 1.0f - fabsf(x - 0.5f) * 2.0f; out[2] = 1.0f - x;
+  } else if (pc->mode == MODE_DEPTH) {
+    float d = first_hit < 0.0f ? 0.0f : 1.0f - om_sat(first_hit / 2.0f);
+    out[0] = d; out[1] = d; out[2] = d;
+  } else if (composite) {
+    float ta = 1.0f - acc[3];
+    out[0] = acc[0] + ta * 0.02f; out[1] = acc[1] + ta * 0.02f; out[2] = acc[2] + ta * 0.03f;
+  } else {
+    float m = mip_max * pc->density;
+    out[0] = m; out[1] = m; out[2] = m;
+  }
+  out[3] = 1.0f;
+}
+
+int r3do_raycast_bricks(const r3do_bricks *B, const float *tf, const r3do_frame_params *views, uint32_t nviews,
+                        float alpha_cutoff, float *img, uint32_t img_w, uint32_t img_h, uint32_t *px_steps,
+                        uint32_t row0, uint32_t row_step, int nthreads, r3do_counters *cnt) {
+  for (uint32_t v = 0; v < nviews; v++) {
+    const r3do_frame_params *pc = &views[v];
+    if (!(pc->brick_mode & 0x20000u) || (pc->brick_mode & 0xffu) != B->abpa) return -1;
+    if (pc->slab_grid || pc->clip_valid || (pc->view_flags & 2u) || pc->mode > MODE_FLAT) return -1;
+    if (pc->viewport[0] == 0 || pc->viewport[1] == 0 || !(pc->step_voxels > 0.0f)) return -1;
+  }
+  if (row_step == 0) row_step = 1;
+  if (nthreads < 1) nthreads = 1;
+  uint64_t tot_s = 0, tot_k = 0, tot_h = 0, tot_r = 0;
+  for (uint32_t v = 0; v < nviews; v++) {
+    const r3do_frame_params *pc = &views[v];
+    const int half = (pc->view_flags & 32u) != 0u;
+    const uint32_t ox = pc->view_org & 0xffffu, oy = pc->view_org >> 16;
+    const uint32_t W = pc->viewport[0], H = pc->viewport[1];
+    const uint32_t TW = half ? (W + 1) / 2 : W, TH = half ? (H + 1) / 2 : H;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(+ : tot_s, tot_k, tot_h, tot_r)
+    for (int64_t ty = 0; ty < (int64_t)TH; ty++) {
+      if ((uint32_t)ty % row_step != row0 % row_step) continue;
+      for (uint32_t tx = 0; tx < TW; tx++) {
+        uint32_t lx = half ? tx * 2 : tx, ly = half ? (uint32_t)ty * 2 : (uint32_t)ty;
+        if (lx >= W || ly >= H) continue;
+        float c[4];
+        uint32_t ns = 0;
+        uint64_t nk = 0, nh = 0;
+        march_bricks(B, tf, pc, alpha_cutoff, lx, ly, c, &ns, &nk, &nh);
+        tot_s += ns; tot_k += nk; tot_h += nh; tot_r += 1;
+        for (uint32_t dy = 0; dy < (half ? 2u : 1u); dy++)
+          for (uint32_t dx = 0; dx < (half ? 2u : 1u); dx++) {
+            uint32_t qx = lx + dx, qy = ly + dy;
+            if (qx >= W || qy >= H) continue;
+            uint32_t gx = ox + qx, gy = oy + qy;
+            if (gx >= img_w || gy >= img_h) continue;
+            memcpy(img + ((size_t)gy * img_w + gx) * 4, c, sizeof c);
+            if (px_steps) px_steps[(size_t)gy * img_w + gx] = ns;
+          }
+      }
+    }
+  }
+  if (cnt) {
+    cnt->samples = tot_s; cnt->skips = tot_k; cnt->shaded = tot_h; cnt->rays = tot_r;
+  }
+  return 0;
+}
+
+/* The page table and atlas of a pyramid (levels as linear [z][y][x] arrays,
+ * level l of scale 2^l): per level ceil(dim / 128) bricks per axis, entries
+ * level by level after the 64-word header; a brick is resident unless
+ * resident[] (one flag per entry, in page order, or NULL) says 0. Resident
+ * bricks take slots in page order; a slot holds the brick's voxels and zeros
+ * past the level's edge; its entry is slot | max << 24. Returns the number of
+ * page words, or 0 when page_cap / nslots are too small. */
+uint32_t r3do_bricks_build(const r3do_volume *V, const uint8_t *resident, uint32_t *page, uint32_t page_cap,
+                           uint8_t *atlas, uint32_t nslots) {
+  uint32_t words = 64u, offs[16];
+  for (uint32_t l = 0; l < V->nlev; l++) {
+    offs[l] = words;
+    const uint32_t bx = (V->dims[l][0] + 127u) / 128u, by = (V->dims[l][1] + 127u) / 128u,
+                   bz = (V->dims[l][2] + 127u) / 128u;
+    words += bx * by * bz;
+  }
+  if (words > page_cap) return 0;
+  memset(page, 0, 64u * sizeof *page);
+  page[0] = V->nlev;
+  page[1] = V->dims[0][0];
+  page[2] = V->dims[0][1];
+  page[3] = V->dims[0][2];
+  uint32_t slot = 0;
+  for (uint32_t l = 0; l < V->nlev; l++) {
+    const uint32_t nx = V->dims[l][0], ny = V->dims[l][1], nz = V->dims[l][2];
+    const uint32_t bx = (nx + 127u) / 128u, by = (ny + 127u) / 128u, bz = (nz + 127u) / 128u;
+    page[4u + l * 4u] = offs[l];
+    page[5u + l * 4u] = bx | (by << 10) | (bz << 20);
+    page[6u + l * 4u] = 1u << l;
+    for (uint32_t k = 0; k < bx * by * bz; k++) {
+      const uint32_t w = offs[l] + k;
+      if (resident && !resident[w - 64u]) {
+        page[w] = 0xFFFFFFFFu;
+        continue;
+      }
+      if (slot >= nslots) return 0;
+      const uint32_t cx = k % bx, cy = (k / bx) % by, cz = k / (bx * by);
+      uint8_t *S = atlas + (size_t)slot * BR_SLOT_VOX;
+      uint32_t mx = 0;
+      for (uint32_t z = 0; z < 128u; z++)
+        for (uint32_t y = 0; y < 128u; y++)
+          for (uint32_t x = 0; x < 128u; x++) {
+            const uint32_t gx = cx * 128u + x, gy = cy * 128u + y, gz = cz * 128u + z;
+            const uint8_t v = gx < nx && gy < ny && gz < nz ? V->lev[l][((size_t)gz * ny + gy) * nx + gx] : 0u;
+            S[((size_t)z * 128u + y) * 128u + x] = v;
+            if (v > mx) mx = v;
+          }
+      page[w] = slot | (mx << 24);
+      slot++;
+    }
+  }
+  return words;
+}
+
 /* ---- pyramid and occupancy --------------------------------------------- */
 
 void r3do_downsample(const uint8_t *src, uint32_t nx, uint32_t ny, uint32_t nz, uint8_t *dst) {

@@ -138,6 +138,22 @@ int r3do_raycast(const r3do_volume *vol, const float *tf, const r3do_frame_param
                  uint32_t *px_steps, uint32_t row0, uint32_t row_step, int nthreads,
                  r3do_counters *cnt);
 
+/* Bricks-LOD mode (raycast.slang R3D_MODE 3 with the LOD flag): the
+ * reference's page table (64-word header, per-level entries) and 128^3 slots,
+ * slot s at atlas + s * 2 MiB. Views must carry brick_mode = abpa | 0x20000. */
+typedef struct r3do_bricks {
+  const uint32_t *page;
+  uint32_t npage;
+  const uint8_t *atlas;
+  uint32_t abpa;
+} r3do_bricks;
+int r3do_raycast_bricks(const r3do_bricks *B, const float *tf, const r3do_frame_params *views, uint32_t nviews,
+                        float alpha_cutoff, float *img, uint32_t img_w, uint32_t img_h, uint32_t *px_steps,
+                        uint32_t row0, uint32_t row_step, int nthreads, r3do_counters *cnt);
+/* Page table + atlas of a pyramid; see oracle_raycast.c. Returns page words or 0. */
+uint32_t r3do_bricks_build(const r3do_volume *V, const uint8_t *resident, uint32_t *page, uint32_t page_cap,
+                           uint8_t *atlas, uint32_t nslots);
+
 /* Differentiable raycast in FP64 (parameter-dependent parts): renders
  * img (double RGBA, alpha = accumulated opacity) and, with dimg != NULL,
  * adds dL/dtf (256x4) and dL/dgrid. FULL or FLAT views only. */

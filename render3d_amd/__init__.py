@@ -5,9 +5,9 @@ from . import scene
 from ._lib import (FrameParams, R3DError, MODE_FULL, MODE_MIP, MODE_DEPTH, MODE_HEATMAP, MODE_RAYDIR,
                    MODE_FLAT, VIEW_ORTHO, VIEW_HALF, R3D_DECODE_EXACT_F32, R3D_DECODE_MFMA_F32, R3D_DECODE_MFMA_F16,
                    R3D_DECODE_MFMA_F16X3, R3D_DECODE_F32_OUT, load as load_library, LIB_PATH)
-from .api import Context, ShardStore, Volume, headers_tensor, headers_from_tensor
+from .api import Bricks, Context, ShardStore, Volume, headers_tensor, headers_from_tensor
 
-__all__ = ["scene", "FrameParams", "R3DError", "Context", "ShardStore", "Volume", "headers_tensor", "headers_from_tensor",
+__all__ = ["scene", "Bricks", "FrameParams", "R3DError", "Context", "ShardStore", "Volume", "headers_tensor", "headers_from_tensor",
            "load_library", "LIB_PATH", "MODE_FULL", "MODE_MIP", "MODE_DEPTH", "MODE_HEATMAP", "MODE_RAYDIR",
            "MODE_FLAT", "VIEW_ORTHO", "VIEW_HALF", "R3D_DECODE_EXACT_F32", "R3D_DECODE_MFMA_F32", "R3D_DECODE_MFMA_F16",
            "R3D_DECODE_MFMA_F16X3", "R3D_DECODE_F32_OUT"]

@@ -146,6 +146,13 @@ class ShardRegion(C.Structure):
                 ("reserved0", C.c_uint32)]
 
 
+class BricksDesc(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32), ("nlev", C.c_uint32),
+                ("abpa", C.c_uint32), ("reserved0", C.c_uint32)]
+
+
+BR_LOD_FLAG = 0x20000
+
 SHARD_GRID = 64
 SHARD_INDEX_BYTES = SHARD_GRID ** 3 * 16 + 4
 
@@ -205,6 +212,12 @@ SIGNATURES = {
     "r3d_hip_shard_decode_region": (C.c_int, [P, P, C.c_uint32, C.POINTER(ShardRegion), P, P, P, P]),
     "r3d_hip_crc32c": (C.c_int, [P, P, C.c_uint64, P, P]),
     "r3d_hip_deblock_u8": (C.c_int, [P, P, P, C.c_float, C.c_float, P]),
+    "r3d_hip_bricks_create": (C.c_int, [P, C.POINTER(BricksDesc), C.POINTER(P)]),
+    "r3d_hip_bricks_destroy": (None, [P]),
+    "r3d_hip_bricks_build": (C.c_int, [P, P, P, P, P]),
+    "r3d_hip_bricks_tables": (C.c_int, [P, C.POINTER(P), C.POINTER(C.c_uint32), C.POINTER(P), C.POINTER(C.c_uint32)]),
+    "r3d_hip_bricks_raycast": (C.c_int, [P, P, C.POINTER(FrameParams), C.c_uint32, C.POINTER(Image),
+                                         C.POINTER(RaycastOpts), C.POINTER(Stats), P]),
     "r3d_hip_deblock_chunks_u8": (C.c_int, [P, C.c_uint32, P, P, C.c_float, C.c_float, P, P]),
 }
 

@@ -277,6 +277,11 @@ class Context:
     def raycast(self, vol, views, width, height, *, alpha_cutoff=0.98, rgba=True, rgba8=False, tf=None,
                 counters=None, px_steps=False, stats=False, out=None, stream=None):
         """Render views into a [height, width, 4] float32 image (and/or RGBA8)."""
+        return self._raycast(self.lib.r3d_hip_raycast, vol.h, views, width, height, alpha_cutoff, rgba, rgba8, tf,
+                             counters, px_steps, stats, out, stream)
+
+    def _raycast(self, fn, handle, views, width, height, alpha_cutoff, rgba, rgba8, tf, counters, px_steps, stats,
+                 out, stream):
         if isinstance(views, FrameParams):
             views = [views]
         arr = (FrameParams * len(views))(*views)
@@ -291,8 +296,8 @@ class Context:
                            None if counters is None else counters.data_ptr(),
                            None if steps is None else steps.data_ptr())
         st = Stats()
-        self.check(self.lib.r3d_hip_raycast(self.h, vol.h, arr, len(views), C.byref(im), C.byref(opts),
-                                            C.byref(st) if stats else None, _stream(stream)))
+        self.check(fn(self.h, handle, arr, len(views), C.byref(im), C.byref(opts), C.byref(st) if stats else None,
+                      _stream(stream)))
         res = {"rgba": img, "rgba8": img8, "px_steps": steps}
         if stats:
             res["stats"] = {"samples": st.samples, "skips": st.skips, "shaded": st.shaded, "rays": st.rays}
@@ -446,6 +451,113 @@ class Volume:
                                                                   out.ctypes.data_as(C.c_void_p), _lib.R3D_SRC_HOST,
                                                                   _stream(stream)))
         return out
+
+
+class Bricks:
+    """r3d_hip_bricks: a volume as a pyramid of 128^3 bricks in the slots of
+    an atlas, found through the reference's page table (vkbackend.c:4185-4205),
+    rendered in bricks-LOD mode (raycast.slang R3D_MODE 3 with BR_LOD_FLAG)."""
+
+    def __init__(self, ctx, nx, ny, nz, nlev, abpa):
+        self.ctx = ctx
+        d = _lib.BricksDesc(nx, ny, nz, nlev, abpa, 0)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.r3d_hip_bricks_create(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.dims_xyz, self.nlev, self.abpa = (nx, ny, nz), nlev, abpa
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.r3d_hip_bricks_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def brick_mode(self):
+        """The brick_mode a view of this atlas carries."""
+        return self.abpa | _lib.BR_LOD_FLAG
+
+    def tables(self):
+        """(page table device address, its words, atlas device address, slots)."""
+        page, atlas = C.c_void_p(), C.c_void_p()
+        words, nslots = C.c_uint32(), C.c_uint32()
+        self.ctx.check(self.ctx.lib.r3d_hip_bricks_tables(self.h, C.byref(page), C.byref(words), C.byref(atlas),
+                                                          C.byref(nslots)))
+        return page.value, words.value, atlas.value, nslots.value
+
+    def build(self, vol, resident=None, stream=None):
+        """Pyramid of vol (uint8 [nz, ny, nx] on the device); bricks whose
+        resident flag (one per page entry, page order) is 0 stay absent."""
+        nx, ny, nz = self.dims_xyz
+        if not (isinstance(vol, torch.Tensor) and vol.is_cuda and vol.dtype == torch.uint8 and vol.is_contiguous()
+                and vol.numel() == nx * ny * nz):
+            raise ValueError(f"bricks build: need a contiguous uint8 device tensor of {nz}x{ny}x{nx}")
+        res = None
+        if resident is not None:
+            res = np.ascontiguousarray(resident, np.uint8)
+            if res.size != self.tables()[1] - 64:
+                raise ValueError("bricks build: one resident flag per page entry")
+        self.ctx.check(self.ctx.lib.r3d_hip_bricks_build(self.ctx.h, self.h, _ptr(vol),
+                                                         None if res is None else res.ctypes.data_as(C.c_void_p),
+                                                         _stream(stream)))
+
+    def read_page(self):
+        """The page table as uint32 on the host (synchronous)."""
+        page, words, _, _ = self.tables()
+        out = np.empty(words, np.uint32)
+        _hip_copy(out.ctypes.data, page, out.nbytes)
+        return out
+
+    def read_slot(self, slot):
+        """Slot `slot` of the atlas as uint8 [128, 128, 128] (z, y, x) on the host."""
+        _, _, atlas, nslots = self.tables()
+        if not 0 <= slot < nslots:
+            raise IndexError(slot)
+        out = np.empty((128, 128, 128), np.uint8)
+        _hip_copy(out.ctypes.data, atlas + slot * out.nbytes, out.nbytes)
+        return out
+
+    def write_page(self, page):
+        """Replace the whole page table (uint32, the table's length) from the host (synchronous)."""
+        dst, words, _, _ = self.tables()
+        page = np.ascontiguousarray(page, np.uint32)
+        if page.size != words:
+            raise ValueError(f"bricks write_page: {page.size} words for a table of {words}")
+        _hip_copy(dst, page.ctypes.data, page.nbytes)
+
+    def write_slots(self, first, voxels):
+        """Fill slots first.. from host uint8 data of whole 128^3 slots (synchronous)."""
+        _, _, atlas, nslots = self.tables()
+        v = np.ascontiguousarray(voxels, np.uint8).reshape(-1)
+        n = v.size // (128 ** 3)
+        if v.size % (128 ** 3) or first < 0 or first + n > nslots:
+            raise ValueError("bricks write_slots: whole slots inside the atlas")
+        _hip_copy(atlas + first * 128 ** 3, v.ctypes.data, v.nbytes)
+
+    def raycast(self, views, width, height, *, alpha_cutoff=0.98, rgba=True, rgba8=False, tf=None, counters=None,
+                px_steps=False, stats=False, out=None, stream=None):
+        """Render bricks-LOD views (brick_mode = self.brick_mode); as Context.raycast."""
+        return self.ctx._raycast(self.ctx.lib.r3d_hip_bricks_raycast, self.h, views, width, height, alpha_cutoff,
+                                 rgba, rgba8, tf, counters, px_steps, stats, out, stream)
+
+
+_hip = None
+
+
+def _hip_copy(dst, src, nbytes):
+    """hipMemcpy(dst, src, nbytes, hipMemcpyDefault) from the HIP runtime the library uses."""
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.restype = C.c_int
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    if _hip.hipMemcpy(dst, src, nbytes, 4) != 0:
+        raise R3DError(-1, "hipMemcpy failed")
 
 
 class ShardStore:

@@ -1,0 +1,255 @@
+// r3d_bricks.hip -- bricks-LOD volumes: the page table and slot atlas of
+// raycast.slang's R3D_MODE 3 with BR_LOD_FLAG (the render is in
+// r3d_raycast.hip, march_bricks).
+//
+// The page table is the reference's layout (vkbackend.c:4185-4205), so a
+// streamer written against it can fill it: a 64-word header (nlev, base
+// shape, per level {entry offset, packed brick dims, scale}) and one entry
+// per 128^3 brick of every level, slot | max << 24 or 0xFFFFFFFF. The atlas is
+// linear HBM: slot s is 2 MiB at atlas + s * 128^3, z-major, so a trilinear
+// footprint is 8 byte loads at fixed offsets.
+//
+// r3d_hip_bricks_build makes the pyramid with the rounded 2x box
+// ((sum of 8 + 4) / 8, edge voxels repeated: tools/lodpack/main.c:216-227)
+// into temporary linear levels, then one workgroup per resident brick copies
+// its 128^3 voxels into its slot (zeros past the level's edge) and writes its
+// entry with the brick's maximum.
+#include <new>
+#include <vector>
+
+#include "r3d_internal.h"
+
+namespace r3d {
+
+constexpr uint32_t BK = 128u;
+constexpr uint64_t BK_VOX = 128ull * 128ull * 128ull;
+
+__global__ __launch_bounds__(256) void k_bk_down(const uint8_t *src, uint32_t nx, uint32_t ny, uint32_t nz,
+                                                 uint8_t *dst, uint32_t mx, uint32_t my, uint32_t mz) {
+  const uint64_t n = (uint64_t)mx * my * mz;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) {
+    const uint32_t x = (uint32_t)(i % mx), y = (uint32_t)((i / mx) % my), z = (uint32_t)(i / ((uint64_t)mx * my));
+    uint32_t sum = 4u;
+#pragma unroll
+    for (uint32_t d = 0; d < 8u; d++) {
+      uint32_t sx = 2u * x + (d & 1u), sy = 2u * y + ((d >> 1) & 1u), sz = 2u * z + (d >> 2);
+      sx = sx < nx ? sx : nx - 1u;
+      sy = sy < ny ? sy : ny - 1u;
+      sz = sz < nz ? sz : nz - 1u;
+      sum += src[((uint64_t)sz * ny + sy) * nx + sx];
+    }
+    dst[i] = (uint8_t)(sum / 8u);
+  }
+}
+
+struct BkJob {
+  uint32_t level, cx, cy, cz;
+  uint32_t word, slot;
+};
+
+struct BkFill {
+  const uint8_t *lev[R3D_MAX_LEVELS];
+  uint32_t dims[R3D_MAX_LEVELS][3];
+  const BkJob *jobs;
+  uint32_t njobs;
+  uint8_t *atlas;
+  uint32_t *page;
+};
+
+// One workgroup per resident brick: 16,384 rows of 128 voxels, 64 per lane.
+__global__ __launch_bounds__(256) void k_bk_fill(BkFill a) {
+  __shared__ uint32_t s_max[4];
+  for (uint32_t j = blockIdx.x; j < a.njobs; j += gridDim.x) {
+    const BkJob job = a.jobs[j];
+    const uint8_t *L = a.lev[job.level];
+    const uint32_t nx = a.dims[job.level][0], ny = a.dims[job.level][1], nz = a.dims[job.level][2];
+    uint8_t *S = a.atlas + (uint64_t)job.slot * BK_VOX;
+    const uint32_t x0 = job.cx * BK;
+    const bool whole_x = x0 + BK <= nx && nx % 16u == 0u && ((uintptr_t)L & 15u) == 0u;
+    uint32_t mx = 0;
+    for (uint32_t r = threadIdx.x; r < BK * BK; r += 256u) {
+      const uint32_t y = r & 127u, z = r >> 7;
+      const uint32_t gy = job.cy * BK + y, gz = job.cz * BK + z;
+      uint4 *dst = (uint4 *)(S + ((uint64_t)z * BK + y) * BK);
+      if (gy >= ny || gz >= nz) {
+        for (int k = 0; k < 8; k++) dst[k] = make_uint4(0, 0, 0, 0);
+        continue;
+      }
+      const uint8_t *row = L + ((uint64_t)gz * ny + gy) * nx + x0;
+      if (whole_x) {
+        for (int k = 0; k < 8; k++) {
+          const uint4 q = ((const uint4 *)row)[k];
+          dst[k] = q;
+          const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+          for (int i = 0; i < 4; i++)
+            for (int b = 0; b < 4; b++) {
+              const uint32_t v = (w[i] >> (8 * b)) & 255u;
+              mx = v > mx ? v : mx;
+            }
+        }
+      } else {
+        uint8_t *d8 = (uint8_t *)dst;
+        for (uint32_t x = 0; x < BK; x++) {
+          const uint32_t v = x0 + x < nx ? row[x] : 0u;
+          d8[x] = (uint8_t)v;
+          mx = v > mx ? v : mx;
+        }
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint32_t t = __shfl_xor(mx, o);
+      mx = t > mx ? t : mx;
+    }
+    if ((threadIdx.x & 63u) == 0u) s_max[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t m = s_max[0];
+      for (int w = 1; w < 4; w++) m = s_max[w] > m ? s_max[w] : m;
+      a.page[job.word] = job.slot | (m << 24);
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace r3d
+
+using namespace r3d;
+
+extern "C" int r3d_hip_bricks_create(r3d_hip_ctx *ctx, const r3d_bricks_desc *d, r3d_hip_bricks **out) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!d || !out) return r3d_fail(ctx, R3D_ERR_INVALID, "bricks_create: NULL argument");
+  *out = nullptr;
+  if (d->nx == 0 || d->ny == 0 || d->nz == 0)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "bricks_create: empty shape %ux%ux%u", d->nx, d->ny, d->nz);
+  if (d->nx > 1023u * BK || d->ny > 1023u * BK || d->nz > 1023u * BK)
+    return r3d_fail(ctx, R3D_ERR_RANGE, "bricks_create: more than 1023 bricks per axis");
+  if (d->nlev == 0 || d->nlev > 15u) return r3d_fail(ctx, R3D_ERR_INVALID, "bricks_create: nlev %u outside 1..15", d->nlev);
+  if (d->abpa == 0 || d->abpa > 255u) return r3d_fail(ctx, R3D_ERR_INVALID, "bricks_create: abpa %u outside 1..255", d->abpa);
+  r3d_hip_bricks *b = new (std::nothrow) r3d_hip_bricks;
+  if (!b) return r3d_fail(ctx, R3D_ERR_NOMEM, "bricks_create: handle");
+  b->ctx = ctx;
+  b->desc = *d;
+  uint64_t words = 64u;
+  for (uint32_t l = 0; l < d->nlev; l++) {
+    for (int a = 0; a < 3; a++) {
+      const uint32_t base = a == 0 ? d->nx : a == 1 ? d->ny : d->nz;
+      b->dims[l][a] = l == 0 ? base : (b->dims[l - 1][a] > 1u ? b->dims[l - 1][a] / 2u : 1u);
+      b->nb[l][a] = (b->dims[l][a] + BK - 1u) / BK;
+    }
+    b->off[l] = (uint32_t)words;
+    words += (uint64_t)b->nb[l][0] * b->nb[l][1] * b->nb[l][2];
+  }
+  if (words > 0xFFFFFFFFull) {
+    delete b;
+    return r3d_fail(ctx, R3D_ERR_RANGE, "bricks_create: page table too large");
+  }
+  b->page_words = (uint32_t)words;
+  b->nslots = d->abpa * d->abpa * d->abpa;
+  if (hipSetDevice(ctx->device) != hipSuccess || hipMalloc(&b->d_page, 4ull * b->page_words) != hipSuccess ||
+      hipMalloc(&b->d_atlas, (uint64_t)b->nslots * BK_VOX) != hipSuccess) {
+    r3d_hip_bricks_destroy(b);
+    return r3d_fail(ctx, R3D_ERR_NOMEM, "bricks_create: %u slots of 2 MiB", d->abpa * d->abpa * d->abpa);
+  }
+  std::vector<uint32_t> page(b->page_words, 0xFFFFFFFFu);
+  for (uint32_t i = 0; i < 64u; i++) page[i] = 0u;
+  page[0] = d->nlev;
+  page[1] = d->nx;
+  page[2] = d->ny;
+  page[3] = d->nz;
+  for (uint32_t l = 0; l < d->nlev; l++) {
+    page[4u + l * 4u] = b->off[l];
+    page[5u + l * 4u] = b->nb[l][0] | (b->nb[l][1] << 10) | (b->nb[l][2] << 20);
+    page[6u + l * 4u] = 1u << l;
+  }
+  if (hipMemcpy(b->d_page, page.data(), 4ull * b->page_words, hipMemcpyHostToDevice) != hipSuccess) {
+    r3d_hip_bricks_destroy(b);
+    return r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_create: page upload");
+  }
+  *out = b;
+  return R3D_OK;
+}
+
+extern "C" void r3d_hip_bricks_destroy(r3d_hip_bricks *b) {
+  if (!b) return;
+  if (b->ctx) hipSetDevice(b->ctx->device);
+  if (b->d_page) hipFree(b->d_page);
+  if (b->d_atlas) hipFree(b->d_atlas);
+  delete b;
+}
+
+extern "C" int r3d_hip_bricks_tables(const r3d_hip_bricks *b, uint32_t **page, uint32_t *page_words, uint8_t **atlas,
+                                     uint32_t *nslots) {
+  if (!b) return R3D_ERR_INVALID;
+  if (page) *page = b->d_page;
+  if (page_words) *page_words = b->page_words;
+  if (atlas) *atlas = b->d_atlas;
+  if (nslots) *nslots = b->nslots;
+  return R3D_OK;
+}
+
+extern "C" int r3d_hip_bricks_build(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const uint8_t *vol, const uint8_t *resident,
+                                    void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!b || b->ctx != ctx || !vol) return r3d_fail(ctx, R3D_ERR_INVALID, "bricks_build: bad handle or volume");
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = (hipStream_t)stream;
+  const uint32_t nlev = b->desc.nlev;
+  // resident bricks in page order, slots in that order
+  std::vector<BkJob> jobs;
+  std::vector<uint32_t> page(b->page_words - 64u, 0xFFFFFFFFu);
+  for (uint32_t l = 0; l < nlev; l++) {
+    const uint32_t n = b->nb[l][0] * b->nb[l][1] * b->nb[l][2];
+    for (uint32_t k = 0; k < n; k++) {
+      const uint32_t w = b->off[l] + k;
+      if (resident && !resident[w - 64u]) continue;
+      if (jobs.size() >= b->nslots)
+        return r3d_fail(ctx, R3D_ERR_RANGE, "bricks_build: more resident bricks than the %u slots", b->nslots);
+      jobs.push_back(BkJob{l, k % b->nb[l][0], (k / b->nb[l][0]) % b->nb[l][1], k / (b->nb[l][0] * b->nb[l][1]), w,
+                           (uint32_t)jobs.size()});
+    }
+  }
+  // levels 1.. as temporary linear arrays
+  BkFill f{};
+  f.lev[0] = vol;
+  std::vector<uint8_t *> tmp;
+  int rc = R3D_OK;
+  for (uint32_t l = 0; l < nlev; l++)
+    for (int a = 0; a < 3; a++) f.dims[l][a] = b->dims[l][a];
+  for (uint32_t l = 1; l < nlev && rc == R3D_OK; l++) {
+    uint8_t *p = nullptr;
+    const uint64_t n = (uint64_t)b->dims[l][0] * b->dims[l][1] * b->dims[l][2];
+    if (hipMalloc(&p, n) != hipSuccess) {
+      rc = r3d_fail(ctx, R3D_ERR_NOMEM, "bricks_build: level %u", l);
+      break;
+    }
+    tmp.push_back(p);
+    f.lev[l] = p;
+    const uint64_t blocks = (n + 255u) / 256u;
+    const uint32_t grid = (uint32_t)(blocks < 65536u ? blocks : 65536u);
+    hipLaunchKernelGGL(k_bk_down, dim3(grid), dim3(256), 0, st, f.lev[l - 1], b->dims[l - 1][0], b->dims[l - 1][1],
+                       b->dims[l - 1][2], p, b->dims[l][0], b->dims[l][1], b->dims[l][2]);
+    if (hipGetLastError() != hipSuccess) rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: downsample launch");
+  }
+  BkJob *djobs = nullptr;
+  if (rc == R3D_OK && !jobs.empty() && hipMalloc(&djobs, sizeof(BkJob) * jobs.size()) != hipSuccess)
+    rc = r3d_fail(ctx, R3D_ERR_NOMEM, "bricks_build: job list");
+  if (rc == R3D_OK && hipStreamSynchronize(st) != hipSuccess) rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: sync");
+  if (rc == R3D_OK && hipMemcpy(b->d_page + 64, page.data(), 4ull * page.size(), hipMemcpyHostToDevice) != hipSuccess)
+    rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: page reset");
+  if (rc == R3D_OK && djobs &&
+      hipMemcpy(djobs, jobs.data(), sizeof(BkJob) * jobs.size(), hipMemcpyHostToDevice) != hipSuccess)
+    rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: job upload");
+  if (rc == R3D_OK && djobs) {
+    f.jobs = djobs;
+    f.njobs = (uint32_t)jobs.size();
+    f.atlas = b->d_atlas;
+    f.page = b->d_page;
+    const uint32_t cap = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4u;
+    hipLaunchKernelGGL(k_bk_fill, dim3(f.njobs < cap ? f.njobs : cap), dim3(256), 0, st, f);
+    if (hipGetLastError() != hipSuccess) rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: fill launch");
+  }
+  if (hipStreamSynchronize(st) != hipSuccess && rc == R3D_OK) rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: sync");
+  if (djobs) hipFree(djobs);
+  for (uint8_t *p : tmp) hipFree(p);
+  return rc;
+}

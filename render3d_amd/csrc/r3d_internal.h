@@ -78,6 +78,18 @@ struct r3d_hip_volume {
   uint8_t *d_occ_raw = nullptr;
 };
 
+struct r3d_hip_bricks {
+  r3d_hip_ctx *ctx = nullptr;
+  r3d_bricks_desc desc{};
+  uint32_t dims[R3D_MAX_LEVELS][3] = {};   // level dims (x, y, z)
+  uint32_t nb[R3D_MAX_LEVELS][3] = {};     // bricks per axis per level
+  uint32_t off[R3D_MAX_LEVELS] = {};       // page word of each level's first entry
+  uint32_t page_words = 0;
+  uint32_t nslots = 0;
+  uint32_t *d_page = nullptr;
+  uint8_t *d_atlas = nullptr;
+};
+
 // error helpers
 int r3d_fail(r3d_hip_ctx *ctx, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
 #define R3D_HIP_CHECK(ctx, expr)                                                        \

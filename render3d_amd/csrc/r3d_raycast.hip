@@ -31,8 +31,18 @@
 
 namespace r3d {
 
+// Bricks-LOD volume (R3D_MODE 3 with the LOD flag): the reference's page
+// table and 128^3 slots, slot s at atlas + s * 2 MiB.
+struct BricksDev {
+  const uint32_t *page;
+  const uint8_t *atlas;
+  uint32_t abpa;
+  uint32_t npage, nslots;  // an entry past the table or a slot past the atlas reads as absent
+};
+
 struct RayArgs {
   VolDev vol;
+  BricksDev bk;
   const r3d_frame_params *views;  // this launch's views
   const uint32_t *tile_start;     // nviews + 1 prefix sums of tiles
   uint32_t nviews;
@@ -206,6 +216,328 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
   return R;
 }
 
+// ---- bricks-LOD mode -----------------------------------------------------
+// raycast.slang main() with R3D_MODE 3 and BR_LOD_FLAG: page-table locate
+// with a per-ray cache (lines 198-279), single-level fetch inside a 128^3
+// slot, whole-brick skips, taps in brick-local voxels. The fetch is the
+// pinned trilinear of the cube path: t = clamp(bl, 0.5, 127.5) - 0.5,
+// i1 = min(i0 + 1, 127), lerp over x, y, z, times 1/255.
+constexpr float BR_SLOT = 128.0f;
+constexpr uint64_t BR_SLOT_VOX = 128ull * 128ull * 128ull;
+
+struct BCtx {  // flags as full words: a bool's padding bytes were copied through scratch
+  uint32_t ok;
+  float maxv;
+  V3 bl;
+  uint32_t slot;
+  float grad_vox, world_brick;
+};
+
+struct LodCache {
+  uint32_t valid;
+  int desired;
+  uint32_t scale, bx, by, bz;
+  BCtx c;
+};
+
+__device__ __forceinline__ float brick_fetch(const uint8_t *atlas, uint32_t slot, V3 bl) {
+  const float tx = fclamp_sel(bl.x, 0.5f, 127.5f) - 0.5f;
+  const float ty = fclamp_sel(bl.y, 0.5f, 127.5f) - 0.5f;
+  const float tz = fclamp_sel(bl.z, 0.5f, 127.5f) - 0.5f;
+  const uint32_t x0 = (uint32_t)tx, y0 = (uint32_t)ty, z0 = (uint32_t)tz;
+  const uint32_t dx = x0 < 127u ? 1u : 0u, dy = y0 < 127u ? 128u : 0u, dz = z0 < 127u ? 16384u : 0u;
+  const float fx = tx - (float)x0, fy = ty - (float)y0, fz = tz - (float)z0;
+  const uint8_t *q = atlas + (uint64_t)slot * BR_SLOT_VOX + ((z0 * 128u + y0) * 128u + x0);
+  const float v000 = (float)q[0], v001 = (float)q[dx], v010 = (float)q[dy], v011 = (float)q[dy + dx];
+  const float v100 = (float)q[dz], v101 = (float)q[dz + dx], v110 = (float)q[dz + dy], v111 = (float)q[dz + dy + dx];
+  const float c00 = flerp(v000, v001, fx);
+  const float c10 = flerp(v010, v011, fx);
+  const float c01 = flerp(v100, v101, fx);
+  const float c11 = flerp(v110, v111, fx);
+  return flerp(flerp(c00, c10, fy), flerp(c01, c11, fy), fz) * K255;
+}
+
+__device__ __forceinline__ V3 base_voxel(const uint32_t *hdr, V3 p, float md) {
+  const float sx = (float)hdr[1], sy = (float)hdr[2], sz = (float)hdr[3];
+  return mk(fclamp_sel(p.x * md, 0.0f, fmax_sel(sx - 0.001f, 0.0f)), fclamp_sel(p.y * md, 0.0f, fmax_sel(sy - 0.001f, 0.0f)),
+            fclamp_sel(p.z * md, 0.0f, fmax_sel(sz - 0.001f, 0.0f)));
+}
+
+// bricks_lod_locate_ex: desired level, +1, +2, then the coarsest. hdr is the
+// page header in LDS; entries are read from the page table.
+__device__ __forceinline__ BCtx locate_ex(const BricksDev &B, const uint32_t *hdr, float md, V3 p, float lod,
+                                          uint32_t *scale_out, uint32_t *bx_o, uint32_t *by_o, uint32_t *bz_o) {
+  BCtx c;
+  c.ok = 0u;
+  c.maxv = 0.0f;
+  c.bl = mk(0.0f, 0.0f, 0.0f);
+  c.slot = 0;
+  c.grad_vox = 1.0f;
+  *scale_out = 1u;
+  *bx_o = *by_o = *bz_o = 0u;
+  const uint32_t nlev = hdr[0];
+  const V3 bv = base_voxel(hdr, p, md);
+  c.world_brick = BR_SLOT * (float)(1u << (nlev - 1u)) / md;
+  const int desired = (int)fclamp_sel(__builtin_floorf(lod), 0.0f, (float)(nlev - 1u));
+  const int last = (int)nlev - 1;
+  for (uint32_t step = 0; step < 4u; step++) {
+    const int li = step < 3u ? desired + (int)step : last;
+    if (li < 0 || li > last || (step == 3u && li <= desired + 2)) continue;
+    const uint32_t h = 4u + (uint32_t)li * 4u;
+    const uint32_t off = hdr[h], packed = hdr[h + 1u], scale = hdr[h + 2u];
+    const uint32_t bdx = packed & 1023u, bdy = (packed >> 10) & 1023u, bdz = packed >> 20;
+    const float fs = (float)scale;
+    const V3 lv = mk(bv.x / fs, bv.y / fs, bv.z / fs);
+    const uint32_t bcx = min((uint32_t)lv.x >> 7, bdx - 1u), bcy = min((uint32_t)lv.y >> 7, bdy - 1u),
+                   bcz = min((uint32_t)lv.z >> 7, bdz - 1u);
+    const uint32_t w = off + (bcz * bdy + bcy) * bdx + bcx;
+    const uint32_t e = w < B.npage ? B.page[w] : 0xFFFFFFFFu;
+    if (e == 0xFFFFFFFFu || (e & 0xFFFFFFu) >= B.nslots) continue;
+    c.ok = 1u;
+    c.maxv = (float)(e >> 24) / 255.0f;
+    c.slot = e & 0xFFFFFFu;
+    c.bl = mk(lv.x - (float)(bcx << 7), lv.y - (float)(bcy << 7), lv.z - (float)(bcz << 7));
+    c.grad_vox = exp2_pinned(lod) / fs;
+    c.world_brick = BR_SLOT * fs / md;
+    *scale_out = scale;
+    *bx_o = bcx;
+    *by_o = bcy;
+    *bz_o = bcz;
+    return c;
+  }
+  return c;
+}
+
+// bricks_lod_locate_cached: the walk is skipped while the sample stays in the
+// cached brick at the same desired level (its level, grad_vox and brick size
+// are then the cached ones).
+__device__ __forceinline__ BCtx locate_cached(const BricksDev &B, const uint32_t *hdr, float md, V3 p, float lod,
+                                              LodCache &lc) {
+  const uint32_t nlev = hdr[0];
+  const int desired = (int)fclamp_sel(__builtin_floorf(lod), 0.0f, (float)(nlev - 1u));
+  if (lc.valid && lc.desired == desired) {
+    const V3 bv = base_voxel(hdr, p, md);
+    const float fs = (float)lc.scale;
+    const V3 lv = mk(bv.x / fs, bv.y / fs, bv.z / fs);
+    const uint32_t bcx = (uint32_t)lv.x >> 7, bcy = (uint32_t)lv.y >> 7, bcz = (uint32_t)lv.z >> 7;
+    if (bcx == lc.bx && bcy == lc.by && bcz == lc.bz) {
+      BCtx c = lc.c;
+      c.bl = mk(lv.x - (float)(bcx << 7), lv.y - (float)(bcy << 7), lv.z - (float)(bcz << 7));
+      return c;
+    }
+  }
+  uint32_t scale, bx, by, bz;
+  const BCtx c = locate_ex(B, hdr, md, p, lod, &scale, &bx, &by, &bz);
+  lc.valid = c.ok;
+  lc.desired = desired;
+  lc.scale = scale;
+  lc.bx = bx;
+  lc.by = by;
+  lc.bz = bz;
+  lc.c = c;
+  return c;
+}
+
+template <int KIND>
+__device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_t *hdr, const float4 *tf,
+                                               const r3d_frame_params &pc, float cutoff, uint32_t px, uint32_t py) {
+  RayOut R;
+  R.steps = R.skips = R.shaded = 0;
+  const uint32_t mode = KIND == KIND_FULL ? MODE_FULL : KIND == KIND_MIP ? MODE_MIP : KIND == KIND_FLAT ? MODE_FLAT : pc.mode;
+  const V3 fwd = mk(pc.cam_forward[0], pc.cam_forward[1], pc.cam_forward[2]);
+  const V3 right = mk(pc.cam_right[0], pc.cam_right[1], pc.cam_right[2]);
+  const V3 up = mk(pc.cam_up[0], pc.cam_up[1], pc.cam_up[2]);
+  const bool ortho = (pc.view_flags & 1u) != 0u;
+  const float u = ((float)px + 0.5f) / (float)pc.viewport[0];
+  const float vv = ((float)py + 0.5f) / (float)pc.viewport[1];
+  const float ndx = u * 2.0f - 1.0f, ndy = 1.0f - vv * 2.0f;
+  V3 dir = ortho ? vnorm(fwd) : vnorm(vadd(vadd(fwd, vscale(right, ndx)), vscale(up, ndy)));
+  if (KIND == KIND_OTHER && mode == MODE_RAYDIR) {
+    R.c[0] = dir.x * 0.5f + 0.5f;
+    R.c[1] = dir.y * 0.5f + 0.5f;
+    R.c[2] = dir.z * 0.5f + 0.5f;
+    R.c[3] = 1.0f;
+    return R;
+  }
+  const float sx = (float)hdr[1], sy = (float)hdr[2], sz = (float)hdr[3];
+  const uint32_t mdu = max(hdr[1], max(hdr[2], hdr[3]));
+  const float md = (float)mdu;
+  const float voxel = 1.0f / md;
+  const float max_lod = (float)(hdr[0] - 1u);
+  const float emax = fmax_sel(sx, fmax_sel(sy, sz));
+  const V3 ext = mk(sx / emax, sy / emax, sz / emax);
+  V3 org = mk(pc.cam_origin[0], pc.cam_origin[1], pc.cam_origin[2]);
+  if (ortho) org = vadd(vadd(org, vscale(right, ndx)), vscale(up, ndy));
+  const V3 ctr = vscale(ext, 0.5f);
+  const V3 c0 = mk(pc.vol_r0[0], pc.vol_r1[0], pc.vol_r2[0]);
+  const V3 c1 = mk(pc.vol_r0[1], pc.vol_r1[1], pc.vol_r2[1]);
+  const V3 c2 = mk(pc.vol_r0[2], pc.vol_r1[2], pc.vol_r2[2]);
+  const V3 q = vsub(vsub(org, mk(pc.vol_tx, pc.vol_ty, pc.vol_tz)), ctr);
+  const V3 ro = vadd(mk(vdot(c0, q), vdot(c1, q), vdot(c2, q)), ctr);
+  dir = mk(vdot(c0, dir), vdot(c1, dir), vdot(c2, dir));
+  // raycast.slang:690-718: zoom-crop box or slab range
+  V3 lo = mk(0.0f, 0.0f, 0.0f), hi = ext;
+  float t_lo = 0.0f, t_hi = 1e30f;
+  if ((pc.view_flags & 16u) != 0u) {
+    const V3 clo = mk(pc.slab_x0 / md, pc.slab_y0 / md, pc.slab_z0 / md);
+    const V3 chi = vadd(clo, mk(pc.slab_px / md, pc.slab_py / md, pc.slab_nx / md));
+    lo = mk(fclamp_sel(clo.x, 0.0f, ext.x), fclamp_sel(clo.y, 0.0f, ext.y), fclamp_sel(clo.z, 0.0f, ext.z));
+    hi = mk(fclamp_sel(chi.x, lo.x, ext.x), fclamp_sel(chi.y, lo.y, ext.y), fclamp_sel(chi.z, lo.z, ext.z));
+  } else if (pc.slab_depth > 0u) {
+    if ((pc.view_flags & 4u) != 0u) {
+      t_lo = pc.slab_z0 / md;
+      t_hi = (pc.slab_z0 + (float)pc.slab_depth) / md;
+    } else {
+      const uint32_t axc = (pc.view_flags >> 8) & 3u;
+      const float a0 = fclamp_sel(pc.slab_z0 / md, 0.0f, axc == 1u ? ext.x : axc == 2u ? ext.y : ext.z);
+      const float a1 = fclamp_sel((pc.slab_z0 + (float)pc.slab_depth) / md, a0, axc == 1u ? ext.x : axc == 2u ? ext.y : ext.z);
+      if (axc == 1u) {
+        lo.x = a0;
+        hi.x = a1;
+      } else if (axc == 2u) {
+        lo.y = a0;
+        hi.y = a1;
+      } else {
+        lo.z = a0;
+        hi.z = a1;
+      }
+    }
+  }
+  const V3 inv = mk(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+  const V3 t0 = mk((lo.x - ro.x) * inv.x, (lo.y - ro.y) * inv.y, (lo.z - ro.z) * inv.z);
+  const V3 t1 = mk((hi.x - ro.x) * inv.x, (hi.y - ro.y) * inv.y, (hi.z - ro.z) * inv.z);
+  const float hx = fmax_sel(fmax_sel(fmin_sel(t0.x, t1.x), fmin_sel(t0.y, t1.y)), fmin_sel(t0.z, t1.z));
+  const float hy = fmin_sel(fmin_sel(fmax_sel(t0.x, t1.x), fmax_sel(t0.y, t1.y)), fmax_sel(t0.z, t1.z));
+  const float tn = fmax_sel(hx, t_lo), tfar = fmin_sel(hy, t_hi);
+  if (tn >= tfar) {
+    R.c[0] = 0.02f;
+    R.c[1] = 0.02f;
+    R.c[2] = 0.03f;
+    R.c[3] = 1.0f;
+    return R;
+  }
+  const float cone = 2.0f * vlen(up) / (float)pc.viewport[1];
+  const float dt = pc.step_voxels * voxel;
+  float t = tn + hash12(px, py, pc.frame_index) * dt;
+  float mip_max = 0.0f, first_hit = -1.0f;
+  float ar = 0.0f, ag = 0.0f, ab = 0.0f, aa = 0.0f;
+  uint32_t steps = 0, skips = 0, shaded_n = 0;
+  const bool composite = mode == MODE_FULL || mode == MODE_FLAT;
+  const bool shaded = mode == MODE_FULL;
+  const uint32_t budget = shaded ? 192u : 256u;  // raycast.slang:743
+  const float span_step = fmax_sel((tfar - tn) / ((float)budget * dt), 1.0f);
+  const float occ_gate = fmax_sel(pc.skip_gate, 0.5f / 255.0f);
+  const float bgate = fmax_sel(occ_gate, 4.5f / 255.0f);
+  const float opac_k = pc.density * pc.step_voxels;
+  const V3 lvec = vscale(dir, -1.0f);
+  const V3 hvec = vnorm(vsub(lvec, dir));
+  const V3 key = mk(0x1.a20bd8p-2f, 0x1.a20bd8p-1f, -0x1.a20bd8p-2f);
+  const float sgx = dir.x > 0.0f ? 1.0f : 0.0f, sgy = dir.y > 0.0f ? 1.0f : 0.0f,
+              sgz = dir.z > 0.0f ? 1.0f : 0.0f;
+  LodCache lc;
+  lc.valid = 0u;
+  lc.desired = -1;
+  lc.scale = 1u;
+  lc.bx = lc.by = lc.bz = 0u;
+  lc.c = BCtx{false, 0.0f, mk(0.0f, 0.0f, 0.0f), 0u, 1.0f, 0.0f};
+  while (t < tfar && steps < budget) {
+    steps++;
+    const float foot = ortho ? cone : t * cone;
+    const float lod = fclamp_sel(log2_pinned(fmax_sel(foot / voxel, 1.0f)) + pc.lod_bias, 0.0f, max_lod);
+    const float lod_step = fmax_sel(exp2_pinned(lod), span_step);
+    const V3 p = vadd(ro, vscale(dir, t));
+    const BCtx bc = locate_cached(B, hdr, md, p, lod, lc);
+    float v = bc.ok ? brick_fetch(B.atlas, bc.slot, bc.bl) : 0.0f;
+    const bool cut = v < pc.threshold;
+    if (cut) v = 0.0f;
+    // raycast.slang:834-868: whole-brick skips
+    float bsz = 0.0f;
+    if (mode == MODE_MIP && bc.ok && bc.maxv <= mip_max)
+      bsz = bc.world_brick;
+    else if (v < bgate && bc.maxv < bgate)
+      bsz = bc.world_brick;
+    if (bsz > 0.0f) {
+      const float nx = (__builtin_floorf(p.x / bsz) + sgx) * bsz;
+      const float ny = (__builtin_floorf(p.y / bsz) + sgy) * bsz;
+      const float nz = (__builtin_floorf(p.z / bsz) + sgz) * bsz;
+      const float ex = __builtin_fabsf(dir.x) > 1e-8f ? (nx - ro.x) / dir.x : 1e30f;
+      const float ey = __builtin_fabsf(dir.y) > 1e-8f ? (ny - ro.y) / dir.y : 1e30f;
+      const float ez = __builtin_fabsf(dir.z) > 1e-8f ? (nz - ro.z) / dir.z : 1e30f;
+      t = fmax_sel(fmin_sel(fmin_sel(ex, ey), ez), t) + bsz * 0.01f;
+      skips++;
+      continue;
+    }
+    if (v > mip_max) mip_max = v;
+    if (mode == MODE_MIP && mip_max >= 0.999f) break;
+    if (KIND == KIND_OTHER && first_hit < 0.0f && v > 0.05f) first_hit = t;
+    if (composite && !cut) {
+      const float4 s = tf_sample(tf, v);
+      const float a = 1.0f - pow_unit(1.0f - s.w, opac_k * lod_step);
+      if (a > 0.003f) {
+        float cr = s.x, cg = s.y, cb = s.z;
+        if (shaded) {
+          shaded_n++;
+          const float ev = bc.grad_vox;
+          const V3 b = bc.bl;
+          const float gx = brick_fetch(B.atlas, bc.slot, mk(b.x + ev, b.y, b.z)) -
+                           brick_fetch(B.atlas, bc.slot, mk(b.x + -ev, b.y, b.z));
+          const float gy = brick_fetch(B.atlas, bc.slot, mk(b.x, b.y + ev, b.z)) -
+                           brick_fetch(B.atlas, bc.slot, mk(b.x, b.y + -ev, b.z));
+          const float gz = brick_fetch(B.atlas, bc.slot, mk(b.x, b.y, b.z + ev)) -
+                           brick_fetch(B.atlas, bc.slot, mk(b.x, b.y, b.z + -ev));
+          const V3 g = mk(gx, gy, gz);
+          const float gm = vlen(g);
+          if (gm > 1e-4f) {
+            const V3 n = mk(g.x / gm, g.y / gm, g.z / gm);
+            const float diff = 0.7f * __builtin_fabsf(vdot(n, lvec)) + 0.3f * __builtin_fabsf(vdot(n, key));
+            const float spec = pow24(__builtin_fabsf(vdot(n, hvec))) * 0.3f;
+            const float lit = 0.25f + 0.75f * diff;
+            const float w = fsat(gm * 6.0f);
+            const float m = flerp(1.0f, lit, w);
+            const float sw = spec * w;
+            cr = cr * m + sw;
+            cg = cg * m + sw;
+            cb = cb * m + sw;
+          }
+        }
+        const float wa = (1.0f - aa) * a;
+        ar = ar + wa * cr;
+        ag = ag + wa * cg;
+        ab = ab + wa * cb;
+        aa = aa + wa;
+        if (aa >= cutoff) break;
+      }
+    }
+    t += dt * lod_step;
+  }
+  R.steps = steps;
+  R.skips = skips;
+  R.shaded = shaded_n;
+  if (KIND == KIND_OTHER && mode == MODE_HEATMAP) {
+    const float x = (float)steps / 1024.0f;
+    R.c[0] = x;
+    R.c[1] = 1.0f - __builtin_fabsf(x - 0.5f) * 2.0f;
+    R.c[2] = 1.0f - x;
+  } else if (KIND == KIND_OTHER && mode == MODE_DEPTH) {
+    const float d = first_hit < 0.0f ? 0.0f : 1.0f - fsat(first_hit / 2.0f);
+    R.c[0] = d;
+    R.c[1] = d;
+    R.c[2] = d;
+  } else if (composite) {
+    const float ta = 1.0f - aa;
+    R.c[0] = ar + ta * 0.02f;
+    R.c[1] = ag + ta * 0.02f;
+    R.c[2] = ab + ta * 0.03f;
+  } else {
+    const float m = mip_max * pc.density;
+    R.c[0] = m;
+    R.c[1] = m;
+    R.c[2] = m;
+  }
+  R.c[3] = 1.0f;
+  return R;
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -229,18 +561,21 @@ static int shape_index(uint32_t w, uint32_t h) {
 // persistent and takes tiles from the queue counter until they run out (the
 // TF and level tables are then staged once per workgroup, not once per tile).
 // Counter partials: one uint4 per workgroup.
-template <int KIND, int TW, int TH>
+template <int KIND, int TW, int TH, bool BRICKS>
 __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
   constexpr uint32_t NT = TW * TH, NW = NT / 64u;
   __shared__ float4 s_tf[256];
   __shared__ LevelDev s_lev[R3D_MAX_LEVELS];
+  __shared__ uint32_t s_hdr[BRICKS ? 64 : 1];  // bricks: the page header
   __shared__ uint32_t s_ts[R3D_MAX_VIEWS_PER_CALL + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint4 s_red[NW];
   const uint32_t lane = threadIdx.x;
   for (uint32_t i = lane; i < 256u; i += NT)
     s_tf[i] = make_float4(a.tf[i * 4 + 0], a.tf[i * 4 + 1], a.tf[i * 4 + 2], a.tf[i * 4 + 3]);
-  if (lane < a.vol.nlev) s_lev[lane] = a.vol.levels[lane];
+  if (!BRICKS && lane < a.vol.nlev) s_lev[lane] = a.vol.levels[lane];
+  if (BRICKS)  // nlev held to 1..15: the per-level words stay inside the header
+    for (uint32_t i = lane; i < 64u; i += NT) s_hdr[i] = i ? a.bk.page[i] : min(max(a.bk.page[0], 1u), 15u);
   for (uint32_t i = lane; i <= a.nviews; i += NT) s_ts[i] = a.tile_start[i];
   __syncthreads();
   const Sampler S{a.vol.base, s_lev, a.vol.nlev};
@@ -272,7 +607,8 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
     const uint32_t ty = (lt / tiles_x) * TH + (lane / TW);
     if (tx < RW && ty < RH) {
       const uint32_t lx = half ? tx * 2u : tx, ly = half ? ty * 2u : ty;
-      const RayOut R = march<KIND>(S, a.vol, s_tf, pc, a.cutoff, lx, ly);
+      const RayOut R = BRICKS ? march_bricks<KIND>(a.bk, s_hdr, s_tf, pc, a.cutoff, lx, ly)
+                              : march<KIND>(S, a.vol, s_tf, pc, a.cutoff, lx, ly);
       st += R.steps;
       sk += R.skips;
       sh += R.shaded;
@@ -356,10 +692,13 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 }
 
 typedef void (*RayKernel)(RayArgs);
-#define R3D_RAY_SHAPES(K)                                                                        \
-  { k_raycast<K, 8, 8>, k_raycast<K, 16, 4>, k_raycast<K, 16, 8>, k_raycast<K, 16, 16> }
-static const RayKernel ray_kernels[NKINDS][NSHAPES] = {R3D_RAY_SHAPES(KIND_FULL), R3D_RAY_SHAPES(KIND_MIP),
-                                                       R3D_RAY_SHAPES(KIND_FLAT), R3D_RAY_SHAPES(KIND_OTHER)};
+#define R3D_RAY_SHAPES(K, B)                                                                     \
+  { k_raycast<K, 8, 8, B>, k_raycast<K, 16, 4, B>, k_raycast<K, 16, 8, B>, k_raycast<K, 16, 16, B> }
+static const RayKernel ray_kernels[2][NKINDS][NSHAPES] = {
+    {R3D_RAY_SHAPES(KIND_FULL, false), R3D_RAY_SHAPES(KIND_MIP, false), R3D_RAY_SHAPES(KIND_FLAT, false),
+     R3D_RAY_SHAPES(KIND_OTHER, false)},
+    {R3D_RAY_SHAPES(KIND_FULL, true), R3D_RAY_SHAPES(KIND_MIP, true), R3D_RAY_SHAPES(KIND_FLAT, true),
+     R3D_RAY_SHAPES(KIND_OTHER, true)}};
 #undef R3D_RAY_SHAPES
 
 static int kind_of(uint32_t mode) {
@@ -442,7 +781,7 @@ static int check_tuning(r3d_hip_ctx *ctx, const r3d_raycast_tuning *t) {
 // counter reduction. `partials` holds partial_slots() uint4 entries.
 static int launch_groups(r3d_hip_ctx *ctx, RayArgs a, const r3d_frame_params *dviews, const uint32_t *dtables,
                          const ViewGroups &g, const r3d_raycast_tuning &t, uint32_t *partials, uint32_t *queue,
-                         unsigned long long *counters, hipStream_t s) {
+                         unsigned long long *counters, hipStream_t s, bool bricks = false) {
   const int sh = shape_index(t.tile_w, t.tile_h);
   const uint32_t nt = SHAPE_W[sh] * SHAPE_H[sh];
   if (t.blocks_per_cu) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, queue, (uint32_t)NKINDS);
@@ -463,7 +802,7 @@ static int launch_groups(r3d_hip_ctx *ctx, RayArgs a, const r3d_frame_params *dv
     a.nviews = nv;
     a.ntiles = tiles;
     a.queue = t.blocks_per_cu ? queue + k : nullptr;
-    hipLaunchKernelGGL(ray_kernels[k][sh], dim3(grid), dim3(nt), 0, s, a);
+    hipLaunchKernelGGL(ray_kernels[bricks ? 1 : 0][k][sh], dim3(grid), dim3(nt), 0, s, a);
     R3D_HIP_CHECK(ctx, hipGetLastError());
   }
   if (counters && pbase) {
@@ -474,8 +813,10 @@ static int launch_groups(r3d_hip_ctx *ctx, RayArgs a, const r3d_frame_params *dv
   return R3D_OK;
 }
 
+// abpa = 0: cube-mode views; otherwise bricks-LOD views of an atlas with abpa
+// slots per axis (brick_mode = abpa | BR_LOD_FLAG, raycast.slang:157-161).
 static int validate_views(r3d_hip_ctx *ctx, const r3d_frame_params *views, uint32_t nviews,
-                          const r3d_hip_image *out) {
+                          const r3d_hip_image *out, uint32_t abpa = 0) {
   if (!views || nviews == 0 || nviews > R3D_MAX_VIEWS_PER_CALL)
     return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: nviews %u outside 1..%u", nviews, R3D_MAX_VIEWS_PER_CALL);
   if (!out || (!out->rgba && !out->rgba8) || out->width == 0 || out->height == 0)
@@ -484,13 +825,18 @@ static int validate_views(r3d_hip_ctx *ctx, const r3d_frame_params *views, uint3
     return r3d_fail(ctx, R3D_ERR_RANGE, "raycast: image above 65535 pixels per side");
   for (uint32_t i = 0; i < nviews; i++) {
     const r3d_frame_params &p = views[i];
-    if (p.slab_grid || p.clip_valid || p.brick_mode || (p.view_flags & 2u))
+    if (abpa) {
+      if (!(p.brick_mode & 0x20000u) || (p.brick_mode & 0xffu) != abpa || p.slab_grid || p.clip_valid ||
+          (p.view_flags & 2u))
+        return r3d_fail(ctx, R3D_ERR_UNSUPPORTED, "raycast: view %u is not a bricks-LOD view of this atlas", i);
+    } else if (p.slab_grid || p.clip_valid || p.brick_mode || (p.view_flags & 2u)) {
       return r3d_fail(ctx, R3D_ERR_UNSUPPORTED, "raycast: view %u is not a cube-mode view", i);
+    }
     if (p.mode > MODE_FLAT) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: view %u mode %u", i, p.mode);
     if (p.viewport[0] == 0 || p.viewport[1] == 0 || p.viewport[0] > 65535u || p.viewport[1] > 65535u)
       return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: view %u viewport %ux%u", i, p.viewport[0], p.viewport[1]);
     if (!(p.step_voxels > 0.0f)) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: view %u step_voxels", i);
-    if (!(p.max_mip >= 0.0f)) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: view %u max_mip", i);
+    if (!abpa && !(p.max_mip >= 0.0f)) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: view %u max_mip", i);
   }
   return R3D_OK;
 }
@@ -498,7 +844,7 @@ static int validate_views(r3d_hip_ctx *ctx, const r3d_frame_params *views, uint3
 static RayArgs base_args(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_hip_image *out,
                          const r3d_raycast_opts *opts) {
   RayArgs a{};
-  a.vol = vol_dev(vol);
+  if (vol) a.vol = vol_dev(vol);  // a bricks-LOD render has no cube volume (nlev 0)
   a.tf = opts && opts->tf ? opts->tf : ctx->d_tf;
   a.img = out->rgba;
   a.img8 = out->rgba8;
@@ -553,12 +899,11 @@ struct r3d_hip_raycast_plan {
   r3d_raycast_tuning tuning{8u, 8u, 0u, 0.0f};
 };
 
-extern "C" int r3d_hip_raycast(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_frame_params *views,
-                               uint32_t nviews, const r3d_hip_image *out, const r3d_raycast_opts *opts,
-                               r3d_hip_stats *stats, void *stream) {
-  if (!ctx) return R3D_ERR_INVALID;
-  if (!vol || vol->ctx != ctx) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: bad volume");
-  int rc = validate_views(ctx, views, nviews, out);
+// r3d_hip_raycast and r3d_hip_bricks_raycast: one of vol / bk is given.
+static int raycast_impl(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_hip_bricks *bk,
+                        const r3d_frame_params *views, uint32_t nviews, const r3d_hip_image *out,
+                        const r3d_raycast_opts *opts, r3d_hip_stats *stats, void *stream) {
+  int rc = validate_views(ctx, views, nviews, out, bk ? bk->desc.abpa : 0u);
   if (rc != R3D_OK) return rc;
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   hipStream_t s = (hipStream_t)stream;
@@ -589,8 +934,10 @@ extern "C" int r3d_hip_raycast(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, cons
   }
   const r3d_frame_params *dviews = (const r3d_frame_params *)slot->dev;
   const uint32_t *dtables = (const uint32_t *)((const uint8_t *)slot->dev + g.view_bytes());
-  rc = launch_groups(ctx, base_args(ctx, vol, out, opts), dviews, dtables, g, ctx->tuning, ctx->d_partials,
-                     ctx->d_queue, counters, s);
+  RayArgs a = base_args(ctx, vol, out, opts);
+  if (bk) a.bk = BricksDev{bk->d_page, bk->d_atlas, bk->desc.abpa, bk->page_words, bk->nslots};
+  rc = launch_groups(ctx, a, dviews, dtables, g, ctx->tuning, ctx->d_partials, ctx->d_queue, counters, s,
+                     bk != nullptr);
   if (rc != R3D_OK) return rc;
   R3D_HIP_CHECK(ctx, hipEventRecord(slot->ev, s));
   slot->recorded = true;
@@ -604,6 +951,22 @@ extern "C" int r3d_hip_raycast(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, cons
     stats->rays = h[3];
   }
   return R3D_OK;
+}
+
+extern "C" int r3d_hip_raycast(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_frame_params *views,
+                               uint32_t nviews, const r3d_hip_image *out, const r3d_raycast_opts *opts,
+                               r3d_hip_stats *stats, void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!vol || vol->ctx != ctx) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: bad volume");
+  return raycast_impl(ctx, vol, nullptr, views, nviews, out, opts, stats, stream);
+}
+
+extern "C" int r3d_hip_bricks_raycast(r3d_hip_ctx *ctx, const r3d_hip_bricks *b, const r3d_frame_params *views,
+                                      uint32_t nviews, const r3d_hip_image *out, const r3d_raycast_opts *opts,
+                                      r3d_hip_stats *stats, void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!b || b->ctx != ctx) return r3d_fail(ctx, R3D_ERR_INVALID, "bricks_raycast: bad bricks handle");
+  return raycast_impl(ctx, nullptr, b, views, nviews, out, opts, stats, stream);
 }
 
 extern "C" int r3d_hip_set_raycast_tuning(r3d_hip_ctx *ctx, const r3d_raycast_tuning *t) {

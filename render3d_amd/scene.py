@@ -160,9 +160,9 @@ def tf_unorm(lut):
 
 
 def frame_params(cam, width, height, *, mode=MODE_FULL, step_voxels=1.0, density=1.0, lod_bias=0.0,
-                 max_mip=10.0, frame_index=0, threshold=0.0, gate=0.0, view_org=(0, 0), view_flags=0):
-    """Assemble r3d_frame_params the way main.c:6788-6818 does (cube mode,
-    identity model transform)."""
+                 max_mip=10.0, frame_index=0, threshold=0.0, gate=0.0, view_org=(0, 0), view_flags=0, brick_mode=0):
+    """Assemble r3d_frame_params the way main.c:6788-6818 does (identity
+    model transform; cube mode, or bricks-LOD with brick_mode = abpa | 0x20000)."""
     right, up, fwd = cam.basis(f32(f32(width) / f32(height)))
     p = FrameParams()
     p.cam_origin[:] = [float(v) for v in cam.pos]
@@ -184,6 +184,7 @@ def frame_params(cam, width, height, *, mode=MODE_FULL, step_voxels=1.0, density
     p.vol_r2[:] = [0.0, 0.0, 1.0]
     p.view_org = (view_org[0] & 0xFFFF) | ((view_org[1] & 0xFFFF) << 16)
     p.view_flags = view_flags
+    p.brick_mode = brick_mode
     return p
 
 

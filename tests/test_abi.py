@@ -34,18 +34,18 @@ def test_ctypes_mirror_covers_the_header():
 
 def test_struct_sizes_match_the_c_compiler(tmp_path):
     src = tmp_path / "sz.c"
-    src.write_text('#include "r3d_hip.h"\n#include <stdio.h>\n#include <stddef.h>\nint main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
+    src.write_text('#include "r3d_hip.h"\n#include <stdio.h>\n#include <stddef.h>\nint main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
                    'sizeof(r3d_frame_params), sizeof(r3d_chunk_batch), sizeof(r3d_chunk_header), sizeof(r3d_hip_image),'
                    'sizeof(r3d_raycast_opts), sizeof(r3d_diff_params), offsetof(r3d_chunk_batch, out), '
                    'sizeof(r3d_raycast_tuning), sizeof(r3d_blob_batch), sizeof(r3d_blob_parse_out), '
-                   'sizeof(r3d_shard_src), sizeof(r3d_shard_region));return 0;}\n')
+                   'sizeof(r3d_shard_src), sizeof(r3d_shard_region), sizeof(r3d_bricks_desc));return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-std=c2x", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [C.sizeof(_lib.FrameParams), C.sizeof(_lib.ChunkBatch), C.sizeof(_lib.ChunkHeader), C.sizeof(_lib.Image),
             C.sizeof(_lib.RaycastOpts), C.sizeof(_lib.DiffParams), _lib.ChunkBatch.out.offset,
             C.sizeof(_lib.RaycastTuning), C.sizeof(_lib.BlobBatch), C.sizeof(_lib.BlobParseOut),
-            C.sizeof(_lib.ShardSrc), C.sizeof(_lib.ShardRegion)]
+            C.sizeof(_lib.ShardSrc), C.sizeof(_lib.ShardRegion), C.sizeof(_lib.BricksDesc)]
     assert got == want
 
 
