@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--no-cfg0", action="store_true", help="skip the 256^3 / 512^2 workload")
     ap.add_argument("--no-deblock", action="store_true", help="skip the deblocking workload")
     ap.add_argument("--no-shard", action="store_true", help="skip the zarr shard region-decode workload")
+    ap.add_argument("--no-bricks", action="store_true", help="skip the bricks-LOD (page table) workload")
     ap.add_argument("--no-blobs", action="store_true", help="skip the 3ddct blob (entropy stage) workload")
     ap.add_argument("--no-fit", action="store_true", help="skip the differentiable 256^3 fit (200 Adam steps)")
     ap.add_argument("--profile-decode", action="store_true",
@@ -194,6 +195,7 @@ def main():
     deb = None
     if shd is not None and not args.no_deblock:
         deb = deblock_stage(ctx, args, dev, stream, shd["_state"]["dst"], BLOB_QSTEP)
+    brk = None if args.no_bricks else bricks_stage(ctx, args, dev, stream, n, nsheets, seed, lut, cam, W, H)
     fit = None if args.no_fit else fit_256(ctx, dev)
 
     cpu = None
@@ -208,6 +210,8 @@ def main():
             shd["cpu_baseline"], shd["parity_vs_reference"] = cpu_shard(shd["_state"], args)
         if deb is not None:
             deb["cpu_baseline"], deb["parity_vs_reference"] = cpu_deblock(deb["_state"], args)
+        if brk is not None:
+            brk["cpu_baseline"], brk["parity_vs_oracle"] = cpu_bricks(brk["_state"], args)
     if c0 is not None:
         c0.pop("_state", None)
     if blob is not None:
@@ -216,6 +220,8 @@ def main():
         shd.pop("_state", None)
     if deb is not None:
         deb.pop("_state", None)
+    if brk is not None:
+        brk.pop("_state", None)
 
     out = {
         "metric": "ray samples/s + fps @1024³ 1080p; chunk decode GB/s; adjoint/fwd time ratio",
@@ -247,6 +253,7 @@ def main():
         "blobs": blob,
         "shard": shd,
         "deblock": deb,
+        "bricks": brk,
         "samples_per_frame": samples,
         "skips_per_frame": skips,
         "shaded_per_frame": shaded,
@@ -704,6 +711,105 @@ def cpu_deblock(state, args):
     cpu = {"value": reps * sub.size / dt, "unit": "voxels/s", "cores": 1, "kind": kind,
            "sample": f"{reps} x dct3d_deblock_u8 of a 256^3 sub-volume, {dt:.1f} s"}
     return cpu, {"subvolume_equal": bool(np.array_equal(state["gpu"], want))}
+
+
+def bricks_stage(ctx, args, dev, stream, n, nsheets, seed, lut, cam, W, H):
+    """Bricks-LOD: the n^3 sheet volume as 128^3 bricks of an args.levels
+    pyramid (1024^3, 4 levels: 512 + 64 + 8 + 1 = 585 bricks, all resident in
+    9^3 = 729 slots), built on the GPU from the linear volume, then the cube
+    workload's 1920x1080 FULL frame rendered through the page table
+    (raycast.slang R3D_MODE 3 with the LOD flag). Build timed by the host
+    clock (it synchronises), frames by HIP events on the stream."""
+    import numpy as np
+    import torch
+
+    import render3d_amd as r3d
+    from render3d_amd import scene
+
+    lin = ctx.synth_sheets(n, n, n, nsheets, seed)
+    nb = sum(-(-max(n >> l, 1) // 128) ** 3 for l in range(args.levels))
+    abpa = 1
+    while abpa ** 3 < nb:
+        abpa += 1
+    b = r3d.Bricks(ctx, n, n, n, args.levels, abpa)
+    reps = max(3, args.steps)
+    build_ms = []
+    for i in range(reps + 1):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        b.build(lin, stream=stream)
+        torch.cuda.synchronize()
+        if i:
+            build_ms.append((time.perf_counter() - t) * 1e3)
+    del lin
+    view = scene.frame_params(cam, W, H, step_voxels=1.0, gate=scene.skip_gate(lut), brick_mode=b.brick_mode)
+    img = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    counters = torch.zeros(4, dtype=torch.int64, device=dev)
+    for _ in range(args.warmup):
+        b.raycast([view], W, H, alpha_cutoff=0.99, out=img, stream=stream)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        b.raycast([view], W, H, alpha_cutoff=0.99, out=img, counters=counters, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    cnt = counters.cpu().numpy().astype(np.uint64) // np.uint64(reps)
+    samples, skips, shaded, rays = (int(x) for x in cnt)
+    t = float(np.mean(ms))
+    # build: level 0 read once, each coarser level written and read once, every slot written
+    lv = [max(n >> l, 1) ** 3 for l in range(args.levels)]
+    build_bytes = lv[0] + 2 * sum(lv[1:]) + nb * 128 ** 3
+    bt = float(np.mean(build_ms))
+    res = {"bricks": nb, "slots": abpa ** 3, "atlas_bytes": abpa ** 3 * 128 ** 3,
+           "build_ms": bt, "build_GBps": build_bytes / (bt / 1e3) / 1e9,
+           "raycast_ms": t, "fps": 1000.0 / t, "samples_per_s": samples / (t / 1e3),
+           "samples_per_frame": samples, "skips_per_frame": skips, "shaded_per_frame": shaded,
+           "rays_per_frame": rays}
+    page = b.read_page()
+    atlas = b.read_slots(0, nb)
+    res["_state"] = {"page": page, "atlas": atlas, "abpa": abpa, "view": view, "lut": lut, "img": img.cpu().numpy(),
+                     "W": W, "H": H}
+    b.close()
+    return res
+
+
+def cpu_bricks(state, args):
+    """The oracle's bricks-LOD raycaster (r3do_raycast_bricks) on the same
+    page table and atlas (read back from HBM) and the same frame, every 4th
+    row, repeated for at least a quarter of `cpu_seconds`, on the host's cores
+    (at most 16); and the GPU frame's rows against it."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    from render3d_amd import scene
+
+    cores = min(16, os.cpu_count() or 1)
+    nslots = state["abpa"] ** 3
+    atlas = np.zeros(nslots * orc.SLOT_VOX, np.uint8)
+    atlas[:state["atlas"].size] = state["atlas"].reshape(-1)
+    ob = orc.OracleBricks(state["page"], atlas, state["abpa"])
+    W, H, rs = state["W"], state["H"], 4
+    tf = scene.tf_unorm(state["lut"])
+    t = time.perf_counter()
+    reps, samples, rays = 0, 0, 0
+    while True:
+        ref, cnt, _ = orc.raycast_bricks(ob, tf, [state["view"]], W, H, alpha_cutoff=0.99, row0=0, row_step=rs,
+                                         nthreads=cores)
+        reps += 1
+        samples += cnt["samples"]
+        rays += cnt["rays"]
+        if time.perf_counter() - t >= args.cpu_seconds / 4:
+            break
+    dt = time.perf_counter() - t
+    rows = np.arange(0, H, rs)
+    err = float(np.max(np.abs(state["img"][rows] - ref[rows])))
+    cpu = {"value": samples / dt, "unit": "ray samples/s", "cores": cores, "kind": "port",
+           "sample": f"{reps} x the {W}x{H} bricks-LOD frame (every {rs}th row), {rays} rays, {samples} samples, "
+                     f"{dt:.1f} s"}
+    return cpu, {"raycast_max_abs": err, "raycast_rows_checked": int(rows.size)}
 
 
 def cpu_blobs(state, args):

@@ -25,8 +25,9 @@ def _ptr(t):
 
 
 def _stream(stream):
+    """A HIP stream handle: an int, a torch stream, or None for torch's current stream."""
     if stream is not None:
-        return C.c_void_p(stream)
+        return C.c_void_p(getattr(stream, "cuda_stream", stream))
     if torch is not None and torch.cuda.is_available():
         return C.c_void_p(torch.cuda.current_stream().cuda_stream)
     return None
@@ -515,11 +516,15 @@ class Bricks:
 
     def read_slot(self, slot):
         """Slot `slot` of the atlas as uint8 [128, 128, 128] (z, y, x) on the host."""
+        return self.read_slots(slot, 1)[0]
+
+    def read_slots(self, first, n):
+        """Slots first..first+n-1 as uint8 [n, 128, 128, 128] on the host (synchronous)."""
         _, _, atlas, nslots = self.tables()
-        if not 0 <= slot < nslots:
-            raise IndexError(slot)
-        out = np.empty((128, 128, 128), np.uint8)
-        _hip_copy(out.ctypes.data, atlas + slot * out.nbytes, out.nbytes)
+        if not (0 <= first and n >= 0 and first + n <= nslots):
+            raise IndexError((first, n))
+        out = np.empty((n, 128, 128, 128), np.uint8)
+        _hip_copy(out.ctypes.data, atlas + first * 128 ** 3, out.nbytes)
         return out
 
     def write_page(self, page):
