@@ -33,44 +33,91 @@ struct Sampler {
   const LevelDev *lev;  // LDS
   uint32_t nlev;
 
-  __device__ __forceinline__ float tri(uint32_t l, V3 p) const {
+  // A trilinear footprint: its four x-pairs of voxels as loaded (low byte at
+  // x0, high byte at x0 + 1) and the fractions. The bytes are taken apart
+  // only when the footprint is used, so a footprint can stay in flight.
+  struct Foot {
+    uint32_t pr[4];
+    float fx, fy, fz;
+  };
+
+  __device__ __forceinline__ static uint32_t pair_at(const uint8_t *q) {
+    uint16_t v;
+    __builtin_memcpy(&v, q, 2);
+    return v;
+  }
+
+  __device__ __forceinline__ Foot load(uint32_t l, V3 p) const {
     const LevelDev L = lev[l];
     const float tx = fclamp_sel(p.x * L.fx - 0.5f, 0.0f, L.fx - 1.0f);
     const float ty = fclamp_sel(p.y * L.fy - 0.5f, 0.0f, L.fy - 1.0f);
     const float tz = fclamp_sel(p.z * L.fz - 0.5f, 0.0f, L.fz - 1.0f);
     const uint32_t x0 = (uint32_t)tx, y0 = (uint32_t)ty, z0 = (uint32_t)tz;
-    const float fx = tx - (float)x0, fy = ty - (float)y0, fz = tz - (float)z0;
+    Foot f;
+    f.fx = tx - (float)x0;
+    f.fy = ty - (float)y0;
+    f.fz = tz - (float)z0;
     const uint32_t b = ((z0 >> 4) * L.by + (y0 >> 4)) * L.bx + (x0 >> 4);
     const uint64_t off = ((uint64_t)L.off_hi << 32) | L.off_lo;
     const uint8_t *q = base + off + (uint64_t)b * R3D_BRICK_BYTES +
                        (((z0 & 15u) * 17u + (y0 & 15u)) * 17u + (x0 & 15u));
-    const float v000 = (float)q[0], v001 = (float)q[1], v010 = (float)q[17], v011 = (float)q[18];
-    const float v100 = (float)q[289], v101 = (float)q[290], v110 = (float)q[306], v111 = (float)q[307];
-    const float c00 = flerp(v000, v001, fx);
-    const float c10 = flerp(v010, v011, fx);
-    const float c01 = flerp(v100, v101, fx);
-    const float c11 = flerp(v110, v111, fx);
-    return flerp(flerp(c00, c10, fy), flerp(c01, c11, fy), fz);
+    f.pr[0] = pair_at(q);
+    f.pr[1] = pair_at(q + 17);
+    f.pr[2] = pair_at(q + 289);
+    f.pr[3] = pair_at(q + 306);
+    return f;
   }
 
+  __device__ __forceinline__ static float lerp3(const Foot &f) {
+    const float c00 = flerp((float)(f.pr[0] & 255u), (float)(f.pr[0] >> 8), f.fx);
+    const float c10 = flerp((float)(f.pr[1] & 255u), (float)(f.pr[1] >> 8), f.fx);
+    const float c01 = flerp((float)(f.pr[2] & 255u), (float)(f.pr[2] >> 8), f.fx);
+    const float c11 = flerp((float)(f.pr[3] & 255u), (float)(f.pr[3] >> 8), f.fx);
+    return flerp(flerp(c00, c10, f.fy), flerp(c01, c11, f.fy), f.fz);
+  }
+
+  __device__ __forceinline__ float tri(uint32_t l, V3 p) const { return lerp3(load(l, p)); }
+
+  // Two samples at one lod with all four footprints loaded before any is
+  // used (a scheduling barrier keeps the loads together): one memory round
+  // trip for a pair of gradient taps.
+  __device__ __forceinline__ void sample_pair(V3 pa, V3 pb, float lod, float *sa, float *sb) const {
+    const uint32_t d = (uint32_t)lod;
+    const float w = lod - (float)d;
+    const bool blend = w > 0.0f && d + 1u < nlev;
+    const uint32_t d1 = blend ? d + 1u : d;
+    const Foot a0 = load(d, pa), a1 = load(d1, pa), b0 = load(d, pb), b1 = load(d1, pb);
+    __builtin_amdgcn_sched_barrier(0);
+    const float ra0 = lerp3(a0), ra1 = lerp3(a1), rb0 = lerp3(b0), rb1 = lerp3(b1);
+    *sa = (blend ? flerp(ra0, ra1, w) : ra0) * K255;
+    *sb = (blend ? flerp(rb0, rb1, w) : rb0) * K255;
+  }
+
+  // Both levels' footprints are read unconditionally (the second is level d
+  // again when no blend is taken), so a sample costs one memory round trip
+  // instead of two; the blend is then a select.
   __device__ __forceinline__ float sample(V3 p, float lod) const {
     const uint32_t d = (uint32_t)lod;
     const float w = lod - (float)d;
-    float s = tri(d, p);
-    if (w > 0.0f && d + 1u < nlev) s = flerp(s, tri(d + 1u, p), w);
+    const bool blend = w > 0.0f && d + 1u < nlev;
+    const float s0 = tri(d, p), s1 = tri(blend ? d + 1u : d, p);
+    const float s = blend ? flerp(s0, s1, w) : s0;
     return s * K255;
   }
 };
 
-__device__ __forceinline__ float occ_sample(const VolDev &V, V3 p) {
+// The occupancy byte of the 8^3 block holding p (clamped to the grid).
+__device__ __forceinline__ uint32_t occ_load(const VolDev &V, V3 p) {
   const float ox = __builtin_floorf(p.x * (float)V.ox);
   const float oy = __builtin_floorf(p.y * (float)V.oy);
   const float oz = __builtin_floorf(p.z * (float)V.oz);
   const uint32_t ix = (uint32_t)fclamp_sel(ox, 0.0f, (float)(V.ox - 1u));
   const uint32_t iy = (uint32_t)fclamp_sel(oy, 0.0f, (float)(V.oy - 1u));
   const uint32_t iz = (uint32_t)fclamp_sel(oz, 0.0f, (float)(V.oz - 1u));
-  return (float)V.occ[((size_t)iz * V.oy + iy) * V.ox + ix] * K255;
+  return V.occ[((size_t)iz * V.oy + iy) * V.ox + ix];
 }
+
+__device__ __forceinline__ float occ_sample(const VolDev &V, V3 p) { return (float)occ_load(V, p) * K255; }
 
 __device__ __forceinline__ float4 tf_sample(const float4 *tf, float v) {
   const float t = fclamp_sel(v * 256.0f - 0.5f, 0.0f, 255.0f);

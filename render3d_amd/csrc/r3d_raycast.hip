@@ -57,17 +57,88 @@ struct RayArgs {
   uint32_t *queue;     // tile counter of a persistent launch, or NULL
 };
 
+// Samples a ray issues per memory round trip, per kernel kind.
+#ifndef R3D_RAY_BATCH_FULL
+#define R3D_RAY_BATCH_FULL 2
+#endif
+#ifndef R3D_RAY_BATCH_MIP
+#define R3D_RAY_BATCH_MIP 3
+#endif
+#ifndef R3D_RAY_BATCH_FLAT
+#define R3D_RAY_BATCH_FLAT 3
+#endif
+template <int KIND>
+constexpr int RAY_BATCH = KIND == KIND_FULL ? R3D_RAY_BATCH_FULL
+                          : KIND == KIND_MIP ? R3D_RAY_BATCH_MIP
+                                             : R3D_RAY_BATCH_FLAT;
+
 struct RayOut {
   float c[4];
   uint32_t steps, skips, shaded;
 };
 
+// Wave-cooperative gradient taps (FULL): the lanes whose sample is shaded
+// publish (p, e, lod) in LDS, every active lane evaluates one tap of one of
+// them per round (6 taps per shaded lane: +x, -x, +y, -y, +z, -z, each the
+// same sample call the lane would make itself), and the results come back
+// through LDS. A wave with k shaded lanes and n active lanes pays
+// ceil(6k / n) samples instead of six. buf holds 64 x 5 + 384 floats.
+constexpr uint32_t COOP_FLOATS = 64u * 5u + 384u;
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ void coop_taps(const Sampler &S, float *buf, bool need, V3 p, float e, float lod,
+                                          float g[6]) {
+  const uint64_t act = __ballot(1);
+  const uint64_t m = __ballot(need);
+  if (m == 0) return;
+  const uint32_t k = (uint32_t)__popcll(m), nact = (uint32_t)__popcll(act);
+  const uint32_t rn = lanes_below(m), ra = lanes_below(act);
+  float *pub = buf, *res = buf + 320;
+  if (need) {
+    pub[rn * 5u + 0u] = p.x;
+    pub[rn * 5u + 1u] = p.y;
+    pub[rn * 5u + 2u] = p.z;
+    pub[rn * 5u + 3u] = e;
+    pub[rn * 5u + 4u] = lod;
+  }
+  wave_sync();
+  const uint32_t njobs = 6u * k;
+  for (uint32_t base = 0; base < njobs; base += nact) {
+    const uint32_t job = base + ra;
+    if (job < njobs) {
+      const uint32_t o = job / 6u, tap = job - 6u * o;
+      const float qx = pub[o * 5u + 0u], qy = pub[o * 5u + 1u], qz = pub[o * 5u + 2u];
+      const float qe = pub[o * 5u + 3u], ql = pub[o * 5u + 4u];
+      const bool neg = (tap & 1u) != 0u;
+      const float sh = tap < 2u ? qx : tap < 4u ? qy : qz;
+      const float moved = neg ? sh - qe : sh + qe;
+      const V3 q = mk(tap < 2u ? moved : qx, tap - 2u < 2u ? moved : qy, tap >= 4u ? moved : qz);
+      res[job] = S.sample(q, ql);
+    }
+  }
+  wave_sync();
+  if (need) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) g[i] = res[rn * 6u + (uint32_t)i];
+  }
+  wave_sync();  // the buffer is reused by the next step
+}
+
 // The per-ray march. KIND fixes the compositing / shading branches at compile
 // time, as the reference builds one pipeline per mode (src/shaders/shaders.cmake:17-30).
 template <int KIND>
 __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const float4 *tf,
-                                        const r3d_frame_params &pc, float cutoff, uint32_t px,
-                                        uint32_t py) {
+                                        const r3d_frame_params &pc, float cutoff, uint32_t px, uint32_t py,
+                                        float *coop) {
   RayOut R;
   R.steps = R.skips = R.shaded = 0;
   const uint32_t mode = KIND == KIND_FULL ? MODE_FULL : KIND == KIND_MIP ? MODE_MIP : KIND == KIND_FLAT ? MODE_FLAT : pc.mode;
@@ -119,7 +190,6 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
   float ar = 0.0f, ag = 0.0f, ab = 0.0f, aa = 0.0f;
   uint32_t steps = 0, skips = 0, shaded_n = 0;
   const bool composite = mode == MODE_FULL || mode == MODE_FLAT;
-  const bool shaded = mode == MODE_FULL;
   const float span_step = fmax_sel((tfar - tn) / ((float)MAX_STEPS * dt), 1.0f);
   const float bs = 8.0f * voxel;
   const float occ_gate = fmax_sel(pc.skip_gate, 0.5f / 255.0f);
@@ -129,16 +199,46 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
   const V3 key = mk(0x1.a20bd8p-2f, 0x1.a20bd8p-1f, -0x1.a20bd8p-2f);  // normalize(0.4,0.8,-0.4)
   const float sgx = dir.x > 0.0f ? 1.0f : 0.0f, sgy = dir.y > 0.0f ? 1.0f : 0.0f,
               sgz = dir.z > 0.0f ? 1.0f : 0.0f;
-  while (t < tfar && steps < MAX_STEPS) {
+  // One sample's position, lod and reads (both levels' footprints and the
+  // occupancy byte). Reads are clamped into the volume, so a sample issued
+  // past the end of the ray is always valid to read and simply dropped.
+  struct Pending {
+    float lod, lod_step;
+    V3 p;
+    Sampler::Foot f0, f1;
+    uint32_t occ;
+  };
+  // view fields the loop reads, held in registers (a read of the view in the
+  // loop would wait behind the sample reads in flight)
+  const float lod_bias = pc.lod_bias, threshold = pc.threshold;
+  const auto issue = [&](float tt) {
+    Pending q;
+    const float foot = ortho ? cone : tt * cone;
+    q.lod = fclamp_sel(log2_pinned(fmax_sel(foot / voxel, 1.0f)) + lod_bias, 0.0f, max_lod);
+    q.lod_step = fmax_sel(exp2_pinned(q.lod), span_step);
+    q.p = vadd(ro, vscale(dir, tt));
+    const uint32_t d = (uint32_t)q.lod;
+    const bool blend = q.lod - (float)d > 0.0f && d + 1u < S.nlev;
+    q.f0 = S.load(d, q.p);
+    q.f1 = S.load(blend ? d + 1u : d, q.p);
+    q.occ = occ_load(V, q.p);
+    return q;
+  };
+  // One march step on the sample in cur (issued at the current t): 0 ends
+  // the ray, 1 advanced t by dt * lod_step, 2 skipped to another t.
+  const auto step = [&](const Pending &cur) -> int {
     steps++;
-    const float foot = ortho ? cone : t * cone;
-    const float lod = fclamp_sel(log2_pinned(fmax_sel(foot / voxel, 1.0f)) + pc.lod_bias, 0.0f, max_lod);
-    const float lod_step = fmax_sel(exp2_pinned(lod), span_step);
-    const V3 p = vadd(ro, vscale(dir, t));
-    float v = S.sample(p, lod);
-    const bool cut = v < pc.threshold;
+    const float lod = cur.lod, lod_step = cur.lod_step;
+    const V3 p = cur.p;
+    const float t_next = t + dt * lod_step;
+    const uint32_t d = (uint32_t)lod;
+    const float w = lod - (float)d;
+    const float s0 = Sampler::lerp3(cur.f0), s1 = Sampler::lerp3(cur.f1);
+    float v = (w > 0.0f && d + 1u < S.nlev ? flerp(s0, s1, w) : s0) * K255;
+    const float occ = (float)cur.occ * K255;
+    const bool cut = v < threshold;
     if (cut) v = 0.0f;
-    if (v < occ_gate && occ_sample(V, p) < occ_gate) {  // raycast.slang:866-869
+    if ((v < occ_gate) & (occ < occ_gate)) {  // raycast.slang:866-869
       const float nx = (__builtin_floorf(p.x / bs) + sgx) * bs;
       const float ny = (__builtin_floorf(p.y / bs) + sgy) * bs;
       const float nz = (__builtin_floorf(p.z / bs) + sgz) * bs;
@@ -147,46 +247,79 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
       const float ez = __builtin_fabsf(dir.z) > 1e-8f ? (nz - ro.z) / dir.z : 1e30f;
       t = fmax_sel(fmin_sel(fmin_sel(ex, ey), ez), t) + bs * 0.01f;
       skips++;
-      continue;
+      return 2;
     }
     if (v > mip_max) mip_max = v;
-    if (mode == MODE_MIP && mip_max >= 0.999f) break;
+    if (mode == MODE_MIP && mip_max >= 0.999f) return 0;
     if (KIND == KIND_OTHER && first_hit < 0.0f && v > 0.05f) first_hit = t;
-    if (composite && !cut) {
+    if (KIND == KIND_FULL) {
+      // every active lane takes part in the taps, so the TF and opacity are
+      // evaluated for all of them (a cut sample simply does not use them)
       const float4 s = tf_sample(tf, v);
       const float a = 1.0f - pow_unit(1.0f - s.w, opac_k * lod_step);
-      if (a > 0.003f) {
+      const bool lit_here = !cut && a > 0.003f;
+      float gt[6];
+      coop_taps(S, coop, lit_here, p, voxel * lod_step, lod, gt);
+      if (lit_here) {
         float cr = s.x, cg = s.y, cb = s.z;
-        if (shaded) {
-          shaded_n++;
-          const float e = voxel * lod_step;
-          const float gx = S.sample(mk(p.x + e, p.y, p.z), lod) - S.sample(mk(p.x - e, p.y, p.z), lod);
-          const float gy = S.sample(mk(p.x, p.y + e, p.z), lod) - S.sample(mk(p.x, p.y - e, p.z), lod);
-          const float gz = S.sample(mk(p.x, p.y, p.z + e), lod) - S.sample(mk(p.x, p.y, p.z - e), lod);
-          const V3 g = mk(gx, gy, gz);
-          const float gm = vlen(g);
-          if (gm > 1e-4f) {
-            const V3 n = mk(g.x / gm, g.y / gm, g.z / gm);
-            const float diff = 0.7f * __builtin_fabsf(vdot(n, lvec)) + 0.3f * __builtin_fabsf(vdot(n, key));
-            const float spec = pow24(__builtin_fabsf(vdot(n, hvec))) * 0.3f;
-            const float lit = 0.25f + 0.75f * diff;
-            const float w = fsat(gm * 6.0f);
-            const float m = flerp(1.0f, lit, w);
-            const float sw = spec * w;
-            cr = cr * m + sw;
-            cg = cg * m + sw;
-            cb = cb * m + sw;
-          }
+        shaded_n++;
+        const V3 g = mk(gt[0] - gt[1], gt[2] - gt[3], gt[4] - gt[5]);
+        const float gm = vlen(g);
+        if (gm > 1e-4f) {
+          const V3 n = mk(g.x / gm, g.y / gm, g.z / gm);
+          const float diff = 0.7f * __builtin_fabsf(vdot(n, lvec)) + 0.3f * __builtin_fabsf(vdot(n, key));
+          const float spec = pow24(__builtin_fabsf(vdot(n, hvec))) * 0.3f;
+          const float lit = 0.25f + 0.75f * diff;
+          const float w = fsat(gm * 6.0f);
+          const float m = flerp(1.0f, lit, w);
+          const float sw = spec * w;
+          cr = cr * m + sw;
+          cg = cg * m + sw;
+          cb = cb * m + sw;
         }
         const float wa = (1.0f - aa) * a;
         ar = ar + wa * cr;
         ag = ag + wa * cg;
         ab = ab + wa * cb;
         aa = aa + wa;
-        if (aa >= cutoff) break;
+        if (aa >= cutoff) return 0;
+      }
+    } else if (composite && !cut) {
+      const float4 s = tf_sample(tf, v);
+      const float a = 1.0f - pow_unit(1.0f - s.w, opac_k * lod_step);
+      if (a > 0.003f) {
+        const float wa = (1.0f - aa) * a;
+        ar = ar + wa * s.x;
+        ag = ag + wa * s.y;
+        ab = ab + wa * s.z;
+        aa = aa + wa;
+        if (aa >= cutoff) return 0;
       }
     }
-    t += dt * lod_step;
+    t = t_next;
+    return 1;
+  };
+  // BATCH consecutive samples are issued together (one memory round trip
+  // for all); each is used only if the ones before it neither ended the ray
+  // nor skipped (a skip issues a new batch from the new t).
+  constexpr int BATCH = RAY_BATCH<KIND>;
+  bool live = true;
+  while (live && t < tfar && steps < MAX_STEPS) {
+    Pending q[BATCH];
+    float tq = t;
+#pragma unroll
+    for (int i = 0; i < BATCH; i++) {
+      q[i] = issue(tq);
+      tq = tq + dt * q[i].lod_step;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < BATCH; i++) {
+      if (i > 0 && !(t < tfar && steps < MAX_STEPS)) break;
+      const int r = step(q[i]);
+      if (r == 0) live = false;
+      if (r != 1) break;
+    }
   }
   R.steps = steps;
   R.skips = skips;
@@ -570,6 +703,7 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
   __shared__ uint32_t s_ts[R3D_MAX_VIEWS_PER_CALL + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint4 s_red[NW];
+  __shared__ float s_coop[KIND == KIND_FULL && !BRICKS ? NW * COOP_FLOATS : 1];  // per wave: gradient taps
   const uint32_t lane = threadIdx.x;
   for (uint32_t i = lane; i < 256u; i += NT)
     s_tf[i] = make_float4(a.tf[i * 4 + 0], a.tf[i * 4 + 1], a.tf[i * 4 + 2], a.tf[i * 4 + 3]);
@@ -608,7 +742,8 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
     if (tx < RW && ty < RH) {
       const uint32_t lx = half ? tx * 2u : tx, ly = half ? ty * 2u : ty;
       const RayOut R = BRICKS ? march_bricks<KIND>(a.bk, s_hdr, s_tf, pc, a.cutoff, lx, ly)
-                              : march<KIND>(S, a.vol, s_tf, pc, a.cutoff, lx, ly);
+                              : march<KIND>(S, a.vol, s_tf, pc, a.cutoff, lx, ly,
+                                            s_coop + (KIND == KIND_FULL && !BRICKS ? (lane >> 6) * COOP_FLOATS : 0u));
       st += R.steps;
       sk += R.skips;
       sh += R.shaded;
