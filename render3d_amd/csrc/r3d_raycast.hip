@@ -78,9 +78,9 @@ struct RayOut {
 };
 
 // Wave-cooperative gradient taps (FULL): the lanes whose sample is shaded
-// publish (p, e, lod) in LDS, every active lane evaluates one tap of one of
+// publish (p, e, aux) in LDS, every active lane evaluates one tap of one of
 // them per round (6 taps per shaded lane: +x, -x, +y, -y, +z, -z, each the
-// same sample call the lane would make itself), and the results come back
+// same fetch the lane would make itself), and the results come back
 // through LDS. A wave with k shaded lanes and n active lanes pays
 // ceil(6k / n) samples instead of six. buf holds 64 x 5 + 384 floats.
 constexpr uint32_t COOP_FLOATS = 64u * 5u + 384u;
@@ -95,7 +95,10 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-__device__ __forceinline__ void coop_taps(const Sampler &S, float *buf, bool need, V3 p, float e, float lod,
+// fetch(q, aux) evaluates one tap at position q; aux is the owner's fifth
+// published word (the cube path's lod, the bricks path's slot).
+template <class Fetch>
+__device__ __forceinline__ void coop_taps(const Fetch &fetch, float *buf, bool need, V3 p, float e, float aux,
                                           float g[6]) {
   const uint64_t act = __ballot(1);
   const uint64_t m = __ballot(need);
@@ -108,7 +111,7 @@ __device__ __forceinline__ void coop_taps(const Sampler &S, float *buf, bool nee
     pub[rn * 5u + 1u] = p.y;
     pub[rn * 5u + 2u] = p.z;
     pub[rn * 5u + 3u] = e;
-    pub[rn * 5u + 4u] = lod;
+    pub[rn * 5u + 4u] = aux;
   }
   wave_sync();
   const uint32_t njobs = 6u * k;
@@ -122,7 +125,7 @@ __device__ __forceinline__ void coop_taps(const Sampler &S, float *buf, bool nee
       const float sh = tap < 2u ? qx : tap < 4u ? qy : qz;
       const float moved = neg ? sh - qe : sh + qe;
       const V3 q = mk(tap < 2u ? moved : qx, tap - 2u < 2u ? moved : qy, tap >= 4u ? moved : qz);
-      res[job] = S.sample(q, ql);
+      res[job] = fetch(q, ql);
     }
   }
   wave_sync();
@@ -259,7 +262,7 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
       const float a = 1.0f - pow_unit(1.0f - s.w, opac_k * lod_step);
       const bool lit_here = !cut && a > 0.003f;
       float gt[6];
-      coop_taps(S, coop, lit_here, p, voxel * lod_step, lod, gt);
+      coop_taps([&](V3 q, float ql) { return S.sample(q, ql); }, coop, lit_here, p, voxel * lod_step, lod, gt);
       if (lit_here) {
         float cr = s.x, cg = s.y, cb = s.z;
         shaded_n++;
@@ -473,7 +476,8 @@ __device__ __forceinline__ BCtx locate_cached(const BricksDev &B, const uint32_t
 
 template <int KIND>
 __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_t *hdr, const float4 *tf,
-                                               const r3d_frame_params &pc, float cutoff, uint32_t px, uint32_t py) {
+                                               const r3d_frame_params &pc, float cutoff, uint32_t px, uint32_t py,
+                                               float *coop) {
   RayOut R;
   R.steps = R.skips = R.shaded = 0;
   const uint32_t mode = KIND == KIND_FULL ? MODE_FULL : KIND == KIND_MIP ? MODE_MIP : KIND == KIND_FLAT ? MODE_FLAT : pc.mode;
@@ -603,40 +607,46 @@ __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_
     if (v > mip_max) mip_max = v;
     if (mode == MODE_MIP && mip_max >= 0.999f) break;
     if (KIND == KIND_OTHER && first_hit < 0.0f && v > 0.05f) first_hit = t;
-    if (composite && !cut) {
+    if (KIND == KIND_FULL) {  // every active lane takes part in the taps (see march)
       const float4 s = tf_sample(tf, v);
       const float a = 1.0f - pow_unit(1.0f - s.w, opac_k * lod_step);
-      if (a > 0.003f) {
+      const bool lit_here = !cut && a > 0.003f;
+      float gt[6];
+      // taps reuse the located brick (raycast.slang:964-973); x + -ev is x - ev
+      coop_taps([&](V3 q, float slot) { return brick_fetch(B.atlas, __builtin_bit_cast(uint32_t, slot), q); }, coop,
+                lit_here, bc.bl, bc.grad_vox, __builtin_bit_cast(float, bc.slot), gt);
+      if (lit_here) {
         float cr = s.x, cg = s.y, cb = s.z;
-        if (shaded) {
-          shaded_n++;
-          const float ev = bc.grad_vox;
-          const V3 b = bc.bl;
-          const float gx = brick_fetch(B.atlas, bc.slot, mk(b.x + ev, b.y, b.z)) -
-                           brick_fetch(B.atlas, bc.slot, mk(b.x + -ev, b.y, b.z));
-          const float gy = brick_fetch(B.atlas, bc.slot, mk(b.x, b.y + ev, b.z)) -
-                           brick_fetch(B.atlas, bc.slot, mk(b.x, b.y + -ev, b.z));
-          const float gz = brick_fetch(B.atlas, bc.slot, mk(b.x, b.y, b.z + ev)) -
-                           brick_fetch(B.atlas, bc.slot, mk(b.x, b.y, b.z + -ev));
-          const V3 g = mk(gx, gy, gz);
-          const float gm = vlen(g);
-          if (gm > 1e-4f) {
-            const V3 n = mk(g.x / gm, g.y / gm, g.z / gm);
-            const float diff = 0.7f * __builtin_fabsf(vdot(n, lvec)) + 0.3f * __builtin_fabsf(vdot(n, key));
-            const float spec = pow24(__builtin_fabsf(vdot(n, hvec))) * 0.3f;
-            const float lit = 0.25f + 0.75f * diff;
-            const float w = fsat(gm * 6.0f);
-            const float m = flerp(1.0f, lit, w);
-            const float sw = spec * w;
-            cr = cr * m + sw;
-            cg = cg * m + sw;
-            cb = cb * m + sw;
-          }
+        shaded_n++;
+        const V3 g = mk(gt[0] - gt[1], gt[2] - gt[3], gt[4] - gt[5]);
+        const float gm = vlen(g);
+        if (gm > 1e-4f) {
+          const V3 n = mk(g.x / gm, g.y / gm, g.z / gm);
+          const float diff = 0.7f * __builtin_fabsf(vdot(n, lvec)) + 0.3f * __builtin_fabsf(vdot(n, key));
+          const float spec = pow24(__builtin_fabsf(vdot(n, hvec))) * 0.3f;
+          const float lit = 0.25f + 0.75f * diff;
+          const float w = fsat(gm * 6.0f);
+          const float m = flerp(1.0f, lit, w);
+          const float sw = spec * w;
+          cr = cr * m + sw;
+          cg = cg * m + sw;
+          cb = cb * m + sw;
         }
         const float wa = (1.0f - aa) * a;
         ar = ar + wa * cr;
         ag = ag + wa * cg;
         ab = ab + wa * cb;
+        aa = aa + wa;
+        if (aa >= cutoff) break;
+      }
+    } else if (composite && !cut) {
+      const float4 s = tf_sample(tf, v);
+      const float a = 1.0f - pow_unit(1.0f - s.w, opac_k * lod_step);
+      if (a > 0.003f) {
+        const float wa = (1.0f - aa) * a;
+        ar = ar + wa * s.x;
+        ag = ag + wa * s.y;
+        ab = ab + wa * s.z;
         aa = aa + wa;
         if (aa >= cutoff) break;
       }
@@ -703,7 +713,7 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
   __shared__ uint32_t s_ts[R3D_MAX_VIEWS_PER_CALL + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint4 s_red[NW];
-  __shared__ float s_coop[KIND == KIND_FULL && !BRICKS ? NW * COOP_FLOATS : 1];  // per wave: gradient taps
+  __shared__ float s_coop[KIND == KIND_FULL ? NW * COOP_FLOATS : 1];  // per wave: gradient taps
   const uint32_t lane = threadIdx.x;
   for (uint32_t i = lane; i < 256u; i += NT)
     s_tf[i] = make_float4(a.tf[i * 4 + 0], a.tf[i * 4 + 1], a.tf[i * 4 + 2], a.tf[i * 4 + 3]);
@@ -741,9 +751,9 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
     const uint32_t ty = (lt / tiles_x) * TH + (lane / TW);
     if (tx < RW && ty < RH) {
       const uint32_t lx = half ? tx * 2u : tx, ly = half ? ty * 2u : ty;
-      const RayOut R = BRICKS ? march_bricks<KIND>(a.bk, s_hdr, s_tf, pc, a.cutoff, lx, ly)
-                              : march<KIND>(S, a.vol, s_tf, pc, a.cutoff, lx, ly,
-                                            s_coop + (KIND == KIND_FULL && !BRICKS ? (lane >> 6) * COOP_FLOATS : 0u));
+      float *coop = s_coop + (KIND == KIND_FULL ? (lane >> 6) * COOP_FLOATS : 0u);
+      const RayOut R = BRICKS ? march_bricks<KIND>(a.bk, s_hdr, s_tf, pc, a.cutoff, lx, ly, coop)
+                              : march<KIND>(S, a.vol, s_tf, pc, a.cutoff, lx, ly, coop);
       st += R.steps;
       sk += R.skips;
       sh += R.shaded;
