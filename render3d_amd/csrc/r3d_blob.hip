@@ -267,6 +267,13 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
     // iterations later.
     enum : uint32_t { PH_SIG, PH_MAG, PH_EGP, PH_SIGN, PH_DONE };
     uint32_t p = 0, recent = 0, m = 0, nb = 0;
+    // Pairs go out 8 at a time (two 16-byte stores filling a 32-byte
+    // sector) from a per-lane shift register, when the capacity and the
+    // array keep such groups aligned: a scattered 4-byte store per pair made
+    // the memory system fetch and write whole sectors, and on this ISA every
+    // store also holds up the next wait for a window refill.
+    const bool vec8 = (a.cap & 7u) == 0u && ((uintptr_t)a.pairs & 31u) == 0u;
+    uint32_t pb0 = 0, pb1 = 0, pb2 = 0, pb3 = 0, pb4 = 0, pb5 = 0, pb6 = 0, pb7 = 0;
     uint32_t e = s_scan[0];
     uint32_t ph = eob == 0u ? PH_DONE : eob == 1u ? PH_MAG : PH_SIG;
     if (eob == 1u) recent = 1u;
@@ -294,7 +301,24 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
           st = st == 1 ? -2 : st;  // more nonzero levels than the capacity
         } else {
           const uint32_t lv = (uint32_t)(uint16_t)(int16_t)(b ? -(int32_t)m : (int32_t)m);
-          a.pairs[base + np] = (e & 4095u) | (lv << 16);
+          const uint32_t pr = (e & 4095u) | (lv << 16);
+          if (vec8) {
+            pb0 = pb1;
+            pb1 = pb2;
+            pb2 = pb3;
+            pb3 = pb4;
+            pb4 = pb5;
+            pb5 = pb6;
+            pb6 = pb7;
+            pb7 = pr;
+            if ((np & 7u) == 7u) {
+              uint4 *q = (uint4 *)(a.pairs + base + np - 7u);
+              q[0] = make_uint4(pb0, pb1, pb2, pb3);
+              q[1] = make_uint4(pb4, pb5, pb6, pb7);
+            }
+          } else {
+            a.pairs[base + np] = pr;
+          }
           np++;
         }
       }
@@ -320,6 +344,11 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
         d.merge();
         d.refill();
       }
+    }
+    if (vec8) {  // the last np % 8 pairs, oldest first, are the newest registers
+      const uint32_t r = np & 7u;
+      const uint32_t tail[8] = {pb0, pb1, pb2, pb3, pb4, pb5, pb6, pb7};
+      for (uint32_t k = 0; k < r; k++) a.pairs[base + np - r + k] = tail[8u - r + k];
     }
     if (has_corr) {  // dct3d.c:572-587
       uint32_t cnt = d.eg() + 1u, pos = 0;
