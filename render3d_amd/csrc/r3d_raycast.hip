@@ -161,6 +161,9 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
     return R;
   }
   const float voxel = 1.0f / (float)V.maxdim;
+  // with a power-of-two largest dim, voxel is exact and x / voxel == x * maxdim
+  const bool dim_p2 = (V.maxdim & (V.maxdim - 1u)) == 0u;
+  const float fdim = (float)V.maxdim;
   const float max_lod = fmin_sel(pc.max_mip, (float)(V.nlev - 1u));
   V3 org = mk(pc.cam_origin[0], pc.cam_origin[1], pc.cam_origin[2]);
   if (ortho) org = vadd(vadd(org, vscale(right, ndx)), vscale(up, ndy));
@@ -217,7 +220,8 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
   const auto issue = [&](float tt) {
     Pending q;
     const float foot = ortho ? cone : tt * cone;
-    q.lod = fclamp_sel(log2_pinned(fmax_sel(foot / voxel, 1.0f)) + lod_bias, 0.0f, max_lod);
+    const float fv = dim_p2 ? foot * fdim : foot / voxel;
+    q.lod = fclamp_sel(log2_pinned(fmax_sel(fv, 1.0f)) + lod_bias, 0.0f, max_lod);
     q.lod_step = fmax_sel(exp2_pinned(q.lod), span_step);
     q.p = vadd(ro, vscale(dir, tt));
     const uint32_t d = (uint32_t)q.lod;
@@ -370,6 +374,7 @@ struct BCtx {  // flags as full words: a bool's padding bytes were copied throug
 };
 
 struct LodCache {
+  float inv;  // 1 / scale of the cached level
   uint32_t valid;
   int desired;
   uint32_t scale, bx, by, bz;
@@ -401,8 +406,11 @@ __device__ __forceinline__ V3 base_voxel(const uint32_t *hdr, V3 p, float md) {
 
 // bricks_lod_locate_ex: desired level, +1, +2, then the coarsest. hdr is the
 // page header in LDS; entries are read from the page table.
+// scales_p2: every level's scale is a power of two (as built here), so
+// x / scale == x * (1 / scale) exactly and the divisions become products.
 __device__ __forceinline__ BCtx locate_ex(const BricksDev &B, const uint32_t *hdr, float md, V3 p, float lod,
-                                          uint32_t *scale_out, uint32_t *bx_o, uint32_t *by_o, uint32_t *bz_o) {
+                                          bool scales_p2, uint32_t *scale_out, float *inv_out, uint32_t *bx_o,
+                                          uint32_t *by_o, uint32_t *bz_o) {
   BCtx c;
   c.ok = 0u;
   c.maxv = 0.0f;
@@ -410,6 +418,7 @@ __device__ __forceinline__ BCtx locate_ex(const BricksDev &B, const uint32_t *hd
   c.slot = 0;
   c.grad_vox = 1.0f;
   *scale_out = 1u;
+  *inv_out = 1.0f;
   *bx_o = *by_o = *bz_o = 0u;
   const uint32_t nlev = hdr[0];
   const V3 bv = base_voxel(hdr, p, md);
@@ -422,8 +431,8 @@ __device__ __forceinline__ BCtx locate_ex(const BricksDev &B, const uint32_t *hd
     const uint32_t h = 4u + (uint32_t)li * 4u;
     const uint32_t off = hdr[h], packed = hdr[h + 1u], scale = hdr[h + 2u];
     const uint32_t bdx = packed & 1023u, bdy = (packed >> 10) & 1023u, bdz = packed >> 20;
-    const float fs = (float)scale;
-    const V3 lv = mk(bv.x / fs, bv.y / fs, bv.z / fs);
+    const float fs = (float)scale, inv = 1.0f / fs;
+    const V3 lv = scales_p2 ? mk(bv.x * inv, bv.y * inv, bv.z * inv) : mk(bv.x / fs, bv.y / fs, bv.z / fs);
     const uint32_t bcx = min((uint32_t)lv.x >> 7, bdx - 1u), bcy = min((uint32_t)lv.y >> 7, bdy - 1u),
                    bcz = min((uint32_t)lv.z >> 7, bdz - 1u);
     const uint32_t w = off + (bcz * bdy + bcy) * bdx + bcx;
@@ -433,9 +442,10 @@ __device__ __forceinline__ BCtx locate_ex(const BricksDev &B, const uint32_t *hd
     c.maxv = (float)(e >> 24) / 255.0f;
     c.slot = e & 0xFFFFFFu;
     c.bl = mk(lv.x - (float)(bcx << 7), lv.y - (float)(bcy << 7), lv.z - (float)(bcz << 7));
-    c.grad_vox = exp2_pinned(lod) / fs;
+    c.grad_vox = scales_p2 ? exp2_pinned(lod) * inv : exp2_pinned(lod) / fs;
     c.world_brick = BR_SLOT * fs / md;
     *scale_out = scale;
+    *inv_out = inv;
     *bx_o = bcx;
     *by_o = bcy;
     *bz_o = bcz;
@@ -448,13 +458,13 @@ __device__ __forceinline__ BCtx locate_ex(const BricksDev &B, const uint32_t *hd
 // cached brick at the same desired level (its level, grad_vox and brick size
 // are then the cached ones).
 __device__ __forceinline__ BCtx locate_cached(const BricksDev &B, const uint32_t *hdr, float md, V3 p, float lod,
-                                              LodCache &lc) {
+                                              bool scales_p2, LodCache &lc) {
   const uint32_t nlev = hdr[0];
   const int desired = (int)fclamp_sel(__builtin_floorf(lod), 0.0f, (float)(nlev - 1u));
   if (lc.valid && lc.desired == desired) {
     const V3 bv = base_voxel(hdr, p, md);
-    const float fs = (float)lc.scale;
-    const V3 lv = mk(bv.x / fs, bv.y / fs, bv.z / fs);
+    const float fs = (float)lc.scale, inv = lc.inv;
+    const V3 lv = scales_p2 ? mk(bv.x * inv, bv.y * inv, bv.z * inv) : mk(bv.x / fs, bv.y / fs, bv.z / fs);
     const uint32_t bcx = (uint32_t)lv.x >> 7, bcy = (uint32_t)lv.y >> 7, bcz = (uint32_t)lv.z >> 7;
     if (bcx == lc.bx && bcy == lc.by && bcz == lc.bz) {
       BCtx c = lc.c;
@@ -463,10 +473,12 @@ __device__ __forceinline__ BCtx locate_cached(const BricksDev &B, const uint32_t
     }
   }
   uint32_t scale, bx, by, bz;
-  const BCtx c = locate_ex(B, hdr, md, p, lod, &scale, &bx, &by, &bz);
+  float inv;
+  const BCtx c = locate_ex(B, hdr, md, p, lod, scales_p2, &scale, &inv, &bx, &by, &bz);
   lc.valid = c.ok;
   lc.desired = desired;
   lc.scale = scale;
+  lc.inv = inv;
   lc.bx = bx;
   lc.by = by;
   lc.bz = bz;
@@ -500,6 +512,14 @@ __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_
   const uint32_t mdu = max(hdr[1], max(hdr[2], hdr[3]));
   const float md = (float)mdu;
   const float voxel = 1.0f / md;
+  const bool md_p2 = (mdu & (mdu - 1u)) == 0u;  // then x / voxel == x * md
+  // levels' scales, all powers of two in a table built here (wave-uniform)
+  bool scales_p2 = true;
+  for (uint32_t l = 0; l < hdr[0]; l++) {
+    const uint32_t sc = hdr[6u + 4u * l];
+    scales_p2 = scales_p2 && sc != 0u && (sc & (sc - 1u)) == 0u;
+  }
+  scales_p2 = __builtin_amdgcn_readfirstlane(scales_p2 ? 1u : 0u) != 0u;
   const float max_lod = (float)(hdr[0] - 1u);
   const float emax = fmax_sel(sx, fmax_sel(sy, sz));
   const V3 ext = mk(sx / emax, sy / emax, sz / emax);
@@ -575,15 +595,17 @@ __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_
   lc.valid = 0u;
   lc.desired = -1;
   lc.scale = 1u;
+  lc.inv = 1.0f;
   lc.bx = lc.by = lc.bz = 0u;
   lc.c = BCtx{false, 0.0f, mk(0.0f, 0.0f, 0.0f), 0u, 1.0f, 0.0f};
   while (t < tfar && steps < budget) {
     steps++;
     const float foot = ortho ? cone : t * cone;
-    const float lod = fclamp_sel(log2_pinned(fmax_sel(foot / voxel, 1.0f)) + pc.lod_bias, 0.0f, max_lod);
+    const float fv = md_p2 ? foot * md : foot / voxel;
+    const float lod = fclamp_sel(log2_pinned(fmax_sel(fv, 1.0f)) + pc.lod_bias, 0.0f, max_lod);
     const float lod_step = fmax_sel(exp2_pinned(lod), span_step);
     const V3 p = vadd(ro, vscale(dir, t));
-    const BCtx bc = locate_cached(B, hdr, md, p, lod, lc);
+    const BCtx bc = locate_cached(B, hdr, md, p, lod, scales_p2, lc);
     float v = bc.ok ? brick_fetch(B.atlas, bc.slot, bc.bl) : 0.0f;
     const bool cut = v < pc.threshold;
     if (cut) v = 0.0f;
