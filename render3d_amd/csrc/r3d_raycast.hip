@@ -381,21 +381,44 @@ struct LodCache {
   BCtx c;
 };
 
-__device__ __forceinline__ float brick_fetch(const uint8_t *atlas, uint32_t slot, V3 bl) {
+// A slot footprint as loaded (bytes taken apart at use) and its fractions.
+struct BFoot {
+  uint32_t v[8];
+  float fx, fy, fz;
+};
+
+__device__ __forceinline__ BFoot brick_load(const uint8_t *atlas, uint32_t slot, V3 bl) {
   const float tx = fclamp_sel(bl.x, 0.5f, 127.5f) - 0.5f;
   const float ty = fclamp_sel(bl.y, 0.5f, 127.5f) - 0.5f;
   const float tz = fclamp_sel(bl.z, 0.5f, 127.5f) - 0.5f;
   const uint32_t x0 = (uint32_t)tx, y0 = (uint32_t)ty, z0 = (uint32_t)tz;
   const uint32_t dx = x0 < 127u ? 1u : 0u, dy = y0 < 127u ? 128u : 0u, dz = z0 < 127u ? 16384u : 0u;
-  const float fx = tx - (float)x0, fy = ty - (float)y0, fz = tz - (float)z0;
+  BFoot f;
+  f.fx = tx - (float)x0;
+  f.fy = ty - (float)y0;
+  f.fz = tz - (float)z0;
   const uint8_t *q = atlas + (uint64_t)slot * BR_SLOT_VOX + ((z0 * 128u + y0) * 128u + x0);
-  const float v000 = (float)q[0], v001 = (float)q[dx], v010 = (float)q[dy], v011 = (float)q[dy + dx];
-  const float v100 = (float)q[dz], v101 = (float)q[dz + dx], v110 = (float)q[dz + dy], v111 = (float)q[dz + dy + dx];
-  const float c00 = flerp(v000, v001, fx);
-  const float c10 = flerp(v010, v011, fx);
-  const float c01 = flerp(v100, v101, fx);
-  const float c11 = flerp(v110, v111, fx);
-  return flerp(flerp(c00, c10, fy), flerp(c01, c11, fy), fz) * K255;
+  f.v[0] = q[0];
+  f.v[1] = q[dx];
+  f.v[2] = q[dy];
+  f.v[3] = q[dy + dx];
+  f.v[4] = q[dz];
+  f.v[5] = q[dz + dx];
+  f.v[6] = q[dz + dy];
+  f.v[7] = q[dz + dy + dx];
+  return f;
+}
+
+__device__ __forceinline__ float brick_lerp(const BFoot &f) {
+  const float c00 = flerp((float)f.v[0], (float)f.v[1], f.fx);
+  const float c10 = flerp((float)f.v[2], (float)f.v[3], f.fx);
+  const float c01 = flerp((float)f.v[4], (float)f.v[5], f.fx);
+  const float c11 = flerp((float)f.v[6], (float)f.v[7], f.fx);
+  return flerp(flerp(c00, c10, f.fy), flerp(c01, c11, f.fy), f.fz) * K255;
+}
+
+__device__ __forceinline__ float brick_fetch(const uint8_t *atlas, uint32_t slot, V3 bl) {
+  return brick_lerp(brick_load(atlas, slot, bl));
 }
 
 __device__ __forceinline__ V3 base_voxel(const uint32_t *hdr, V3 p, float md) {
@@ -457,21 +480,29 @@ __device__ __forceinline__ BCtx locate_ex(const BricksDev &B, const uint32_t *hd
 // bricks_lod_locate_cached: the walk is skipped while the sample stays in the
 // cached brick at the same desired level (its level, grad_vox and brick size
 // are then the cached ones).
-__device__ __forceinline__ BCtx locate_cached(const BricksDev &B, const uint32_t *hdr, float md, V3 p, float lod,
-                                              bool scales_p2, LodCache &lc) {
+// The cache's answer for a sample at p: true (and the located brick in c)
+// when the sample stays in the cached brick at the same desired level.
+__device__ __forceinline__ bool cache_probe(const uint32_t *hdr, float md, V3 p, float lod, bool scales_p2,
+                                            const LodCache &lc, BCtx *c) {
   const uint32_t nlev = hdr[0];
   const int desired = (int)fclamp_sel(__builtin_floorf(lod), 0.0f, (float)(nlev - 1u));
-  if (lc.valid && lc.desired == desired) {
-    const V3 bv = base_voxel(hdr, p, md);
-    const float fs = (float)lc.scale, inv = lc.inv;
-    const V3 lv = scales_p2 ? mk(bv.x * inv, bv.y * inv, bv.z * inv) : mk(bv.x / fs, bv.y / fs, bv.z / fs);
-    const uint32_t bcx = (uint32_t)lv.x >> 7, bcy = (uint32_t)lv.y >> 7, bcz = (uint32_t)lv.z >> 7;
-    if (bcx == lc.bx && bcy == lc.by && bcz == lc.bz) {
-      BCtx c = lc.c;
-      c.bl = mk(lv.x - (float)(bcx << 7), lv.y - (float)(bcy << 7), lv.z - (float)(bcz << 7));
-      return c;
-    }
-  }
+  if (!(lc.valid && lc.desired == desired)) return false;
+  const V3 bv = base_voxel(hdr, p, md);
+  const float fs = (float)lc.scale, inv = lc.inv;
+  const V3 lv = scales_p2 ? mk(bv.x * inv, bv.y * inv, bv.z * inv) : mk(bv.x / fs, bv.y / fs, bv.z / fs);
+  const uint32_t bcx = (uint32_t)lv.x >> 7, bcy = (uint32_t)lv.y >> 7, bcz = (uint32_t)lv.z >> 7;
+  if (!(bcx == lc.bx && bcy == lc.by && bcz == lc.bz)) return false;
+  *c = lc.c;
+  c->bl = mk(lv.x - (float)(bcx << 7), lv.y - (float)(bcy << 7), lv.z - (float)(bcz << 7));
+  return true;
+}
+
+__device__ __forceinline__ BCtx locate_cached(const BricksDev &B, const uint32_t *hdr, float md, V3 p, float lod,
+                                              bool scales_p2, LodCache &lc) {
+  BCtx hit;
+  if (cache_probe(hdr, md, p, lod, scales_p2, lc, &hit)) return hit;
+  const uint32_t nlev = hdr[0];
+  const int desired = (int)fclamp_sel(__builtin_floorf(lod), 0.0f, (float)(nlev - 1u));
   uint32_t scale, bx, by, bz;
   float inv;
   const BCtx c = locate_ex(B, hdr, md, p, lod, scales_p2, &scale, &inv, &bx, &by, &bz);
@@ -598,15 +629,28 @@ __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_
   lc.inv = 1.0f;
   lc.bx = lc.by = lc.bz = 0u;
   lc.c = BCtx{false, 0.0f, mk(0.0f, 0.0f, 0.0f), 0u, 1.0f, 0.0f};
-  while (t < tfar && steps < budget) {
-    steps++;
-    const float foot = ortho ? cone : t * cone;
+  // A sample: position, lod, located brick and its footprint's reads.
+  struct BPend {
+    float lod, lod_step;
+    V3 p;
+    BCtx bc;
+    BFoot f;
+  };
+  const auto at_t = [&](float tt, BPend &q) {
+    const float foot = ortho ? cone : tt * cone;
     const float fv = md_p2 ? foot * md : foot / voxel;
-    const float lod = fclamp_sel(log2_pinned(fmax_sel(fv, 1.0f)) + pc.lod_bias, 0.0f, max_lod);
-    const float lod_step = fmax_sel(exp2_pinned(lod), span_step);
-    const V3 p = vadd(ro, vscale(dir, t));
-    const BCtx bc = locate_cached(B, hdr, md, p, lod, scales_p2, lc);
-    float v = bc.ok ? brick_fetch(B.atlas, bc.slot, bc.bl) : 0.0f;
+    q.lod = fclamp_sel(log2_pinned(fmax_sel(fv, 1.0f)) + pc.lod_bias, 0.0f, max_lod);
+    q.lod_step = fmax_sel(exp2_pinned(q.lod), span_step);
+    q.p = vadd(ro, vscale(dir, tt));
+  };
+  // One march step on a located, issued sample at the current t: 0 ends the
+  // ray, 1 advanced t by dt * lod_step, 2 skipped.
+  const auto step = [&](const BPend &q) -> int {
+    steps++;
+    const float lod_step = q.lod_step;
+    const V3 p = q.p;
+    const BCtx &bc = q.bc;
+    float v = bc.ok ? brick_lerp(q.f) : 0.0f;
     const bool cut = v < pc.threshold;
     if (cut) v = 0.0f;
     // raycast.slang:834-868: whole-brick skips
@@ -624,10 +668,10 @@ __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_
       const float ez = __builtin_fabsf(dir.z) > 1e-8f ? (nz - ro.z) / dir.z : 1e30f;
       t = fmax_sel(fmin_sel(fmin_sel(ex, ey), ez), t) + bsz * 0.01f;
       skips++;
-      continue;
+      return 2;
     }
     if (v > mip_max) mip_max = v;
-    if (mode == MODE_MIP && mip_max >= 0.999f) break;
+    if (mode == MODE_MIP && mip_max >= 0.999f) return 0;
     if (KIND == KIND_OTHER && first_hit < 0.0f && v > 0.05f) first_hit = t;
     if (KIND == KIND_FULL) {  // every active lane takes part in the taps (see march)
       const float4 s = tf_sample(tf, v);
@@ -659,7 +703,7 @@ __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_
         ag = ag + wa * cg;
         ab = ab + wa * cb;
         aa = aa + wa;
-        if (aa >= cutoff) break;
+        if (aa >= cutoff) return 0;
       }
     } else if (composite && !cut) {
       const float4 s = tf_sample(tf, v);
@@ -670,10 +714,30 @@ __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_
         ag = ag + wa * s.y;
         ab = ab + wa * s.z;
         aa = aa + wa;
-        if (aa >= cutoff) break;
+        if (aa >= cutoff) return 0;
       }
     }
     t += dt * lod_step;
+    return 1;
+  };
+  // The sample after a located one is located from the same cache when it
+  // stays in the cached brick (pure arithmetic), so both footprints are read
+  // in one memory round trip; otherwise it is located in the next iteration.
+  // Reads of an unlocated sample go to slot 0 and are not used.
+  while (t < tfar && steps < budget) {
+    BPend a, b;
+    at_t(t, a);
+    a.bc = locate_cached(B, hdr, md, a.p, a.lod, scales_p2, lc);
+    a.f = brick_load(B.atlas, a.bc.ok ? a.bc.slot : 0u, a.bc.bl);
+    at_t(t + dt * a.lod_step, b);
+    b.bc = BCtx{0u, 0.0f, mk(0.0f, 0.0f, 0.0f), 0u, 1.0f, 0.0f};  // slot 0 unless the probe hits
+    const bool hit_b = cache_probe(hdr, md, b.p, b.lod, scales_p2, lc, &b.bc);
+    b.f = brick_load(B.atlas, b.bc.slot, b.bc.bl);
+    __builtin_amdgcn_sched_barrier(0);
+    const int ra = step(a);
+    if (ra == 0) break;
+    if (ra == 2 || !hit_b || !(t < tfar && steps < budget)) continue;
+    if (step(b) == 0) break;
   }
   R.steps = steps;
   R.skips = skips;
