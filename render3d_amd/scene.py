@@ -7,6 +7,8 @@ through the C library's sinf/cosf/tanf so the floats equal the reference's.
 * camera basis / orbit: src/core/camera.c:5-75
 * transfer LUT build and presets: src/core/transfer.c:5-50
 * first visible TF index and skip gate: src/main.c:1238-1242, 6797-6801
+* projected-footprint LOD pick and fallback order of a bricks streamer:
+  src/core/lod.h:8-26
 """
 import ctypes as C
 import math
@@ -16,7 +18,7 @@ import numpy as np
 from ._lib import FrameParams, MODE_FULL
 
 _libm = C.CDLL("libm.so.6")
-for _fn in ("sinf", "cosf", "tanf", "sqrtf"):
+for _fn in ("sinf", "cosf", "tanf", "sqrtf", "log2f"):
     getattr(_libm, _fn).restype = C.c_float
     getattr(_libm, _fn).argtypes = [C.c_float]
 
@@ -194,3 +196,29 @@ def orbit_camera(yaw_deg=45.0, pitch_deg=0.0, dist=2.0, target=(0.5, 0.5, 0.5)):
     cam.pitch = f32(math.radians(pitch_deg))
     cam.orbit_set(target, dist)
     return cam
+
+
+LOD_NONE = 0xFFFFFFFF
+
+
+def lod_pick(distance, pixel_cone, base_voxels, nlevels):
+    """The level whose voxels match a pixel's footprint at `distance`
+    (src/core/lod.h:8-16): floor(log2(max(distance * cone * base_voxels, 1)))
+    in FP32, clamped to the pyramid."""
+    if not nlevels:
+        return 0
+    footprint = max(f32(f32(f32(distance) * f32(pixel_cone)) * f32(base_voxels)), f32(1.0))
+    level = int(math.floor(_libm.log2f(footprint)))
+    return min(max(level, 0), nlevels - 1)
+
+
+def lod_fallback(desired, ordinal, nlevels):
+    """The ordinal-th level a streamer tries for a brick wanted at `desired`:
+    desired, the coarser ones in order, then the finer ones going down
+    (src/core/lod.h:18-24); LOD_NONE once every level was visited."""
+    if not nlevels or desired >= nlevels or ordinal >= nlevels:
+        return LOD_NONE
+    coarse = nlevels - desired
+    if ordinal < coarse:
+        return desired + ordinal
+    return desired - 1 - (ordinal - coarse)

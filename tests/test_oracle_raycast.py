@@ -394,3 +394,16 @@ def test_unsupported_view_raises():
     p.brick_mode = 3
     with pytest.raises(ValueError):
         orc.raycast(ov, scene.tf_unorm(scene.tf_ramp_40_200()), [p], 8, 8)
+
+
+def test_lod_policy_known_answers():  # tests/test_quick.c:64-71
+    assert scene.lod_pick(0.0, 0.001, 8192.0, 8) == 0
+    assert scene.lod_pick(1.0, 0.001, 8192.0, 8) == 3  # 8.2 voxels per pixel
+    assert scene.lod_pick(100.0, 0.01, 8192.0, 8) == 7  # clamped to the coarsest
+    assert [scene.lod_fallback(2, i, 5) for i in range(5)] == [2, 3, 4, 1, 0]
+    assert scene.lod_fallback(2, 5, 5) == scene.LOD_NONE
+    # every level visited once; the pick at exact powers of two
+    for n in range(1, 7):
+        for d in range(n):
+            assert sorted(scene.lod_fallback(d, i, n) for i in range(n)) == list(range(n))
+    assert [scene.lod_pick(d, 1.0, 1.0, 12) for d in (1.0, 2.0, 3.99, 4.0, 1024.0)] == [0, 1, 1, 2, 10]
