@@ -149,63 +149,88 @@ __global__ __launch_bounds__(256) void k_brickify(const uint8_t *src, LevelGeom 
   }
 }
 
-// One workgroup per brick, one lane per 17-byte row (17 x 17 rows). Apron
-// and padding bytes copy the payload voxel at the clamped global position;
-// payload bytes are only read, so concurrent bricks never race.
-__global__ __launch_bounds__(320) void k_apron(LevelGeom L) {
+// One workgroup per brick. Apron and out-of-level bytes copy the payload
+// voxel at the clamped global position; payload bytes inside the level are
+// only read, so concurrent bricks never race. A full brick's 817 apron bytes
+// (x = 16 face 256, y = 16 face 272, z = 16 face 289) are enumerated
+// directly, up to four independent items per lane with every load issued
+// before any store (a lane walking a 17-byte apron row made a serial chain of
+// 17 dependent memory round trips); partial bricks walk their 4913 bytes.
+__global__ __launch_bounds__(256) void k_apron(LevelGeom L) {
   const uint32_t b = blockIdx.x;
   const uint32_t bxi = b % L.bx, byi = (b / L.bx) % L.by, bzi = b / (L.bx * L.by);
-  const uint32_t r = threadIdx.x;
-  if (r >= 289u) return;
-  const uint32_t lz = r / 17u, ly = r % 17u;
-  const uint32_t gy0 = byi * 16u + ly, gz0 = bzi * 16u + lz;
-  const uint32_t gy = gy0 < L.ny ? gy0 : L.ny - 1u, gz = gz0 < L.nz ? gz0 : L.nz - 1u;
-  const bool row_payload = ly < 16u && lz < 16u && gy0 < L.ny && gz0 < L.nz;
-  const uint32_t xfull = L.nx - bxi * 16u;  // payload x positions inside the level
-  uint8_t *dst = L.base + (size_t)b * R3D_BRICK_BYTES + r * 17u;
-  const uint32_t x0 = row_payload ? (xfull < 16u ? xfull : 16u) : 0u;
-  for (uint32_t lx = x0; lx < 17u; lx++) {
-    const uint32_t gx0 = bxi * 16u + lx;
-    const uint32_t gx = gx0 < L.nx ? gx0 : L.nx - 1u;
-    dst[lx] = L.base[payload_index(L, gx, gy, gz)];
+  const uint32_t t = threadIdx.x;
+  const uint32_t xf = min(16u, L.nx - bxi * 16u), yf = min(16u, L.ny - byi * 16u), zf = min(16u, L.nz - bzi * 16u);
+  uint8_t *dst = L.base + (size_t)b * R3D_BRICK_BYTES;
+  const auto src = [&](uint32_t lx, uint32_t ly, uint32_t lz) {
+    const uint32_t gx = min(bxi * 16u + lx, L.nx - 1u), gy = min(byi * 16u + ly, L.ny - 1u),
+                   gz = min(bzi * 16u + lz, L.nz - 1u);
+    return L.base[payload_index(L, gx, gy, gz)];
+  };
+  if (xf == 16u && yf == 16u && zf == 16u) {
+    uint32_t pos[4];
+    uint8_t val[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t i = t + 256u * (uint32_t)k;
+      uint32_t lx = 16u, ly = i & 15u, lz = i >> 4;  // x face
+      if (i >= 256u && i < 528u) {                  // y face, lz < 16
+        const uint32_t j = i - 256u;
+        lx = j % 17u;
+        ly = 16u;
+        lz = j / 17u;
+      } else if (i >= 528u) {  // z face
+        const uint32_t j = min(i - 528u, 288u);
+        lx = j % 17u;
+        ly = j / 17u;
+        lz = 16u;
+      }
+      pos[k] = (lz * 17u + ly) * 17u + lx;
+      val[k] = i < 817u ? src(lx, ly, lz) : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (t + 256u * (uint32_t)k < 817u) dst[pos[k]] = val[k];
+    return;
+  }
+  for (uint32_t p = t; p < R3D_BRICK_CONTENT; p += 256u) {
+    const uint32_t lz = p / 289u, ly = (p / 17u) % 17u, lx = p % 17u;
+    if (lx < xf && ly < yf && lz < zf) continue;
+    dst[p] = src(lx, ly, lz);
   }
 }
 
-// Level l+1 from level l, one workgroup per destination brick, one lane per
-// 17-byte row; every byte (payload, apron, padding) is the rounded 2x box at
-// its clamped global position.
-__global__ __launch_bounds__(320) void k_downsample(LevelGeom S, LevelGeom D) {
+// Level l+1 from level l, one workgroup per destination brick; every byte
+// (payload, apron) is the rounded 2x box at its clamped global position, the
+// eight sources clamped to the source level. Four destination bytes per
+// round with all 32 loads in flight.
+__global__ __launch_bounds__(256) void k_downsample(LevelGeom S, LevelGeom D) {
   const uint32_t b = blockIdx.x;
   const uint32_t bxi = b % D.bx, byi = (b / D.bx) % D.by, bzi = b / (D.bx * D.by);
-  const uint32_t r = threadIdx.x;
-  if (r >= 289u) return;
-  const uint32_t lz = r / 17u, ly = r % 17u;
-  uint32_t y = byi * 16u + ly, z = bzi * 16u + lz;
-  y = y < D.ny ? y : D.ny - 1u;
-  z = z < D.nz ? z : D.nz - 1u;
-  // the four source rows (2y + dy, 2z + dz), clamped
-  const uint8_t *rowp[4];
+  const uint32_t t = threadIdx.x;
+  uint8_t *dst = D.base + (size_t)b * R3D_BRICK_BYTES;
+  for (uint32_t p0 = t; p0 < R3D_BRICK_CONTENT; p0 += 1024u) {
+    uint32_t sum[4];
 #pragma unroll
-  for (uint32_t q = 0; q < 4; q++) {
-    uint32_t sy = 2u * y + (q & 1u), sz = 2u * z + (q >> 1);
-    sy = sy < S.ny ? sy : S.ny - 1u;
-    sz = sz < S.nz ? sz : S.nz - 1u;
-    rowp[q] = S.base + ((size_t)(sz >> 4) * S.by + (sy >> 4)) * S.bx * (size_t)R3D_BRICK_BYTES +
-              ((sz & 15u) * 17u + (sy & 15u)) * 17u;
-  }
-  uint8_t *dst = D.base + (size_t)b * R3D_BRICK_BYTES + r * 17u;
-  for (uint32_t lx = 0; lx < 17u; lx++) {
-    uint32_t x = bxi * 16u + lx;
-    x = x < D.nx ? x : D.nx - 1u;
-    uint32_t s0 = 2u * x, s1 = 2u * x + 1u;
-    s0 = s0 < S.nx ? s0 : S.nx - 1u;
-    s1 = s1 < S.nx ? s1 : S.nx - 1u;
-    const size_t o0 = (size_t)(s0 >> 4) * R3D_BRICK_BYTES + (s0 & 15u);
-    const size_t o1 = (size_t)(s1 >> 4) * R3D_BRICK_BYTES + (s1 & 15u);
-    uint32_t sum = 4u;
+    for (int k = 0; k < 4; k++) {
+      const uint32_t p = min(p0 + 256u * (uint32_t)k, R3D_BRICK_CONTENT - 1u);
+      const uint32_t lz = p / 289u, ly = (p / 17u) % 17u, lx = p % 17u;
+      const uint32_t x = min(bxi * 16u + lx, D.nx - 1u), y = min(byi * 16u + ly, D.ny - 1u),
+                     z = min(bzi * 16u + lz, D.nz - 1u);
+      uint32_t acc = 4u;
 #pragma unroll
-    for (uint32_t q = 0; q < 4; q++) sum += (uint32_t)rowp[q][o0] + (uint32_t)rowp[q][o1];
-    dst[lx] = (uint8_t)(sum / 8u);
+      for (uint32_t d = 0; d < 8u; d++) {
+        const uint32_t sx = min(2u * x + (d & 1u), S.nx - 1u), sy = min(2u * y + ((d >> 1) & 1u), S.ny - 1u),
+                       sz = min(2u * z + (d >> 2), S.nz - 1u);
+        acc += S.base[payload_index(S, sx, sy, sz)];
+      }
+      sum[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t p = p0 + 256u * (uint32_t)k;
+      if (p < R3D_BRICK_CONTENT) dst[p] = (uint8_t)(sum[k] / 8u);
+    }
   }
 }
 
@@ -458,11 +483,11 @@ static dim3 brick_grid(const r3d_hip_volume *v, uint32_t l) { return dim3(nbrick
 static int build_tail(r3d_hip_ctx *ctx, r3d_hip_volume *v, hipStream_t s, bool l1_payload_done) {
   if (v->nlev > 1) {
     if (l1_payload_done)
-      hipLaunchKernelGGL(k_apron, brick_grid(v, 1), dim3(320), 0, s, geom(v, 1));
+      hipLaunchKernelGGL(k_apron, brick_grid(v, 1), dim3(256), 0, s, geom(v, 1));
     else
-      hipLaunchKernelGGL(k_downsample, brick_grid(v, 1), dim3(320), 0, s, geom(v, 0), geom(v, 1));
+      hipLaunchKernelGGL(k_downsample, brick_grid(v, 1), dim3(256), 0, s, geom(v, 0), geom(v, 1));
     for (uint32_t l = 1; l + 1 < v->nlev; l++)
-      hipLaunchKernelGGL(k_downsample, brick_grid(v, l + 1), dim3(320), 0, s, geom(v, l), geom(v, l + 1));
+      hipLaunchKernelGGL(k_downsample, brick_grid(v, l + 1), dim3(256), 0, s, geom(v, l), geom(v, l + 1));
   }
   hipLaunchKernelGGL(k_occ_dilate, grid1d(ctx, (uint64_t)v->odim[0] * v->odim[1] * v->odim[2], 256), dim3(256), 0, s,
                      (const uint8_t *)v->d_occ_raw, v->d_occ, v->odim[0], v->odim[1], v->odim[2]);
@@ -484,7 +509,7 @@ extern "C" int r3d_hip_volume_build(r3d_hip_ctx *ctx, r3d_hip_volume *v, void *s
   if (!v || v->ctx != ctx) return r3d_fail(ctx, R3D_ERR_INVALID, "volume_build: bad volume");
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_apron, brick_grid(v, 0), dim3(320), 0, s, geom(v, 0));
+  hipLaunchKernelGGL(k_apron, brick_grid(v, 0), dim3(256), 0, s, geom(v, 0));
   hipLaunchKernelGGL(k_occ_raw, brick_grid(v, 0), dim3(256), 0, s, geom(v, 0), v->d_occ_raw, v->odim[0],
                      v->odim[1], v->odim[2]);
   R3D_HIP_CHECK(ctx, hipGetLastError());
