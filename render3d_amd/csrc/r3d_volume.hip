@@ -68,6 +68,7 @@ __global__ __launch_bounds__(256) void k_brickify(const uint8_t *src, LevelGeom 
   const bool vec = full && (SRC == SRC_CHUNKS || (L0.nx & 15u) == 0u);
   if (t < 8u) m8[t] = 0u;
   if (t < R3D_BRICK_BYTES - R3D_BRICK_CONTENT) tile[R3D_BRICK_CONTENT + t] = 0;
+  uint32_t row_lo = 0, row_hi = 0;  // maxima of the row's x < 8 and x >= 8 halves (vec rows)
   if (vec) {  // own payload row as one 16-byte load
     const uint8_t *row = SRC == SRC_CHUNKS
                              ? src + (size_t)b * 4096u + t * 16u
@@ -76,7 +77,12 @@ __global__ __launch_bounds__(256) void k_brickify(const uint8_t *src, LevelGeom 
     const uint32_t w[4] = {q.x, q.y, q.z, q.w};
     uint8_t *d = tile + (lz * 17u + ly) * 17u;
 #pragma unroll
-    for (int i = 0; i < 16; i++) d[i] = (uint8_t)(w[i >> 2] >> ((i & 3) * 8));
+    for (int i = 0; i < 16; i++) {
+      const uint32_t v = (w[i >> 2] >> ((i & 3) * 8)) & 255u;
+      d[i] = (uint8_t)v;
+      if (i < 8) row_lo = v > row_lo ? v : row_lo;
+      else row_hi = v > row_hi ? v : row_hi;
+    }
   }
   if (vec) {
     // apron only: 256 z-face + 272 y-face + 289 x-face bytes, enumerated directly
@@ -107,39 +113,52 @@ __global__ __launch_bounds__(256) void k_brickify(const uint8_t *src, LevelGeom 
   __syncthreads();
   uint4 *dst = (uint4 *)(L0.base + (size_t)b * R3D_BRICK_BYTES);
   for (uint32_t i = t; i < R3D_BRICK_BYTES / 16u; i += 256u) dst[i] = ((const uint4 *)tile)[i];
-  // raw occupancy: payload voxels inside the level only
+  // raw occupancy: payload voxels inside the level only (a vec row's maxima
+  // were taken from its registers)
   {
     const uint32_t gy = byi * 16u + ly, gz = bzi * 16u + lz;
     if (gy < L0.ny && gz < L0.nz) {
-      const uint8_t *row = tile + (lz * 17u + ly) * 17u;
-      const uint32_t xn = L0.nx - bxi * 16u < 16u ? L0.nx - bxi * 16u : 16u;
-      uint32_t lo = 0, hi = 0;
-      for (uint32_t x = 0; x < xn; x++) {
-        const uint32_t v = row[x];
-        if (x < 8u) lo = v > lo ? v : lo;
-        else hi = v > hi ? v : hi;
+      uint32_t lo = row_lo, hi = row_hi;
+      if (!vec) {
+        const uint8_t *row = tile + (lz * 17u + ly) * 17u;
+        const uint32_t xn = L0.nx - bxi * 16u < 16u ? L0.nx - bxi * 16u : 16u;
+        for (uint32_t x = 0; x < xn; x++) {
+          const uint32_t v = row[x];
+          if (x < 8u) lo = v > lo ? v : lo;
+          else hi = v > hi ? v : hi;
+        }
       }
       const uint32_t q = ((lz >> 3) * 2u + (ly >> 3)) * 2u;
       atomicMax(&m8[q], lo);
       atomicMax(&m8[q + 1u], hi);
     }
   }
-  // level-1 octant: 8x8 rows of 8 voxels, sources clamped through the tile
-  if (do_l1 && t < 64u && (bxi >> 1) < L1.bx && (byi >> 1) < L1.by && (bzi >> 1) < L1.bz) {
-    const uint32_t z1 = t >> 3, y1 = t & 7u;
-    uint8_t *drow = L1.base + ((size_t)((bzi >> 1) * L1.by + (byi >> 1)) * L1.bx + (bxi >> 1)) * R3D_BRICK_BYTES +
-                    (((bzi & 1u) * 8u + z1) * 17u + (byi & 1u) * 8u + y1) * 17u + (bxi & 1u) * 8u;
-    const uint8_t *r00 = tile + ((2u * z1) * 17u + 2u * y1) * 17u;
-    for (uint32_t x1 = 0; x1 < 8u; x1++) {
-      uint32_t sum = 4u;
+  // level-1 octant: 8^3 voxels, two per lane (all 32 tile reads independent),
+  // sources clamped through the tile
+  if (do_l1 && (bxi >> 1) < L1.bx && (byi >> 1) < L1.by && (bzi >> 1) < L1.bz) {
+    uint8_t *doct = L1.base + ((size_t)((bzi >> 1) * L1.by + (byi >> 1)) * L1.bx + (bxi >> 1)) * R3D_BRICK_BYTES +
+                    (((bzi & 1u) * 8u) * 17u + (byi & 1u) * 8u) * 17u + (bxi & 1u) * 8u;
+    uint32_t sum[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const uint32_t o = t + 256u * (uint32_t)k;
+      const uint32_t x1 = o & 7u, y1 = (o >> 3) & 7u, z1 = o >> 6;
+      const uint8_t *r00 = tile + ((2u * z1) * 17u + 2u * y1) * 17u + 2u * x1;
+      uint32_t acc = 4u;
 #pragma unroll
       for (uint32_t dz = 0; dz < 2; dz++)
 #pragma unroll
         for (uint32_t dy = 0; dy < 2; dy++) {
-          const uint8_t *r = r00 + (dz * 17u + dy) * 17u + 2u * x1;
-          sum += (uint32_t)r[0] + (uint32_t)r[1];
+          const uint8_t *r = r00 + (dz * 17u + dy) * 17u;
+          acc += (uint32_t)r[0] + (uint32_t)r[1];
         }
-      drow[x1] = (uint8_t)(sum / 8u);
+      sum[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const uint32_t o = t + 256u * (uint32_t)k;
+      const uint32_t x1 = o & 7u, y1 = (o >> 3) & 7u, z1 = o >> 6;
+      doct[(z1 * 17u + y1) * 17u + x1] = (uint8_t)(sum[k] / 8u);
     }
   }
   __syncthreads();
