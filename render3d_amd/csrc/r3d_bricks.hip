@@ -42,6 +42,35 @@ __global__ __launch_bounds__(256) void k_bk_down(const uint8_t *src, uint32_t nx
   }
 }
 
+// The 2x box of a level whose x extent is a multiple of 16 and whose base is
+// 16-byte aligned: each lane makes 8 consecutive voxels of a destination row
+// from one 16-byte load of each of its 4 source rows (rows past the source's
+// y / z edge are clamped; x never needs it since 2x + 1 < nx).
+__global__ __launch_bounds__(256) void k_bk_down16(const uint8_t *src, uint32_t nx, uint32_t ny, uint32_t nz,
+                                                   uint8_t *dst, uint32_t mx, uint32_t my, uint32_t mz) {
+  const uint32_t gx = mx / 8u;  // 8-voxel groups per destination row
+  const uint64_t n = (uint64_t)gx * my * mz;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) {
+    const uint32_t g = (uint32_t)(i % gx), y = (uint32_t)((i / gx) % my), z = (uint32_t)(i / ((uint64_t)gx * my));
+    uint32_t acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc[k] = 4u;
+#pragma unroll
+    for (uint32_t d = 0; d < 4u; d++) {
+      const uint32_t sy = min(2u * y + (d & 1u), ny - 1u), sz = min(2u * z + (d >> 1), nz - 1u);
+      const uint4 q = *(const uint4 *)(src + ((uint64_t)sz * ny + sy) * nx + 16u * g);
+      const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        acc[k] += ((w[k >> 1] >> (16 * (k & 1))) & 255u) + ((w[k >> 1] >> (16 * (k & 1) + 8)) & 255u);
+    }
+    uint2 o;
+    o.x = (acc[0] / 8u) | (acc[1] / 8u) << 8 | (acc[2] / 8u) << 16 | (acc[3] / 8u) << 24;
+    o.y = (acc[4] / 8u) | (acc[5] / 8u) << 8 | (acc[6] / 8u) << 16 | (acc[7] / 8u) << 24;
+    *(uint2 *)(dst + ((uint64_t)z * my + y) * mx + 8u * g) = o;
+  }
+}
+
 struct BkJob {
   uint32_t level, cx, cy, cz;
   uint32_t word, slot;
@@ -54,61 +83,81 @@ struct BkFill {
   uint32_t njobs;
   uint8_t *atlas;
   uint32_t *page;
+  uint32_t *maxv;  // per job, zeroed before the fill
 };
 
-// One workgroup per resident brick: 16,384 rows of 128 voxels, 64 per lane.
+constexpr uint32_t BK_PARTS = 64u;  // workgroups per brick: 256 rows of 128 voxels each
+
+// BK_PARTS workgroups per resident brick, each copying 256 of its 16,384
+// rows of 128 voxels (one row per lane, zeros past the level's edge) and
+// folding its maximum into the brick's word with an atomic max.
 __global__ __launch_bounds__(256) void k_bk_fill(BkFill a) {
   __shared__ uint32_t s_max[4];
-  for (uint32_t j = blockIdx.x; j < a.njobs; j += gridDim.x) {
-    const BkJob job = a.jobs[j];
-    const uint8_t *L = a.lev[job.level];
-    const uint32_t nx = a.dims[job.level][0], ny = a.dims[job.level][1], nz = a.dims[job.level][2];
-    uint8_t *S = a.atlas + (uint64_t)job.slot * BK_VOX;
-    const uint32_t x0 = job.cx * BK;
-    const bool whole_x = x0 + BK <= nx && nx % 16u == 0u && ((uintptr_t)L & 15u) == 0u;
-    uint32_t mx = 0;
-    for (uint32_t r = threadIdx.x; r < BK * BK; r += 256u) {
-      const uint32_t y = r & 127u, z = r >> 7;
-      const uint32_t gy = job.cy * BK + y, gz = job.cz * BK + z;
-      uint4 *dst = (uint4 *)(S + ((uint64_t)z * BK + y) * BK);
-      if (gy >= ny || gz >= nz) {
-        for (int k = 0; k < 8; k++) dst[k] = make_uint4(0, 0, 0, 0);
-        continue;
+  const uint32_t j = blockIdx.x / BK_PARTS, part = blockIdx.x % BK_PARTS;
+  if (j >= a.njobs) return;
+  const BkJob job = a.jobs[j];
+  const uint8_t *L = a.lev[job.level];
+  const uint32_t nx = a.dims[job.level][0], ny = a.dims[job.level][1], nz = a.dims[job.level][2];
+  uint8_t *S = a.atlas + (uint64_t)job.slot * BK_VOX;
+  const uint32_t x0 = job.cx * BK;
+  const bool whole_x = x0 + BK <= nx && nx % 16u == 0u && ((uintptr_t)L & 15u) == 0u;
+  const uint32_t r = part * 256u + threadIdx.x;  // row of the brick
+  const uint32_t y = r & 127u, z = r >> 7;
+  const uint32_t gy = job.cy * BK + y, gz = job.cz * BK + z;
+  uint4 *dst = (uint4 *)(S + ((uint64_t)z * BK + y) * BK);
+  uint32_t mx = 0;
+  if (gy >= ny || gz >= nz) {
+    for (int k = 0; k < 8; k++) dst[k] = make_uint4(0, 0, 0, 0);
+  } else {
+    const uint8_t *row = L + ((uint64_t)gz * ny + gy) * nx + x0;
+    if (whole_x) {
+      uint4 q[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) q[k] = ((const uint4 *)row)[k];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        dst[k] = q[k];
+        const uint32_t w[4] = {q[k].x, q[k].y, q[k].z, q[k].w};
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int bb = 0; bb < 4; bb++) {
+            const uint32_t v = (w[i] >> (8 * bb)) & 255u;
+            mx = v > mx ? v : mx;
+          }
       }
-      const uint8_t *row = L + ((uint64_t)gz * ny + gy) * nx + x0;
-      if (whole_x) {
-        for (int k = 0; k < 8; k++) {
-          const uint4 q = ((const uint4 *)row)[k];
-          dst[k] = q;
-          const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-          for (int i = 0; i < 4; i++)
-            for (int b = 0; b < 4; b++) {
-              const uint32_t v = (w[i] >> (8 * b)) & 255u;
-              mx = v > mx ? v : mx;
-            }
-        }
-      } else {
-        uint8_t *d8 = (uint8_t *)dst;
-        for (uint32_t x = 0; x < BK; x++) {
-          const uint32_t v = x0 + x < nx ? row[x] : 0u;
-          d8[x] = (uint8_t)v;
-          mx = v > mx ? v : mx;
-        }
+    } else {
+      uint8_t *d8 = (uint8_t *)dst;
+      for (uint32_t x = 0; x < BK; x++) {
+        const uint32_t v = x0 + x < nx ? row[x] : 0u;
+        d8[x] = (uint8_t)v;
+        mx = v > mx ? v : mx;
       }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      const uint32_t t = __shfl_xor(mx, o);
-      mx = t > mx ? t : mx;
-    }
-    if ((threadIdx.x & 63u) == 0u) s_max[threadIdx.x >> 6] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t m = s_max[0];
-      for (int w = 1; w < 4; w++) m = s_max[w] > m ? s_max[w] : m;
-      a.page[job.word] = job.slot | (m << 24);
-    }
-    __syncthreads();
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t t = __shfl_xor(mx, o);
+    mx = t > mx ? t : mx;
+  }
+  if ((threadIdx.x & 63u) == 0u) s_max[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t m = s_max[0];
+    for (int w = 1; w < 4; w++) m = s_max[w] > m ? s_max[w] : m;
+    atomicMax(&a.maxv[j], m);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bk_zero(uint32_t *p, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// The entries of the filled bricks, once every part has folded its maximum.
+__global__ __launch_bounds__(256) void k_bk_entries(const BkJob *jobs, uint32_t njobs, const uint32_t *maxv,
+                                                    uint32_t *page) {
+  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+  if (j < njobs) page[jobs[j].word] = jobs[j].slot | (maxv[j] << 24);
 }
 
 }  // namespace r3d
@@ -224,14 +273,21 @@ extern "C" int r3d_hip_bricks_build(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const u
     }
     tmp.push_back(p);
     f.lev[l] = p;
-    const uint64_t blocks = (n + 255u) / 256u;
+    const uint32_t snx = b->dims[l - 1][0], mxl = b->dims[l][0];
+    const bool vec16 = snx % 16u == 0u && mxl * 2u == snx && ((uintptr_t)f.lev[l - 1] & 15u) == 0u;
+    const uint64_t items = vec16 ? n / 8u : n;
+    const uint64_t blocks = (items + 255u) / 256u;
     const uint32_t grid = (uint32_t)(blocks < 65536u ? blocks : 65536u);
-    hipLaunchKernelGGL(k_bk_down, dim3(grid), dim3(256), 0, st, f.lev[l - 1], b->dims[l - 1][0], b->dims[l - 1][1],
-                       b->dims[l - 1][2], p, b->dims[l][0], b->dims[l][1], b->dims[l][2]);
+    if (vec16)
+      hipLaunchKernelGGL(k_bk_down16, dim3(grid), dim3(256), 0, st, f.lev[l - 1], snx, b->dims[l - 1][1],
+                         b->dims[l - 1][2], p, mxl, b->dims[l][1], b->dims[l][2]);
+    else
+      hipLaunchKernelGGL(k_bk_down, dim3(grid), dim3(256), 0, st, f.lev[l - 1], snx, b->dims[l - 1][1],
+                         b->dims[l - 1][2], p, mxl, b->dims[l][1], b->dims[l][2]);
     if (hipGetLastError() != hipSuccess) rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: downsample launch");
   }
-  BkJob *djobs = nullptr;
-  if (rc == R3D_OK && !jobs.empty() && hipMalloc(&djobs, sizeof(BkJob) * jobs.size()) != hipSuccess)
+  BkJob *djobs = nullptr;  // the job list, then one max word per job
+  if (rc == R3D_OK && !jobs.empty() && hipMalloc(&djobs, (sizeof(BkJob) + 4u) * jobs.size()) != hipSuccess)
     rc = r3d_fail(ctx, R3D_ERR_NOMEM, "bricks_build: job list");
   if (rc == R3D_OK && hipStreamSynchronize(st) != hipSuccess) rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: sync");
   if (rc == R3D_OK && hipMemcpy(b->d_page + 64, page.data(), 4ull * page.size(), hipMemcpyHostToDevice) != hipSuccess)
@@ -244,8 +300,11 @@ extern "C" int r3d_hip_bricks_build(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const u
     f.njobs = (uint32_t)jobs.size();
     f.atlas = b->d_atlas;
     f.page = b->d_page;
-    const uint32_t cap = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4u;
-    hipLaunchKernelGGL(k_bk_fill, dim3(f.njobs < cap ? f.njobs : cap), dim3(256), 0, st, f);
+    f.maxv = (uint32_t *)(djobs + jobs.size());
+    hipLaunchKernelGGL(k_bk_zero, dim3((f.njobs + 255u) / 256u), dim3(256), 0, st, f.maxv, f.njobs);
+    hipLaunchKernelGGL(k_bk_fill, dim3(f.njobs * BK_PARTS), dim3(256), 0, st, f);
+    hipLaunchKernelGGL(k_bk_entries, dim3((f.njobs + 255u) / 256u), dim3(256), 0, st, (const BkJob *)djobs, f.njobs,
+                       (const uint32_t *)f.maxv, f.page);
     if (hipGetLastError() != hipSuccess) rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: fill launch");
   }
   if (hipStreamSynchronize(st) != hipSuccess && rc == R3D_OK) rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: sync");
