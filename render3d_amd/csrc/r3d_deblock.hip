@@ -4,13 +4,14 @@
 // Volume form: the reference filters every internal 16-aligned plane in
 // place, x seams, then y seams, then z seams. Within one pass the seams are 16
 // voxels apart and each touches only the two layers on either side, so every
-// seam sample of a pass is independent: one lane per sample, one kernel per
-// pass (the passes stay in order). Chunk form: one workgroup per chunk
-// filters its six faces in order against pristine neighbours, with the
-// chunk's running values in LDS. The arithmetic is the reference's FP32
-// sequence (unfused, correctly rounded division, roundf, clamp), so the output
-// is bit-identical. The volume passes move about a quarter of the volume's
-// bytes each (the layers at the seams): HBM-bound byte work.
+// seam sample of a pass is independent. Rows that are 16-byte aligned take
+// one fused kernel over tiles that each hold a whole seam neighbourhood per
+// axis (the volume crosses HBM once); any other alignment takes one kernel per
+// pass, one lane per sample (the passes stay in order). Chunk form: one
+// workgroup per chunk filters its six faces in order against pristine
+// neighbours, with the chunk's running values in LDS. The arithmetic is the
+// reference's FP32 sequence (unfused, correctly rounded division, roundf,
+// clamp), so the output is bit-identical. HBM-bound byte work.
 #include <type_traits>
 
 #include "r3d_internal.h"
@@ -196,6 +197,184 @@ __global__ __launch_bounds__(256) void k_db_pass(DbPass a) {
   }
 }
 
+// All three passes in one kernel for rows that are 16-byte aligned. One
+// workgroup per tile of 128 x 16 x 16 voxels shifted by -2 on every axis
+// (x range [128m - 2, 128m + 126), y [16j - 2, 16j + 14), z likewise): a tile
+// holds the whole two-layer neighbourhood of every seam it covers (8 along x,
+// one along y and z), every voxel belongs to one tile, and a tile reads no
+// byte that another tile writes (tiles run concurrently). Eight lanes share a
+// 128-byte row, 16 bytes each (coalesced loads); lane c of a row holds tile
+// bytes 16c .. 16c + 15, whose first four are the x seam 128m + 16c, so the x
+// seams are filtered in registers (the two bytes before each lane's load come
+// from its neighbour lane). The y and z seams follow through the tile in LDS,
+// a dword of four columns per lane, in the reference's order, and only
+// changed dwords go back. The volume is read once; the per-pass kernels read
+// it whole for the x pass (seams every 16 bytes touch every line) and a
+// quarter again for each of y and z.
+struct DbTile {
+  uint8_t *vol;
+  uint32_t nz, ny, nx;
+  uint32_t tx, ty, tz;  // tiles per axis
+  DbLim lim;
+  const DbLim *flim;
+};
+
+__device__ __forceinline__ void db_sample(uint32_t &b0, uint32_t &b1, uint32_t &b2, uint32_t &b3, float alpha,
+                                          float beta) {
+  const float p1 = (float)b0, p0 = (float)b1, q0 = (float)b2, q1 = (float)b3;
+  float d1;
+  const float d0 = db_delta(p1, p0, q0, q1, alpha, beta, &d1);
+  if (d0 != 0.0f) {
+    b1 = db_u8(p0 + d0);
+    b2 = db_u8(q0 - d0);
+    b0 = db_u8(p1 + d1);
+    b3 = db_u8(q1 - d1);
+  }
+}
+
+// Four columns packed in one dword per layer (p1 | p0 | q0 | q1 across the
+// seam); byte k is filtered when bit k of `valid` is set.
+__device__ __forceinline__ void db_sample4(uint32_t &w0, uint32_t &w1, uint32_t &w2, uint32_t &w3, uint32_t valid,
+                                           float alpha, float beta) {
+  uint32_t o0 = 0, o1 = 0, o2 = 0, o3 = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    uint32_t b0 = (w0 >> (8 * k)) & 255u, b1 = (w1 >> (8 * k)) & 255u, b2 = (w2 >> (8 * k)) & 255u,
+             b3 = (w3 >> (8 * k)) & 255u;
+    if ((valid >> k) & 1u) db_sample(b0, b1, b2, b3, alpha, beta);
+    o0 |= b0 << (8 * k);
+    o1 |= b1 << (8 * k);
+    o2 |= b2 << (8 * k);
+    o3 |= b3 << (8 * k);
+  }
+  w0 = o0;
+  w1 = o1;
+  w2 = o2;
+  w3 = o3;
+}
+
+constexpr uint32_t DBT_W = 128u;  // tile width in x
+
+__global__ __launch_bounds__(256) void k_db_tiles(DbTile a) {
+  __shared__ __attribute__((aligned(16))) uint32_t s_t[16 * 16 * DBT_W / 4];  // [lz][ly][lx / 4]
+  const DbLim l = a.flim ? *a.flim : a.lim;
+  if (!l.on) return;
+  const uint32_t b = blockIdx.x;
+  const uint32_t tm = b % a.tx, tj = (b / a.tx) % a.ty, tk = b / (a.tx * a.ty);
+  const uint32_t t = threadIdx.x, c = t & 7u, rs = t >> 3;  // 16-byte column, first row
+  const uint32_t xb = DBT_W * tm;  // the tile starts at x = xb - 2
+  const int64_t y0 = 16 * (int64_t)tj - 2, z0 = 16 * (int64_t)tk - 2;
+  const uint32_t xc = xb + 16u * c;  // x of this lane's load
+  const bool col_in = xc < a.nx, has_lo = c == 0u && tm > 0u;
+  // rows r = rs + 32q of the tile (r = ly + 16 lz)
+  uint4 W[8];
+  uint32_t lo[8];
+  bool rin[8];
+  uint8_t *rowp[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const uint32_t r = rs + 32u * (uint32_t)q;
+    const int64_t gy = y0 + (r & 15u), gz = z0 + (r >> 4);
+    rin[q] = gy >= 0 && gy < a.ny && gz >= 0 && gz < a.nz;
+    rowp[q] = a.vol + ((uint64_t)(rin[q] ? gz : 0) * a.ny + (uint64_t)(rin[q] ? gy : 0)) * a.nx;
+    W[q] = make_uint4(0, 0, 0, 0);
+    lo[q] = 0;
+    if (rin[q] && col_in) W[q] = *(const uint4 *)(rowp[q] + xc);
+    if (rin[q] && has_lo) lo[q] = *(const uint32_t *)(rowp[q] + xb - 4u);
+  }
+  // this lane's x seam is xc (xc + 1 < nx follows from col_in: nx is a multiple of 16)
+  const bool seam_x = col_in && xc >= 16u;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const uint32_t up = __shfl_up(W[q].w, 1u, 8);
+    const uint32_t pw = c == 0u ? lo[q] : up;
+    uint32_t r0 = (pw >> 16) | (W[q].x << 16);
+    const uint32_t r1 = (W[q].x >> 16) | (W[q].y << 16), r2 = (W[q].y >> 16) | (W[q].z << 16),
+                   r3 = (W[q].z >> 16) | (W[q].w << 16);
+    if (rin[q] && seam_x) {
+      uint32_t b0 = r0 & 255u, b1 = (r0 >> 8) & 255u, b2 = (r0 >> 16) & 255u, b3 = r0 >> 24;
+      db_sample(b0, b1, b2, b3, l.alpha, l.beta);
+      r0 = b0 | b1 << 8 | b2 << 16 | b3 << 24;
+    }
+    const uint32_t r = rs + 32u * (uint32_t)q;
+    *(uint4 *)(s_t + r * (DBT_W / 4u) + 4u * c) = make_uint4(r0, r1, r2, r3);
+  }
+  __syncthreads();
+  // y seam at 16j: dword columns (x4, lz), 2 per lane
+  if (tj > 0u && 16u * tj + 1u < a.ny) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const uint32_t it = t + 256u * (uint32_t)q, x4 = it & 31u, cz = it >> 5;
+      const int64_t gzc = z0 + cz;
+      if (gzc >= 0 && gzc < a.nz) {
+        uint32_t valid = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int64_t gx = (int64_t)xb - 2 + 4 * x4 + k;
+          valid |= (gx >= 0 && gx < a.nx ? 1u : 0u) << k;
+        }
+        uint32_t *cc = s_t + cz * 16u * (DBT_W / 4u) + x4;
+        uint32_t w0 = cc[0], w1 = cc[DBT_W / 4u], w2 = cc[2u * (DBT_W / 4u)], w3 = cc[3u * (DBT_W / 4u)];
+        db_sample4(w0, w1, w2, w3, valid, l.alpha, l.beta);
+        cc[0] = w0;
+        cc[DBT_W / 4u] = w1;
+        cc[2u * (DBT_W / 4u)] = w2;
+        cc[3u * (DBT_W / 4u)] = w3;
+      }
+    }
+  }
+  __syncthreads();
+  // z seam at 16k: dword columns (x4, ly), 2 per lane
+  if (tk > 0u && 16u * tk + 1u < a.nz) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const uint32_t it = t + 256u * (uint32_t)q, x4 = it & 31u, cy = it >> 5;
+      const int64_t gyc = y0 + cy;
+      if (gyc >= 0 && gyc < a.ny) {
+        uint32_t valid = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int64_t gx = (int64_t)xb - 2 + 4 * x4 + k;
+          valid |= (gx >= 0 && gx < a.nx ? 1u : 0u) << k;
+        }
+        uint32_t *cc = s_t + cy * (DBT_W / 4u) + x4;
+        constexpr uint32_t PL = 16u * (DBT_W / 4u);
+        uint32_t w0 = cc[0], w1 = cc[PL], w2 = cc[2u * PL], w3 = cc[3u * PL];
+        db_sample4(w0, w1, w2, w3, valid, l.alpha, l.beta);
+        cc[0] = w0;
+        cc[PL] = w1;
+        cc[2u * PL] = w2;
+        cc[3u * PL] = w3;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const uint32_t r = rs + 32u * (uint32_t)q;
+    const uint4 f = *(const uint4 *)(s_t + r * (DBT_W / 4u) + 4u * c);
+    const uint32_t nf = __shfl_down(f.x, 1u, 8);  // tile bytes 16c + 16, 16c + 17 (c < 7)
+    if (!rin[q]) continue;
+    uint8_t *row = rowp[q];
+    // x = xb - 2, xb - 1: tile bytes 0, 1 (the previous tile owns xb - 4, xb - 3)
+    if (has_lo && (f.x & 0xFFFFu) != (lo[q] >> 16)) *(uint16_t *)(row + xb - 2u) = (uint16_t)(f.x & 0xFFFFu);
+    if (!col_in) continue;
+    // x = xc .. xc + 15: tile bytes 16c + 2 .. 16c + 17; for c = 7 the last two
+    // (x = xb + 126, 127) belong to the next tile
+    const uint32_t n0 = (f.x >> 16) | (f.y << 16), n1 = (f.y >> 16) | (f.z << 16), n2 = (f.z >> 16) | (f.w << 16),
+                   n3 = (f.w >> 16) | (nf << 16);
+    uint32_t *d = (uint32_t *)(row + xc);
+    if (n0 != W[q].x) d[0] = n0;
+    if (n1 != W[q].y) d[1] = n1;
+    if (n2 != W[q].z) d[2] = n2;
+    if (c < 7u) {
+      if (n3 != W[q].w) d[3] = n3;
+    } else if ((n3 & 0xFFFFu) != (W[q].w & 0xFFFFu)) {
+      *(uint16_t *)(row + xc + 12u) = (uint16_t)(n3 & 0xFFFFu);
+    }
+  }
+}
+
 __device__ __forceinline__ void block_minmax(const uint8_t *c, uint32_t t, uint32_t *s_red, float *mn, float *mx) {
   uint32_t lo = 255u, hi = 0u;
   const uint4 q = ((const uint4 *)c)[t];
@@ -309,6 +488,24 @@ extern "C" int r3d_hip_deblock_u8(r3d_hip_ctx *ctx, uint8_t *vol, const uint64_t
   }
   const uint64_t seams[3] = {nx >= 2u ? (nx - 2u) / DB_N : 0u, ny >= 2u ? (ny - 2u) / DB_N : 0u,
                              nz >= 2u ? (nz - 2u) / DB_N : 0u};
+  if (nx % 16u == 0u && (uintptr_t)vol % 16u == 0u && nx <= 0xFFFFFFF0u && ny <= 0xFFFFFFF0u && nz <= 0xFFFFFFF0u) {
+    DbTile tl{};
+    tl.vol = vol;
+    tl.nx = (uint32_t)nx;
+    tl.ny = (uint32_t)ny;
+    tl.nz = (uint32_t)nz;
+    tl.tx = (uint32_t)((nx + 2u + DBT_W - 1u) / DBT_W);
+    tl.ty = (uint32_t)((ny + 2u + 15u) / 16u);
+    tl.tz = (uint32_t)((nz + 2u + 15u) / 16u);
+    tl.lim = p.lim;
+    tl.flim = p.flim;
+    const uint64_t tiles = (uint64_t)tl.tx * tl.ty * tl.tz;
+    if (tiles <= 0x7FFFFFFFull) {
+      hipLaunchKernelGGL(k_db_tiles, dim3((uint32_t)tiles), dim3(256), 0, st, tl);
+      R3D_HIP_CHECK(ctx, hipGetLastError());
+      return R3D_OK;
+    }
+  }
   // vector width: rows aligned to 16 (or 4) bytes take 16 (or 4) samples per lane
   const uintptr_t base = (uintptr_t)vol;
   const int w = (nx % 16u == 0u && base % 16u == 0u) ? 16 : (nx % 4u == 0u && base % 4u == 0u) ? 4 : 1;

@@ -81,3 +81,30 @@ def test_no_op_cases(ctx):
     torch.cuda.synchronize()
     assert torch.equal(v.cpu(), torch.from_numpy(D["a"]))
     assert torch.equal(small.cpu(), torch.from_numpy(D["a"][:17, :17, :17].copy()))
+
+
+def test_both_kernel_forms_against_the_oracle(ctx):
+    """A row length that is a multiple of 128 (the tile pass's last tile holds
+    only the two voxels before the end of each row) on a 16-byte aligned
+    volume, and the same bytes at an odd address, which takes the per-pass
+    kernels."""
+    n = 256
+    step = torch.from_numpy(workloads.step_table_qstep(32.0)).cuda()
+    v = ctx.synth_sheets(n, n, n, 20, 9)
+    levels, headers = ctx.encode_chunks(v, n, n, n, step)
+    flat = ctx.decode_chunks(headers=headers, step_tables=step, levels=levels)
+    full = flat.view(16, 16, 16, 16, 16, 16).permute(0, 3, 1, 4, 2, 5).reshape(n, n, n)
+    base = full[3:70, 14:99, :].contiguous()
+    want = orc.deblock(base.cpu().numpy(), 32.0, 1.0)
+    assert (want != base.cpu().numpy()).sum() > 1000
+    g = base.clone()
+    assert g.data_ptr() % 16 == 0
+    ctx.deblock(g, 32.0, 1.0)
+    buf = torch.zeros(base.numel() + 16, dtype=torch.uint8, device="cuda")
+    odd = buf[1:1 + base.numel()].view(base.shape)
+    odd.copy_(base)
+    ctx.deblock(odd, 32.0, 1.0)
+    torch.cuda.synchronize()
+    assert np.array_equal(g.cpu().numpy(), want)
+    assert np.array_equal(odd.cpu().numpy(), want)
+    assert int(buf[0]) == 0 and int(buf[-15:].sum()) == 0
