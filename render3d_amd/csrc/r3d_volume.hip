@@ -55,6 +55,19 @@ __device__ __forceinline__ uint32_t src_voxel(const uint8_t *src, const LevelGeo
   return src[((size_t)z * L.ny + y) * L.nx + x];
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin in blockIdx order, and
+// each XCD has its own L2. Brick b's apron reads the +x neighbour chunk's
+// every line (its x = 0 column) and half the +y one's; in brick order those
+// neighbours would be read by workgroups on other XCDs. This map gives XCD k
+// the contiguous band of bricks [k n/8, (k + 1) n/8) in order, so a brick's +x
+// and +y neighbours are read on the same XCD a few slots later (L2 hits);
+// the remainder past 8 (n / 8) keeps its own index. Every brick costs the same,
+// so the bands stay balanced.
+__device__ __forceinline__ uint32_t xcd_band(uint32_t b, uint32_t n) {
+  const uint32_t per = n / 8u;
+  return b < 8u * per ? (b % 8u) * per + b / 8u : b;
+}
+
 template <int SRC>
 __global__ __launch_bounds__(256) void k_brickify(const uint8_t *src, LevelGeom L0, LevelGeom L1, int do_l1,
                                                   uint8_t *raw, uint32_t ox, uint32_t oy, uint32_t oz) {
@@ -168,6 +181,153 @@ __global__ __launch_bounds__(256) void k_brickify(const uint8_t *src, LevelGeom 
   }
 }
 
+// Level 0 from chunks, four bricks along x per workgroup (L0.bx a multiple of
+// 4, dims multiples of 16, so every brick is full). Each lane loads its row
+// of all four payloads (16 bytes each) up front; the z = 16 and y = 16 apron
+// faces and their edge are 16-byte rows of the neighbour chunks (or of the
+// brick's own last layer at the volume's end, as the clamp gives); only the
+// group's last brick reads its x = 16 face from the next chunk, and bricks
+// 0..2 copy theirs from the next brick's x = 0 plane in LDS. The four bricks
+// are contiguous in the level, so they go out as one run of 16-byte stores.
+// Same bytes, occupancy maxima and level-1 octants as k_brickify: a lane has
+// four times the bytes in flight, and the +x chunk's strided column is read
+// once per four bricks instead of once per brick.
+constexpr uint32_t BK_GROUP = 4u;
+
+__global__ __launch_bounds__(256) void k_brickify4(const uint8_t *src, LevelGeom L0, LevelGeom L1, int do_l1,
+                                                   uint8_t *raw, uint32_t ox, uint32_t oy, uint32_t oz) {
+  __shared__ __attribute__((aligned(16))) uint8_t tile[BK_GROUP * R3D_BRICK_BYTES];
+  __shared__ uint32_t m8[BK_GROUP * 8u];
+  const uint32_t ng = L0.bx / BK_GROUP;
+  const uint32_t b = blockIdx.x;
+  const uint32_t bx0 = (b % ng) * BK_GROUP, byi = (b / ng) % L0.by, bzi = b / (ng * L0.by);
+  const uint32_t t = threadIdx.x, lz = t >> 4, ly = t & 15u;
+  const size_t b0 = ((size_t)bzi * L0.by + byi) * L0.bx + bx0;  // the group's first brick (= chunk)
+  const bool yn = byi + 1u < L0.by, zn = bzi + 1u < L0.bz;
+  const size_t sy = (size_t)L0.bx * 4096u, sz = (size_t)L0.bx * L0.by * 4096u;  // chunk strides in bytes
+  const uint8_t *c0 = src + b0 * 4096u;
+  uint4 P[BK_GROUP];
+#pragma unroll
+  for (uint32_t k = 0; k < BK_GROUP; k++) P[k] = *(const uint4 *)(c0 + k * 4096u + t * 16u);
+  // lanes 0..63: the z = 16 face rows (brick fk, y = fi); 64..127: the y = 16
+  // face rows (brick fk, z = fi); 128..131: the y = z = 16 edge rows
+  const uint32_t fk = (t >> 4) & 3u, fi = t & 15u;
+  uint4 F = make_uint4(0, 0, 0, 0);
+  uint32_t fo = 0;  // LDS offset of the row within the group's tiles
+  if (t < 64u) {
+    F = *(const uint4 *)(c0 + fk * 4096u + (zn ? sz : 15u * 256u) + fi * 16u);
+    fo = fk * R3D_BRICK_BYTES + (16u * 17u + fi) * 17u;
+  } else if (t < 128u) {
+    F = *(const uint4 *)(c0 + fk * 4096u + (yn ? sy : 15u * 16u) + fi * 256u);
+    fo = fk * R3D_BRICK_BYTES + (fi * 17u + 16u) * 17u;
+  } else if (t < 128u + BK_GROUP) {
+    const uint32_t k = t - 128u;
+    F = *(const uint4 *)(c0 + k * 4096u + (yn ? sy : 15u * 16u) + (zn ? sz : 15u * 256u));
+    fo = k * R3D_BRICK_BYTES + (16u * 17u + 16u) * 17u;
+  }
+  // the last brick's x = 16 face (17 x 17, edges included) from the next chunk's
+  // x = 0 column, clamped to the volume
+  uint32_t xv[2] = {0u, 0u};
+  {
+    const uint32_t X = min(16u * (bx0 + BK_GROUP), L0.nx - 1u);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const uint32_t i = t + 256u * (uint32_t)h;
+      if (i < 289u) {
+        const uint32_t y = min(byi * 16u + i % 17u, L0.ny - 1u), z = min(bzi * 16u + i / 17u, L0.nz - 1u);
+        xv[h] = src_voxel<SRC_CHUNKS>(src, L0, X, y, z);
+      }
+    }
+  }
+  if (t < BK_GROUP * 8u) m8[t] = 0u;
+  if (t < BK_GROUP * (R3D_BRICK_BYTES - R3D_BRICK_CONTENT)) {
+    const uint32_t pad = R3D_BRICK_BYTES - R3D_BRICK_CONTENT;
+    tile[(t / pad) * R3D_BRICK_BYTES + R3D_BRICK_CONTENT + t % pad] = 0;
+  }
+  uint32_t row_lo[BK_GROUP], row_hi[BK_GROUP];  // maxima of the row's x < 8 and x >= 8 halves
+#pragma unroll
+  for (uint32_t k = 0; k < BK_GROUP; k++) {
+    const uint32_t w[4] = {P[k].x, P[k].y, P[k].z, P[k].w};
+    uint8_t *d = tile + k * R3D_BRICK_BYTES + (lz * 17u + ly) * 17u;
+    row_lo[k] = 0u;
+    row_hi[k] = 0u;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const uint32_t v = (w[i >> 2] >> ((i & 3) * 8)) & 255u;
+      d[i] = (uint8_t)v;
+      if (i < 8) row_lo[k] = v > row_lo[k] ? v : row_lo[k];
+      else row_hi[k] = v > row_hi[k] ? v : row_hi[k];
+    }
+  }
+  if (t < 128u + BK_GROUP) {
+    const uint32_t w[4] = {F.x, F.y, F.z, F.w};
+#pragma unroll
+    for (int i = 0; i < 16; i++) tile[fo + i] = (uint8_t)((w[i >> 2] >> ((i & 3) * 8)) & 255u);
+  }
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const uint32_t i = t + 256u * (uint32_t)h;
+    if (i < 289u) tile[(BK_GROUP - 1u) * R3D_BRICK_BYTES + i * 17u + 16u] = (uint8_t)xv[h];
+  }
+  __syncthreads();
+  // bricks 0..2: x = 16 face = the next brick's x = 0 plane (its y and z
+  // aprons included, clamped the same way)
+  for (uint32_t i = t; i < (BK_GROUP - 1u) * 289u; i += 256u) {
+    const uint32_t k = i / 289u, j = i % 289u;
+    tile[k * R3D_BRICK_BYTES + j * 17u + 16u] = tile[(k + 1u) * R3D_BRICK_BYTES + j * 17u];
+  }
+  {
+    const uint32_t q = ((lz >> 3) * 2u + (ly >> 3)) * 2u;
+#pragma unroll
+    for (uint32_t k = 0; k < BK_GROUP; k++) {
+      atomicMax(&m8[k * 8u + q], row_lo[k]);
+      atomicMax(&m8[k * 8u + q + 1u], row_hi[k]);
+    }
+  }
+  __syncthreads();
+  uint4 *dst = (uint4 *)(L0.base + b0 * R3D_BRICK_BYTES);
+  for (uint32_t i = t; i < BK_GROUP * R3D_BRICK_BYTES / 16u; i += 256u) dst[i] = ((const uint4 *)tile)[i];
+  // level-1 octants: 8^3 voxels per brick, two per lane and brick
+  if (do_l1 && (byi >> 1) < L1.by && (bzi >> 1) < L1.bz) {
+#pragma unroll
+    for (uint32_t k = 0; k < BK_GROUP; k++) {
+      const uint32_t bxi = bx0 + k;
+      if ((bxi >> 1) >= L1.bx) continue;
+      uint8_t *doct = L1.base +
+                      ((size_t)((bzi >> 1) * L1.by + (byi >> 1)) * L1.bx + (bxi >> 1)) * R3D_BRICK_BYTES +
+                      (((bzi & 1u) * 8u) * 17u + (byi & 1u) * 8u) * 17u + (bxi & 1u) * 8u;
+      const uint8_t *tk = tile + k * R3D_BRICK_BYTES;
+      uint32_t sum[2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const uint32_t o = t + 256u * (uint32_t)h;
+        const uint32_t x1 = o & 7u, y1 = (o >> 3) & 7u, z1 = o >> 6;
+        const uint8_t *r00 = tk + ((2u * z1) * 17u + 2u * y1) * 17u + 2u * x1;
+        uint32_t acc = 4u;
+#pragma unroll
+        for (uint32_t dz = 0; dz < 2; dz++)
+#pragma unroll
+          for (uint32_t dy = 0; dy < 2; dy++) {
+            const uint8_t *r = r00 + (dz * 17u + dy) * 17u;
+            acc += (uint32_t)r[0] + (uint32_t)r[1];
+          }
+        sum[h] = acc;
+      }
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const uint32_t o = t + 256u * (uint32_t)h;
+        const uint32_t x1 = o & 7u, y1 = (o >> 3) & 7u, z1 = o >> 6;
+        doct[(z1 * 17u + y1) * 17u + x1] = (uint8_t)(sum[h] / 8u);
+      }
+    }
+  }
+  if (t < BK_GROUP * 8u) {
+    const uint32_t k = t >> 3, j = t & 7u;
+    const uint32_t X = (bx0 + k) * 2u + (j & 1u), Y = byi * 2u + ((j >> 1) & 1u), Z = bzi * 2u + (j >> 2);
+    if (X < ox && Y < oy && Z < oz) raw[((size_t)Z * oy + Y) * ox + X] = (uint8_t)m8[t];
+  }
+}
+
 // One workgroup per brick. Apron and out-of-level bytes copy the payload
 // voxel at the clamped global position; payload bytes inside the level are
 // only read, so concurrent bricks never race. A full brick's 817 apron bytes
@@ -176,7 +336,7 @@ __global__ __launch_bounds__(256) void k_brickify(const uint8_t *src, LevelGeom 
 // before any store (a lane walking a 17-byte apron row made a serial chain of
 // 17 dependent memory round trips); partial bricks walk their 4913 bytes.
 __global__ __launch_bounds__(256) void k_apron(LevelGeom L) {
-  const uint32_t b = blockIdx.x;
+  const uint32_t b = xcd_band(blockIdx.x, gridDim.x);  // the +x neighbour brick on the same XCD
   const uint32_t bxi = b % L.bx, byi = (b / L.bx) % L.by, bzi = b / (L.bx * L.by);
   const uint32_t t = threadIdx.x;
   const uint32_t xf = min(16u, L.nx - bxi * 16u), yf = min(16u, L.ny - byi * 16u), zf = min(16u, L.nz - bzi * 16u);
@@ -517,8 +677,13 @@ static int build_tail(r3d_hip_ctx *ctx, r3d_hip_volume *v, hipStream_t s, bool l
 template <int SRC>
 static int brickify_and_build(r3d_hip_ctx *ctx, r3d_hip_volume *v, const uint8_t *src, hipStream_t s) {
   const LevelGeom l1 = v->nlev > 1 ? geom(v, 1) : geom(v, 0);
-  hipLaunchKernelGGL(k_brickify<SRC>, brick_grid(v, 0), dim3(256), 0, s, src, geom(v, 0), l1, v->nlev > 1 ? 1 : 0,
-                     v->d_occ_raw, v->odim[0], v->odim[1], v->odim[2]);
+  const LevelGeom l0 = geom(v, 0);
+  if (SRC == SRC_CHUNKS && l0.bx % BK_GROUP == 0u)
+    hipLaunchKernelGGL(k_brickify4, dim3(nbricks(v, 0) / BK_GROUP), dim3(256), 0, s, src, l0, l1,
+                       v->nlev > 1 ? 1 : 0, v->d_occ_raw, v->odim[0], v->odim[1], v->odim[2]);
+  else
+    hipLaunchKernelGGL(k_brickify<SRC>, brick_grid(v, 0), dim3(256), 0, s, src, l0, l1, v->nlev > 1 ? 1 : 0,
+                       v->d_occ_raw, v->odim[0], v->odim[1], v->odim[2]);
   R3D_HIP_CHECK(ctx, hipGetLastError());
   return build_tail(ctx, v, s, v->nlev > 1);
 }

@@ -77,6 +77,29 @@ def test_volume_decode_and_build_from_chunks(ctx, n, nlev):
         assert np.array_equal(v.read_occupancy(), occ)
 
 
+@pytest.mark.parametrize("dims", [(128, 48, 80), (64, 16, 32), (192, 32, 16), (80, 48, 32)])
+def test_build_from_chunks_shapes(ctx, dims):
+    """Level 0..n and the occupancy grid from flat chunks of non-cubic volumes
+    against the oracle's pyramid: rows of 8, 4 and 12 bricks (taken four at a
+    time), a single brick row or plane (the y and z aprons clamp to the
+    brick's own last layer), and a row of 5 (one brick at a time)."""
+    nx, ny, nz = dims
+    bx, by, bz = nx // 16, ny // 16, nz // 16
+    rng = np.random.default_rng(nx * 7 + ny * 3 + nz)
+    lin = orc.synth_sheets(nx, ny, nz, 4, 31 + nx)
+    lin[rng.random(lin.shape) < 0.02] = 255  # isolated bright voxels: every octant and apron byte matters
+    chunks = lin.reshape(bz, 16, by, 16, bx, 16).transpose(0, 2, 4, 1, 3, 5).reshape(-1, 4096).copy()
+    ref = orc.pyramid(lin, nx, ny, nz, orc.full_levels(nx, ny, nz))
+    odim, occ = orc.occupancy(lin, nx, ny, nz)
+    v = ctx.volume(nx, ny, nz, 0)
+    v.build_from_chunks(torch.from_numpy(chunks).cuda())
+    assert v.nlevels == len(ref)
+    for l in range(v.nlevels):
+        assert np.array_equal(v.read_level(l), ref[l][1]), l
+    assert v.odim == odim
+    assert np.array_equal(v.read_occupancy(), occ)
+
+
 def _views(width, height):
     lut = scene.tf_ramp_40_200()
     gate = scene.skip_gate(lut)
