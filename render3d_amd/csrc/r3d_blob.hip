@@ -52,8 +52,8 @@ __constant__ uint16_t c_prior[NCTX] = {
 struct RDec {
   const uint8_t *p;  // stream start
   uint32_t n;        // stream length
-  uint32_t at;       // next byte to consume
-  uint64_t w;        // bytes [at, at + wn), stream order from bit 63 down
+  uint32_t qn;       // the stream byte after the window (take() leaves it unchanged)
+  uint64_t w;        // the window: bytes [qn - wn, qn), stream order from bit 63 down
   uint32_t wn;
   uint32_t stg_k, stg_skip, stg_cnt;  // a staged dword (dword stg_k of blk), pending while stg_cnt > 0
   uint4 blk;                          // the aligned 16-byte block the window is read from
@@ -75,6 +75,7 @@ struct RDec {
     be &= ~(0xFFFFFFFFu >> (8u * cnt - 1u) >> 1);  // keep the first cnt bytes (cnt in 1..4)
     w |= (uint64_t)be << (32u - 8u * wn);
     wn += cnt;
+    qn += cnt;
   }
   __device__ __forceinline__ void merge() {
     if (stg_cnt) {
@@ -90,8 +91,9 @@ struct RDec {
   // holding a stream byte lies in the buffer's pages, like the dword did.
   __device__ __forceinline__ void refill() {
     if (stg_cnt == 0u && wn <= 4u) {
-      const uint32_t q = at + wn;
+      const uint32_t q = qn;
       if (q >= n) {
+        qn += 8u - wn;
         wn = 8;  // the rest of the window is the stream's zero tail
       } else {
         const uintptr_t a = (uintptr_t)(p + q);
@@ -110,8 +112,9 @@ struct RDec {
   __device__ __forceinline__ void fill_sync(uint32_t need) {
     merge();
     while (wn < need) {
-      const uint32_t q = at + wn;
+      const uint32_t q = qn;
       if (q >= n) {
+        qn += 8u - wn;
         wn = 8;
         break;
       }
@@ -126,12 +129,11 @@ struct RDec {
     code = (uint32_t)((((uint64_t)code << 32) | (w >> 32)) >> (32u - 8u * k));
     w <<= 8u * k;
     wn -= k;
-    at += k;
   }
   __device__ __forceinline__ void init(const uint8_t *b, uint32_t len) {
     p = b;
     n = len;
-    at = 0;
+    qn = 0;
     w = 0;
     wn = 0;
     stg_cnt = 0;
@@ -142,10 +144,13 @@ struct RDec {
     take(1);  // the first byte is shifted out of code, as in dct3d.c:227
     range = 0xFFFFFFFFu;
   }
-  // Renormalise: range >= 2^8 on entry (every caller guarantees it), so at
-  // most two bytes.
+  // Renormalise: bytes until range >= 2^24. range >= 2^8 on entry (a context
+  // decision leaves at least range >> 12 >= 2^12 since its probability lies in
+  // [1, 4095], a bypass 2^23, a raw-bits step of s <= 16 bits 2^8), so at most
+  // two bytes, and their count is the leading-zero count / 8 (0 for
+  // [2^24, 2^32), 1 for [2^16, 2^24), 2 for [2^8, 2^16)).
   __device__ __forceinline__ void norm() {
-    const uint32_t k = (range < RC_TOP ? 1u : 0u) + (range < (1u << 16) ? 1u : 0u);
+    const uint32_t k = (uint32_t)__clz((int)range) >> 3;
     take(k);
     range <<= 8u * k;
   }
@@ -289,7 +294,9 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
     // sector) from a per-lane shift register, when the capacity and the
     // array keep such groups aligned: a scattered 4-byte store per pair made
     // the memory system fetch and write whole sectors, and on this ISA every
-    // store also holds up the next wait for a window refill.
+    // store also holds up the next wait for a window refill. (Groups of 4,
+    // one 16-byte store each, save four register moves per iteration but
+    // write each sector twice: 0.93 GB more per 262,144 blobs.)
     const bool vec8 = (a.cap & 7u) == 0u && ((uintptr_t)a.pairs & 31u) == 0u;
     uint32_t pb0 = 0, pb1 = 0, pb2 = 0, pb3 = 0, pb4 = 0, pb5 = 0, pb6 = 0, pb7 = 0;
     uint32_t e = s_scan[0];

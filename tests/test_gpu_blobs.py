@@ -129,6 +129,22 @@ def test_volume_round_trip_and_oracle_encoder_agreement(ctx):
         assert np.array_equal(d[o[c]:o[c] + ln[c]], want), c
 
 
+def test_pair_output_paths(ctx):
+    """Pairs leave a lane eight at a time (32-byte groups) when the capacity is
+    a multiple of 8, one at a time otherwise; both give the reference's levels
+    for every golden blob, whatever its count modulo 8."""
+    data, offs, lens = upload(GOLD["blob_bytes"], GOLD["blob_offsets"])
+    counts = (GOLD["blob_levels"] != 0).sum(axis=1)
+    assert len(set(int(c) % 8 for c in counts)) > 4
+    cap8 = int(counts.max()) + 8 - int(counts.max()) % 8
+    for cap in (cap8, cap8 + 1, cap8 + 4, cap8 + 8):
+        parsed = ctx.parse_blobs(data, offs, lens, max_pairs=cap)
+        torch.cuda.synchronize()
+        assert np.all(parsed["status"].cpu().numpy() == 1), cap
+        assert np.array_equal(parsed["pair_counts"].cpu().numpy(), counts), cap
+        assert np.array_equal(levels_of(parsed, cap), GOLD["blob_levels"]), cap
+
+
 def test_capacity_dtype_and_empty_blobs(ctx):
     offs_np, pairs, hdr = workloads.sparse_laplacian_batch(8, 3)
     lv = workloads.densify(offs_np, pairs)
