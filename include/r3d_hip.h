@@ -515,13 +515,17 @@ typedef struct r3d_bricks_desc {
   uint32_t reserved0;
 } r3d_bricks_desc;
 
+/* Allocates the page table, the abpa^3-slot atlas and the build's scratch
+ * (levels 1.. as linear arrays, about 1/7 of the base volume, and the job
+ * list), so that a build allocates nothing. */
 int r3d_hip_bricks_create(r3d_hip_ctx *ctx, const r3d_bricks_desc *desc, r3d_hip_bricks **out);
 void r3d_hip_bricks_destroy(r3d_hip_bricks *b);
 /* Builds the pyramid of vol (device, nz x ny x nx u8, x fastest) with the
  * rounded 2x box (tools/lodpack/main.c:216-227) and makes resident the bricks
  * whose resident[] flag is set (host, one byte per page entry in page order
  * after the 64-word header; NULL = all), in slots taken in page order; the
- * others read 0xFFFFFFFF. Voxels of a slot past its level's edge are 0. */
+ * others read 0xFFFFFFFF. Voxels of a slot past its level's edge are 0.
+ * Runs on the stream after its earlier work and returns when done. */
 int r3d_hip_bricks_build(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const uint8_t *vol, const uint8_t *resident,
                          void *stream);
 /* The device page table and atlas, for a streamer that manages residency

@@ -89,8 +89,11 @@ struct BkFill {
 constexpr uint32_t BK_PARTS = 64u;  // workgroups per brick: 256 rows of 128 voxels each
 
 // BK_PARTS workgroups per resident brick, each copying 256 of its 16,384
-// rows of 128 voxels (one row per lane, zeros past the level's edge) and
-// folding its maximum into the brick's word with an atomic max.
+// rows of 128 voxels (zeros past the level's edge) and folding its maximum
+// into the brick's word with an atomic max. Rows that lie whole and 16-byte
+// aligned in the level go eight lanes to a row, 16 bytes each, so that a wave
+// instruction touches 8 lines rather than 64; other bricks take one row per
+// lane.
 __global__ __launch_bounds__(256) void k_bk_fill(BkFill a) {
   __shared__ uint32_t s_max[4];
   const uint32_t j = blockIdx.x / BK_PARTS, part = blockIdx.x % BK_PARTS;
@@ -101,33 +104,39 @@ __global__ __launch_bounds__(256) void k_bk_fill(BkFill a) {
   uint8_t *S = a.atlas + (uint64_t)job.slot * BK_VOX;
   const uint32_t x0 = job.cx * BK;
   const bool whole_x = x0 + BK <= nx && nx % 16u == 0u && ((uintptr_t)L & 15u) == 0u;
-  const uint32_t r = part * 256u + threadIdx.x;  // row of the brick
-  const uint32_t y = r & 127u, z = r >> 7;
-  const uint32_t gy = job.cy * BK + y, gz = job.cz * BK + z;
-  uint4 *dst = (uint4 *)(S + ((uint64_t)z * BK + y) * BK);
   uint32_t mx = 0;
-  if (gy >= ny || gz >= nz) {
-    for (int k = 0; k < 8; k++) dst[k] = make_uint4(0, 0, 0, 0);
+  if (whole_x) {
+    const uint32_t c = threadIdx.x & 7u, rs = threadIdx.x >> 3;  // 16-byte column, first row
+    uint4 q[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const uint32_t r = part * 256u + rs + 32u * (uint32_t)i, y = r & 127u, z = r >> 7;
+      const uint32_t gy = job.cy * BK + y, gz = job.cz * BK + z;
+      q[i] = make_uint4(0, 0, 0, 0);
+      if (gy < ny && gz < nz) q[i] = *(const uint4 *)(L + ((uint64_t)gz * ny + gy) * nx + x0 + 16u * c);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const uint32_t r = part * 256u + rs + 32u * (uint32_t)i, y = r & 127u, z = r >> 7;
+      *(uint4 *)(S + ((uint64_t)z * BK + y) * BK + 16u * c) = q[i];
+      const uint32_t w[4] = {q[i].x, q[i].y, q[i].z, q[i].w};
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int bb = 0; bb < 4; bb++) {
+          const uint32_t v = (w[k] >> (8 * bb)) & 255u;
+          mx = v > mx ? v : mx;
+        }
+    }
   } else {
-    const uint8_t *row = L + ((uint64_t)gz * ny + gy) * nx + x0;
-    if (whole_x) {
-      uint4 q[8];
-#pragma unroll
-      for (int k = 0; k < 8; k++) q[k] = ((const uint4 *)row)[k];
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        dst[k] = q[k];
-        const uint32_t w[4] = {q[k].x, q[k].y, q[k].z, q[k].w};
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-          for (int bb = 0; bb < 4; bb++) {
-            const uint32_t v = (w[i] >> (8 * bb)) & 255u;
-            mx = v > mx ? v : mx;
-          }
-      }
+    const uint32_t r = part * 256u + threadIdx.x;  // row of the brick
+    const uint32_t y = r & 127u, z = r >> 7;
+    const uint32_t gy = job.cy * BK + y, gz = job.cz * BK + z;
+    uint8_t *d8 = S + ((uint64_t)z * BK + y) * BK;
+    if (gy >= ny || gz >= nz) {
+      for (int k = 0; k < 8; k++) ((uint4 *)d8)[k] = make_uint4(0, 0, 0, 0);
     } else {
-      uint8_t *d8 = (uint8_t *)dst;
+      const uint8_t *row = L + ((uint64_t)gz * ny + gy) * nx + x0;
       for (uint32_t x = 0; x < BK; x++) {
         const uint32_t v = x0 + x < nx ? row[x] : 0u;
         d8[x] = (uint8_t)v;
@@ -199,6 +208,16 @@ extern "C" int r3d_hip_bricks_create(r3d_hip_ctx *ctx, const r3d_bricks_desc *d,
     r3d_hip_bricks_destroy(b);
     return r3d_fail(ctx, R3D_ERR_NOMEM, "bricks_create: %u slots of 2 MiB", d->abpa * d->abpa * d->abpa);
   }
+  // the build's scratch, allocated once so that a build allocates nothing
+  const uint64_t entries = words - 64u;
+  b->max_jobs = (uint32_t)(entries < b->nslots ? entries : b->nslots);
+  bool ok = hipMalloc(&b->d_jobs, (sizeof(BkJob) + 4u) * (uint64_t)b->max_jobs) == hipSuccess;
+  for (uint32_t l = 1; l < d->nlev && ok; l++)
+    ok = hipMalloc(&b->d_lev[l], (uint64_t)b->dims[l][0] * b->dims[l][1] * b->dims[l][2]) == hipSuccess;
+  if (!ok) {
+    r3d_hip_bricks_destroy(b);
+    return r3d_fail(ctx, R3D_ERR_NOMEM, "bricks_create: build scratch (coarser levels and job list)");
+  }
   std::vector<uint32_t> page(b->page_words, 0xFFFFFFFFu);
   for (uint32_t i = 0; i < 64u; i++) page[i] = 0u;
   page[0] = d->nlev;
@@ -223,6 +242,9 @@ extern "C" void r3d_hip_bricks_destroy(r3d_hip_bricks *b) {
   if (b->ctx) hipSetDevice(b->ctx->device);
   if (b->d_page) hipFree(b->d_page);
   if (b->d_atlas) hipFree(b->d_atlas);
+  if (b->d_jobs) hipFree(b->d_jobs);
+  for (uint32_t l = 0; l < R3D_MAX_LEVELS; l++)
+    if (b->d_lev[l]) hipFree(b->d_lev[l]);
   delete b;
 }
 
@@ -257,22 +279,15 @@ extern "C" int r3d_hip_bricks_build(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const u
                            (uint32_t)jobs.size()});
     }
   }
-  // levels 1.. as temporary linear arrays
+  // levels 1.. as linear arrays in the handle's scratch
   BkFill f{};
   f.lev[0] = vol;
-  std::vector<uint8_t *> tmp;
-  int rc = R3D_OK;
   for (uint32_t l = 0; l < nlev; l++)
     for (int a = 0; a < 3; a++) f.dims[l][a] = b->dims[l][a];
-  for (uint32_t l = 1; l < nlev && rc == R3D_OK; l++) {
-    uint8_t *p = nullptr;
-    const uint64_t n = (uint64_t)b->dims[l][0] * b->dims[l][1] * b->dims[l][2];
-    if (hipMalloc(&p, n) != hipSuccess) {
-      rc = r3d_fail(ctx, R3D_ERR_NOMEM, "bricks_build: level %u", l);
-      break;
-    }
-    tmp.push_back(p);
+  for (uint32_t l = 1; l < nlev; l++) {
+    uint8_t *p = b->d_lev[l];
     f.lev[l] = p;
+    const uint64_t n = (uint64_t)b->dims[l][0] * b->dims[l][1] * b->dims[l][2];
     const uint32_t snx = b->dims[l - 1][0], mxl = b->dims[l][0];
     const bool vec16 = snx % 16u == 0u && mxl * 2u == snx && ((uintptr_t)f.lev[l - 1] & 15u) == 0u;
     const uint64_t items = vec16 ? n / 8u : n;
@@ -284,31 +299,26 @@ extern "C" int r3d_hip_bricks_build(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const u
     else
       hipLaunchKernelGGL(k_bk_down, dim3(grid), dim3(256), 0, st, f.lev[l - 1], snx, b->dims[l - 1][1],
                          b->dims[l - 1][2], p, mxl, b->dims[l][1], b->dims[l][2]);
-    if (hipGetLastError() != hipSuccess) rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: downsample launch");
+    R3D_HIP_CHECK(ctx, hipGetLastError());
   }
-  BkJob *djobs = nullptr;  // the job list, then one max word per job
-  if (rc == R3D_OK && !jobs.empty() && hipMalloc(&djobs, (sizeof(BkJob) + 4u) * jobs.size()) != hipSuccess)
-    rc = r3d_fail(ctx, R3D_ERR_NOMEM, "bricks_build: job list");
-  if (rc == R3D_OK && hipStreamSynchronize(st) != hipSuccess) rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: sync");
-  if (rc == R3D_OK && hipMemcpy(b->d_page + 64, page.data(), 4ull * page.size(), hipMemcpyHostToDevice) != hipSuccess)
-    rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: page reset");
-  if (rc == R3D_OK && djobs &&
-      hipMemcpy(djobs, jobs.data(), sizeof(BkJob) * jobs.size(), hipMemcpyHostToDevice) != hipSuccess)
-    rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: job upload");
-  if (rc == R3D_OK && djobs) {
+  // the entries reset to absent, then the resident ones written by the fill;
+  // the copies are ordered on the stream after work that may read the page
+  R3D_HIP_CHECK(ctx, hipMemcpyAsync(b->d_page + 64, page.data(), 4ull * page.size(), hipMemcpyHostToDevice, st));
+  if (!jobs.empty()) {
+    BkJob *djobs = (BkJob *)b->d_jobs;
+    R3D_HIP_CHECK(ctx, hipMemcpyAsync(djobs, jobs.data(), sizeof(BkJob) * jobs.size(), hipMemcpyHostToDevice, st));
     f.jobs = djobs;
     f.njobs = (uint32_t)jobs.size();
     f.atlas = b->d_atlas;
     f.page = b->d_page;
-    f.maxv = (uint32_t *)(djobs + jobs.size());
+    f.maxv = (uint32_t *)(djobs + b->max_jobs);
     hipLaunchKernelGGL(k_bk_zero, dim3((f.njobs + 255u) / 256u), dim3(256), 0, st, f.maxv, f.njobs);
     hipLaunchKernelGGL(k_bk_fill, dim3(f.njobs * BK_PARTS), dim3(256), 0, st, f);
     hipLaunchKernelGGL(k_bk_entries, dim3((f.njobs + 255u) / 256u), dim3(256), 0, st, (const BkJob *)djobs, f.njobs,
                        (const uint32_t *)f.maxv, f.page);
-    if (hipGetLastError() != hipSuccess) rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: fill launch");
+    R3D_HIP_CHECK(ctx, hipGetLastError());
   }
-  if (hipStreamSynchronize(st) != hipSuccess && rc == R3D_OK) rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build: sync");
-  if (djobs) hipFree(djobs);
-  for (uint8_t *p : tmp) hipFree(p);
-  return rc;
+  // the host's page and job vectors live until the copies have run
+  R3D_HIP_CHECK(ctx, hipStreamSynchronize(st));
+  return R3D_OK;
 }

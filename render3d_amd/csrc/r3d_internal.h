@@ -88,6 +88,11 @@ struct r3d_hip_bricks {
   uint32_t nslots = 0;
   uint32_t *d_page = nullptr;
   uint8_t *d_atlas = nullptr;
+  // build scratch, sized at create and kept: levels 1.. as linear arrays, and
+  // the job list with one max word per job (at most max_jobs)
+  uint8_t *d_lev[R3D_MAX_LEVELS] = {};
+  void *d_jobs = nullptr;
+  uint32_t max_jobs = 0;
 };
 
 // error helpers
