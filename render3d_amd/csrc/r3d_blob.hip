@@ -245,7 +245,7 @@ struct ParseArgs {
 
 __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
   __shared__ uint16_t s_scan[4096];
-  __shared__ uint16_t s_ctx[NCTX * BLOB_NT];
+  __shared__ __attribute__((aligned(16))) uint16_t s_ctx[NCTX * BLOB_NT];
   const uint32_t t = threadIdx.x;
   for (uint32_t i = t; i < 4096u; i += BLOB_NT) s_scan[i] = c_scan[i];
   __syncthreads();
@@ -291,14 +291,17 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
     enum : uint32_t { PH_SIG, PH_MAG, PH_EGP, PH_SIGN, PH_DONE };
     uint32_t p = 0, recent = 0, m = 0, nb = 0;
     // Pairs go out 8 at a time (two 16-byte stores filling a 32-byte
-    // sector) from a per-lane shift register, when the capacity and the
-    // array keep such groups aligned: a scattered 4-byte store per pair made
-    // the memory system fetch and write whole sectors, and on this ISA every
-    // store also holds up the next wait for a window refill. (Groups of 4,
-    // one 16-byte store each, save four register moves per iteration but
-    // write each sector twice: 0.93 GB more per 262,144 blobs.)
+    // sector) when the capacity and the array keep such groups aligned: a
+    // scattered 4-byte store per pair made the memory system fetch and write
+    // whole sectors, and on this ISA every store also holds up the next wait
+    // for a window refill. A lane stages its pairs 0..6 of a group in LDS, in
+    // the EOB context rows (44..57: 14 x 256 halfwords = 7 x 256 dwords),
+    // which the level loop and everything after it never touch; the eighth
+    // completes the group from a register. (A register shift of 8 costs 8
+    // moves in nearly every iteration, since some lane of the wave is nearly
+    // always at a sign; groups of 4 in registers wrote each sector twice.)
     const bool vec8 = (a.cap & 7u) == 0u && ((uintptr_t)a.pairs & 31u) == 0u;
-    uint32_t pb0 = 0, pb1 = 0, pb2 = 0, pb3 = 0, pb4 = 0, pb5 = 0, pb6 = 0, pb7 = 0;
+    uint32_t *stg = (uint32_t *)(s_ctx + CTX_EOB * BLOB_NT) + t;  // slot j at stg[j * BLOB_NT]
     uint32_t e = s_scan[0];
     uint32_t ph = eob == 0u ? PH_DONE : eob == 1u ? PH_MAG : PH_SIG;
     if (eob == 1u) recent = 1u;
@@ -328,18 +331,13 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
           const uint32_t lv = (uint32_t)(uint16_t)(int16_t)(b ? -(int32_t)m : (int32_t)m);
           const uint32_t pr = (e & 4095u) | (lv << 16);
           if (vec8) {
-            pb0 = pb1;
-            pb1 = pb2;
-            pb2 = pb3;
-            pb3 = pb4;
-            pb4 = pb5;
-            pb5 = pb6;
-            pb6 = pb7;
-            pb7 = pr;
-            if ((np & 7u) == 7u) {
+            const uint32_t slot = np & 7u;
+            if (slot == 7u) {
               uint4 *q = (uint4 *)(a.pairs + base + np - 7u);
-              q[0] = make_uint4(pb0, pb1, pb2, pb3);
-              q[1] = make_uint4(pb4, pb5, pb6, pb7);
+              q[0] = make_uint4(stg[0], stg[BLOB_NT], stg[2u * BLOB_NT], stg[3u * BLOB_NT]);
+              q[1] = make_uint4(stg[4u * BLOB_NT], stg[5u * BLOB_NT], stg[6u * BLOB_NT], pr);
+            } else {
+              stg[slot * BLOB_NT] = pr;
             }
           } else {
             a.pairs[base + np] = pr;
@@ -370,10 +368,9 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
         d.refill();
       }
     }
-    if (vec8) {  // the last np % 8 pairs, oldest first, are the newest registers
+    if (vec8) {  // the last np % 8 pairs are staged in slots 0 .. np % 8 - 1
       const uint32_t r = np & 7u;
-      const uint32_t tail[8] = {pb0, pb1, pb2, pb3, pb4, pb5, pb6, pb7};
-      for (uint32_t k = 0; k < r; k++) a.pairs[base + np - r + k] = tail[8u - r + k];
+      for (uint32_t k = 0; k < r; k++) a.pairs[base + np - r + k] = stg[k * BLOB_NT];
     }
     if (has_corr) {  // dct3d.c:572-587
       uint32_t cnt = d.eg() + 1u, pos = 0;
