@@ -396,14 +396,15 @@ __global__ __launch_bounds__(256) void k_downsample(LevelGeom S, LevelGeom D) {
       const uint32_t lz = p / 289u, ly = (p / 17u) % 17u, lx = p % 17u;
       const uint32_t x = min(bxi * 16u + lx, D.nx - 1u), y = min(byi * 16u + ly, D.ny - 1u),
                      z = min(bzi * 16u + lz, D.nz - 1u);
-      uint32_t acc = 4u;
-#pragma unroll
-      for (uint32_t d = 0; d < 8u; d++) {
-        const uint32_t sx = min(2u * x + (d & 1u), S.nx - 1u), sy = min(2u * y + ((d >> 1) & 1u), S.ny - 1u),
-                       sz = min(2u * z + (d >> 2), S.nz - 1u);
-        acc += S.base[payload_index(S, sx, sy, sz)];
-      }
-      sum[k] = acc;
+      // The box's sources 2x, 2x + 1 (per axis) lie in one source brick (2x is
+      // even) and 2x <= S.n - 1 (level dims are floor halvings); 2x + 1 is
+      // clamped to 2x only on a one-voxel source axis. So one index and the
+      // payload strides (1, 17, 289), zero where clamped, give all eight.
+      const uint8_t *s0 = S.base + payload_index(S, 2u * x, 2u * y, 2u * z);
+      const uint32_t ox = 2u * x + 1u < S.nx ? 1u : 0u, oy = 2u * y + 1u < S.ny ? 17u : 0u,
+                     oz = 2u * z + 1u < S.nz ? 289u : 0u;
+      sum[k] = 4u + (uint32_t)s0[0] + s0[ox] + s0[oy] + s0[ox + oy] + s0[oz] + s0[ox + oz] + s0[oy + oz] +
+               s0[ox + oy + oz];
     }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
