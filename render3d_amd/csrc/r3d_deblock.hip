@@ -364,13 +364,13 @@ __global__ __launch_bounds__(256) void k_db_tiles(DbTile a) {
     const uint32_t n0 = (f.x >> 16) | (f.y << 16), n1 = (f.y >> 16) | (f.z << 16), n2 = (f.z >> 16) | (f.w << 16),
                    n3 = (f.w >> 16) | (nf << 16);
     uint32_t *d = (uint32_t *)(row + xc);
-    if (n0 != W[q].x) d[0] = n0;
-    if (n1 != W[q].y) d[1] = n1;
-    if (n2 != W[q].z) d[2] = n2;
-    if (c < 7u) {
-      if (n3 != W[q].w) d[3] = n3;
-    } else if ((n3 & 0xFFFFu) != (W[q].w & 0xFFFFu)) {
-      *(uint16_t *)(row + xc + 12u) = (uint16_t)(n3 & 0xFFFFu);
+    if (c < 7u) {  // all 16 bytes are this tile's: one store when any changed
+      if (n0 != W[q].x || n1 != W[q].y || n2 != W[q].z || n3 != W[q].w) *(uint4 *)d = make_uint4(n0, n1, n2, n3);
+    } else {
+      if (n0 != W[q].x) d[0] = n0;
+      if (n1 != W[q].y) d[1] = n1;
+      if (n2 != W[q].z) d[2] = n2;
+      if ((n3 & 0xFFFFu) != (W[q].w & 0xFFFFu)) *(uint16_t *)(row + xc + 12u) = (uint16_t)(n3 & 0xFFFFu);
     }
   }
 }
