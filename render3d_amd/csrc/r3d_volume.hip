@@ -401,10 +401,20 @@ __global__ __launch_bounds__(256) void k_downsample(LevelGeom S, LevelGeom D) {
       // clamped to 2x only on a one-voxel source axis. So one index and the
       // payload strides (1, 17, 289), zero where clamped, give all eight.
       const uint8_t *s0 = S.base + payload_index(S, 2u * x, 2u * y, 2u * z);
-      const uint32_t ox = 2u * x + 1u < S.nx ? 1u : 0u, oy = 2u * y + 1u < S.ny ? 17u : 0u,
-                     oz = 2u * z + 1u < S.nz ? 289u : 0u;
-      sum[k] = 4u + (uint32_t)s0[0] + s0[ox] + s0[oy] + s0[ox + oy] + s0[oz] + s0[ox + oz] + s0[oy + oz] +
-               s0[ox + oy + oz];
+      const uint32_t oy = 2u * y + 1u < S.ny ? 17u : 0u, oz = 2u * z + 1u < S.nz ? 289u : 0u;
+      if (S.nx >= 2u) {  // wave-uniform: 2x + 1 < S.nx, so each x pair is one 2-byte load
+        uint16_t q[4];
+        __builtin_memcpy(&q[0], s0, 2);
+        __builtin_memcpy(&q[1], s0 + oy, 2);
+        __builtin_memcpy(&q[2], s0 + oz, 2);
+        __builtin_memcpy(&q[3], s0 + oy + oz, 2);
+        uint32_t acc = 4u;
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc += (uint32_t)(q[i] & 255u) + (uint32_t)(q[i] >> 8);
+        sum[k] = acc;
+      } else {
+        sum[k] = 4u + 2u * ((uint32_t)s0[0] + s0[oy] + s0[oz] + s0[oy + oz]);
+      }
     }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
