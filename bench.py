@@ -66,6 +66,11 @@ def parse():
     return ap.parse_args()
 
 
+def rows_desc(rs):
+    """The rows a CPU-baseline sample renders."""
+    return "every row" if rs == 1 else f"every {rs}th row"
+
+
 def read_traffic(path, key="traffic_bytes_per_launch"):
     """`<key>: N` (HBM bytes or instructions per launch) from a committed
     rocprof summary, or None."""
@@ -808,7 +813,7 @@ def cpu_bricks(state, args):
     rows = np.arange(0, H, rs)
     err = float(np.max(np.abs(state["img"][rows] - ref[rows])))
     cpu = {"value": samples / dt, "unit": "ray samples/s", "cores": cores, "kind": "port",
-           "sample": f"{reps} x the {W}x{H} bricks-LOD frame (every {rs}th row), {rays} rays, {samples} samples, "
+           "sample": f"{reps} x the {W}x{H} bricks-LOD frame ({rows_desc(rs)}), {rays} rays, {samples} samples, "
                      f"{dt:.1f} s"}
     return cpu, {"raycast_max_abs": err, "raycast_rows_checked": int(rows.size)}
 
@@ -1045,7 +1050,7 @@ def cpu_baseline(ctx, vol, view, lut, img, args, levels, headers, step_tab, n, W
             break
     dt2 = time.perf_counter() - t2
     cpu = {"value": samples / dt, "unit": "ray samples/s", "cores": cores, "kind": "port",
-           "sample": f"{reps} x the {W}x{H} frame (every {rs}th row), {rays} rays, {samples} samples, {dt:.1f} s",
+           "sample": f"{reps} x the {W}x{H} frame ({rows_desc(rs)}), {rays} rays, {samples} samples, {dt:.1f} s",
            "decode_GBps": dreps * k * CHUNK_BYTES / dt2 / 1e9,
            "decode_sample": f"{dreps} x {k} chunks of the batch, {dt2:.1f} s"}
     parity = {"raycast_max_abs": err, "raycast_rows_checked": int(rows.size)}
