@@ -302,23 +302,29 @@ extern "C" int r3d_hip_bricks_build(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const u
     R3D_HIP_CHECK(ctx, hipGetLastError());
   }
   // the entries reset to absent, then the resident ones written by the fill;
-  // the copies are ordered on the stream after work that may read the page
+  // the copies are ordered on the stream after work that may read the page.
+  // Once a copy is queued, every exit synchronizes first: the copies read the
+  // host's page and job vectors, which end with this call.
   R3D_HIP_CHECK(ctx, hipMemcpyAsync(b->d_page + 64, page.data(), 4ull * page.size(), hipMemcpyHostToDevice, st));
+  hipError_t err = hipSuccess;
   if (!jobs.empty()) {
     BkJob *djobs = (BkJob *)b->d_jobs;
-    R3D_HIP_CHECK(ctx, hipMemcpyAsync(djobs, jobs.data(), sizeof(BkJob) * jobs.size(), hipMemcpyHostToDevice, st));
-    f.jobs = djobs;
-    f.njobs = (uint32_t)jobs.size();
-    f.atlas = b->d_atlas;
-    f.page = b->d_page;
-    f.maxv = (uint32_t *)(djobs + b->max_jobs);
-    hipLaunchKernelGGL(k_bk_zero, dim3((f.njobs + 255u) / 256u), dim3(256), 0, st, f.maxv, f.njobs);
-    hipLaunchKernelGGL(k_bk_fill, dim3(f.njobs * BK_PARTS), dim3(256), 0, st, f);
-    hipLaunchKernelGGL(k_bk_entries, dim3((f.njobs + 255u) / 256u), dim3(256), 0, st, (const BkJob *)djobs, f.njobs,
-                       (const uint32_t *)f.maxv, f.page);
-    R3D_HIP_CHECK(ctx, hipGetLastError());
+    err = hipMemcpyAsync(djobs, jobs.data(), sizeof(BkJob) * jobs.size(), hipMemcpyHostToDevice, st);
+    if (err == hipSuccess) {
+      f.jobs = djobs;
+      f.njobs = (uint32_t)jobs.size();
+      f.atlas = b->d_atlas;
+      f.page = b->d_page;
+      f.maxv = (uint32_t *)(djobs + b->max_jobs);
+      hipLaunchKernelGGL(k_bk_zero, dim3((f.njobs + 255u) / 256u), dim3(256), 0, st, f.maxv, f.njobs);
+      hipLaunchKernelGGL(k_bk_fill, dim3(f.njobs * BK_PARTS), dim3(256), 0, st, f);
+      hipLaunchKernelGGL(k_bk_entries, dim3((f.njobs + 255u) / 256u), dim3(256), 0, st, (const BkJob *)djobs,
+                         f.njobs, (const uint32_t *)f.maxv, f.page);
+      err = hipGetLastError();
+    }
   }
-  // the host's page and job vectors live until the copies have run
-  R3D_HIP_CHECK(ctx, hipStreamSynchronize(st));
+  const hipError_t sync = hipStreamSynchronize(st);
+  R3D_HIP_CHECK(ctx, err);
+  R3D_HIP_CHECK(ctx, sync);
   return R3D_OK;
 }
