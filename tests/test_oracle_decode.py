@@ -159,3 +159,34 @@ def test_a_planted_mistake_fails_the_golden_check():
     orc.load().r3do_idct3(r, out)
     assert not np.array_equal((out + np.float32(GOLD["blob_dcq"][i] / 16)).view(np.uint32),
                               GOLD["blob_pre"][i].view(np.uint32))
+
+
+def test_round_to_u8_as_one_biased_add():
+    """The decoder rounds to u8 as trunc(clamp(v + c, 0, 255)) with
+    c = 0.49999997f, the float just below 0.5, and the sum rounded to nearest
+    even in FP32. That equals the reference's clamp(roundf(v), 0, 255) with
+    NaN -> 0 (dct3d.c:625-635): here on every float within 4,096 ulps of each
+    tie k + 0.5 (k = 0 .. 300) and of 0, on 200,000 random floats in
+    [-2, 300], and on NaN, the infinities and the largest floats."""
+    c = np.float32(0.49999997)
+    assert c == np.nextafter(np.float32(0.5), np.float32(0)) and c < np.float32(0.5)
+
+    def reference(v):
+        w = v.astype(np.float64)
+        r = np.where(np.abs(w) >= 2.0 ** 23, w, np.trunc(w + np.copysign(0.5, w)))
+        return np.clip(np.where(np.isnan(w), 0.0, r), 0, 255).astype(np.uint32)
+
+    def biased(v):
+        s = (v + c).astype(np.float32)
+        s = np.minimum(np.where(np.isnan(s), np.float32(0), np.maximum(s, np.float32(0))), np.float32(255))
+        return np.trunc(s).astype(np.uint32)
+
+    centres = np.concatenate([np.arange(301, dtype=np.float32) + np.float32(0.5), np.float32([0.0, 255.0])])
+    bits = centres.view(np.uint32).astype(np.int64)[:, None] + np.arange(-4096, 4097)[None, :]
+    near = bits[bits >= 0].astype(np.uint32).view(np.float32)
+    rng = np.random.default_rng(3)
+    rand = rng.uniform(-2.0, 300.0, 200_000).astype(np.float32)
+    special = np.float32([np.nan, np.inf, -np.inf, 3.4e38, -3.4e38, -0.0, 1e-45, 0.49999997, 0.5, 255.5, 256.0])
+    v = np.concatenate([near, -near[near < 2], rand, special])
+    with np.errstate(invalid="ignore", over="ignore"):
+        assert np.array_equal(reference(v), biased(v))
