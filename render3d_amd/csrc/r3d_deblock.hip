@@ -259,7 +259,10 @@ __global__ __launch_bounds__(256) void k_db_tiles(DbTile a) {
   __shared__ __attribute__((aligned(16))) uint32_t s_t[16 * 16 * DBT_W / 4];  // [lz][ly][lx / 4]
   const DbLim l = a.flim ? *a.flim : a.lim;
   if (!l.on) return;
-  const uint32_t b = blockIdx.x;
+  // XCD bands: a row's x tiles run on one XCD one after another, so the line
+  // holding a tile's first two bytes (also the tile before it's) is an L2 hit;
+  // in blockIdx order it was fetched again, doubling the reads
+  const uint32_t b = xcd_band(blockIdx.x, gridDim.x);
   const uint32_t tm = b % a.tx, tj = (b / a.tx) % a.ty, tk = b / (a.tx * a.ty);
   const uint32_t t = threadIdx.x, c = t & 7u, rs = t >> 3;  // 16-byte column, first row
   const uint32_t xb = DBT_W * tm;  // the tile starts at x = xb - 2

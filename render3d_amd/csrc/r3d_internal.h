@@ -105,6 +105,17 @@ int r3d_fail(r3d_hip_ctx *ctx, int code, const char *fmt, ...) __attribute__((fo
   } while (0)
 
 namespace r3d {
+// Workgroups are dispatched to the 8 XCDs round-robin in blockIdx order, and
+// each XCD has its own L2. For a grid of n workgroups of equal cost, this map
+// gives XCD k the contiguous band of work items [k n/8, (k + 1) n/8) in order
+// (the remainder past 8 (n / 8) keeps its own index), so neighbouring items,
+// which read each other's edge lines, run on the same XCD a few slots apart
+// and find those lines in its L2.
+__device__ __forceinline__ uint32_t xcd_band(uint32_t b, uint32_t n) {
+  const uint32_t per = n / 8u;
+  return b < 8u * per ? (b % 8u) * per + b / 8u : b;
+}
+
 // Copies the FP32 DCT basis (dct3d.c:38-49) into the decoder's constant bank.
 int upload_basis(r3d_hip_ctx *ctx);
 // Uploads the 3ddct scan order and the powf(1 + f, 0.65f) weights (computed by

@@ -55,19 +55,6 @@ __device__ __forceinline__ uint32_t src_voxel(const uint8_t *src, const LevelGeo
   return src[((size_t)z * L.ny + y) * L.nx + x];
 }
 
-// Workgroups are dispatched to the 8 XCDs round-robin in blockIdx order, and
-// each XCD has its own L2. Brick b's apron reads the +x neighbour chunk's
-// every line (its x = 0 column) and half the +y one's; in brick order those
-// neighbours would be read by workgroups on other XCDs. This map gives XCD k
-// the contiguous band of bricks [k n/8, (k + 1) n/8) in order, so a brick's +x
-// and +y neighbours are read on the same XCD a few slots later (L2 hits);
-// the remainder past 8 (n / 8) keeps its own index. Every brick costs the same,
-// so the bands stay balanced.
-__device__ __forceinline__ uint32_t xcd_band(uint32_t b, uint32_t n) {
-  const uint32_t per = n / 8u;
-  return b < 8u * per ? (b % 8u) * per + b / 8u : b;
-}
-
 template <int SRC>
 __global__ __launch_bounds__(256) void k_brickify(const uint8_t *src, LevelGeom L0, LevelGeom L1, int do_l1,
                                                   uint8_t *raw, uint32_t ox, uint32_t oy, uint32_t oz) {
