@@ -84,6 +84,7 @@ extern "C" void r3d_hip_destroy(r3d_hip_ctx *c) {
   if (c->d_partials) hipFree(c->d_partials);
   if (c->d_gtf) hipFree(c->d_gtf);
   if (c->d_ws) hipFree(c->d_ws);
+  if (c->d_sort) hipFree(c->d_sort);
   delete c;
 }
 

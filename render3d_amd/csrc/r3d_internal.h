@@ -60,6 +60,8 @@ struct r3d_hip_ctx {
   uint32_t *d_queue = nullptr;                   // tile queues of r3d_hip_raycast, one per kind
   void *d_ws = nullptr;                          // workspace of r3d_hip_shard_decode_region
   size_t ws_cap = 0;
+  void *d_sort = nullptr;                        // r3d_hip_parse_blobs' lane order and its sort
+  size_t sort_cap = 0;
 };
 
 struct r3d_hip_volume {

@@ -192,3 +192,26 @@ def test_parse_is_independent_of_blob_alignment(ctx):
             assert np.array_equal(got["levels"], GOLD["blob_levels"])
         for k in ref:
             assert np.array_equal(got[k], ref[k]), (shift, k)
+
+
+def test_large_batches_parse_in_length_order(ctx):
+    """Batches of 512 blobs or more are parsed longest blob first (the lanes
+    take a radix-sorted order); the outputs stay indexed by blob. The
+    reference's 40 blobs repeated 16 times, each copy at its own offset."""
+    n = len(GOLD["blob_names"])
+    byts, offs = GOLD["blob_bytes"], GOLD["blob_offsets"].astype(np.int64)
+    rep = 16
+    data = np.tile(byts, rep)
+    big_offs = np.concatenate([offs[:n] + k * byts.size for k in range(rep)])
+    lens = np.tile(np.diff(offs).astype(np.int32), rep)
+    parsed = ctx.parse_blobs(cuda(data), cuda(big_offs), cuda(lens))
+    torch.cuda.synchronize()
+    assert np.all(parsed["status"].cpu().numpy() == 1)
+    assert np.array_equal(levels_of(parsed, 4096), np.tile(GOLD["blob_levels"], (rep, 1)))
+    hdr = parsed["headers"].cpu().numpy()
+    assert np.array_equal(hdr[:, 2], np.tile(GOLD["blob_dcq"], rep)) and np.array_equal(hdr[:, 3], np.arange(n * rep))
+    steps = parsed["step_tables"].cpu().numpy().reshape(n * rep, 46)
+    assert np.array_equal(steps.view(np.uint32), np.tile(GOLD["blob_steps"].view(np.uint32), (rep, 1)))
+    assert np.array_equal(parsed["corr_counts"].cpu().numpy(), np.tile(np.diff(GOLD["blob_corr_off"]), rep))
+    po = parsed["pair_offsets"].cpu().numpy()
+    assert np.array_equal(po, np.arange(n * rep + 1) * 4096)
