@@ -99,6 +99,12 @@ enum {
 #define R3D_VIEW_ORTHO 1u
 #define R3D_VIEW_HALF 32u
 
+/* shading quality, render_types.h:35-38 */
+enum {
+  R3D_QUALITY_FULL = 0, /* six-tap central-difference gradient */
+  R3D_QUALITY_FAST = 1, /* four-tap tetrahedral gradient */
+};
+
 #endif /* R3D_RENDER_TYPES_H */
 
 /* ---- context ------------------------------------------------------------- */
@@ -285,6 +291,16 @@ int r3d_hip_raycast(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_frame
                     uint32_t nviews, const r3d_hip_image *out, const r3d_raycast_opts *opts,
                     r3d_hip_stats *stats, void *stream);
 
+/* r3d_set_quality (src/render/render.h:241-242): the gradient of FULL-mode
+ * views, six central-difference taps (R3D_QUALITY_FULL, the default) or four
+ * tetrahedral taps renormalised by sqrt(3)/2 (R3D_QUALITY_FAST,
+ * src/shaders/raycast.slang:919-955), for later r3d_hip_raycast and
+ * r3d_hip_bricks_raycast calls and for plans created after it (a plan keeps
+ * the quality it was created with). Any other value selects FULL, as
+ * src/vk/vkbackend.c:2075-2077 does. The differentiable path always uses six
+ * taps. */
+void r3d_hip_set_quality(r3d_hip_ctx *ctx, uint32_t quality);
+
 /* Launch shape of the raycast kernel: the workgroup renders tile_w x
  * tile_h-pixel tiles (8x8, 16x4, 16x8 or 16x16); blocks_per_cu > 0 makes
  * the launch persistent, with that many workgroups per CU taking tiles from a
@@ -410,7 +426,16 @@ typedef struct r3d_blob_parse_out {
  * GPU lane per blob. The result feeds r3d_hip_decode_chunks as a sparse batch
  * (R3D_COEF_SPARSE, ntables = nchunks, with pair_counts / corr_counts), which
  * together give dct3d_decode_u8's output. Replaces the per-chunk CPU loop of
- * r3d_shard_decode_region (src/shard/shardio.c:269-306). */
+ * r3d_shard_decode_region (src/shard/shardio.c:269-306).
+ * Batches of 512 blobs or more are parsed longest first: the lane order and
+ * its radix sort live in a scratch buffer of the context (12 bytes per blob
+ * plus the sort's own), allocated the first time a batch needs more than the
+ * context holds. A larger batch later takes a new buffer and keeps the old
+ * one until r3d_hip_destroy, so a HIP graph that captured a parse stays
+ * valid. Growing is refused (R3D_ERR_INVALID, nothing launched) while the
+ * stream is capturing: parse a batch at least as large before capture.
+ * Calls on one context share that buffer, so they must not overlap in time
+ * (one stream, or ordered by the caller). */
 int r3d_hip_parse_blobs(r3d_hip_ctx *ctx, const r3d_blob_batch *in, const r3d_blob_parse_out *out, void *stream);
 
 /* Workload encoder: the entropy-coding back half of encode_float

@@ -58,10 +58,12 @@ def load():
     L.r3do_occupancy.argtypes = [u8p, C.c_uint32, C.c_uint32, C.c_uint32, u8p]
     L.r3do_raycast.restype = C.c_int
     L.r3do_raycast.argtypes = [C.POINTER(VolumeC), f32p, VP, C.c_uint32, C.c_float, f32p, C.c_uint32,
-                               C.c_uint32, VP, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Counters)]
+                               C.c_uint32, VP, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Counters), C.c_int]
     L.r3do_raycast_bricks.restype = C.c_int
     L.r3do_raycast_bricks.argtypes = [C.POINTER(BricksC), f32p, VP, C.c_uint32, C.c_float, f32p, C.c_uint32,
-                                      C.c_uint32, VP, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Counters)]
+                                      C.c_uint32, VP, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Counters),
+                                      C.c_int]
+    L.r3do_plant.argtypes = [C.c_int]
     L.r3do_bricks_build.restype = C.c_uint32
     L.r3do_bricks_build.argtypes = [C.POINTER(VolumeC), VP, u32p, C.c_uint32, u8p, C.c_uint32]
     for fn in ("r3do_log2f", "r3do_exp2f"):
@@ -223,8 +225,30 @@ class OracleVolume:
         self.c = s
 
 
+QUALITY_FULL, QUALITY_FAST = 0, 1  # render_types.h:35-38
+
+# planted mistakes (r3d_oracle.h R3DO_PLANT_*): sensitivity checks of the tests only
+PLANT_NONE, PLANT_SPEC_EXP, PLANT_LIT_LERP, PLANT_TAP_RADIUS, PLANT_MIP_WEIGHT = 0, 1, 2, 3, 4
+PLANT_SKIP_NUDGE, PLANT_SKIP_FACE, PLANT_FAST_RENORM, PLANT_KEY_LIGHT = 5, 6, 7, 8
+
+
+class planted:
+    """Context manager: the oracle with one planted mistake switched on."""
+
+    def __init__(self, k):
+        self.k = k
+
+    def __enter__(self):
+        load().r3do_plant(self.k)
+        return self
+
+    def __exit__(self, *exc):
+        load().r3do_plant(PLANT_NONE)
+        return False
+
+
 def raycast(ovol, tf_f32, views, width, height, alpha_cutoff=0.98, row0=0, row_step=1, nthreads=8,
-            px_steps=False):
+            px_steps=False, quality=QUALITY_FULL):
     """Returns (rgba float32 [h,w,4] (unrendered rows zero), counters dict, px_steps or None)."""
     L = load()
     n = len(views)
@@ -234,7 +258,7 @@ def raycast(ovol, tf_f32, views, width, height, alpha_cutoff=0.98, row0=0, row_s
     cnt = Counters()
     rc = L.r3do_raycast(C.byref(ovol.c), np.ascontiguousarray(tf_f32, np.float32).reshape(-1), C.cast(arr, VP), n,
                         alpha_cutoff, img, width, height, None if steps is None else steps.ctypes.data, row0,
-                        row_step, nthreads, C.byref(cnt))
+                        row_step, nthreads, C.byref(cnt), quality)
     if rc != 0:
         raise ValueError("oracle raycast: unsupported view")
     return (img.reshape(height, width, 4),
@@ -277,7 +301,7 @@ def bricks_build(ovol, abpa, resident=None):
 
 
 def raycast_bricks(ob, tf_f32, views, width, height, alpha_cutoff=0.98, row0=0, row_step=1, nthreads=8,
-                   px_steps=False):
+                   px_steps=False, quality=QUALITY_FULL):
     """r3do_raycast_bricks -> (rgba float32 [h,w,4], counters dict, px_steps or None)."""
     n = len(views)
     arr = (type(views[0]) * n)(*views)
@@ -287,7 +311,7 @@ def raycast_bricks(ob, tf_f32, views, width, height, alpha_cutoff=0.98, row0=0, 
     rc = load().r3do_raycast_bricks(C.byref(ob.c), np.ascontiguousarray(tf_f32, np.float32).reshape(-1),
                                     C.cast(arr, VP), n, alpha_cutoff, img, width, height,
                                     None if steps is None else steps.ctypes.data, row0, row_step, nthreads,
-                                    C.byref(cnt))
+                                    C.byref(cnt), quality)
     if rc != 0:
         raise ValueError("oracle raycast_bricks: unsupported view")
     return (img.reshape(height, width, 4),

@@ -43,6 +43,13 @@ static inline float vdot(v3f a, v3f b) { return a.x * b.x + a.y * b.y + a.z * b.
 static inline float vlen(v3f a) { return sqrtf(vdot(a, a)); }
 static inline v3f vnorm(v3f a) { float l = vlen(a); return mk(a.x / l, a.y / l, a.z / l); }
 
+/* ---- planted mistakes ----------------------------------------------------
+ * The tests' sensitivity checks switch one plausible mistake on (r3do_plant)
+ * and require the closed-form check it targets to fail; 0 (the default) is
+ * the oracle as transcribed. */
+static int g_plant = R3DO_PLANT_NONE;
+void r3do_plant(int k) { g_plant = k; }
+
 /* ---- sampling ---------------------------------------------------------- */
 
 static float tri(const r3do_volume *V, uint32_t l, v3f p) {
@@ -70,7 +77,7 @@ static float vol_sample(const r3do_volume *V, v3f p, float lod) {
   uint32_t d = (uint32_t)lod;
   float w = lod - (float)d;
   float s = tri(V, d, p);
-  if (w > 0.0f && d + 1 < V->nlev) s = om_lerp(s, tri(V, d + 1, p), w);
+  if (w > 0.0f && d + 1 < V->nlev) s = om_lerp(s, tri(V, d + 1, p), g_plant == R3DO_PLANT_MIP_WEIGHT ? 1.0f - w : w);
   return s * K255;
 }
 
@@ -95,6 +102,56 @@ static void tf_sample(const float *tf, float v, float out[4]) {
   for (int c = 0; c < 4; c++) out[c] = om_lerp(tf[i0 * 4 + c], tf[i1 * 4 + c], f);
 }
 
+/* ---- gradient shading (raycast.slang:915-999) -------------------------- */
+
+#define IR3 0.57735026919f    /* 1/sqrt(3): tetrahedron tap radius (raycast.slang:926) */
+#define RENORM 0.86602540378f /* sqrt(3)/2 (raycast.slang:927) */
+
+/* Four tetrahedron-corner taps q0..q3 at p + ee*(+,-,-), (-,-,+), (-,+,-),
+ * (+,+,+) to a gradient (raycast.slang:952-953), left to right in FP32. */
+static v3f tet_grad(float q0, float q1, float q2, float q3) {
+  const float r = g_plant == R3DO_PLANT_FAST_RENORM ? 1.0f : RENORM;
+  return mk((q0 - q1 - q2 + q3) * r, (-q0 - q1 + q2 + q3) * r, (-q0 + q1 - q2 + q3) * r);
+}
+
+/* Cube mode: six central-difference taps at radius e, or (fast) the four
+ * tetrahedral taps at radius e/sqrt(3) (raycast.slang:946-951, 991-994). */
+static v3f grad_cube(const r3do_volume *V, v3f p, float lod, float e, int fast) {
+  if (fast) {
+    const float ee = e * IR3;
+    const float q0 = vol_sample(V, mk(p.x + ee, p.y + -ee, p.z + -ee), lod);
+    const float q1 = vol_sample(V, mk(p.x + -ee, p.y + -ee, p.z + ee), lod);
+    const float q2 = vol_sample(V, mk(p.x + -ee, p.y + ee, p.z + -ee), lod);
+    const float q3 = vol_sample(V, mk(p.x + ee, p.y + ee, p.z + ee), lod);
+    return tet_grad(q0, q1, q2, q3);
+  }
+  return mk(vol_sample(V, mk(p.x + e, p.y, p.z), lod) - vol_sample(V, mk(p.x - e, p.y, p.z), lod),
+            vol_sample(V, mk(p.x, p.y + e, p.z), lod) - vol_sample(V, mk(p.x, p.y - e, p.z), lod),
+            vol_sample(V, mk(p.x, p.y, p.z + e), lod) - vol_sample(V, mk(p.x, p.y, p.z - e), lod));
+}
+
+/* Gradient-modulated Blinn-Phong (raycast.slang:985-997): headlight l = -dir,
+ * key light normalize(0.4, 0.8, -0.4) rounded once, half vector
+ * h = normalize(l - dir); pow(x, 24) by squaring. */
+static void shade_col(v3f g, v3f lvec, v3f hvec, float col[3]) {
+  const v3f key = mk(0x1.a20bd8p-2f, 0x1.a20bd8p-1f, -0x1.a20bd8p-2f);
+  const float gm = vlen(g);
+  if (!(gm > 1e-4f)) return;
+  const v3f n = mk(g.x / gm, g.y / gm, g.z / gm);
+  const float kd = g_plant == R3DO_PLANT_KEY_LIGHT ? 0.0f : 0.3f * fabsf(vdot(n, key));
+  const float diff = 0.7f * fabsf(vdot(n, lvec)) + kd;
+  const float nh = fabsf(vdot(n, hvec));
+  const float nh2 = nh * nh, nh4 = nh2 * nh2, nh8 = nh4 * nh4;
+  const float spec = (g_plant == R3DO_PLANT_SPEC_EXP ? nh8 * nh8 : om_pow24(nh)) * 0.3f;
+  const float lit = 0.25f + 0.75f * diff;
+  const float w = om_sat(gm * 6.0f);
+  const float m = g_plant == R3DO_PLANT_LIT_LERP ? om_lerp(lit, 1.0f, w) : om_lerp(1.0f, lit, w);
+  const float sw = spec * w;
+  col[0] = col[0] * m + sw;
+  col[1] = col[1] * m + sw;
+  col[2] = col[2] * m + sw;
+}
+
 /* ---- ray setup (raycast.slang:469-495, common.slang:45-52) ------------- */
 
 static v3f to_volume_dir(const r3do_frame_params *pc, v3f d) {
@@ -117,7 +174,7 @@ static void pix_ndc(const r3do_frame_params *pc, uint32_t px, uint32_t py, float
 /* ---- one pixel ---------------------------------------------------------- */
 
 static void march(const r3do_volume *V, const float *tf, const r3do_frame_params *pc,
-                  float cutoff, uint32_t px, uint32_t py, float out[4], uint32_t *nsteps,
+                  float cutoff, int fast, uint32_t px, uint32_t py, float out[4], uint32_t *nsteps,
                   uint64_t *nskip, uint64_t *nshade) {
   const v3f fwd = mk(pc->cam_forward[0], pc->cam_forward[1], pc->cam_forward[2]);
   const v3f right = mk(pc->cam_right[0], pc->cam_right[1], pc->cam_right[2]);
@@ -172,7 +229,6 @@ static void march(const r3do_volume *V, const float *tf, const r3do_frame_params
   const float opac_k = pc->density * pc->step_voxels;
   const v3f lvec = vscale(dir, -1.0f);
   const v3f hvec = vnorm(vsub(lvec, dir));
-  const v3f key = mk(0x1.a20bd8p-2f, 0x1.a20bd8p-1f, -0x1.a20bd8p-2f); /* normalize(0.4,0.8,-0.4) */
   const v3f sgn = mk(dir.x > 0.0f ? 1.0f : 0.0f, dir.y > 0.0f ? 1.0f : 0.0f, dir.z > 0.0f ? 1.0f : 0.0f);
 
   while (t < tfar && steps < MAX_STEPS) {
@@ -186,12 +242,13 @@ static void march(const r3do_volume *V, const float *tf, const r3do_frame_params
     if (cut) v = 0.0f;
     /* empty-space skip (raycast.slang:866-869) */
     if (v < occ_gate && occ_sample(V, p) < occ_gate) {
-      v3f nextp = mk((floorf(p.x / bs) + sgn.x) * bs, (floorf(p.y / bs) + sgn.y) * bs,
-                     (floorf(p.z / bs) + sgn.z) * bs);
+      const v3f sg = g_plant == R3DO_PLANT_SKIP_FACE ? mk(0.0f, 0.0f, 0.0f) : sgn;
+      v3f nextp = mk((floorf(p.x / bs) + sg.x) * bs, (floorf(p.y / bs) + sg.y) * bs,
+                     (floorf(p.z / bs) + sg.z) * bs);
       float tx = fabsf(dir.x) > 1e-8f ? (nextp.x - ro.x) / dir.x : 1e30f;
       float ty = fabsf(dir.y) > 1e-8f ? (nextp.y - ro.y) / dir.y : 1e30f;
       float tz = fabsf(dir.z) > 1e-8f ? (nextp.z - ro.z) / dir.z : 1e30f;
-      t = om_max(om_min(om_min(tx, ty), tz), t) + bs * 0.01f;
+      t = om_max(om_min(om_min(tx, ty), tz), t) + (g_plant == R3DO_PLANT_SKIP_NUDGE ? 0.0f : bs * 0.01f);
       (*nskip)++;
       continue;
     }
@@ -206,24 +263,8 @@ static void march(const r3do_volume *V, const float *tf, const r3do_frame_params
         float col[3] = {s[0], s[1], s[2]};
         if (shaded) {
           (*nshade)++;
-          float e = voxel * lod_step;
-          float gx = vol_sample(V, mk(p.x + e, p.y, p.z), lod) - vol_sample(V, mk(p.x - e, p.y, p.z), lod);
-          float gy = vol_sample(V, mk(p.x, p.y + e, p.z), lod) - vol_sample(V, mk(p.x, p.y - e, p.z), lod);
-          float gz = vol_sample(V, mk(p.x, p.y, p.z + e), lod) - vol_sample(V, mk(p.x, p.y, p.z - e), lod);
-          v3f g = mk(gx, gy, gz);
-          float gm = vlen(g);
-          if (gm > 1e-4f) {
-            v3f n = mk(g.x / gm, g.y / gm, g.z / gm);
-            float diff = 0.7f * fabsf(vdot(n, lvec)) + 0.3f * fabsf(vdot(n, key));
-            float spec = om_pow24(fabsf(vdot(n, hvec))) * 0.3f;
-            float lit = 0.25f + 0.75f * diff;
-            float w = om_sat(gm * 6.0f);
-            float m = om_lerp(1.0f, lit, w);
-            float sw = spec * w;
-            col[0] = col[0] * m + sw;
-            col[1] = col[1] * m + sw;
-            col[2] = col[2] * m + sw;
-          }
+          shade_col(grad_cube(V, p, lod, g_plant == R3DO_PLANT_TAP_RADIUS ? voxel : voxel * lod_step, fast), lvec,
+                    hvec, col);
         }
         float wa = (1.0f - acc[3]) * a;
         acc[0] = acc[0] + wa * col[0];
@@ -266,7 +307,7 @@ static int view_supported(const r3do_frame_params *pc) {
 int r3do_raycast(const r3do_volume *vol, const float *tf, const r3do_frame_params *views,
                  uint32_t nviews, float alpha_cutoff, float *img, uint32_t img_w, uint32_t img_h,
                  uint32_t *px_steps, uint32_t row0, uint32_t row_step, int nthreads,
-                 r3do_counters *cnt) {
+                 r3do_counters *cnt, int quality) {
   for (uint32_t v = 0; v < nviews; v++)
     if (!view_supported(&views[v])) return -1;
   if (row_step == 0) row_step = 1;
@@ -287,7 +328,7 @@ int r3do_raycast(const r3do_volume *vol, const float *tf, const r3do_frame_param
         float c[4];
         uint32_t ns = 0;
         uint64_t nk = 0, nh = 0;
-        march(vol, tf, pc, alpha_cutoff, lx, ly, c, &ns, &nk, &nh);
+        march(vol, tf, pc, alpha_cutoff, quality == 1, lx, ly, c, &ns, &nk, &nh);
         tot_s += ns; tot_k += nk; tot_h += nh; tot_r += 1;
         for (uint32_t dy = 0; dy < (half ? 2u : 1u); dy++)
           for (uint32_t dx = 0; dx < (half ? 2u : 1u); dx++) {
@@ -430,8 +471,24 @@ static bctx locate_cached(const r3do_bricks *B, v3f p, float lod, lodcache *lc) 
   return c;
 }
 
+/* Bricks-LOD taps inside the located brick, in brick-local voxels
+ * (raycast.slang:933-937, 964-973). */
+static v3f grad_brick(const r3do_bricks *B, uint32_t slot, v3f b, float ev, int fast) {
+  if (fast) {
+    const float e = ev * IR3;
+    const float q0 = brick_fetch(B, slot, mk(b.x + e, b.y + -e, b.z + -e));
+    const float q1 = brick_fetch(B, slot, mk(b.x + -e, b.y + -e, b.z + e));
+    const float q2 = brick_fetch(B, slot, mk(b.x + -e, b.y + e, b.z + -e));
+    const float q3 = brick_fetch(B, slot, mk(b.x + e, b.y + e, b.z + e));
+    return tet_grad(q0, q1, q2, q3);
+  }
+  return mk(brick_fetch(B, slot, mk(b.x + ev, b.y, b.z)) - brick_fetch(B, slot, mk(b.x + -ev, b.y, b.z)),
+            brick_fetch(B, slot, mk(b.x, b.y + ev, b.z)) - brick_fetch(B, slot, mk(b.x, b.y + -ev, b.z)),
+            brick_fetch(B, slot, mk(b.x, b.y, b.z + ev)) - brick_fetch(B, slot, mk(b.x, b.y, b.z + -ev)));
+}
+
 static void march_bricks(const r3do_bricks *B, const float *tf, const r3do_frame_params *pc, float cutoff,
-                         uint32_t px, uint32_t py, float out[4], uint32_t *nsteps, uint64_t *nskip,
+                         int fast, uint32_t px, uint32_t py, float out[4], uint32_t *nsteps, uint64_t *nskip,
                          uint64_t *nshade) {
   const uint32_t *page = B->page;
   const v3f fwd = mk(pc->cam_forward[0], pc->cam_forward[1], pc->cam_forward[2]);
@@ -504,7 +561,6 @@ static void march_bricks(const r3do_bricks *B, const float *tf, const r3do_frame
   const float opac_k = pc->density * pc->step_voxels;
   const v3f lvec = vscale(dir, -1.0f);
   const v3f hvec = vnorm(vsub(lvec, dir));
-  const v3f key = mk(0x1.a20bd8p-2f, 0x1.a20bd8p-1f, -0x1.a20bd8p-2f);
   const v3f sgn = mk(dir.x > 0.0f ? 1.0f : 0.0f, dir.y > 0.0f ? 1.0f : 0.0f, dir.z > 0.0f ? 1.0f : 0.0f);
   lodcache lc;
   memset(&lc, 0, sizeof lc);
@@ -547,25 +603,8 @@ static void march_bricks(const r3do_bricks *B, const float *tf, const r3do_frame
         float col[3] = {s[0], s[1], s[2]};
         if (shaded) {
           (*nshade)++;
-          const float ev = bc.grad_vox; /* taps reuse the located brick (raycast.slang:964-973) */
-          const v3f b = bc.bl;
-          float gx = brick_fetch(B, bc.slot, mk(b.x + ev, b.y, b.z)) - brick_fetch(B, bc.slot, mk(b.x + -ev, b.y, b.z));
-          float gy = brick_fetch(B, bc.slot, mk(b.x, b.y + ev, b.z)) - brick_fetch(B, bc.slot, mk(b.x, b.y + -ev, b.z));
-          float gz = brick_fetch(B, bc.slot, mk(b.x, b.y, b.z + ev)) - brick_fetch(B, bc.slot, mk(b.x, b.y, b.z + -ev));
-          v3f g = mk(gx, gy, gz);
-          float gm = vlen(g);
-          if (gm > 1e-4f) {
-            v3f n = mk(g.x / gm, g.y / gm, g.z / gm);
-            float diff = 0.7f * fabsf(vdot(n, lvec)) + 0.3f * fabsf(vdot(n, key));
-            float spec = om_pow24(fabsf(vdot(n, hvec))) * 0.3f;
-            float lit = 0.25f + 0.75f * diff;
-            float w = om_sat(gm * 6.0f);
-            float m = om_lerp(1.0f, lit, w);
-            float sw = spec * w;
-            col[0] = col[0] * m + sw;
-            col[1] = col[1] * m + sw;
-            col[2] = col[2] * m + sw;
-          }
+          /* taps reuse the located brick (raycast.slang:964-973) */
+          shade_col(grad_brick(B, bc.slot, bc.bl, bc.grad_vox, fast), lvec, hvec, col);
         }
         float wa = (1.0f - acc[3]) * a;
         acc[0] = acc[0] + wa * col[0];
@@ -596,7 +635,7 @@ static void march_bricks(const r3do_bricks *B, const float *tf, const r3do_frame
 
 int r3do_raycast_bricks(const r3do_bricks *B, const float *tf, const r3do_frame_params *views, uint32_t nviews,
                         float alpha_cutoff, float *img, uint32_t img_w, uint32_t img_h, uint32_t *px_steps,
-                        uint32_t row0, uint32_t row_step, int nthreads, r3do_counters *cnt) {
+                        uint32_t row0, uint32_t row_step, int nthreads, r3do_counters *cnt, int quality) {
   for (uint32_t v = 0; v < nviews; v++) {
     const r3do_frame_params *pc = &views[v];
     if (!(pc->brick_mode & 0x20000u) || (pc->brick_mode & 0xffu) != B->abpa) return -1;
@@ -621,7 +660,7 @@ int r3do_raycast_bricks(const r3do_bricks *B, const float *tf, const r3do_frame_
         float c[4];
         uint32_t ns = 0;
         uint64_t nk = 0, nh = 0;
-        march_bricks(B, tf, pc, alpha_cutoff, lx, ly, c, &ns, &nk, &nh);
+        march_bricks(B, tf, pc, alpha_cutoff, quality == 1, lx, ly, c, &ns, &nk, &nh);
         tot_s += ns; tot_k += nk; tot_h += nh; tot_r += 1;
         for (uint32_t dy = 0; dy < (half ? 2u : 1u); dy++)
           for (uint32_t dx = 0; dx < (half ? 2u : 1u); dx++) {

@@ -129,14 +129,17 @@ typedef struct r3do_counters {
   uint64_t rays;    /* rays marched */
 } r3do_counters;
 
-/* Render views into an RGBA FP32 image of img_w x img_h (each view at its
+/* quality: 0 = the six-tap central-difference gradient, 1 = the four-tap
+ * tetrahedral gradient (R3D_QUALITY_FULL / _FAST, render_types.h:35-38,
+ * selected per renderer by r3d_set_quality, render.h:241-242).
+ * Render views into an RGBA FP32 image of img_w x img_h (each view at its
  * view_org). tf is 256x4 floats. Thread rows ty with ty % row_step == row0 are
  * rendered (row_step 1 = all). px_steps (optional, img_w*img_h) receives the
  * per-pixel loop-iteration count. Returns 0, or -1 for an unsupported view. */
 int r3do_raycast(const r3do_volume *vol, const float *tf, const r3do_frame_params *views,
                  uint32_t nviews, float alpha_cutoff, float *img, uint32_t img_w, uint32_t img_h,
                  uint32_t *px_steps, uint32_t row0, uint32_t row_step, int nthreads,
-                 r3do_counters *cnt);
+                 r3do_counters *cnt, int quality);
 
 /* Bricks-LOD mode (raycast.slang R3D_MODE 3 with the LOD flag): the
  * reference's page table (64-word header, per-level entries) and 128^3 slots,
@@ -149,7 +152,7 @@ typedef struct r3do_bricks {
 } r3do_bricks;
 int r3do_raycast_bricks(const r3do_bricks *B, const float *tf, const r3do_frame_params *views, uint32_t nviews,
                         float alpha_cutoff, float *img, uint32_t img_w, uint32_t img_h, uint32_t *px_steps,
-                        uint32_t row0, uint32_t row_step, int nthreads, r3do_counters *cnt);
+                        uint32_t row0, uint32_t row_step, int nthreads, r3do_counters *cnt, int quality);
 /* Page table + atlas of a pyramid; see oracle_raycast.c. Returns page words or 0. */
 uint32_t r3do_bricks_build(const r3do_volume *V, const uint8_t *resident, uint32_t *page, uint32_t page_cap,
                            uint8_t *atlas, uint32_t nslots);
@@ -165,6 +168,20 @@ int r3do_raycast_diff_window(const r3do_volume *vol, const double *tf, const dou
                              const r3do_frame_params *views, uint32_t nviews, double cutoff, double *img,
                              uint32_t img_w, uint32_t img_h, const double *dimg, double *dtf, double *dgrid,
                              uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1);
+
+/* Planted mistakes (test sensitivity checks only; see oracle_raycast.c). */
+enum {
+  R3DO_PLANT_NONE = 0,
+  R3DO_PLANT_SPEC_EXP = 1,     /* specular |n.h|^16 instead of ^24 */
+  R3DO_PLANT_LIT_LERP = 2,     /* lerp(lit, 1, w) instead of lerp(1, lit, w) */
+  R3DO_PLANT_TAP_RADIUS = 3,   /* cube taps at one voxel, not voxel * lod_step */
+  R3DO_PLANT_MIP_WEIGHT = 4,   /* mip blend weight 1 - w */
+  R3DO_PLANT_SKIP_NUDGE = 5,   /* no 1% nudge past the block face */
+  R3DO_PLANT_SKIP_FACE = 6,    /* jump to the block's lower face, not the next one */
+  R3DO_PLANT_FAST_RENORM = 7,  /* four-tap gradient without the sqrt(3)/2 renorm */
+  R3DO_PLANT_KEY_LIGHT = 8,    /* diffuse without the key light term */
+};
+void r3do_plant(int k);
 
 /* Pinned math, exported so tests can compare against libm and the GPU. */
 float r3do_log2f(float x);

@@ -133,6 +133,7 @@ class BlobParseOut(C.Structure):
 
 
 R3D_DT_U8 = 1
+QUALITY_FULL, QUALITY_FAST = 0, 1  # render_types.h:35-38
 
 
 class ShardSrc(C.Structure):
@@ -191,6 +192,7 @@ SIGNATURES = {
     "r3d_hip_raycast": (C.c_int, [P, P, C.POINTER(FrameParams), C.c_uint32, C.POINTER(Image),
                                   C.POINTER(RaycastOpts), C.POINTER(Stats), P]),
     "r3d_hip_set_raycast_tuning": (C.c_int, [P, C.POINTER(RaycastTuning)]),
+    "r3d_hip_set_quality": (None, [P, C.c_uint32]),
     "r3d_hip_raycast_plan_create": (C.c_int, [P, P, C.POINTER(FrameParams), C.c_uint32, C.POINTER(Image),
                                               C.POINTER(RaycastOpts), C.POINTER(P)]),
     "r3d_hip_raycast_plan_launch": (C.c_int, [P, P]),

@@ -304,6 +304,11 @@ class Context:
             res["stats"] = {"samples": st.samples, "skips": st.skips, "shaded": st.shaded, "rays": st.rays}
         return res
 
+    def set_quality(self, quality):
+        """r3d_set_quality: QUALITY_FULL (six gradient taps) or QUALITY_FAST
+        (four tetrahedral taps) for later renders and plans."""
+        _lib.load().r3d_hip_set_quality(self.h, quality)
+
     def set_raycast_tuning(self, tile_w, tile_h, blocks_per_cu=0):
         """Launch shape of raycast() (r3d_hip_set_raycast_tuning)."""
         self.check(self.lib.r3d_hip_set_raycast_tuning(self.h, C.byref(RaycastTuning(tile_w, tile_h,

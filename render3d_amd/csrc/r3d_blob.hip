@@ -600,17 +600,29 @@ __global__ __launch_bounds__(256) void k_iota(uint32_t *v, uint32_t n) {
 
 using namespace r3d;
 
-// The context's scratch for the parse's lane order (grown on demand, kept).
-static int sort_scratch(r3d_hip_ctx *ctx, size_t bytes, void **out) {
+// The context's scratch for the parse's lane order. A larger batch takes a
+// new buffer (at least twice the old one) and the old one is retired, not
+// freed, until r3d_hip_destroy: a HIP graph that captured an earlier parse
+// keeps its address and stays valid. A capture cannot allocate, so growing
+// is refused while the stream is capturing.
+static int sort_scratch(r3d_hip_ctx *ctx, size_t bytes, hipStream_t st, void **out) {
   if (ctx->sort_cap < bytes) {
-    if (ctx->d_sort) R3D_HIP_CHECK(ctx, hipFree(ctx->d_sort));
-    ctx->d_sort = nullptr;
-    ctx->sort_cap = 0;
-    if (hipMalloc(&ctx->d_sort, bytes) != hipSuccess) {
-      ctx->d_sort = nullptr;
-      return r3d_fail(ctx, R3D_ERR_NOMEM, "parse_blobs: order scratch of %zu bytes", bytes);
-    }
-    ctx->sort_cap = bytes;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    R3D_HIP_CHECK(ctx, hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+      return r3d_fail(ctx, R3D_ERR_INVALID,
+                      "parse_blobs: the lane-order scratch would grow to %zu bytes while the stream is capturing; "
+                      "parse a batch this large once before capturing", bytes);
+    const uint32_t nret = (uint32_t)(sizeof ctx->retired / sizeof ctx->retired[0]);
+    if (ctx->d_sort && ctx->nretired == nret)
+      return r3d_fail(ctx, R3D_ERR_NOMEM, "parse_blobs: %u retired order buffers", nret);
+    const size_t cap = bytes > 2u * ctx->sort_cap ? bytes : 2u * ctx->sort_cap;
+    void *p = nullptr;
+    if (hipMalloc(&p, cap) != hipSuccess)
+      return r3d_fail(ctx, R3D_ERR_NOMEM, "parse_blobs: order scratch of %zu bytes", cap);
+    if (ctx->d_sort) ctx->retired[ctx->nretired++] = ctx->d_sort;
+    ctx->d_sort = p;
+    ctx->sort_cap = cap;
   }
   *out = ctx->d_sort;
   return R3D_OK;
@@ -647,7 +659,7 @@ extern "C" int r3d_hip_parse_blobs(r3d_hip_ctx *ctx, const r3d_blob_batch *in, c
                                                                     (int)n, 0, 16, st));
     const size_t arr = ((size_t)n * 4u + 255u) & ~(size_t)255u;
     void *ws = nullptr;
-    const int rc = sort_scratch(ctx, 3u * arr + temp, &ws);
+    const int rc = sort_scratch(ctx, 3u * arr + temp, st, &ws);
     if (rc) return rc;
     uint8_t *p = (uint8_t *)ws;
     uint32_t *keys = (uint32_t *)p, *idx = (uint32_t *)(p + arr), *ord = (uint32_t *)(p + 2u * arr);

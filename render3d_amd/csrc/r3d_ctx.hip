@@ -45,7 +45,7 @@ extern "C" int r3d_hip_init(r3d_hip_ctx **out, int device) {
   int rc = R3D_OK;
   if (hipMalloc(&c->d_tf, 256 * 4 * sizeof(float)) != hipSuccess ||
       hipMalloc(&c->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess ||
-      hipMalloc(&c->d_queue, 64 * sizeof(uint32_t)) != hipSuccess) {
+      hipMalloc(&c->d_queue, 64 * sizeof(uint32_t)) != hipSuccess || hipMalloc(&c->d_dblim, 16) != hipSuccess) {
     rc = R3D_ERR_NOMEM;
   }
   if (rc == R3D_OK) rc = r3d::upload_basis(c);
@@ -84,7 +84,9 @@ extern "C" void r3d_hip_destroy(r3d_hip_ctx *c) {
   if (c->d_partials) hipFree(c->d_partials);
   if (c->d_gtf) hipFree(c->d_gtf);
   if (c->d_ws) hipFree(c->d_ws);
+  if (c->d_dblim) hipFree(c->d_dblim);
   if (c->d_sort) hipFree(c->d_sort);
+  for (uint32_t i = 0; i < c->nretired; i++) hipFree(c->retired[i]);
   delete c;
 }
 

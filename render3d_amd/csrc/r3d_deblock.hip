@@ -483,11 +483,11 @@ extern "C" int r3d_hip_deblock_u8(r3d_hip_ctx *ctx, uint8_t *vol, const uint64_t
     p.lim.on = 1u;
     p.flim = nullptr;
   } else {
-    void *ws = nullptr;
-    int rc = workspace(ctx, sizeof(DbLim), &ws);
-    if (rc) return rc;
-    p.flim = (const DbLim *)ws;
-    hipLaunchKernelGGL(k_db_range, dim3(1), dim3(1024), 0, st, vol, nz * ny * nx, strength, (DbLim *)ws);
+    // the context's own 16 bytes, not the shard workspace: a region decode in
+    // flight keeps its parsed levels there
+    static_assert(sizeof(DbLim) <= 16u, "DbLim fits the context's limits word");
+    p.flim = (const DbLim *)ctx->d_dblim;
+    hipLaunchKernelGGL(k_db_range, dim3(1), dim3(1024), 0, st, vol, nz * ny * nx, strength, (DbLim *)ctx->d_dblim);
   }
   const uint64_t seams[3] = {nx >= 2u ? (nx - 2u) / DB_N : 0u, ny >= 2u ? (ny - 2u) / DB_N : 0u,
                              nz >= 2u ? (nz - 2u) / DB_N : 0u};

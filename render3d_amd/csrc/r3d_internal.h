@@ -57,11 +57,17 @@ struct r3d_hip_ctx {
   r3d::ViewSlot slots[4];
   int slot_next = 0;
   r3d_raycast_tuning tuning{8u, 8u, 0u, 0.0f};  // r3d_hip_raycast's launch shape
+  uint32_t quality = R3D_QUALITY_FULL;           // gradient of FULL-mode views (r3d_hip_set_quality)
   uint32_t *d_queue = nullptr;                   // tile queues of r3d_hip_raycast, one per kind
   void *d_ws = nullptr;                          // workspace of r3d_hip_shard_decode_region
   size_t ws_cap = 0;
+  void *d_dblim = nullptr;                       // 16 bytes: r3d_hip_deblock_u8's data-range thresholds
   void *d_sort = nullptr;                        // r3d_hip_parse_blobs' lane order and its sort
   size_t sort_cap = 0;
+  // order buffers a larger batch replaced: kept until r3d_hip_destroy, since a
+  // HIP graph that captured an earlier parse still reads them
+  void *retired[48] = {};
+  uint32_t nretired = 0;
 };
 
 struct r3d_hip_volume {

@@ -8,7 +8,15 @@
 namespace r3d {
 
 enum { MODE_FULL = 0, MODE_MIP = 1, MODE_DEPTH = 2, MODE_HEATMAP = 3, MODE_RAYDIR = 4, MODE_FLAT = 5 };
-enum { KIND_FULL = 0, KIND_MIP = 1, KIND_FLAT = 2, KIND_OTHER = 3, NKINDS = 4 };
+// Kernel kinds: FULL-mode views with the six-tap gradient, MIP, FLAT, the
+// remaining modes, and FULL-mode views with the four-tap gradient (the
+// reference's R3D_QUALITY_FAST pipelines, src/shaders/shaders.cmake:24-28).
+enum { KIND_FULL = 0, KIND_MIP = 1, KIND_FLAT = 2, KIND_OTHER = 3, KIND_FAST = 4, NKINDS = 5 };
+template <int KIND>
+constexpr bool SHADED_KIND = KIND == KIND_FULL || KIND == KIND_FAST;
+// The gradient's taps per shaded sample.
+template <int KIND>
+constexpr int KIND_TAPS = KIND == KIND_FAST ? 4 : 6;
 
 constexpr float K255 = 1.0f / 255.0f;
 constexpr uint32_t MAX_STEPS = 4096u;  // raycast.slang:120

@@ -68,7 +68,7 @@ struct RayArgs {
 #define R3D_RAY_BATCH_FLAT 3
 #endif
 template <int KIND>
-constexpr int RAY_BATCH = KIND == KIND_FULL ? R3D_RAY_BATCH_FULL
+constexpr int RAY_BATCH = SHADED_KIND<KIND> ? R3D_RAY_BATCH_FULL
                           : KIND == KIND_MIP ? R3D_RAY_BATCH_MIP
                                              : R3D_RAY_BATCH_FLAT;
 
@@ -79,11 +79,16 @@ struct RayOut {
 
 // Wave-cooperative gradient taps (FULL): the lanes whose sample is shaded
 // publish (p, e, aux) in LDS, every active lane evaluates one tap of one of
-// them per round (6 taps per shaded lane: +x, -x, +y, -y, +z, -z, each the
-// same fetch the lane would make itself), and the results come back
-// through LDS. A wave with k shaded lanes and n active lanes pays
-// ceil(6k / n) samples instead of six. buf holds 64 x 5 + 384 floats.
+// them per round, and the results come back through LDS. A wave with k
+// shaded lanes and n active lanes pays ceil(NTAP k / n) samples instead of
+// NTAP. Each tap is the fetch the owner would make itself:
+//   six taps (R3D_QUALITY_FULL): p +x, -x, +y, -y, +z, -z at radius e;
+//   four taps (R3D_QUALITY_FAST): p + e (+,-,-), (-,-,+), (-,+,-), (+,+,+),
+//   the owner publishing the tetrahedron radius (raycast.slang:925-953).
+// buf holds 64 x 5 + 384 floats.
 constexpr uint32_t COOP_FLOATS = 64u * 5u + 384u;
+constexpr float IR3 = 0.57735026919f;     // 1/sqrt(3) (raycast.slang:926)
+constexpr float RENORM = 0.86602540378f;  // sqrt(3)/2 (raycast.slang:927)
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -97,9 +102,9 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
 
 // fetch(q, aux) evaluates one tap at position q; aux is the owner's fifth
 // published word (the cube path's lod, the bricks path's slot).
-template <class Fetch>
+template <int NTAP, class Fetch>
 __device__ __forceinline__ void coop_taps(const Fetch &fetch, float *buf, bool need, V3 p, float e, float aux,
-                                          float g[6]) {
+                                          float g[NTAP]) {
   const uint64_t act = __ballot(1);
   const uint64_t m = __ballot(need);
   if (m == 0) return;
@@ -114,26 +119,70 @@ __device__ __forceinline__ void coop_taps(const Fetch &fetch, float *buf, bool n
     pub[rn * 5u + 4u] = aux;
   }
   wave_sync();
-  const uint32_t njobs = 6u * k;
+  const uint32_t njobs = (uint32_t)NTAP * k;
   for (uint32_t base = 0; base < njobs; base += nact) {
     const uint32_t job = base + ra;
     if (job < njobs) {
-      const uint32_t o = job / 6u, tap = job - 6u * o;
+      const uint32_t o = job / (uint32_t)NTAP, tap = job - (uint32_t)NTAP * o;
       const float qx = pub[o * 5u + 0u], qy = pub[o * 5u + 1u], qz = pub[o * 5u + 2u];
       const float qe = pub[o * 5u + 3u], ql = pub[o * 5u + 4u];
-      const bool neg = (tap & 1u) != 0u;
-      const float sh = tap < 2u ? qx : tap < 4u ? qy : qz;
-      const float moved = neg ? sh - qe : sh + qe;
-      const V3 q = mk(tap < 2u ? moved : qx, tap - 2u < 2u ? moved : qy, tap >= 4u ? moved : qz);
+      V3 q;
+      if (NTAP == 6) {
+        const bool neg = (tap & 1u) != 0u;
+        const float sh = tap < 2u ? qx : tap < 4u ? qy : qz;
+        const float moved = neg ? sh - qe : sh + qe;
+        q = mk(tap < 2u ? moved : qx, tap - 2u < 2u ? moved : qy, tap >= 4u ? moved : qz);
+      } else {  // x + -e is x - e
+        const bool px = tap == 0u || tap == 3u, py = tap >= 2u, pz = tap == 1u || tap == 3u;
+        q = mk(px ? qx + qe : qx - qe, py ? qy + qe : qy - qe, pz ? qz + qe : qz - qe);
+      }
       res[job] = fetch(q, ql);
     }
   }
   wave_sync();
   if (need) {
 #pragma unroll
-    for (int i = 0; i < 6; i++) g[i] = res[rn * 6u + (uint32_t)i];
+    for (int i = 0; i < NTAP; i++) g[i] = res[rn * (uint32_t)NTAP + (uint32_t)i];
   }
   wave_sync();  // the buffer is reused by the next step
+}
+
+// The tap radius a shaded lane publishes: e for the six taps, e / sqrt(3)
+// for the tetrahedron (raycast.slang:941-945).
+template <int NTAP>
+__device__ __forceinline__ float tap_radius(float e) {
+  return NTAP == 4 ? e * IR3 : e;
+}
+
+// The gradient from the taps of coop_taps: central differences, or the
+// tetrahedral combination renormalised by sqrt(3)/2 (raycast.slang:952-953),
+// left to right in FP32.
+template <int NTAP>
+__device__ __forceinline__ V3 tap_gradient(const float *q) {
+  if (NTAP == 4)
+    return mk((q[0] - q[1] - q[2] + q[3]) * RENORM, (-q[0] - q[1] + q[2] + q[3]) * RENORM,
+              (-q[0] + q[1] - q[2] + q[3]) * RENORM);
+  return mk(q[0] - q[1], q[2] - q[3], q[4] - q[5]);
+}
+
+// Gradient-modulated Blinn-Phong (raycast.slang:985-997): headlight
+// l = -dir, key light normalize(0.4, 0.8, -0.4) rounded once, half vector
+// normalize(l - dir); homogeneous regions (|g| <= 1e-4) stay unshaded.
+__device__ __forceinline__ void blinn_phong(V3 g, V3 lvec, V3 hvec, float &cr, float &cg, float &cb) {
+  const V3 key = mk(0x1.a20bd8p-2f, 0x1.a20bd8p-1f, -0x1.a20bd8p-2f);
+  const float gm = vlen(g);
+  if (gm > 1e-4f) {
+    const V3 n = mk(g.x / gm, g.y / gm, g.z / gm);
+    const float diff = 0.7f * __builtin_fabsf(vdot(n, lvec)) + 0.3f * __builtin_fabsf(vdot(n, key));
+    const float spec = pow24(__builtin_fabsf(vdot(n, hvec))) * 0.3f;
+    const float lit = 0.25f + 0.75f * diff;
+    const float w = fsat(gm * 6.0f);
+    const float m = flerp(1.0f, lit, w);
+    const float sw = spec * w;
+    cr = cr * m + sw;
+    cg = cg * m + sw;
+    cb = cb * m + sw;
+  }
 }
 
 // The per-ray march. KIND fixes the compositing / shading branches at compile
@@ -144,7 +193,7 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
                                         float *coop) {
   RayOut R;
   R.steps = R.skips = R.shaded = 0;
-  const uint32_t mode = KIND == KIND_FULL ? MODE_FULL : KIND == KIND_MIP ? MODE_MIP : KIND == KIND_FLAT ? MODE_FLAT : pc.mode;
+  const uint32_t mode = SHADED_KIND<KIND> ? MODE_FULL : KIND == KIND_MIP ? MODE_MIP : KIND == KIND_FLAT ? MODE_FLAT : pc.mode;
   const V3 fwd = mk(pc.cam_forward[0], pc.cam_forward[1], pc.cam_forward[2]);
   const V3 right = mk(pc.cam_right[0], pc.cam_right[1], pc.cam_right[2]);
   const V3 up = mk(pc.cam_up[0], pc.cam_up[1], pc.cam_up[2]);
@@ -202,7 +251,6 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
   const float opac_k = pc.density * pc.step_voxels;
   const V3 lvec = vscale(dir, -1.0f);
   const V3 hvec = vnorm(vsub(lvec, dir));
-  const V3 key = mk(0x1.a20bd8p-2f, 0x1.a20bd8p-1f, -0x1.a20bd8p-2f);  // normalize(0.4,0.8,-0.4)
   const float sgx = dir.x > 0.0f ? 1.0f : 0.0f, sgy = dir.y > 0.0f ? 1.0f : 0.0f,
               sgz = dir.z > 0.0f ? 1.0f : 0.0f;
   // One sample's position, lod and reads (both levels' footprints and the
@@ -259,31 +307,20 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
     if (v > mip_max) mip_max = v;
     if (mode == MODE_MIP && mip_max >= 0.999f) return 0;
     if (KIND == KIND_OTHER && first_hit < 0.0f && v > 0.05f) first_hit = t;
-    if (KIND == KIND_FULL) {
+    if (SHADED_KIND<KIND>) {
       // every active lane takes part in the taps, so the TF and opacity are
       // evaluated for all of them (a cut sample simply does not use them)
+      constexpr int NTAP = KIND_TAPS<KIND>;
       const float4 s = tf_sample(tf, v);
       const float a = 1.0f - pow_unit(1.0f - s.w, opac_k * lod_step);
       const bool lit_here = !cut && a > 0.003f;
-      float gt[6];
-      coop_taps([&](V3 q, float ql) { return S.sample(q, ql); }, coop, lit_here, p, voxel * lod_step, lod, gt);
+      float gt[NTAP];
+      coop_taps<NTAP>([&](V3 q, float ql) { return S.sample(q, ql); }, coop, lit_here, p,
+                      tap_radius<NTAP>(voxel * lod_step), lod, gt);
       if (lit_here) {
         float cr = s.x, cg = s.y, cb = s.z;
         shaded_n++;
-        const V3 g = mk(gt[0] - gt[1], gt[2] - gt[3], gt[4] - gt[5]);
-        const float gm = vlen(g);
-        if (gm > 1e-4f) {
-          const V3 n = mk(g.x / gm, g.y / gm, g.z / gm);
-          const float diff = 0.7f * __builtin_fabsf(vdot(n, lvec)) + 0.3f * __builtin_fabsf(vdot(n, key));
-          const float spec = pow24(__builtin_fabsf(vdot(n, hvec))) * 0.3f;
-          const float lit = 0.25f + 0.75f * diff;
-          const float w = fsat(gm * 6.0f);
-          const float m = flerp(1.0f, lit, w);
-          const float sw = spec * w;
-          cr = cr * m + sw;
-          cg = cg * m + sw;
-          cb = cb * m + sw;
-        }
+        blinn_phong(tap_gradient<NTAP>(gt), lvec, hvec, cr, cg, cb);
         const float wa = (1.0f - aa) * a;
         ar = ar + wa * cr;
         ag = ag + wa * cg;
@@ -523,7 +560,7 @@ __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_
                                                float *coop) {
   RayOut R;
   R.steps = R.skips = R.shaded = 0;
-  const uint32_t mode = KIND == KIND_FULL ? MODE_FULL : KIND == KIND_MIP ? MODE_MIP : KIND == KIND_FLAT ? MODE_FLAT : pc.mode;
+  const uint32_t mode = SHADED_KIND<KIND> ? MODE_FULL : KIND == KIND_MIP ? MODE_MIP : KIND == KIND_FLAT ? MODE_FLAT : pc.mode;
   const V3 fwd = mk(pc.cam_forward[0], pc.cam_forward[1], pc.cam_forward[2]);
   const V3 right = mk(pc.cam_right[0], pc.cam_right[1], pc.cam_right[2]);
   const V3 up = mk(pc.cam_up[0], pc.cam_up[1], pc.cam_up[2]);
@@ -619,7 +656,6 @@ __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_
   const float opac_k = pc.density * pc.step_voxels;
   const V3 lvec = vscale(dir, -1.0f);
   const V3 hvec = vnorm(vsub(lvec, dir));
-  const V3 key = mk(0x1.a20bd8p-2f, 0x1.a20bd8p-1f, -0x1.a20bd8p-2f);
   const float sgx = dir.x > 0.0f ? 1.0f : 0.0f, sgy = dir.y > 0.0f ? 1.0f : 0.0f,
               sgz = dir.z > 0.0f ? 1.0f : 0.0f;
   LodCache lc;
@@ -673,31 +709,19 @@ __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_
     if (v > mip_max) mip_max = v;
     if (mode == MODE_MIP && mip_max >= 0.999f) return 0;
     if (KIND == KIND_OTHER && first_hit < 0.0f && v > 0.05f) first_hit = t;
-    if (KIND == KIND_FULL) {  // every active lane takes part in the taps (see march)
+    if (SHADED_KIND<KIND>) {  // every active lane takes part in the taps (see march)
+      constexpr int NTAP = KIND_TAPS<KIND>;
       const float4 s = tf_sample(tf, v);
       const float a = 1.0f - pow_unit(1.0f - s.w, opac_k * lod_step);
       const bool lit_here = !cut && a > 0.003f;
-      float gt[6];
-      // taps reuse the located brick (raycast.slang:964-973); x + -ev is x - ev
-      coop_taps([&](V3 q, float slot) { return brick_fetch(B.atlas, __builtin_bit_cast(uint32_t, slot), q); }, coop,
-                lit_here, bc.bl, bc.grad_vox, __builtin_bit_cast(float, bc.slot), gt);
+      float gt[NTAP];
+      // taps reuse the located brick (raycast.slang:933-937, 964-973); x + -ev is x - ev
+      coop_taps<NTAP>([&](V3 q, float slot) { return brick_fetch(B.atlas, __builtin_bit_cast(uint32_t, slot), q); },
+                      coop, lit_here, bc.bl, tap_radius<NTAP>(bc.grad_vox), __builtin_bit_cast(float, bc.slot), gt);
       if (lit_here) {
         float cr = s.x, cg = s.y, cb = s.z;
         shaded_n++;
-        const V3 g = mk(gt[0] - gt[1], gt[2] - gt[3], gt[4] - gt[5]);
-        const float gm = vlen(g);
-        if (gm > 1e-4f) {
-          const V3 n = mk(g.x / gm, g.y / gm, g.z / gm);
-          const float diff = 0.7f * __builtin_fabsf(vdot(n, lvec)) + 0.3f * __builtin_fabsf(vdot(n, key));
-          const float spec = pow24(__builtin_fabsf(vdot(n, hvec))) * 0.3f;
-          const float lit = 0.25f + 0.75f * diff;
-          const float w = fsat(gm * 6.0f);
-          const float m = flerp(1.0f, lit, w);
-          const float sw = spec * w;
-          cr = cr * m + sw;
-          cg = cg * m + sw;
-          cb = cb * m + sw;
-        }
+        blinn_phong(tap_gradient<NTAP>(gt), lvec, hvec, cr, cg, cb);
         const float wa = (1.0f - aa) * a;
         ar = ar + wa * cr;
         ag = ag + wa * cg;
@@ -799,7 +823,7 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
   __shared__ uint32_t s_ts[R3D_MAX_VIEWS_PER_CALL + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint4 s_red[NW];
-  __shared__ float s_coop[KIND == KIND_FULL ? NW * COOP_FLOATS : 1];  // per wave: gradient taps
+  __shared__ float s_coop[SHADED_KIND<KIND> ? NW * COOP_FLOATS : 1];  // per wave: gradient taps
   const uint32_t lane = threadIdx.x;
   for (uint32_t i = lane; i < 256u; i += NT)
     s_tf[i] = make_float4(a.tf[i * 4 + 0], a.tf[i * 4 + 1], a.tf[i * 4 + 2], a.tf[i * 4 + 3]);
@@ -837,7 +861,7 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
     const uint32_t ty = (lt / tiles_x) * TH + (lane / TW);
     if (tx < RW && ty < RH) {
       const uint32_t lx = half ? tx * 2u : tx, ly = half ? ty * 2u : ty;
-      float *coop = s_coop + (KIND == KIND_FULL ? (lane >> 6) * COOP_FLOATS : 0u);
+      float *coop = s_coop + (SHADED_KIND<KIND> ? (lane >> 6) * COOP_FLOATS : 0u);
       const RayOut R = BRICKS ? march_bricks<KIND>(a.bk, s_hdr, s_tf, pc, a.cutoff, lx, ly, coop)
                               : march<KIND>(S, a.vol, s_tf, pc, a.cutoff, lx, ly, coop);
       st += R.steps;
@@ -927,14 +951,14 @@ typedef void (*RayKernel)(RayArgs);
   { k_raycast<K, 8, 8, B>, k_raycast<K, 16, 4, B>, k_raycast<K, 16, 8, B>, k_raycast<K, 16, 16, B> }
 static const RayKernel ray_kernels[2][NKINDS][NSHAPES] = {
     {R3D_RAY_SHAPES(KIND_FULL, false), R3D_RAY_SHAPES(KIND_MIP, false), R3D_RAY_SHAPES(KIND_FLAT, false),
-     R3D_RAY_SHAPES(KIND_OTHER, false)},
+     R3D_RAY_SHAPES(KIND_OTHER, false), R3D_RAY_SHAPES(KIND_FAST, false)},
     {R3D_RAY_SHAPES(KIND_FULL, true), R3D_RAY_SHAPES(KIND_MIP, true), R3D_RAY_SHAPES(KIND_FLAT, true),
-     R3D_RAY_SHAPES(KIND_OTHER, true)}};
+     R3D_RAY_SHAPES(KIND_OTHER, true), R3D_RAY_SHAPES(KIND_FAST, true)}};
 #undef R3D_RAY_SHAPES
 
-static int kind_of(uint32_t mode) {
+static int kind_of(uint32_t mode, uint32_t quality) {
   switch (mode) {
-  case MODE_FULL: return KIND_FULL;
+  case MODE_FULL: return quality == R3D_QUALITY_FAST ? KIND_FAST : KIND_FULL;
   case MODE_MIP: return KIND_MIP;
   case MODE_FLAT: return KIND_FLAT;
   default: return KIND_OTHER;
@@ -953,7 +977,8 @@ struct ViewGroups {
   size_t bytes() const { return view_bytes() + sizeof(uint32_t) * table_words() * NSHAPES; }
 };
 
-static void group_views(const r3d_frame_params *views, uint32_t nviews, uint8_t *buf, ViewGroups *g) {
+static void group_views(const r3d_frame_params *views, uint32_t nviews, uint32_t quality, uint8_t *buf,
+                        ViewGroups *g) {
   g->nviews = nviews;
   r3d_frame_params *pv = (r3d_frame_params *)buf;
   uint32_t *tab = (uint32_t *)(buf + g->view_bytes());
@@ -964,7 +989,7 @@ static void group_views(const r3d_frame_params *views, uint32_t nviews, uint8_t 
     g->tpos[k] = tcur;
     for (int s = 0; s < NSHAPES; s++) tiles[s] = 0;
     for (uint32_t i = 0; i < nviews; i++) {
-      if (kind_of(views[i].mode) != k) continue;
+      if (kind_of(views[i].mode, quality) != k) continue;
       pv[n] = views[i];
       const bool half = (views[i].view_flags & 32u) != 0u;
       const uint32_t RW = half ? (views[i].viewport[0] + 1u) / 2u : views[i].viewport[0];
@@ -1141,7 +1166,7 @@ static int raycast_impl(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_h
   uint8_t buf[sizeof(r3d_frame_params) * R3D_MAX_VIEWS_PER_CALL +
               sizeof(uint32_t) * (R3D_MAX_VIEWS_PER_CALL + NKINDS) * NSHAPES];
   ViewGroups g;
-  group_views(views, nviews, buf, &g);
+  group_views(views, nviews, ctx->quality, buf, &g);
   ViewSlot *slot = nullptr;
   rc = stage(ctx, buf, g.bytes(), s, &slot);
   if (rc != R3D_OK) return rc;
@@ -1200,6 +1225,12 @@ extern "C" int r3d_hip_bricks_raycast(r3d_hip_ctx *ctx, const r3d_hip_bricks *b,
   return raycast_impl(ctx, nullptr, b, views, nviews, out, opts, stats, stream);
 }
 
+// r3d_set_quality (src/render/render.h:241-242; src/vk/vkbackend.c:2075-2077):
+// an unknown quality selects FULL.
+extern "C" void r3d_hip_set_quality(r3d_hip_ctx *ctx, uint32_t quality) {
+  if (ctx) ctx->quality = quality == R3D_QUALITY_FAST ? R3D_QUALITY_FAST : R3D_QUALITY_FULL;
+}
+
 extern "C" int r3d_hip_set_raycast_tuning(r3d_hip_ctx *ctx, const r3d_raycast_tuning *t) {
   if (!ctx) return R3D_ERR_INVALID;
   const int rc = check_tuning(ctx, t);
@@ -1232,7 +1263,7 @@ extern "C" int r3d_hip_raycast_plan_create(r3d_hip_ctx *ctx, const r3d_hip_volum
     delete p;
     return r3d_fail(ctx, R3D_ERR_NOMEM, "raycast plan: host tables");
   }
-  group_views(views, nviews, buf, &p->g);
+  group_views(views, nviews, ctx->quality, buf, &p->g);  // the quality is fixed at create
   // partial slots for the largest launch any tuning can make
   uint32_t slots = 0;
   for (int s = 0; s < NSHAPES; s++) {

@@ -236,9 +236,10 @@ int upload_crc_tables(r3d_hip_ctx *ctx) {
 
 int workspace(r3d_hip_ctx *ctx, size_t bytes, void **out) {
   if (ctx->ws_cap < bytes) {
-    if (ctx->d_ws) R3D_HIP_CHECK(ctx, hipFree(ctx->d_ws));
+    void *old = ctx->d_ws;
     ctx->d_ws = nullptr;
     ctx->ws_cap = 0;
+    if (old) R3D_HIP_CHECK(ctx, hipFree(old));
     if (hipMalloc(&ctx->d_ws, bytes) != hipSuccess) {
       ctx->d_ws = nullptr;
       return r3d_fail(ctx, R3D_ERR_NOMEM, "workspace of %zu bytes", bytes);

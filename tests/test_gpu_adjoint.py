@@ -127,6 +127,39 @@ def test_adam_fit_lowers_the_loss(ctx):
     assert losses[-1] < 0.5 * losses[0], losses[::8]
 
 
+def test_two_renders_of_one_renderer_in_one_loss(ctx):
+    """R(tf1, g) and R(tf2, g) of the same renderer summed into one loss:
+    each autograd call keeps its own checkpoints, so the gradients equal those
+    of two separate renderers (which cannot share a workspace)."""
+    from render3d_amd.autograd import DiffRenderer
+    n, W, H = 32, 24, 24
+    vol = ctx.synth_sheets(n, n, n, 2, 77)
+    v = ctx.volume(n, n, n)
+    v.upload(vol)
+    view = scene.frame_params(scene.orbit_camera(40.0, 10.0, 1.8), W, H, step_voxels=0.5)
+    g0 = torch.ones((8, 8, 8), dtype=torch.float32, device="cuda")
+    tf1_0 = torch.from_numpy(gaussian_tf().astype(np.float32)).cuda()
+    tf2_0 = torch.full((256, 4), 0.35, dtype=torch.float32, device="cuda")
+    w1 = torch.linspace(0.0, 1.0, H * W * 4, device="cuda").reshape(H, W, 4)
+    w2 = torch.linspace(1.0, -0.5, H * W * 4, device="cuda").reshape(H, W, 4)
+
+    def grads(renderers):
+        tf1 = tf1_0.clone().requires_grad_(True)
+        tf2 = tf2_0.clone().requires_grad_(True)
+        g = (g0 * 1.1).requires_grad_(True)
+        loss = (renderers[0](tf1, g) * w1).sum() + (renderers[1](tf2, g) * w2).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        return tf1.grad.cpu().numpy(), tf2.grad.cpu().numpy(), g.grad.cpu().numpy()
+
+    R = DiffRenderer(ctx, v, [view], W, H)
+    shared = grads([R, R])
+    separate = grads([DiffRenderer(ctx, v, [view], W, H), DiffRenderer(ctx, v, [view], W, H)])
+    for a, b in zip(shared, separate):
+        assert np.abs(b).max() > 0
+        assert np.array_equal(a, b) or rel(a, b) <= 1e-5, rel(a, b)  # grid sums use float atomics
+
+
 def test_full_scale_gradient_in_a_window(ctx):
     """1024^3 volume decoded from its chunks (4 levels), a 128^3 grid and one
     1920x1080 view: the GPU adjoint of a linear loss on a 16x16 ray window

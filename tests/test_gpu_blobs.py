@@ -215,3 +215,107 @@ def test_large_batches_parse_in_length_order(ctx):
     assert np.array_equal(parsed["corr_counts"].cpu().numpy(), np.tile(np.diff(GOLD["blob_corr_off"]), rep))
     po = parsed["pair_offsets"].cpu().numpy()
     assert np.array_equal(po, np.arange(n * rep + 1) * 4096)
+
+
+def _tiled(byts, offs, rep, zero_every=0):
+    """The blobs of (byts, offs) repeated rep times, each copy at its own
+    offset; with zero_every > 0 a zero-length entry (a missing chunk, as the
+    shard gather emits) follows every zero_every-th blob."""
+    n = offs.size - 1
+    lens1 = np.diff(offs).astype(np.int64)
+    starts, lens = [], []
+    for k in range(rep):
+        for i in range(n):
+            starts.append(offs[i] + k * byts.size)
+            lens.append(lens1[i])
+            if zero_every and (k * n + i) % zero_every == 0:
+                starts.append(0)
+                lens.append(0)
+    return np.tile(byts, rep), np.array(starts, np.int64), np.array(lens, np.int32)
+
+
+def _parse_np(c, data, starts, lens, cap=4096):
+    p = c.parse_blobs(cuda(data), cuda(starts), cuda(lens), max_pairs=cap)
+    torch.cuda.synchronize()
+    return {"status": p["status"].cpu().numpy(), "headers": p["headers"].cpu().numpy(),
+            "pair_counts": p["pair_counts"].cpu().numpy(), "corr_counts": p["corr_counts"].cpu().numpy(),
+            "levels": levels_of(p, cap)}
+
+
+def test_damaged_blobs_in_length_order_match_the_unsorted_parse(ctx):
+    """The 160 corrupted blobs and 8 low-quality blobs of the reference (with
+    their verdicts) tiled past 512 entries with zero-length entries mixed in:
+    the length-ordered parse gives every blob the status, header, counts and
+    levels of the same blob parsed alone in a batch below 512 (chunk order)."""
+    byts, offs = CORRUPT["bytes"], CORRUPT["offsets"].astype(np.int64)
+    data, starts, lens = _tiled(byts, offs, 4, zero_every=7)
+    assert starts.size >= 512 and np.sum(lens == 0) > 50
+    big = _parse_np(ctx, data, starts, lens)
+    ref = {k: [] for k in big}
+    for a in range(0, starts.size, 300):  # batches below 512: lanes in chunk order
+        part = _parse_np(ctx, data, starts[a:a + 300], lens[a:a + 300])
+        for k in ref:
+            ref[k].append(part[k])
+    for k in ref:
+        assert np.array_equal(big[k], np.concatenate(ref[k])), k
+    assert np.all(big["status"][lens == 0] == 0)
+    assert {-1, 0, 1} <= set(big["status"].tolist())
+
+
+def test_captured_parse_survives_a_larger_eager_parse():
+    """A HIP graph holding a length-ordered parse keeps its lane-order buffer
+    when a later eager parse on the same context needs a larger one (the old
+    buffer is retired until the context is destroyed), and a capture that
+    would have to grow the buffer is refused before anything is recorded."""
+    import render3d_amd as r3d
+
+    c = r3d.Context(0)  # a fresh context: its order buffer starts empty
+    try:
+        byts, offs = GOLD["blob_bytes"], GOLD["blob_offsets"].astype(np.int64)
+        ds, ss, ls = _tiled(byts, offs, 13)  # 520 blobs
+        db, sb, lb = _tiled(byts, offs, 52)  # 2,080 blobs
+        want = _parse_np(c, ds, ss, ls)
+        d_data, d_st, d_ln = cuda(ds), cuda(ss), cuda(ls)
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                gp = c.parse_blobs(d_data, d_st, d_ln, stream=s)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        big = _parse_np(c, db, sb, lb)  # the order buffer grows here
+        assert np.all(big["status"] == 1)
+        assert np.array_equal(big["levels"], np.tile(GOLD["blob_levels"], (52, 1)))
+        for rep in range(3):
+            for k in ("status", "pair_counts", "headers"):
+                gp[k].zero_()
+            gp["pairs"].zero_()
+            junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+            _parse_np(c, db[:byts.size * 20], sb[:800], lb[:800])
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert np.array_equal(gp["status"].cpu().numpy(), want["status"]), rep
+            assert np.array_equal(gp["pair_counts"].cpu().numpy(), want["pair_counts"]), rep
+            assert np.array_equal(gp["headers"].cpu().numpy(), want["headers"]), rep
+            assert np.array_equal(levels_of(gp, 4096), want["levels"]), rep
+            del junk
+        del g
+        # a capture that would grow the buffer: refused, nothing launched
+        dx, sx, lx = _tiled(byts, offs, 128)  # 5,120 blobs: more than the buffer holds
+        d_dx, d_sx, d_lx = cuda(dx), cuda(sx), cuda(lx)
+        g2 = torch.cuda.CUDAGraph()
+        s.wait_stream(torch.cuda.current_stream())
+        marker = torch.zeros(1, device="cuda")
+        with pytest.raises(r3d.R3DError, match="capturing"):
+            with torch.cuda.stream(s):
+                with torch.cuda.graph(g2, stream=s):
+                    marker.add_(1.0)  # the graph is not empty
+                    c.parse_blobs(d_dx, d_sx, d_lx, stream=s)
+        torch.cuda.synchronize()
+        after = _parse_np(c, dx, sx, lx)  # eagerly the same batch grows the buffer and parses
+        assert np.all(after["status"] == 1)
+    finally:
+        torch.cuda.synchronize()
+        c.close()

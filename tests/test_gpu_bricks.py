@@ -102,12 +102,17 @@ CASES = ["full", "flat", "mip", "depth", "heatmap", "raydir", "lodbias", "lodbia
          "fine_step", "ortho", "rotated", "inside", "crop", "slab_z", "slab_x", "slab_y", "oblique", "multi"]
 
 
-def check(ctx, b, ob, lut, vs, W, H):
+def check(ctx, b, ob, lut, vs, W, H, quality=r3d.QUALITY_FULL):
     ctx.set_transfer(lut)
     out = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
-    res = b.raycast(vs, W, H, alpha_cutoff=0.99, out=out, px_steps=True, stats=True, rgba8=True)
+    ctx.set_quality(quality)
+    try:
+        res = b.raycast(vs, W, H, alpha_cutoff=0.99, out=out, px_steps=True, stats=True, rgba8=True)
+    finally:
+        ctx.set_quality(r3d.QUALITY_FULL)
     img = res["rgba"].cpu().numpy()
-    ref, cnt, steps = orc.raycast_bricks(ob, scene.tf_unorm(lut), vs, W, H, alpha_cutoff=0.99, px_steps=True)
+    ref, cnt, steps = orc.raycast_bricks(ob, scene.tf_unorm(lut), vs, W, H, alpha_cutoff=0.99, px_steps=True,
+                                         quality=quality)
     err = float(np.max(np.abs(img - ref)))
     assert err <= TOL, f"max abs {err}"
     assert np.array_equal(res["px_steps"].cpu().numpy().astype(np.uint32), steps)
@@ -118,6 +123,17 @@ def check(ctx, b, ob, lut, vs, W, H):
     want8[ref[..., 3] == 0] = 0
     assert np.array_equal(res["rgba8"].cpu().numpy(), want8)
     return ref, cnt
+
+
+@pytest.mark.parametrize("case", ["full", "lodbias", "half", "rotated", "crop", "oblique", "multi"])
+def test_bricks_fast_quality_matches_oracle(ctx, bscene, case):
+    """R3D_QUALITY_FAST: the four tetrahedral taps inside the located brick
+    (raycast.slang:933-937), against the oracle's restatement."""
+    b, ob, _ = bscene
+    W, H = 120, 88
+    lut, views = _views(W, H, b.brick_mode)
+    _, cnt = check(ctx, b, ob, lut, views[case], W, H, quality=r3d.QUALITY_FAST)
+    assert cnt["shaded"] > 0
 
 
 @pytest.mark.parametrize("case", CASES)

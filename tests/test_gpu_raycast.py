@@ -176,6 +176,77 @@ def test_raycast_matches_oracle(ctx, scene96, case):
     assert np.array_equal(res["rgba8"].cpu().numpy(), want8)
 
 
+@pytest.mark.parametrize("case", ["full", "lodbias", "half", "ortho", "rotated", "inside", "multi"])
+def test_raycast_fast_quality_matches_oracle(ctx, scene96, case):
+    """R3D_QUALITY_FAST (r3d_set_quality): FULL-mode views take the four
+    tetrahedral gradient taps (raycast.slang:919-955); the other modes of a
+    mixed call are unchanged."""
+    v, ov = scene96
+    W, H = 120, 88
+    lut, views = _views(W, H)
+    vs = views[case]
+    ctx.set_transfer(lut)
+    ctx.set_quality(r3d.QUALITY_FAST)
+    try:
+        res = ctx.raycast(v, vs, W, H, alpha_cutoff=0.99, px_steps=True, stats=True)
+    finally:
+        ctx.set_quality(r3d.QUALITY_FULL)
+    img = res["rgba"].cpu().numpy()
+    ref, cnt, steps = orc.raycast(ov, scene.tf_unorm(lut), vs, W, H, alpha_cutoff=0.99, px_steps=True,
+                                  quality=orc.QUALITY_FAST)
+    assert float(np.max(np.abs(img - ref))) <= TOL
+    assert np.array_equal(res["px_steps"].cpu().numpy().astype(np.uint32), steps)
+    st = res["stats"]
+    assert (st["samples"], st["skips"], st["shaded"], st["rays"]) == \
+        (cnt["samples"], cnt["skips"], cnt["shaded"], cnt["rays"])
+    assert cnt["shaded"] > 0
+    full, _, _ = orc.raycast(ov, scene.tf_unorm(lut), vs, W, H, alpha_cutoff=0.99)
+    assert np.max(np.abs(full - ref)) > 1e-3  # the four taps do shade differently
+
+
+@pytest.fixture(scope="module")
+def sparse96(ctx):
+    """Two sheets (radii 20 and 29 about the z axis) with everything below 90
+    cleared: sheets on an empty background, so the dilated 8^3 occupancy is
+    zero beyond the outer sheet (15% of the blocks) and rays take the
+    empty-space jump (raycast.slang:829-869) before and after their hits."""
+    dims = (96, 80, 72)
+    vol = orc.synth_sheets(*dims, 2, 31)
+    vol[vol < 90] = 0
+    v = ctx.volume(*dims)
+    v.upload(vol)
+    return v, orc.OracleVolume(vol, *dims)
+
+
+@pytest.mark.parametrize("case", ["full", "fast", "flat", "mip", "depth", "heatmap", "lodbias", "ortho", "rotated",
+                                  "multi"])
+def test_raycast_with_skips_matches_oracle(ctx, sparse96, case):
+    v, ov = sparse96
+    W, H = 120, 88
+    lut, views = _views(W, H)
+    vs = views["full" if case == "fast" else case]
+    quality = r3d.QUALITY_FAST if case == "fast" else r3d.QUALITY_FULL
+    ctx.set_transfer(lut)
+    ctx.set_quality(quality)
+    try:
+        res = ctx.raycast(v, vs, W, H, alpha_cutoff=0.99, px_steps=True, stats=True)
+    finally:
+        ctx.set_quality(r3d.QUALITY_FULL)
+    img = res["rgba"].cpu().numpy()
+    ref, cnt, steps = orc.raycast(ov, scene.tf_unorm(lut), vs, W, H, alpha_cutoff=0.99, px_steps=True,
+                                  quality=quality)
+    err = float(np.max(np.abs(img - ref)))
+    print(f"{case}: max abs {err:g}, samples {cnt['samples']}, skips {cnt['skips']}, shaded {cnt['shaded']}")
+    assert cnt["skips"] > cnt["rays"] // 4, cnt  # the jump is taken, and not only at the faces
+    assert err <= TOL
+    assert np.array_equal(res["px_steps"].cpu().numpy().astype(np.uint32), steps)
+    st = res["stats"]
+    assert (st["samples"], st["skips"], st["shaded"], st["rays"]) == \
+        (cnt["samples"], cnt["skips"], cnt["shaded"], cnt["rays"])
+    if case in ("full", "fast", "lodbias", "rotated", "multi"):
+        assert cnt["shaded"] > 0
+
+
 def test_raycast_f32_tf_and_empty_volume(ctx):
     n = 48
     v = ctx.volume(n, n, n)
