@@ -188,7 +188,8 @@ def test_raycast_fast_quality_matches_oracle(ctx, scene96, case):
     ctx.set_transfer(lut)
     ctx.set_quality(r3d.QUALITY_FAST)
     try:
-        res = ctx.raycast(v, vs, W, H, alpha_cutoff=0.99, px_steps=True, stats=True)
+        out = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")  # pixels no view covers stay 0
+        res = ctx.raycast(v, vs, W, H, alpha_cutoff=0.99, out=out, px_steps=True, stats=True)
     finally:
         ctx.set_quality(r3d.QUALITY_FULL)
     img = res["rgba"].cpu().numpy()
@@ -229,7 +230,8 @@ def test_raycast_with_skips_matches_oracle(ctx, sparse96, case):
     ctx.set_transfer(lut)
     ctx.set_quality(quality)
     try:
-        res = ctx.raycast(v, vs, W, H, alpha_cutoff=0.99, px_steps=True, stats=True)
+        out = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")  # pixels no view covers stay 0
+        res = ctx.raycast(v, vs, W, H, alpha_cutoff=0.99, out=out, px_steps=True, stats=True)
     finally:
         ctx.set_quality(r3d.QUALITY_FULL)
     img = res["rgba"].cpu().numpy()
