@@ -438,6 +438,19 @@ typedef struct r3d_blob_parse_out {
  * (one stream, or ordered by the caller). */
 int r3d_hip_parse_blobs(r3d_hip_ctx *ctx, const r3d_blob_batch *in, const r3d_blob_parse_out *out, void *stream);
 
+/* No counterpart in the reference: r3d_hip_parse_blobs for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The same parse and outputs, with
+ * the lane order and its radix sort in a fixed device array of 16 MiB per
+ * GPU (held until the process ends) instead of context memory: the call
+ * allocates nothing. At most R3D_PARSE_NOALLOC_MAX_BLOBS blobs per call
+ * (R3D_ERR_RANGE beyond, nothing launched). Every call on one GPU shares that
+ * array, whatever its context: calls that overlap in time on one GPU (other
+ * streams, contexts or threads) race. */
+#define R3D_PARSE_NOALLOC_MAX_BLOBS 262144u
+int r3d_hip_parse_blobs_noalloc(r3d_hip_ctx *ctx, const r3d_blob_batch *in, const r3d_blob_parse_out *out,
+                                void *stream);
+
 /* Workload encoder: the entropy-coding back half of encode_float
  * (dct3d.c:499-537) without corrections, one lane per chunk. Blob c is written
  * at out + c * slot_bytes and lengths[c] is its length (longer than

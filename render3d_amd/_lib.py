@@ -133,6 +133,7 @@ class BlobParseOut(C.Structure):
 
 
 R3D_DT_U8 = 1
+R3D_PARSE_NOALLOC_MAX_BLOBS = 262144
 QUALITY_FULL, QUALITY_FAST = 0, 1  # render_types.h:35-38
 
 
@@ -209,6 +210,7 @@ SIGNATURES = {
                                        C.c_uint64, P]),
     "r3d_hip_encode_chunks": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P]),
     "r3d_hip_parse_blobs": (C.c_int, [P, C.POINTER(BlobBatch), C.POINTER(BlobParseOut), P]),
+    "r3d_hip_parse_blobs_noalloc": (C.c_int, [P, C.POINTER(BlobBatch), C.POINTER(BlobParseOut), P]),
     "r3d_hip_encode_blobs": (C.c_int, [P, C.c_uint32, P, P, P, C.c_uint32, P, C.c_uint64, P, P]),
     "r3d_hip_shard_region_chunks": (C.c_uint64, [C.POINTER(ShardRegion)]),
     "r3d_hip_shard_decode_region": (C.c_int, [P, P, C.c_uint32, C.POINTER(ShardRegion), P, P, P, P]),

@@ -151,9 +151,11 @@ class Context:
         offsets = torch.arange(n, dtype=torch.int64, device=dev) * slot_bytes
         return out, offsets, lengths
 
-    def parse_blobs(self, data, offsets, lengths, *, max_pairs=4096, dtype=_lib.R3D_DT_U8, stream=None):
-        """r3d_hip_parse_blobs: blobs -> the arrays of a sparse decode batch
-        (dict of tensors, plus 'status')."""
+    def parse_blobs(self, data, offsets, lengths, *, max_pairs=4096, dtype=_lib.R3D_DT_U8, stream=None,
+                    noalloc=False):
+        """r3d_hip_parse_blobs (or, with noalloc, r3d_hip_parse_blobs_noalloc):
+        blobs -> the arrays of a sparse decode batch (dict of tensors, plus
+        'status')."""
         n = int(offsets.shape[0])
         dev = data.device
         o = {"headers": torch.empty((n, 4), dtype=torch.int32, device=dev),
@@ -170,7 +172,8 @@ class Context:
         po = BlobParseOut(*[_ptr(o[k]) for k in ("headers", "step_tables", "pair_offsets", "pair_counts", "pairs",
                                                   "corr_offsets", "corr_counts", "corr_pos", "corr_delta",
                                                   "status")])
-        self.check(self.lib.r3d_hip_parse_blobs(self.h, C.byref(bb), C.byref(po), _stream(stream)))
+        fn = self.lib.r3d_hip_parse_blobs_noalloc if noalloc else self.lib.r3d_hip_parse_blobs
+        self.check(fn(self.h, C.byref(bb), C.byref(po), _stream(stream)))
         return o
 
     def parsed_batch(self, parsed, out, **kw):

@@ -21,6 +21,7 @@
 // context, so lanes touching the same context hit consecutive addresses; the
 // ascending-L1 scan table is staged in LDS once per workgroup.
 #include <hipcub/hipcub.hpp>
+#include <mutex>
 
 #include "r3d_internal.h"
 
@@ -628,20 +629,59 @@ static int sort_scratch(r3d_hip_ctx *ctx, size_t bytes, hipStream_t st, void **o
   return R3D_OK;
 }
 
-extern "C" int r3d_hip_parse_blobs(r3d_hip_ctx *ctx, const r3d_blob_batch *in, const r3d_blob_parse_out *out,
-                                   void *stream) {
+// ---- the allocation-free parse ----------------------------------------------
+// r3d_hip_parse_blobs_noalloc keeps the lane order and the radix sort's
+// temporary storage in this file-scope device array instead of context
+// memory: one copy per GPU, resident from module load until the process
+// ends. In host code `scratch` names a host-side stand-in, never the device
+// memory: the device address comes from hipGetSymbolAddress, taken once per
+// device (parse_noalloc_setup, from r3d_hip_init), and only that address is
+// handed to kernels and to the sort. Nothing in it needs to start at zero:
+// the iota and the sort write every word they read.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 16u << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];  // per device: the address of `scratch`
+
+int r3d::parse_noalloc_setup(r3d_hip_ctx *ctx) {
+  if (ctx->device >= NOALLOC_MAX_DEVICES)
+    return r3d_fail(ctx, R3D_ERR_RANGE, "device %d beyond the %d the parse scratch table holds", ctx->device,
+                    NOALLOC_MAX_DEVICES);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[ctx->device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return r3d_fail(ctx, R3D_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[ctx->device] = p;
+  }
+  ctx->d_noalloc = g_scratch_ptr[ctx->device];
+  return R3D_OK;
+}
+
+// r3d_hip_parse_blobs and r3d_hip_parse_blobs_noalloc: the lane order comes
+// from the context's scratch (grown on demand) or from the fixed array.
+static int parse_impl(r3d_hip_ctx *ctx, const r3d_blob_batch *in, const r3d_blob_parse_out *out, void *stream,
+                      bool noalloc) {
+  const char *fn = noalloc ? "parse_blobs_noalloc" : "parse_blobs";
   if (!ctx) return R3D_ERR_INVALID;
-  if (!in || !out) return r3d_fail(ctx, R3D_ERR_INVALID, "parse_blobs: NULL batch or output");
+  if (!in || !out) return r3d_fail(ctx, R3D_ERR_INVALID, "%s: NULL batch or output", fn);
   if (in->nchunks == 0) return R3D_OK;
   if (!in->bytes || !in->offsets || !in->lengths)
-    return r3d_fail(ctx, R3D_ERR_INVALID, "parse_blobs: bytes, offsets and lengths are required");
+    return r3d_fail(ctx, R3D_ERR_INVALID, "%s: bytes, offsets and lengths are required", fn);
   if (in->max_pairs == 0 || in->max_pairs > 4096u)
-    return r3d_fail(ctx, R3D_ERR_INVALID, "parse_blobs: max_pairs %u outside 1..4096", in->max_pairs);
+    return r3d_fail(ctx, R3D_ERR_INVALID, "%s: max_pairs %u outside 1..4096", fn, in->max_pairs);
   if ((uint64_t)in->nchunks * in->max_pairs > 0xFFFFFFFFull)
-    return r3d_fail(ctx, R3D_ERR_RANGE, "parse_blobs: nchunks * max_pairs exceeds 32-bit offsets");
+    return r3d_fail(ctx, R3D_ERR_RANGE, "%s: nchunks * max_pairs exceeds 32-bit offsets", fn);
+  if (noalloc && in->nchunks > R3D_PARSE_NOALLOC_MAX_BLOBS)
+    return r3d_fail(ctx, R3D_ERR_RANGE, "%s: %u blobs, above the %u its fixed scratch holds", fn, in->nchunks,
+                    R3D_PARSE_NOALLOC_MAX_BLOBS);
   if (!out->headers || !out->step_tables || !out->pair_offsets || !out->pair_counts || !out->pairs ||
       !out->corr_offsets || !out->corr_counts || !out->corr_pos || !out->corr_delta)
-    return r3d_fail(ctx, R3D_ERR_INVALID, "parse_blobs: every output array is required");
+    return r3d_fail(ctx, R3D_ERR_INVALID, "%s: every output array is required", fn);
+  if (noalloc && !ctx->d_noalloc) return r3d_fail(ctx, R3D_ERR_DEVICE, "%s: no scratch address", fn);
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   hipStream_t st = (hipStream_t)stream;
   // Lanes take the blobs longest first (a stable radix sort of the lengths):
@@ -658,9 +698,17 @@ extern "C" int r3d_hip_parse_blobs(r3d_hip_ctx *ctx, const r3d_blob_batch *in, c
                                                                     (const uint32_t *)nullptr, (uint32_t *)nullptr,
                                                                     (int)n, 0, 16, st));
     const size_t arr = ((size_t)n * 4u + 255u) & ~(size_t)255u;
+    const size_t need = 3u * arr + temp;
     void *ws = nullptr;
-    const int rc = sort_scratch(ctx, 3u * arr + temp, st, &ws);
-    if (rc) return rc;
+    if (noalloc) {
+      if (need > NOALLOC_SCRATCH_BYTES)
+        return r3d_fail(ctx, R3D_ERR_RANGE, "%s: the lane order of %u blobs needs %zu bytes, above the fixed %zu",
+                        fn, n, need, NOALLOC_SCRATCH_BYTES);
+      ws = ctx->d_noalloc;
+    } else {
+      const int rc = sort_scratch(ctx, need, st, &ws);
+      if (rc) return rc;
+    }
     uint8_t *p = (uint8_t *)ws;
     uint32_t *keys = (uint32_t *)p, *idx = (uint32_t *)(p + arr), *ord = (uint32_t *)(p + 2u * arr);
     hipLaunchKernelGGL(k_iota, dim3((n + 255u) / 256u), dim3(256), 0, st, idx, n);
@@ -675,6 +723,16 @@ extern "C" int r3d_hip_parse_blobs(r3d_hip_ctx *ctx, const r3d_blob_batch *in, c
   hipLaunchKernelGGL(k_parse_blobs, dim3((in->nchunks + BLOB_NT - 1) / BLOB_NT), dim3(BLOB_NT), 0, st, a);
   R3D_HIP_CHECK(ctx, hipGetLastError());
   return R3D_OK;
+}
+
+extern "C" int r3d_hip_parse_blobs(r3d_hip_ctx *ctx, const r3d_blob_batch *in, const r3d_blob_parse_out *out,
+                                   void *stream) {
+  return parse_impl(ctx, in, out, stream, false);
+}
+
+extern "C" int r3d_hip_parse_blobs_noalloc(r3d_hip_ctx *ctx, const r3d_blob_batch *in,
+                                           const r3d_blob_parse_out *out, void *stream) {
+  return parse_impl(ctx, in, out, stream, true);
 }
 
 extern "C" int r3d_hip_encode_blobs(r3d_hip_ctx *ctx, uint32_t nchunks, const int16_t *levels,

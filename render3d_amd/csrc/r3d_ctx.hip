@@ -51,6 +51,7 @@ extern "C" int r3d_hip_init(r3d_hip_ctx **out, int device) {
   if (rc == R3D_OK) rc = r3d::upload_basis(c);
   if (rc == R3D_OK) rc = r3d::upload_blob_tables(c);
   if (rc == R3D_OK) rc = r3d::upload_crc_tables(c);
+  if (rc == R3D_OK) rc = r3d::parse_noalloc_setup(c);
   if (rc == R3D_OK) {
     // default transfer function: the grayscale identity ramp (r3d_tf_preset 0,
     // src/core/transfer.c:28-32), as the reference uploads before any TF is set

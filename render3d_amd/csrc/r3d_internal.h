@@ -68,6 +68,7 @@ struct r3d_hip_ctx {
   // HIP graph that captured an earlier parse still reads them
   void *retired[48] = {};
   uint32_t nretired = 0;
+  void *d_noalloc = nullptr;  // this device's fixed parse scratch (r3d_hip_parse_blobs_noalloc)
 };
 
 struct r3d_hip_volume {
@@ -129,6 +130,9 @@ int upload_basis(r3d_hip_ctx *ctx);
 // Uploads the 3ddct scan order and the powf(1 + f, 0.65f) weights (computed by
 // the host's libm) for the blob parser and encoder.
 int upload_blob_tables(r3d_hip_ctx *ctx);
+// Takes the device address of r3d_hip_parse_blobs_noalloc's fixed scratch
+// on the context's (current) device, once per device.
+int parse_noalloc_setup(r3d_hip_ctx *ctx);
 // Uploads the CRC-32C byte table and the x^(2^k) mod P powers used to fold
 // piecewise CRCs.
 int upload_crc_tables(r3d_hip_ctx *ctx);
