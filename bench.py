@@ -71,6 +71,43 @@ def rows_desc(rs):
     return "every row" if rs == 1 else f"every {rs}th row"
 
 
+def host_cpu():
+    """The host's CPU: model (/proc/cpuinfo), online cores (sysconf) and the
+    threads the baselines use: the share of host cores this GPU is given
+    (OMP_NUM_THREADS, set per GPU on a shared box), else every core."""
+    total = os.sysconf("SC_NPROCESSORS_ONLN")
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    cores = max(1, min(int(share), total)) if share.isdigit() else total
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cores": cores, "cores_total": total, "model": model,
+            "cores_note": "OMP_NUM_THREADS (this GPU's share of the host)" if share.isdigit() else "all online cores"}
+
+
+def median_rate(run, work, runs=5, min_s=1.0):
+    """`runs` timed runs of `run` (each repeated until it has taken min_s),
+    each giving work / seconds; returns (median, all rates)."""
+    import numpy as np
+    rates = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        reps = 0
+        while True:
+            run()
+            reps += 1
+            if time.perf_counter() - t >= min_s:
+                break
+        rates.append(reps * work / (time.perf_counter() - t))
+    return float(np.median(rates)), [float(r) for r in rates]
+
+
 def read_traffic(path, key="traffic_bytes_per_launch"):
     """`<key>: N` (HBM bytes or instructions per launch) from a committed
     rocprof summary, or None."""
@@ -189,6 +226,7 @@ def main():
     dec_bytes = CHUNK_BYTES * nch
     dec_gbps = dec_bytes / (dec_avg / 1000.0) / 1e9
 
+    ray_roof = raycast_roofline(ctx, vol, view, W, H, img, ray_avg, dev)
     variants = decode_variants(ctx, batch, levels, headers, step_tab, flat, stream, dev, max(3, args.steps))
     ctx.decode(batch)  # leave the exact decode in `flat`
 
@@ -211,6 +249,7 @@ def main():
             c0["cpu_baseline"] = cpu_cfg0(c0.pop("_state"), args)
         if blob is not None:
             blob["cpu_baseline"], blob["parity_vs_oracle"] = cpu_blobs(blob["_state"], args)
+            cpu["decode_reference"] = blob["cpu_baseline"].get("reference")
         if shd is not None:
             shd["cpu_baseline"], shd["parity_vs_reference"] = cpu_shard(shd["_state"], args)
         if deb is not None:
@@ -267,11 +306,47 @@ def main():
         "roofline": {"kernel": "k_decode_exact", "bound": "hbm", "achieved": dec_gbps,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": dec_gbps / HBM_PEAK_GBPS,
                      "traffic": read_traffic(args.traffic)},
+        "raycast_roofline": ray_roof,
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
     }
     print(json.dumps(out))
     return 0
+
+
+def raycast_roofline(ctx, vol, view, W, H, img, ray_ms, dev):
+    """The raycast's algorithmic bytes for the timed frame, from one run of
+    the measurement instances (same image, checked bit for bit):
+      gather bytes     = 8 B per trilinear footprint (16 B when the sample
+                         blends two mip levels) x (1 + 6 taps per shaded
+                         sample) + 16 B of FP32 output per pixel;
+      compulsory bytes = the distinct bricks the footprints read x 4,928 B
+                         (16^3 + apron, padded) + the output: the least the
+                         frame can move through HBM.
+    achieved = compulsory bytes / the timed frame's mean ms, against HBM."""
+    import numpy as np
+    import torch
+
+    touched = torch.zeros((vol.brick_count + 31) // 32, dtype=torch.int32, device=dev)
+    mc = torch.zeros(6, dtype=torch.int64, device=dev)
+    img_m = torch.empty_like(img)
+    ctx.raycast(vol, [view], W, H, alpha_cutoff=0.99, out=img_m, counters=mc, measure=True, touched=touched)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(img_m, img))
+    samples, skips, shaded, rays, blended, blended_sh = (int(x) for x in mc.cpu().numpy())
+    uniq = int(np.unpackbits(touched.cpu().numpy().view(np.uint8)).sum())
+    out_bytes = W * H * 16
+    gather = 8 * (samples + blended) + 48 * (shaded + blended_sh) + out_bytes
+    compulsory = uniq * 4928 + out_bytes
+    comp_gbps = compulsory / (ray_ms / 1e3) / 1e9
+    prof = os.path.join(ROOT, "profiles", "raycast_1080p.md")
+    return {"kernel": "k_raycast<FULL,8x8>", "bound": "hbm", "achieved": comp_gbps, "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": comp_gbps / HBM_PEAK_GBPS,
+            "traffic": read_traffic(prof) if os.path.exists(prof) else None,
+            "compulsory_bytes": compulsory, "unique_bricks": uniq, "bricks_in_pyramid": vol.brick_count,
+            "gather_bytes": gather, "gather_GBps": gather / (ray_ms / 1e3) / 1e9,
+            "samples": samples, "blended": blended, "shaded": shaded, "blended_shaded": blended_sh,
+            "measurement_image_equal": same}
 
 
 def fp64_decode(levels, headers, step_tab):
@@ -581,7 +656,7 @@ def cpu_shard(state, args):
     """The reference reader r3d_shard_decode_region (src/shard/shardio.c, built
     from the reference's sources in oracle/_ref) on a 64 x 1024 x 1024 slab
     of the same shard (16,384 chunks), on the host's cores (its own pthread
-    pool, at most 16), repeated for at least half of `cpu_seconds`; its
+    pool, host_cpu()'s threads), repeated for at least half of `cpu_seconds`; its
     output against the GPU's slab. The oracle's restatement stands in when
     the reference build is absent."""
     import ctypes as C
@@ -593,7 +668,7 @@ def cpu_shard(state, args):
 
     shard = state["shard"].cpu().numpy()
     gpu = state["dst"][:64].cpu().numpy()
-    cores = min(16, os.cpu_count() or 1)
+    cores = host_cpu()["cores"]
     ref_path = os.path.join(ROOT, "oracle", "_ref", "libshard_ref.so")
     region = (0, 0, 0, 64, 1024, 1024)
     out = np.zeros(64 * 1024 * 1024, np.uint8)
@@ -785,14 +860,14 @@ def bricks_stage(ctx, args, dev, stream, n, nsheets, seed, lut, cam, W, H):
 def cpu_bricks(state, args):
     """The oracle's bricks-LOD raycaster (r3do_raycast_bricks) on the same
     page table and atlas (read back from HBM) and the same frame, every 4th
-    row, repeated for at least a quarter of `cpu_seconds`, on the host's cores
-    (at most 16); and the GPU frame's rows against it."""
+    row, repeated for at least a quarter of `cpu_seconds`, on host_cpu()'s
+    threads; and the GPU frame's rows against it."""
     import numpy as np
 
     from oracle import oracle as orc
     from render3d_amd import scene
 
-    cores = min(16, os.cpu_count() or 1)
+    cores = host_cpu()["cores"]
     nslots = state["abpa"] ** 3
     atlas = np.zeros(nslots * orc.SLOT_VOX, np.uint8)
     atlas[:state["atlas"].size] = state["atlas"].reshape(-1)
@@ -819,9 +894,12 @@ def cpu_bricks(state, args):
 
 
 def cpu_blobs(state, args):
-    """The oracle's blob decode (parse, qstep table, decode: dct3d_decode_u8
-    per chunk) on a 4096-blob sample of the same blobs, on the host's cores,
-    repeated for at least half of `cpu_seconds`; and its output against the
+    """On a 4,096-blob sample of the same blobs: the oracle's blob decode
+    (parse, qstep table, decode, on the threads host_cpu() gives) and the
+    reference's own dct3d_decode_u8 (built from its source in oracle/_ref,
+    -O2 -ffp-contract=off), timed as tools/3ddct/perf.c:55-60 does (passes over
+    the blobs, one thread; then the same passes over the host threads), each
+    the median of 5 runs of at least 1 s; the oracle's output against the
     GPU's for that sample."""
     import numpy as np
 
@@ -833,18 +911,35 @@ def cpu_blobs(state, args):
     data = state["data"].cpu().numpy()
     sub = np.concatenate([data[o:o + ln] for o, ln in zip(offs, lens)])
     sub_off = np.concatenate([[0], np.cumsum(lens.astype(np.int64))[:-1]])
-    cores = min(16, os.cpu_count() or 1)
-    t = time.perf_counter()
-    reps = 0
-    while True:
-        u8, st = orc.decode_blobs(sub, sub_off, lens, nthreads=cores)
-        reps += 1
-        if time.perf_counter() - t >= args.cpu_seconds / 2:
-            break
-    dt = time.perf_counter() - t
-    cpu = {"value": reps * idx.size / dt, "unit": "blobs/s", "cores": cores, "kind": "port",
-           "voxel_GBps": reps * idx.size * CHUNK_BYTES / dt / 1e9,
-           "sample": f"{reps} x {idx.size} blobs ({int(lens.sum())} bytes), {dt:.1f} s"}
+    host = host_cpu()
+    cores = host["cores"]
+    u8, st = orc.decode_blobs(sub, sub_off, lens, nthreads=cores)
+    rate, runs = median_rate(lambda: orc.decode_blobs(sub, sub_off, lens, nthreads=cores), idx.size, runs=5,
+                             min_s=1.0)
+    cpu = {"value": rate, "unit": "blobs/s", "cores": cores, "cores_total": host["cores_total"],
+           "model": host["model"], "kind": "port", "voxel_GBps": rate * 4096 / 1e9, "runs": runs,
+           "sample": f"{idx.size} blobs ({int(lens.sum())} bytes), median of 5 runs of >= 1 s"}
+    fn = orc.reference_decoder()
+    if fn is not None:
+        def timed(nthreads):
+            def run():
+                dt, ok = orc.time_decoder(fn, sub, sub_off, lens, 1, nthreads)
+                assert ok == idx.size
+                run.t += dt
+            run.t = 0.0
+            return run
+        one, one_runs = median_rate(timed(1), idx.size, runs=5, min_s=1.0)
+        allc, all_runs = median_rate(timed(cores), idx.size, runs=5, min_s=1.0)
+        cpu["reference"] = {"kind": "reference", "function": "dct3d_decode_u8 (tools/3ddct/dct3d.c, with range decoding)",
+                            "build": "-O2 -ffp-contract=off, baseline x86-64 (oracle/Makefile target ref)",
+                            "single_core": {"value": one, "unit": "blobs/s", "voxel_MBps": one * 4096 / 1e6,
+                                            "ns_per_block": 1e9 / one, "runs": one_runs},
+                            "all_cores": {"value": allc, "unit": "blobs/s", "cores": cores,
+                                          "voxel_GBps": allc * 4096 / 1e9, "runs": all_runs},
+                            "sample": f"passes over {idx.size} blobs of the qstep-{BLOB_QSTEP:g} volume, "
+                                      f"median of 5 runs of >= 1 s"}
+    else:
+        cpu["reference"] = None
     parity = {"blobs_checked": int(idx.size), "u8_equal": bool(np.array_equal(u8, state["out"])),
               "status_ok": bool(np.all(st == 1))}
     return cpu, parity
@@ -949,7 +1044,7 @@ def cpu_cfg0(state, args):
     from render3d_amd import scene
 
     vol = state["vol"]
-    cores = min(16, os.cpu_count() or 1)
+    cores = host_cpu()["cores"]
     lv = [((d), vol.read_level(l)) for l, d in enumerate(vol.level_dims)]
     ov = orc.OracleVolume(None, 256, 256, 256, levels=lv, occ=(vol.odim, vol.read_occupancy()))
     tf = scene.tf_unorm(state["lut"])
@@ -964,95 +1059,125 @@ def cpu_cfg0(state, args):
 
 
 def adjoint_ratio(ctx, vol, lut, args, W, H, dev):
-    """Differentiable forward vs forward + adjoint, `adj_views` views at WxH
-    over the same volume with an adj_grid^3 density grid and the FP32 TF."""
+    """The 1024^3 differentiable workload: `adj_views` views at WxH, each
+    rendered into its own region of one image (two columns), an adj_grid^3
+    density grid and the FP32 TF. Timed (mean of 3 after 2 warm-ups):
+      - forward-only: the differentiable model's forward without checkpoints
+        (r3d_hip_raycast_diff with no workspace), the same image;
+      - the forward an adjoint follows (it writes the checkpoints), and the
+        adjoint;
+      - the product's plain forward (r3d_hip_raycast, FULL, same views and
+        FP32 TF, empty-space skipping on, no grid) for context.
+    ratio = (checkpointing forward + adjoint) / forward-only."""
     import numpy as np
     import torch
 
     from render3d_amd import scene
     from render3d_amd.autograd import DiffRenderer
 
-    views = [scene.frame_params(scene.orbit_camera(45.0 + 360.0 * i / args.adj_views, 10.0 * (i % 3 - 1), 2.0),
-                                W, H, step_voxels=1.0) for i in range(args.adj_views)]
-    R = DiffRenderer(ctx, vol, views, W, H * 1, alpha_cutoff=0.99)
+    nv = args.adj_views
+    cols = 2
+    rows = (nv + cols - 1) // cols
+    IW, IH = W * cols, H * rows
+    views = [scene.frame_params(scene.orbit_camera(45.0 + 360.0 * i / nv, 10.0 * (i % 3 - 1), 2.0), W, H,
+                                step_voxels=1.0, view_org=((i % cols) * W, (i // cols) * H)) for i in range(nv)]
+    R = DiffRenderer(ctx, vol, views, IW, IH, alpha_cutoff=0.99)
     tf = torch.from_numpy(scene.tf_unorm(lut)).to(dev)
     g = args.adj_grid
     grid = torch.ones((g, g, g), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
-    # one image per view: views render side by side is not needed; each view
-    # writes the same WxH image (the loss gradient is per view)
-    img = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
-    dimg = torch.full((H, W, 4), 1e-3, dtype=torch.float32, device=dev)
+    img = torch.empty((IH, IW, 4), dtype=torch.float32, device=dev)
+    img_nock = torch.empty_like(img)
+    img_plain = torch.empty_like(img)
+    dimg = torch.full((IH, IW, 4), 1e-3, dtype=torch.float32, device=dev)
     dtf = torch.empty_like(tf)
     dgrid = torch.zeros_like(grid)
     for _ in range(2):
+        R.forward_raw(tf, grid, out=img_nock, checkpoints=False)
         R.forward_raw(tf, grid, out=img)
         R.adjoint_raw(tf, grid, dimg, dtf, dgrid)
+        ctx.raycast(vol, views, IW, IH, alpha_cutoff=0.99, tf=tf, out=img_plain)
     torch.cuda.synchronize()
     free0, total = torch.cuda.mem_get_info()
     reps = 3
-    e = [torch.cuda.Event(enable_timing=True) for _ in range(3 * reps)]
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(5 * reps)]
     for r in range(reps):
-        e[3 * r].record(stream)
+        e[5 * r].record(stream)
+        R.forward_raw(tf, grid, out=img_nock, checkpoints=False)
+        e[5 * r + 1].record(stream)
         R.forward_raw(tf, grid, out=img)
-        e[3 * r + 1].record(stream)
+        e[5 * r + 2].record(stream)
         R.adjoint_raw(tf, grid, dimg, dtf, dgrid)
-        e[3 * r + 2].record(stream)
+        e[5 * r + 3].record(stream)
+        ctx.raycast(vol, views, IW, IH, alpha_cutoff=0.99, tf=tf, out=img_plain)
+        e[5 * r + 4].record(stream)
     torch.cuda.synchronize()
-    fwd = float(np.mean([e[3 * r].elapsed_time(e[3 * r + 1]) for r in range(reps)]))
-    bwd = float(np.mean([e[3 * r + 1].elapsed_time(e[3 * r + 2]) for r in range(reps)]))
-    return {"views": args.adj_views, "width": W, "height": H, "grid": g, "forward_ms": fwd, "adjoint_ms": bwd,
-            "ratio": (fwd + bwd) / fwd, "hbm_used_GB": (total - free0) / 1e9,
+
+    def mean(a):
+        return float(np.mean([e[5 * r + a].elapsed_time(e[5 * r + a + 1]) for r in range(reps)]))
+
+    fwd_only, fwd, bwd, plain = mean(0), mean(1), mean(2), mean(3)
+    same = bool(torch.equal(img, img_nock))
+    return {"views": nv, "width": W, "height": H, "image": [IW, IH], "grid": g,
+            "forward_only_ms": fwd_only, "forward_ms": fwd, "adjoint_ms": bwd, "plain_forward_ms": plain,
+            "ratio": (fwd + bwd) / fwd_only, "ratio_vs_checkpointing_forward": (fwd + bwd) / fwd,
+            "ratio_vs_plain_forward": (fwd + bwd) / plain,
+            "forward_images_equal": same, "hbm_used_GB": (total - free0) / 1e9,
             "workspace_GB": R.workspace.numel() / 1e9,
             "grad_finite": bool(torch.isfinite(dtf).all() and torch.isfinite(dgrid).all())}
 
 
 def cpu_baseline(ctx, vol, view, lut, img, args, levels, headers, step_tab, n, W, H):
     """The oracle's host raycaster on the same frame (same volume, pyramid and
-    occupancy read back from HBM), repeated until at least `cpu_seconds` of
-    work, on the host's cores (at most 16, the GPU's share of the box); plus
-    the oracle decoder on a 4096-chunk sample of the same batch."""
+    occupancy read back from HBM) and the oracle decoder on a 4,096-chunk
+    sample of the same batch, each as the median of 5 runs of at least 2 s
+    (decode 1 s) on the threads host_cpu() gives, and again on one core (the
+    raycast on every 8th row); the GPU frame's rows against the oracle's."""
     import numpy as np
 
     from oracle import oracle as orc
     from render3d_amd import scene
 
-    cores = min(16, os.cpu_count() or 1)
+    host = host_cpu()
+    cores = host["cores"]
     lv = [((d), vol.read_level(l)) for l, d in enumerate(vol.level_dims)]
     occ = (vol.odim, vol.read_occupancy())
     ov = orc.OracleVolume(None, n, n, n, levels=lv, occ=occ)
     rs = args.cpu_rows
     tf = scene.tf_unorm(lut)
-    t = time.perf_counter()
-    reps, samples, rays = 0, 0, 0
-    while True:
-        ref, cnt, _ = orc.raycast(ov, tf, [view], W, H, alpha_cutoff=0.99, row0=0, row_step=rs, nthreads=cores)
-        reps += 1
-        samples += cnt["samples"]
-        rays += cnt["rays"]
-        if time.perf_counter() - t >= args.cpu_seconds:
-            break
-    dt = time.perf_counter() - t
+    ref, cnt, _ = orc.raycast(ov, tf, [view], W, H, alpha_cutoff=0.99, row0=0, row_step=rs, nthreads=cores)
     g = img.cpu().numpy()
     rows = np.arange(0, H, rs)
     err = float(np.max(np.abs(g[rows] - ref[rows])))
+    ray_all, ray_runs = median_rate(
+        lambda: orc.raycast(ov, tf, [view], W, H, alpha_cutoff=0.99, row0=0, row_step=rs, nthreads=cores),
+        cnt["samples"], runs=5, min_s=args.cpu_seconds / 5)
+    _, c1, _ = orc.raycast(ov, tf, [view], W, H, alpha_cutoff=0.99, row0=0, row_step=8, nthreads=1)
+    ray_one, ray_one_runs = median_rate(
+        lambda: orc.raycast(ov, tf, [view], W, H, alpha_cutoff=0.99, row0=0, row_step=8, nthreads=1),
+        c1["samples"], runs=5, min_s=1.0)
     k = min(4096, levels.shape[0])
     idx = np.linspace(0, levels.shape[0] - 1, k).astype(np.int64)
     lvs = levels[idx].cpu().numpy()
     hd = headers[idx].cpu().numpy()
     st = step_tab.cpu().numpy()
-    t2 = time.perf_counter()
-    dreps = 0
-    while True:
-        orc.decode_batch(lvs, hd, st, nthreads=cores)
-        dreps += 1
-        if time.perf_counter() - t2 >= args.cpu_seconds / 2:
-            break
-    dt2 = time.perf_counter() - t2
-    cpu = {"value": samples / dt, "unit": "ray samples/s", "cores": cores, "kind": "port",
-           "sample": f"{reps} x the {W}x{H} frame ({rows_desc(rs)}), {rays} rays, {samples} samples, {dt:.1f} s",
-           "decode_GBps": dreps * k * CHUNK_BYTES / dt2 / 1e9,
-           "decode_sample": f"{dreps} x {k} chunks of the batch, {dt2:.1f} s"}
+    dec_all, dec_runs = median_rate(lambda: orc.decode_batch(lvs, hd, st, nthreads=cores), k * CHUNK_BYTES,
+                                    runs=5, min_s=1.0)
+    lv1, hd1 = lvs[:512], hd[:512]
+    dec_one, _ = median_rate(lambda: orc.decode_batch(lv1, hd1, st, nthreads=1), 512 * CHUNK_BYTES, runs=5,
+                             min_s=0.5)
+    cpu = {"value": ray_all, "unit": "ray samples/s", "cores": cores, "cores_total": host["cores_total"],
+           "model": host["model"], "cores_note": host["cores_note"], "kind": "port",
+           "sample": f"the {W}x{H} frame ({rows_desc(rs)}), {cnt['rays']} rays, {cnt['samples']} samples per run; "
+                     f"median of 5 runs of >= {args.cpu_seconds / 5:.1f} s",
+           "runs": ray_runs,
+           "single_core": {"value": ray_one, "unit": "ray samples/s", "runs": ray_one_runs,
+                           "sample": f"every 8th row of the frame, {c1['samples']} samples per run, "
+                                     f"median of 5 runs of >= 1 s"},
+           "decode_GBps": dec_all / 1e9, "decode_runs_GBps": [r / 1e9 for r in dec_runs],
+           "decode_single_core_GBps": dec_one / 1e9,
+           "decode_sample": f"{k} chunks of the batch (algorithmic bytes {CHUNK_BYTES} per chunk), median of 5 runs "
+                            f"of >= 1 s; single core: 512 chunks"}
     parity = {"raycast_max_abs": err, "raycast_rows_checked": int(rows.size)}
     return cpu, parity
 

@@ -244,6 +244,11 @@ int r3d_hip_volume_build_from_chunks(r3d_hip_ctx *ctx, r3d_hip_volume *vol, cons
  * volume owns, then builds it as r3d_hip_volume_build_from_chunks. */
 int r3d_hip_volume_decode(r3d_hip_ctx *ctx, r3d_hip_volume *vol, const r3d_chunk_batch *batch,
                           const r3d_decode_opts *opts, void *stream);
+/* Bricks over all levels of the pyramid (the bit count of a touched-brick
+ * bitmap, r3d_raycast_opts.touched); each brick is R3D_HIP_BRICK_BYTES in HBM
+ * (16^3 voxels, a one-voxel apron on the +x/+y/+z faces, padded to 16 B). */
+#define R3D_HIP_BRICK_BYTES 4928u
+uint64_t r3d_hip_volume_brick_count(const r3d_hip_volume *vol);
 /* Copies level `level` out as linear [z][y][x] voxels (host or device dst). */
 int r3d_hip_volume_read_level(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, uint32_t level,
                               uint8_t *dst, uint32_t dst_kind, void *stream);
@@ -270,17 +275,32 @@ typedef struct r3d_hip_image {
 typedef struct r3d_hip_stats {
   uint64_t samples; /* loop iterations, each fetching one volume sample */
   uint64_t skips;   /* iterations that took the empty-space jump */
-  uint64_t shaded;  /* samples that ran the 6-tap gradient */
+  uint64_t shaded;  /* samples that ran the gradient taps */
   uint64_t rays;
+  /* with R3D_RAYCAST_MEASURE only (0 otherwise): */
+  uint64_t blended;        /* samples that blended two mip levels (two trilinear footprints) */
+  uint64_t blended_shaded; /* shaded samples among them (their taps blend too) */
 } r3d_hip_stats;
 
 #define R3D_MAX_VIEWS_PER_CALL 256u
+/* flags: measurement mode (r3d_hip_raycast only; no counterpart in the
+ * reference). Slower 8x8 instances render the same image and also count
+ * blended samples: counters then holds six u64 (samples, skips, shaded,
+ * rays, blended, blended_shaded, as r3d_hip_stats) and touched, when given,
+ * gets the bricks the footprints read. Not for timing. */
+#define R3D_RAYCAST_MEASURE 1u
 typedef struct r3d_raycast_opts {
   float alpha_cutoff;     /* early termination: raycast.slang:124 uses 0.98 */
   uint32_t flags;
   const float *tf;        /* optional device FP32 LUT (256x4); NULL = the context's */
-  uint64_t *counters;     /* optional device u64[4]: stats accumulate here */
+  uint64_t *counters;     /* optional device u64[4] (u64[6] with R3D_RAYCAST_MEASURE):
+                             stats accumulate here */
   uint32_t *px_steps;     /* optional device u32[width*height]: per-pixel iterations */
+  /* optional, with R3D_RAYCAST_MEASURE: a device bitmap of
+   * (r3d_hip_volume_brick_count + 31) / 32 words, zeroed by the caller;
+   * every brick a sample's or a gradient tap's footprint reads gets its bit
+   * (brick index: level by level, x fastest within a level). */
+  uint32_t *touched;
 } r3d_raycast_opts;
 
 /* Renders `nviews` cube-mode views (each at its view_org) into `out`: the
@@ -357,7 +377,9 @@ typedef struct r3d_diff_params {
 
 /* Bytes of per-ray checkpoint state (loop state every 64 iterations). */
 size_t r3d_hip_diff_workspace_bytes(const r3d_frame_params *views, uint32_t nviews);
-/* Forward: renders into out->rgba and writes the checkpoints to the workspace. */
+/* Forward: renders into out->rgba and writes the checkpoints to the
+ * workspace. With dp->workspace NULL it renders the same image and keeps no
+ * checkpoints (a forward-only render of the differentiable model). */
 int r3d_hip_raycast_diff(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_frame_params *views,
                          uint32_t nviews, const r3d_hip_image *out, const r3d_diff_params *dp,
                          void *stream);

@@ -21,7 +21,8 @@ VP = C.c_void_p
 
 
 class Counters(C.Structure):
-    _fields_ = [("samples", C.c_uint64), ("skips", C.c_uint64), ("shaded", C.c_uint64), ("rays", C.c_uint64)]
+    _fields_ = [("samples", C.c_uint64), ("skips", C.c_uint64), ("shaded", C.c_uint64), ("rays", C.c_uint64),
+                ("blended", C.c_uint64), ("blended_shaded", C.c_uint64)]
 
 
 class BricksC(C.Structure):
@@ -30,7 +31,7 @@ class BricksC(C.Structure):
 
 class VolumeC(C.Structure):
     _fields_ = [("nlev", C.c_uint32), ("dims", (C.c_uint32 * 3) * 16), ("lev", C.c_void_p * 16),
-                ("odim", C.c_uint32 * 3), ("occ", C.c_void_p)]
+                ("odim", C.c_uint32 * 3), ("occ", C.c_void_p), ("touched", C.c_void_p)]
 
 
 _lib = None
@@ -64,6 +65,9 @@ def load():
                                       C.c_uint32, VP, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Counters),
                                       C.c_int]
     L.r3do_plant.argtypes = [C.c_int]
+    L.r3do_time_decoder.restype = C.c_double
+    L.r3do_time_decoder.argtypes = [VP, u8p, VP, VP, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint64)]
     L.r3do_bricks_build.restype = C.c_uint32
     L.r3do_bricks_build.argtypes = [C.POINTER(VolumeC), VP, u32p, C.c_uint32, u8p, C.c_uint32]
     for fn in ("r3do_log2f", "r3do_exp2f"):
@@ -222,7 +226,24 @@ class OracleVolume:
             s.lev[l] = arr.ctypes.data
         s.odim[0], s.odim[1], s.odim[2] = self.odim
         s.occ = self.occ.ctypes.data
+        s.touched = None
         self.c = s
+        self.touched = None
+
+    def brick_count(self):
+        """16^3 bricks over all levels (the bits of a touched-brick bitmap)."""
+        return sum(-(-x // 16) * -(-y // 16) * -(-z // 16) for (x, y, z), _ in self.levels)
+
+    def track_touched(self, on=True):
+        """Start (or stop) marking the bricks the raycast's footprints read;
+        returns the zeroed bitmap (uint32 words, level by level, x fastest)."""
+        if on:
+            self.touched = np.zeros((self.brick_count() + 31) // 32, np.uint32)
+            self.c.touched = self.touched.ctypes.data
+        else:
+            self.touched = None
+            self.c.touched = None
+        return self.touched
 
 
 QUALITY_FULL, QUALITY_FAST = 0, 1  # render_types.h:35-38
@@ -262,7 +283,8 @@ def raycast(ovol, tf_f32, views, width, height, alpha_cutoff=0.98, row0=0, row_s
     if rc != 0:
         raise ValueError("oracle raycast: unsupported view")
     return (img.reshape(height, width, 4),
-            {"samples": cnt.samples, "skips": cnt.skips, "shaded": cnt.shaded, "rays": cnt.rays},
+            {"samples": cnt.samples, "skips": cnt.skips, "shaded": cnt.shaded, "rays": cnt.rays,
+             "blended": cnt.blended, "blended_shaded": cnt.blended_shaded},
             None if steps is None else steps.reshape(height, width))
 
 
@@ -471,3 +493,32 @@ def raycast_diff(ovol, tf, grid, gdims, views, width, height, alpha_cutoff=0.98,
     if rc != 0:
         raise ValueError("oracle raycast_diff: unsupported view")
     return img.reshape(height, width, 4), (None if dtf is None else dtf.reshape(256, 4)), dgrid
+
+
+REF_DCT3D = os.path.join(HERE, "_ref", "libdct3d_ref.so")
+
+
+def reference_decoder():
+    """dct3d_decode_u8 of the reference codec built in oracle/_ref (the
+    reference's own code, -O2 -ffp-contract=off), or None where it was not
+    built. Test infrastructure: the checker and the timed CPU baseline only."""
+    if not os.path.exists(REF_DCT3D):
+        return None
+    lib = C.CDLL(REF_DCT3D)
+    fn = lib.dct3d_decode_u8
+    fn.restype = C.c_int
+    fn.argtypes = [VP, C.c_size_t, VP]
+    reference_decoder.lib = lib  # keep it loaded
+    return fn
+
+
+def time_decoder(fn, data, offsets, lengths, iters, nthreads):
+    """r3do_time_decoder over blobs (uint8 data, offsets, lengths) ->
+    (seconds, successful decodes)."""
+    data = np.ascontiguousarray(data, np.uint8)
+    offs = np.ascontiguousarray(offsets, np.uint64)
+    lens = np.ascontiguousarray(lengths, np.uint32)
+    ok, sink = C.c_uint64(), C.c_uint64()
+    dt = load().r3do_time_decoder(C.cast(fn, VP), data, offs.ctypes.data, lens.ctypes.data, lens.size, iters, nthreads,
+                                  C.byref(ok), C.byref(sink))
+    return dt, ok.value

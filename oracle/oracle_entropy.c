@@ -11,6 +11,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
+#include <time.h>
 
 #include "r3d_oracle.h"
 
@@ -394,4 +395,30 @@ void r3do_decode_blobs(uint32_t n, const uint8_t *bytes, const uint64_t *offsets
     if (st != 1) memset(o, 0, N3);
     status[i] = st;
   }
+}
+
+/* The timing loop of tools/3ddct/perf.c:55-60 for any decoder of
+ * dct3d_decode_u8's signature (the reference's own, from the build in
+ * oracle/_ref): `iters` passes over the n blobs, each pass split over
+ * nthreads OpenMP threads. Returns the wall seconds; *ok counts successful
+ * decodes and *sink sums the first voxel of each, so no decode is elided. */
+double r3do_time_decoder(int (*decode)(const uint8_t *, size_t, uint8_t *), const uint8_t *bytes,
+                         const uint64_t *offsets, const uint32_t *lengths, uint32_t n, uint32_t iters, int nthreads,
+                         uint64_t *ok, uint64_t *sink) {
+  struct timespec t0, t1;
+  uint64_t good = 0, acc = 0;
+  if (nthreads < 1) nthreads = 1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  for (uint32_t it = 0; it < iters; it++) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads) reduction(+ : good, acc)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+      uint8_t out[4096];
+      good += (uint64_t)(decode(bytes + offsets[i], lengths[i], out) == 1);
+      acc += out[0];
+    }
+  }
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  *ok = good;
+  *sink = acc;
+  return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }

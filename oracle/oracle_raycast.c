@@ -62,6 +62,13 @@ static float tri(const r3do_volume *V, uint32_t l, v3f p) {
   uint32_t y1 = y0 + 1 < ny ? y0 + 1 : ny - 1;
   uint32_t z1 = z0 + 1 < nz ? z0 + 1 : nz - 1;
   float fx = tx - (float)x0, fy = ty - (float)y0, fz = tz - (float)z0;
+  if (V->touched) { /* the footprint's brick: level l's bricks follow those of levels 0..l-1 */
+    uint64_t first = 0;
+    for (uint32_t k = 0; k < l; k++)
+      first += (uint64_t)((V->dims[k][0] + 15) / 16) * ((V->dims[k][1] + 15) / 16) * ((V->dims[k][2] + 15) / 16);
+    const uint64_t gb = first + ((uint64_t)(z0 >> 4) * ((ny + 15) / 16) + (y0 >> 4)) * ((nx + 15) / 16) + (x0 >> 4);
+    __atomic_fetch_or(&V->touched[gb >> 5], 1u << (gb & 31u), __ATOMIC_RELAXED);
+  }
   const uint8_t *L = V->lev[l];
 #define VV(Z, Y, X) ((float)L[((size_t)(Z) * ny + (Y)) * nx + (X)])
   float c00 = om_lerp(VV(z0, y0, x0), VV(z0, y0, x1), fx);
@@ -175,7 +182,7 @@ static void pix_ndc(const r3do_frame_params *pc, uint32_t px, uint32_t py, float
 
 static void march(const r3do_volume *V, const float *tf, const r3do_frame_params *pc,
                   float cutoff, int fast, uint32_t px, uint32_t py, float out[4], uint32_t *nsteps,
-                  uint64_t *nskip, uint64_t *nshade) {
+                  uint64_t *nskip, uint64_t *nshade, uint64_t *nblend, uint64_t *nblend_sh) {
   const v3f fwd = mk(pc->cam_forward[0], pc->cam_forward[1], pc->cam_forward[2]);
   const v3f right = mk(pc->cam_right[0], pc->cam_right[1], pc->cam_right[2]);
   const v3f up = mk(pc->cam_up[0], pc->cam_up[1], pc->cam_up[2]);
@@ -238,6 +245,8 @@ static void march(const r3do_volume *V, const float *tf, const r3do_frame_params
     float lod_step = om_max(om_exp2f(lod), span_step);
     v3f p = vadd(ro, vscale(dir, t));
     float v = vol_sample(V, p, lod);
+    const int blend = lod - (float)(uint32_t)lod > 0.0f && (uint32_t)lod + 1 < V->nlev;
+    *nblend += (uint64_t)blend;
     int cut = v < pc->threshold;
     if (cut) v = 0.0f;
     /* empty-space skip (raycast.slang:866-869) */
@@ -263,6 +272,7 @@ static void march(const r3do_volume *V, const float *tf, const r3do_frame_params
         float col[3] = {s[0], s[1], s[2]};
         if (shaded) {
           (*nshade)++;
+          *nblend_sh += (uint64_t)blend;
           shade_col(grad_cube(V, p, lod, g_plant == R3DO_PLANT_TAP_RADIUS ? voxel : voxel * lod_step, fast), lvec,
                     hvec, col);
         }
@@ -312,14 +322,15 @@ int r3do_raycast(const r3do_volume *vol, const float *tf, const r3do_frame_param
     if (!view_supported(&views[v])) return -1;
   if (row_step == 0) row_step = 1;
   if (nthreads < 1) nthreads = 1;
-  uint64_t tot_s = 0, tot_k = 0, tot_h = 0, tot_r = 0;
+  uint64_t tot_s = 0, tot_k = 0, tot_h = 0, tot_r = 0, tot_b = 0, tot_bh = 0;
   for (uint32_t v = 0; v < nviews; v++) {
     const r3do_frame_params *pc = &views[v];
     const int half = (pc->view_flags & 32u) != 0u;
     const uint32_t ox = pc->view_org & 0xffffu, oy = pc->view_org >> 16;
     const uint32_t W = pc->viewport[0], H = pc->viewport[1];
     const uint32_t TW = half ? (W + 1) / 2 : W, TH = half ? (H + 1) / 2 : H;
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(+ : tot_s, tot_k, tot_h, tot_r)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) \
+    reduction(+ : tot_s, tot_k, tot_h, tot_r, tot_b, tot_bh)
     for (int64_t ty = 0; ty < (int64_t)TH; ty++) {
       if ((uint32_t)ty % row_step != row0 % row_step) continue;
       for (uint32_t tx = 0; tx < TW; tx++) {
@@ -327,9 +338,9 @@ int r3do_raycast(const r3do_volume *vol, const float *tf, const r3do_frame_param
         if (lx >= W || ly >= H) continue;
         float c[4];
         uint32_t ns = 0;
-        uint64_t nk = 0, nh = 0;
-        march(vol, tf, pc, alpha_cutoff, quality == 1, lx, ly, c, &ns, &nk, &nh);
-        tot_s += ns; tot_k += nk; tot_h += nh; tot_r += 1;
+        uint64_t nk = 0, nh = 0, nb = 0, nbh = 0;
+        march(vol, tf, pc, alpha_cutoff, quality == 1, lx, ly, c, &ns, &nk, &nh, &nb, &nbh);
+        tot_s += ns; tot_k += nk; tot_h += nh; tot_r += 1; tot_b += nb; tot_bh += nbh;
         for (uint32_t dy = 0; dy < (half ? 2u : 1u); dy++)
           for (uint32_t dx = 0; dx < (half ? 2u : 1u); dx++) {
             uint32_t qx = lx + dx, qy = ly + dy;
@@ -344,6 +355,7 @@ int r3do_raycast(const r3do_volume *vol, const float *tf, const r3do_frame_param
   }
   if (cnt) {
     cnt->samples = tot_s; cnt->skips = tot_k; cnt->shaded = tot_h; cnt->rays = tot_r;
+    cnt->blended = tot_b; cnt->blended_shaded = tot_bh;
   }
   return 0;
 }
@@ -676,6 +688,7 @@ int r3do_raycast_bricks(const r3do_bricks *B, const float *tf, const r3do_frame_
   }
   if (cnt) {
     cnt->samples = tot_s; cnt->skips = tot_k; cnt->shaded = tot_h; cnt->rays = tot_r;
+    cnt->blended = 0; cnt->blended_shaded = 0; /* one level per sample */
   }
   return 0;
 }

@@ -120,13 +120,19 @@ typedef struct r3do_volume {
   const uint8_t *lev[16];
   uint32_t odim[3];
   const uint8_t *occ;
+  /* optional: one bit per 16^3 brick of every level (level by level, x
+   * fastest), set for each trilinear footprint the raycast reads (the brick
+   * holding its first voxel); NULL = not tracked */
+  uint32_t *touched;
 } r3do_volume;
 
 typedef struct r3do_counters {
   uint64_t samples; /* loop iterations (each fetches one volume sample) */
   uint64_t skips;   /* iterations that took the empty-space jump */
-  uint64_t shaded;  /* samples that ran the 6-tap gradient */
+  uint64_t shaded;  /* samples that ran the gradient taps */
   uint64_t rays;    /* rays marched */
+  uint64_t blended;        /* samples that blended two mip levels */
+  uint64_t blended_shaded; /* shaded samples among them */
 } r3do_counters;
 
 /* quality: 0 = the six-tap central-difference gradient, 1 = the four-tap
@@ -212,6 +218,11 @@ size_t r3do_encode_blob(const r3do_blob_hdr *h, const int16_t lvl[4096], uint32_
  * does per chunk; status[i] as r3do_parse_blob, zero-filled unless 1. */
 void r3do_decode_blobs(uint32_t n, const uint8_t *bytes, const uint64_t *offsets, const uint32_t *lengths,
                        uint32_t dtype, uint8_t *out, int32_t *status, int nthreads);
+/* Wall seconds of `iters` passes of decode (dct3d_decode_u8's signature) over
+ * n blobs on nthreads threads, the loop of tools/3ddct/perf.c:55-60. */
+double r3do_time_decoder(int (*decode)(const uint8_t *, size_t, uint8_t *), const uint8_t *bytes,
+                         const uint64_t *offsets, const uint32_t *lengths, uint32_t n, uint32_t iters, int nthreads,
+                         uint64_t *ok, uint64_t *sink);
 
 /* ---- zarr v3 shard container (src/shard/shardio.c) ------------------- */
 

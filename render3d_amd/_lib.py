@@ -112,12 +112,16 @@ class Image(C.Structure):
 
 
 class Stats(C.Structure):
-    _fields_ = [("samples", C.c_uint64), ("skips", C.c_uint64), ("shaded", C.c_uint64), ("rays", C.c_uint64)]
+    _fields_ = [("samples", C.c_uint64), ("skips", C.c_uint64), ("shaded", C.c_uint64), ("rays", C.c_uint64),
+                ("blended", C.c_uint64), ("blended_shaded", C.c_uint64)]
+
+
+RAYCAST_MEASURE = 1
 
 
 class RaycastOpts(C.Structure):
     _fields_ = [("alpha_cutoff", C.c_float), ("flags", C.c_uint32), ("tf", C.c_void_p),
-                ("counters", C.c_void_p), ("px_steps", C.c_void_p)]
+                ("counters", C.c_void_p), ("px_steps", C.c_void_p), ("touched", C.c_void_p)]
 
 
 class BlobBatch(C.Structure):
@@ -187,6 +191,7 @@ SIGNATURES = {
     "r3d_hip_volume_build_from_chunks": (C.c_int, [P, P, P, P]),
     "r3d_hip_volume_decode": (C.c_int, [P, P, C.POINTER(ChunkBatch), C.POINTER(DecodeOpts), P]),
     "r3d_hip_volume_read_level": (C.c_int, [P, P, C.c_uint32, P, C.c_uint32, P]),
+    "r3d_hip_volume_brick_count": (C.c_uint64, [P]),
     "r3d_hip_volume_read_occupancy": (C.c_int, [P, P, P, C.c_uint32, P]),
     "r3d_hip_set_transfer": (C.c_int, [P, P]),
     "r3d_hip_set_transfer_f32": (C.c_int, [P, P]),

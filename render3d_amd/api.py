@@ -279,13 +279,17 @@ class Context:
         return Volume(self, nx, ny, nz, nlevels)
 
     def raycast(self, vol, views, width, height, *, alpha_cutoff=0.98, rgba=True, rgba8=False, tf=None,
-                counters=None, px_steps=False, stats=False, out=None, stream=None):
-        """Render views into a [height, width, 4] float32 image (and/or RGBA8)."""
+                counters=None, px_steps=False, stats=False, out=None, stream=None, measure=False, touched=None):
+        """Render views into a [height, width, 4] float32 image (and/or RGBA8).
+        measure=True runs the measurement instances (R3D_RAYCAST_MEASURE):
+        stats and counters then include the blended-sample counts, and
+        touched (an int32 device bitmap of vol.brick_count bits, zeroed)
+        receives the bricks the footprints read."""
         return self._raycast(self.lib.r3d_hip_raycast, vol.h, views, width, height, alpha_cutoff, rgba, rgba8, tf,
-                             counters, px_steps, stats, out, stream)
+                             counters, px_steps, stats, out, stream, measure, touched)
 
     def _raycast(self, fn, handle, views, width, height, alpha_cutoff, rgba, rgba8, tf, counters, px_steps, stats,
-                 out, stream):
+                 out, stream, measure=False, touched=None):
         if isinstance(views, FrameParams):
             views = [views]
         arr = (FrameParams * len(views))(*views)
@@ -296,15 +300,18 @@ class Context:
         steps = torch.zeros((height, width), dtype=torch.int32, device=dev) if px_steps else None
         im = Image(img.data_ptr() if img is not None else None, img8.data_ptr() if img8 is not None else None,
                    width, height)
-        opts = RaycastOpts(alpha_cutoff, 0, None if tf is None else tf.data_ptr(),
+        opts = RaycastOpts(alpha_cutoff, _lib.RAYCAST_MEASURE if measure else 0, None if tf is None else tf.data_ptr(),
                            None if counters is None else counters.data_ptr(),
-                           None if steps is None else steps.data_ptr())
+                           None if steps is None else steps.data_ptr(),
+                           None if touched is None else touched.data_ptr())
         st = Stats()
         self.check(fn(self.h, handle, arr, len(views), C.byref(im), C.byref(opts), C.byref(st) if stats else None,
                       _stream(stream)))
         res = {"rgba": img, "rgba8": img8, "px_steps": steps}
         if stats:
             res["stats"] = {"samples": st.samples, "skips": st.skips, "shaded": st.shaded, "rays": st.rays}
+            if measure:
+                res["stats"].update(blended=st.blended, blended_shaded=st.blended_shaded)
         return res
 
     def set_quality(self, quality):
@@ -444,6 +451,11 @@ class Volume:
         opts = DecodeOpts(variant, 0)
         self.ctx.check(self.ctx.lib.r3d_hip_volume_decode(self.ctx.h, self.h, C.byref(batch), C.byref(opts),
                                                           _stream(stream)))
+
+    @property
+    def brick_count(self):
+        """16^3 bricks over all levels (r3d_hip_volume_brick_count)."""
+        return int(self.ctx.lib.r3d_hip_volume_brick_count(self.h))
 
     def read_level(self, level, stream=None):
         nx, ny, nz = self.level_dims[level]

@@ -36,17 +36,19 @@ class DiffRenderer:
     def new_workspace(self):
         return torch.empty(self.workspace_bytes, dtype=torch.uint8, device=f"cuda:{self.ctx.device}")
 
-    def params(self, tf, grid, workspace=None):
+    def params(self, tf, grid, workspace=None, checkpoints=True):
         ws = self.workspace if workspace is None else workspace
         gz, gy, gx = grid.shape
         return DiffParams(tf.data_ptr(), grid.data_ptr(), (C.c_uint32 * 3)(gx, gy, gz), self.cutoff,
-                          ws.data_ptr(), ws.numel())
+                          ws.data_ptr() if checkpoints else None, ws.numel() if checkpoints else 0)
 
-    def forward_raw(self, tf, grid, out=None, workspace=None):
+    def forward_raw(self, tf, grid, out=None, workspace=None, checkpoints=True):
+        """r3d_hip_raycast_diff; checkpoints=False renders without keeping any
+        (no adjoint can follow)."""
         img = out if out is not None else torch.empty((self.height, self.width, 4), dtype=torch.float32,
                                                       device=tf.device)
         im = Image(img.data_ptr(), None, self.width, self.height)
-        dp = self.params(tf, grid, workspace)
+        dp = self.params(tf, grid, workspace, checkpoints)
         self.ctx.check(self.ctx.lib.r3d_hip_raycast_diff(self.ctx.h, self.vol.h, self.views, self.nviews,
                                                          C.byref(im), C.byref(dp), _stream(None)))
         return img

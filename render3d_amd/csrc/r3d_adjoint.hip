@@ -234,7 +234,9 @@ __device__ __forceinline__ bool tile_pixel(const DiffArgs &A, uint32_t tile, uin
   return true;
 }
 
-template <bool SHADED>
+// REC: write the checkpoints (a forward that an adjoint will follow); without
+// them the kernel renders the same image and keeps nothing.
+template <bool SHADED, bool REC>
 __global__ __launch_bounds__(64) void k_diff_forward(DiffArgs A) {
   __shared__ float4 s_tf[256];
   __shared__ LevelDev s_lev[R3D_MAX_LEVELS];
@@ -248,13 +250,13 @@ __global__ __launch_bounds__(64) void k_diff_forward(DiffArgs A) {
   const r3d_frame_params &pc = A.views[v];
   const Sampler S{A.vol.base, s_lev, A.vol.nlev};
   const Ray R = setup_ray(A.vol, pc, px, py);
-  RayRecord &rec = A.rec[ray];
+  RayRecord *rec = REC ? A.rec + ray : nullptr;
   float ar = 0.0f, ag = 0.0f, ab = 0.0f, aa = 0.0f;
   uint32_t steps = 0;
   if (R.hit) {
     float t = R.t0;
     while (t < R.tfar && steps < MAX_STEPS) {
-      if ((steps & (CKPT - 1u)) == 0u) rec.ck[steps / CKPT] = make_float2(t, aa);
+      if (REC && (steps & (CKPT - 1u)) == 0u) rec->ck[steps / CKPT] = make_float2(t, aa);
       steps++;
       float lod_step;
       const SampleEval E = eval_sample<SHADED>(S, A, s_tf, R, pc, t, &lod_step);
@@ -269,7 +271,7 @@ __global__ __launch_bounds__(64) void k_diff_forward(DiffArgs A) {
       t += R.dt * lod_step;
     }
   }
-  rec.nsteps = steps;
+  if (REC) rec->nsteps = steps;
   const float ta = 1.0f - aa;
   const float4 c = R.hit ? make_float4(ar + ta * 0.02f, ag + ta * 0.02f, ab + ta * 0.03f, aa)
                          : make_float4(0.02f, 0.02f, 0.03f, 0.0f);
@@ -463,13 +465,15 @@ struct Plan {
   bool shaded = false;
 };
 
+// need_ws: the adjoint reads the checkpoints; a forward may go without them.
 int validate_diff(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_frame_params *views, uint32_t nviews,
-                  const r3d_diff_params *dp, Plan *plan) {
+                  const r3d_diff_params *dp, Plan *plan, bool need_ws) {
   if (!vol || vol->ctx != ctx) return r3d_fail(ctx, R3D_ERR_INVALID, "diff: bad volume");
   if (!views || nviews == 0 || nviews > R3D_MAX_VIEWS_PER_CALL)
     return r3d_fail(ctx, R3D_ERR_INVALID, "diff: nviews %u outside 1..%u", nviews, R3D_MAX_VIEWS_PER_CALL);
-  if (!dp || !dp->tf || !dp->grid || !dp->workspace)
-    return r3d_fail(ctx, R3D_ERR_INVALID, "diff: tf, grid and workspace are required");
+  if (!dp || !dp->tf || !dp->grid || (need_ws && !dp->workspace))
+    return r3d_fail(ctx, R3D_ERR_INVALID, need_ws ? "adjoint: tf, grid and workspace are required"
+                                                  : "diff: tf and grid are required");
   if (dp->grid_dims[0] == 0 || dp->grid_dims[1] == 0 || dp->grid_dims[2] == 0 || dp->grid_dims[0] > 4096 ||
       dp->grid_dims[1] > 4096 || dp->grid_dims[2] > 4096 ||
       (uint64_t)dp->grid_dims[0] * dp->grid_dims[1] * dp->grid_dims[2] > (1ull << 31))
@@ -494,7 +498,7 @@ int validate_diff(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_frame_p
     rays += TW * TH;
   }
   if (tiles > 0xffffffffull || rays > 0xffffffffull) return r3d_fail(ctx, R3D_ERR_RANGE, "diff: too many rays");
-  if (dp->workspace_bytes < rays * sizeof(RayRecord))
+  if (dp->workspace && dp->workspace_bytes < rays * sizeof(RayRecord))
     return r3d_fail(ctx, R3D_ERR_INVALID, "diff: workspace %llu bytes < %llu needed",
                     (unsigned long long)dp->workspace_bytes, (unsigned long long)(rays * sizeof(RayRecord)));
   plan->nviews = nviews;
@@ -546,7 +550,7 @@ extern "C" int r3d_hip_raycast_diff(r3d_hip_ctx *ctx, const r3d_hip_volume *vol,
                                     void *stream) {
   if (!ctx) return R3D_ERR_INVALID;
   Plan plan;
-  int rc = validate_diff(ctx, vol, views, nviews, dp, &plan);
+  int rc = validate_diff(ctx, vol, views, nviews, dp, &plan, false);
   if (rc != R3D_OK) return rc;
   if (!out || !out->rgba || out->width == 0 || out->height == 0)
     return r3d_fail(ctx, R3D_ERR_INVALID, "diff: FP32 output image required");
@@ -570,10 +574,10 @@ extern "C" int r3d_hip_raycast_diff(r3d_hip_ctx *ctx, const r3d_hip_volume *vol,
   A.img_w = out->width;
   A.img_h = out->height;
   A.rec = (RayRecord *)dp->workspace;
-  if (plan.shaded)
-    hipLaunchKernelGGL(k_diff_forward<true>, dim3(plan.ntiles), dim3(64), 0, s, A);
-  else
-    hipLaunchKernelGGL(k_diff_forward<false>, dim3(plan.ntiles), dim3(64), 0, s, A);
+  typedef void (*FwdKernel)(DiffArgs);
+  static const FwdKernel fwd[2][2] = {{k_diff_forward<false, false>, k_diff_forward<false, true>},
+                                      {k_diff_forward<true, false>, k_diff_forward<true, true>}};
+  hipLaunchKernelGGL(fwd[plan.shaded ? 1 : 0][A.rec ? 1 : 0], dim3(plan.ntiles), dim3(64), 0, s, A);
   R3D_HIP_CHECK(ctx, hipGetLastError());
   R3D_HIP_CHECK(ctx, hipEventRecord(slot->ev, s));
   slot->recorded = true;
@@ -585,7 +589,7 @@ extern "C" int r3d_hip_raycast_adjoint(r3d_hip_ctx *ctx, const r3d_hip_volume *v
                                        float *dL_dtf, float *dL_dgrid, const r3d_diff_params *dp, void *stream) {
   if (!ctx) return R3D_ERR_INVALID;
   Plan plan;
-  int rc = validate_diff(ctx, vol, views, nviews, dp, &plan);
+  int rc = validate_diff(ctx, vol, views, nviews, dp, &plan, true);
   if (rc != R3D_OK) return rc;
   if (!dL_drgba || !dL_dtf || !dL_dgrid || img_w == 0 || img_h == 0)
     return r3d_fail(ctx, R3D_ERR_INVALID, "adjoint: dL_drgba, dL_dtf, dL_dgrid and the image size are required");

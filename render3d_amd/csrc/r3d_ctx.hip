@@ -44,7 +44,7 @@ extern "C" int r3d_hip_init(r3d_hip_ctx **out, int device) {
   build_basis(c->basis);
   int rc = R3D_OK;
   if (hipMalloc(&c->d_tf, 256 * 4 * sizeof(float)) != hipSuccess ||
-      hipMalloc(&c->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&c->d_counters, 8 * sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&c->d_queue, 64 * sizeof(uint32_t)) != hipSuccess || hipMalloc(&c->d_dblim, 16) != hipSuccess) {
     rc = R3D_ERR_NOMEM;
   }

@@ -49,7 +49,7 @@ struct r3d_hip_ctx {
   char err[512] = {0};
   float basis[256];            // host copy of the DCT basis
   float *d_tf = nullptr;       // 256 x 4 FP32 transfer table
-  unsigned long long *d_counters = nullptr;  // 4 x u64 scratch for stats
+  unsigned long long *d_counters = nullptr;  // 8 x u64 scratch for stats
   uint32_t *d_partials = nullptr;            // per-tile raycast counter partials
   size_t partials_cap = 0;                   // in tiles
   float *d_gtf = nullptr;                    // adjoint: per-block TF gradient partials

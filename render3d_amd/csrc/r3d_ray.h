@@ -40,6 +40,9 @@ struct Sampler {
   const uint8_t *base;
   const LevelDev *lev;  // LDS
   uint32_t nlev;
+  // measurement only: one bit per brick of the pyramid, set for every
+  // footprint read (a compile-time null in the rendering kernels)
+  uint32_t *touched = nullptr;
 
   // A trilinear footprint: its four x-pairs of voxels as loaded (low byte at
   // x0, high byte at x0 + 1) and the fractions. The bytes are taken apart
@@ -69,6 +72,10 @@ struct Sampler {
     const uint64_t off = ((uint64_t)L.off_hi << 32) | L.off_lo;
     const uint8_t *q = base + off + (uint64_t)b * R3D_BRICK_BYTES +
                        (((z0 & 15u) * 17u + (y0 & 15u)) * 17u + (x0 & 15u));
+    if (touched) {  // levels start 256-byte aligned, less than a brick past their predecessors' bricks
+      const uint64_t gb = off / R3D_BRICK_BYTES + b;
+      atomicOr(touched + (gb >> 5), 1u << (gb & 31u));
+    }
     f.pr[0] = pair_at(q);
     f.pr[1] = pair_at(q + 17);
     f.pr[2] = pair_at(q + 289);

@@ -51,10 +51,11 @@ struct RayArgs {
   uint8_t *img8;
   uint32_t img_w, img_h;
   float cutoff;
-  uint32_t *partials;  // [workgroup][4] counter sums of this launch, or NULL
+  uint32_t *partials;  // [workgroup][8] counter sums of this launch, or NULL
   uint32_t *px_steps;
   uint32_t ntiles;     // tiles of this launch
   uint32_t *queue;     // tile counter of a persistent launch, or NULL
+  uint32_t *touched;   // measurement: one bit per brick a footprint reads, or NULL
 };
 
 // Samples a ray issues per memory round trip, per kernel kind.
@@ -75,6 +76,7 @@ constexpr int RAY_BATCH = SHADED_KIND<KIND> ? R3D_RAY_BATCH_FULL
 struct RayOut {
   float c[4];
   uint32_t steps, skips, shaded;
+  uint32_t blends, blends_shaded;  // samples (and shaded samples) that blend two mip levels
 };
 
 // Wave-cooperative gradient taps (FULL): the lanes whose sample is shaded
@@ -185,14 +187,15 @@ __device__ __forceinline__ void blinn_phong(V3 g, V3 lvec, V3 hvec, float &cr, f
   }
 }
 
-// The per-ray march. KIND fixes the compositing / shading branches at compile
+// The per-ray march (MEASURE: also count blended samples, for the
+// measurement instances only). KIND fixes the compositing / shading branches at compile
 // time, as the reference builds one pipeline per mode (src/shaders/shaders.cmake:17-30).
-template <int KIND>
+template <int KIND, bool MEASURE>
 __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const float4 *tf,
                                         const r3d_frame_params &pc, float cutoff, uint32_t px, uint32_t py,
                                         float *coop) {
   RayOut R;
-  R.steps = R.skips = R.shaded = 0;
+  R.steps = R.skips = R.shaded = R.blends = R.blends_shaded = 0;
   const uint32_t mode = SHADED_KIND<KIND> ? MODE_FULL : KIND == KIND_MIP ? MODE_MIP : KIND == KIND_FLAT ? MODE_FLAT : pc.mode;
   const V3 fwd = mk(pc.cam_forward[0], pc.cam_forward[1], pc.cam_forward[2]);
   const V3 right = mk(pc.cam_right[0], pc.cam_right[1], pc.cam_right[2]);
@@ -243,7 +246,7 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
   float t = tn + hash12(px, py, pc.frame_index) * dt;
   float mip_max = 0.0f, first_hit = -1.0f;
   float ar = 0.0f, ag = 0.0f, ab = 0.0f, aa = 0.0f;
-  uint32_t steps = 0, skips = 0, shaded_n = 0;
+  uint32_t steps = 0, skips = 0, shaded_n = 0, blends = 0, blends_sh = 0;
   const bool composite = mode == MODE_FULL || mode == MODE_FLAT;
   const float span_step = fmax_sel((tfar - tn) / ((float)MAX_STEPS * dt), 1.0f);
   const float bs = 8.0f * voxel;
@@ -289,7 +292,9 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
     const uint32_t d = (uint32_t)lod;
     const float w = lod - (float)d;
     const float s0 = Sampler::lerp3(cur.f0), s1 = Sampler::lerp3(cur.f1);
-    float v = (w > 0.0f && d + 1u < S.nlev ? flerp(s0, s1, w) : s0) * K255;
+    const bool blend = w > 0.0f && d + 1u < S.nlev;
+    if (MEASURE) blends += blend ? 1u : 0u;
+    float v = (blend ? flerp(s0, s1, w) : s0) * K255;
     const float occ = (float)cur.occ * K255;
     const bool cut = v < threshold;
     if (cut) v = 0.0f;
@@ -320,6 +325,7 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
       if (lit_here) {
         float cr = s.x, cg = s.y, cb = s.z;
         shaded_n++;
+        if (MEASURE) blends_sh += blend ? 1u : 0u;
         blinn_phong(tap_gradient<NTAP>(gt), lvec, hvec, cr, cg, cb);
         const float wa = (1.0f - aa) * a;
         ar = ar + wa * cr;
@@ -346,7 +352,7 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
   // BATCH consecutive samples are issued together (one memory round trip
   // for all); each is used only if the ones before it neither ended the ray
   // nor skipped (a skip issues a new batch from the new t).
-  constexpr int BATCH = RAY_BATCH<KIND>;
+  constexpr int BATCH = MEASURE ? 1 : RAY_BATCH<KIND>;  // measuring: no sample is read and dropped
   bool live = true;
   while (live && t < tfar && steps < MAX_STEPS) {
     Pending q[BATCH];
@@ -368,6 +374,8 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
   R.steps = steps;
   R.skips = skips;
   R.shaded = shaded_n;
+  R.blends = blends;
+  R.blends_shaded = blends_sh;
   if (KIND == KIND_OTHER && mode == MODE_HEATMAP) {
     const float x = (float)steps / 1024.0f;
     R.c[0] = x;
@@ -559,7 +567,7 @@ __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_
                                                const r3d_frame_params &pc, float cutoff, uint32_t px, uint32_t py,
                                                float *coop) {
   RayOut R;
-  R.steps = R.skips = R.shaded = 0;
+  R.steps = R.skips = R.shaded = R.blends = R.blends_shaded = 0;  // one level per sample: no blends
   const uint32_t mode = SHADED_KIND<KIND> ? MODE_FULL : KIND == KIND_MIP ? MODE_MIP : KIND == KIND_FLAT ? MODE_FLAT : pc.mode;
   const V3 fwd = mk(pc.cam_forward[0], pc.cam_forward[1], pc.cam_forward[2]);
   const V3 right = mk(pc.cam_right[0], pc.cam_right[1], pc.cam_right[2]);
@@ -813,8 +821,11 @@ static int shape_index(uint32_t w, uint32_t h) {
 // a.queue == NULL the workgroup renders tile blockIdx.x; otherwise it is
 // persistent and takes tiles from the queue counter until they run out (the
 // TF and level tables are then staged once per workgroup, not once per tile).
-// Counter partials: one uint4 per workgroup.
-template <int KIND, int TW, int TH, bool BRICKS>
+// Counter partials: two uint4 per workgroup (samples, skips, shaded, rays;
+// blended samples, blended shaded samples, 0, 0). MEASURE (8x8 cube-mode
+// measurement instances only) counts the blended samples and marks every
+// brick a footprint reads in a.touched (when set), one bit per brick.
+template <int KIND, int TW, int TH, bool BRICKS, bool MEASURE = false>
 __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
   constexpr uint32_t NT = TW * TH, NW = NT / 64u;
   __shared__ float4 s_tf[256];
@@ -822,7 +833,7 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
   __shared__ uint32_t s_hdr[BRICKS ? 64 : 1];  // bricks: the page header
   __shared__ uint32_t s_ts[R3D_MAX_VIEWS_PER_CALL + 1];
   __shared__ uint32_t s_tile;
-  __shared__ uint4 s_red[NW];
+  __shared__ uint4 s_red[NW][2];
   __shared__ float s_coop[SHADED_KIND<KIND> ? NW * COOP_FLOATS : 1];  // per wave: gradient taps
   const uint32_t lane = threadIdx.x;
   for (uint32_t i = lane; i < 256u; i += NT)
@@ -832,8 +843,8 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
     for (uint32_t i = lane; i < 64u; i += NT) s_hdr[i] = i ? a.bk.page[i] : min(max(a.bk.page[0], 1u), 15u);
   for (uint32_t i = lane; i <= a.nviews; i += NT) s_ts[i] = a.tile_start[i];
   __syncthreads();
-  const Sampler S{a.vol.base, s_lev, a.vol.nlev};
-  uint32_t st = 0, sk = 0, sh = 0, nr = 0;
+  const Sampler S{a.vol.base, s_lev, a.vol.nlev, MEASURE ? a.touched : nullptr};
+  uint32_t st = 0, sk = 0, sh = 0, nr = 0, bl = 0, bs = 0;
   uint32_t tile = blockIdx.x;
   for (;;) {
     if (a.queue) {
@@ -863,11 +874,13 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
       const uint32_t lx = half ? tx * 2u : tx, ly = half ? ty * 2u : ty;
       float *coop = s_coop + (SHADED_KIND<KIND> ? (lane >> 6) * COOP_FLOATS : 0u);
       const RayOut R = BRICKS ? march_bricks<KIND>(a.bk, s_hdr, s_tf, pc, a.cutoff, lx, ly, coop)
-                              : march<KIND>(S, a.vol, s_tf, pc, a.cutoff, lx, ly, coop);
+                              : march<KIND, MEASURE>(S, a.vol, s_tf, pc, a.cutoff, lx, ly, coop);
       st += R.steps;
       sk += R.skips;
       sh += R.shaded;
       nr += 1u;
+      bl += R.blends;
+      bs += R.blends_shaded;
       const uint32_t ox = pc.view_org & 0xffffu, oy = pc.view_org >> 16;
       const uint32_t rep = half ? 2u : 1u;
       for (uint32_t dy = 0; dy < rep; dy++)
@@ -889,48 +902,57 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
     }
     if (!a.queue) break;
   }
-  if (a.partials) {  // one plain store per workgroup; k_reduce_counters sums them
+  if (a.partials) {  // two plain stores per workgroup; k_reduce_counters sums them
     st = wave_sum(st);
     sk = wave_sum(sk);
     sh = wave_sum(sh);
     nr = wave_sum(nr);
-    if ((lane & 63u) == 0) s_red[lane >> 6] = make_uint4(st, sk, sh, nr);
+    bl = wave_sum(bl);
+    bs = wave_sum(bs);
+    if ((lane & 63u) == 0) {
+      s_red[lane >> 6][0] = make_uint4(st, sk, sh, nr);
+      s_red[lane >> 6][1] = make_uint4(bl, bs, 0u, 0u);
+    }
     __syncthreads();
-    if (lane == 0) {
-      uint4 t = s_red[0];
+    if (lane < 2u) {
+      uint4 t = s_red[0][lane];
 #pragma unroll
       for (uint32_t w = 1; w < NW; w++) {
-        t.x += s_red[w].x;
-        t.y += s_red[w].y;
-        t.z += s_red[w].z;
-        t.w += s_red[w].w;
+        t.x += s_red[w][lane].x;
+        t.y += s_red[w][lane].y;
+        t.z += s_red[w][lane].z;
+        t.w += s_red[w][lane].w;
       }
-      ((uint4 *)a.partials)[blockIdx.x] = t;
+      ((uint4 *)a.partials)[2u * blockIdx.x + lane] = t;
     }
   }
 }
 
-// Sums `n` uint4 partials into counters[0..3]: each workgroup reduces a
-// strided share and adds it with 64-bit integer atomics (exact in any order).
-__global__ __launch_bounds__(256) void k_reduce_counters(const uint32_t *partials, uint32_t n,
+// Sums `n` workgroup partials (two uint4 each) into counters[0..ncount):
+// each workgroup reduces a strided share and adds it with 64-bit integer
+// atomics (exact in any order).
+__global__ __launch_bounds__(256) void k_reduce_counters(const uint32_t *partials, uint32_t n, uint32_t ncount,
                                                          unsigned long long *counters) {
-  __shared__ unsigned long long red[4][256];
-  unsigned long long acc[4] = {0, 0, 0, 0};
+  __shared__ unsigned long long red[6][256];
+  unsigned long long acc[6] = {0, 0, 0, 0, 0, 0};
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
-    const uint4 p = ((const uint4 *)partials)[i];
+    const uint4 p = ((const uint4 *)partials)[2u * i];
+    const uint4 q = ((const uint4 *)partials)[2u * i + 1u];
     acc[0] += p.x;
     acc[1] += p.y;
     acc[2] += p.z;
     acc[3] += p.w;
+    acc[4] += q.x;
+    acc[5] += q.y;
   }
-  for (int c = 0; c < 4; c++) red[c][threadIdx.x] = acc[c];
+  for (int c = 0; c < 6; c++) red[c][threadIdx.x] = acc[c];
   __syncthreads();
   for (uint32_t s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s)
-      for (int c = 0; c < 4; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+      for (int c = 0; c < 6; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x < 4 && red[threadIdx.x][0]) atomicAdd(&counters[threadIdx.x], red[threadIdx.x][0]);
+  if (threadIdx.x < ncount && red[threadIdx.x][0]) atomicAdd(&counters[threadIdx.x], red[threadIdx.x][0]);
 }
 
 static uint32_t reduce_grid(uint32_t n) {
@@ -949,6 +971,12 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 typedef void (*RayKernel)(RayArgs);
 #define R3D_RAY_SHAPES(K, B)                                                                     \
   { k_raycast<K, 8, 8, B>, k_raycast<K, 16, 4, B>, k_raycast<K, 16, 8, B>, k_raycast<K, 16, 16, B> }
+// The 8x8 cube-mode measurement instances (blend counts, touched bricks).
+static const RayKernel measure_kernels[NKINDS] = {k_raycast<KIND_FULL, 8, 8, false, true>,
+                                                k_raycast<KIND_MIP, 8, 8, false, true>,
+                                                k_raycast<KIND_FLAT, 8, 8, false, true>,
+                                                k_raycast<KIND_OTHER, 8, 8, false, true>,
+                                                k_raycast<KIND_FAST, 8, 8, false, true>};
 static const RayKernel ray_kernels[2][NKINDS][NSHAPES] = {
     {R3D_RAY_SHAPES(KIND_FULL, false), R3D_RAY_SHAPES(KIND_MIP, false), R3D_RAY_SHAPES(KIND_FLAT, false),
      R3D_RAY_SHAPES(KIND_OTHER, false), R3D_RAY_SHAPES(KIND_FAST, false)},
@@ -1034,10 +1062,13 @@ static int check_tuning(r3d_hip_ctx *ctx, const r3d_raycast_tuning *t) {
 }
 
 // Kernel launches only (capture-safe): queue reset, one launch per kind,
-// counter reduction. `partials` holds partial_slots() uint4 entries.
+// counter reduction into ncount counters. `partials` holds partial_slots()
+// entries of two uint4. ncount 6 runs the 8x8 measurement instances (t must
+// then be 8x8, one workgroup per tile).
 static int launch_groups(r3d_hip_ctx *ctx, RayArgs a, const r3d_frame_params *dviews, const uint32_t *dtables,
                          const ViewGroups &g, const r3d_raycast_tuning &t, uint32_t *partials, uint32_t *queue,
-                         unsigned long long *counters, hipStream_t s, bool bricks = false) {
+                         unsigned long long *counters, uint32_t ncount, hipStream_t s, bool bricks = false) {
+  const bool measure = ncount == 6u;
   const int sh = shape_index(t.tile_w, t.tile_h);
   const uint32_t nt = SHAPE_W[sh] * SHAPE_H[sh];
   if (t.blocks_per_cu) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, queue, (uint32_t)NKINDS);
@@ -1051,19 +1082,20 @@ static int launch_groups(r3d_hip_ctx *ctx, RayArgs a, const r3d_frame_params *dv
       const uint32_t cap = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * t.blocks_per_cu;
       if (cap < grid) grid = cap;
     }
-    a.partials = counters ? partials + (size_t)pbase * 4u : nullptr;
+    a.partials = counters ? partials + (size_t)pbase * 8u : nullptr;
     pbase += grid;
     a.views = dviews + g.gstart[k];
     a.tile_start = dtables + (size_t)sh * g.table_words() + g.tpos[k];
     a.nviews = nv;
     a.ntiles = tiles;
     a.queue = t.blocks_per_cu ? queue + k : nullptr;
-    hipLaunchKernelGGL(ray_kernels[bricks ? 1 : 0][k][sh], dim3(grid), dim3(nt), 0, s, a);
+    hipLaunchKernelGGL(measure ? measure_kernels[k] : ray_kernels[bricks ? 1 : 0][k][sh], dim3(grid), dim3(nt), 0,
+                       s, a);
     R3D_HIP_CHECK(ctx, hipGetLastError());
   }
   if (counters && pbase) {
     hipLaunchKernelGGL(k_reduce_counters, dim3(reduce_grid(pbase)), dim3(256), 0, s, (const uint32_t *)partials,
-                       pbase, counters);
+                       pbase, ncount, counters);
     R3D_HIP_CHECK(ctx, hipGetLastError());
   }
   return R3D_OK;
@@ -1108,6 +1140,7 @@ static RayArgs base_args(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_
   a.img_h = out->height;
   a.cutoff = opts && opts->alpha_cutoff > 0.0f ? opts->alpha_cutoff : 0.98f;
   a.px_steps = opts ? opts->px_steps : nullptr;
+  a.touched = opts ? opts->touched : nullptr;
   return a;
 }
 
@@ -1171,11 +1204,15 @@ static int raycast_impl(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_h
   rc = stage(ctx, buf, g.bytes(), s, &slot);
   if (rc != R3D_OK) return rc;
   unsigned long long *counters = opts ? (unsigned long long *)opts->counters : nullptr;
+  const bool measure = opts && (opts->flags & R3D_RAYCAST_MEASURE);
+  const uint32_t ncount = measure ? 6u : 4u;
   if (stats) {
     counters = ctx->d_counters;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, counters, 4u);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, counters, 6u);
   }
-  const uint32_t slots = partial_slots(ctx, g, ctx->tuning);
+  // measuring: the 8x8 measurement instances, one workgroup per tile
+  const r3d_raycast_tuning tuning = measure ? r3d_raycast_tuning{8u, 8u, 0u, 0.0f} : ctx->tuning;
+  const uint32_t slots = partial_slots(ctx, g, tuning);
   if (counters && ctx->partials_cap < slots) {
     if (ctx->d_partials) {
       R3D_HIP_CHECK(ctx, hipStreamSynchronize(s));
@@ -1184,7 +1221,7 @@ static int raycast_impl(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_h
       ctx->partials_cap = 0;
     }
     const size_t cap = slots < 65536u ? 65536u : slots;
-    if (hipMalloc(&ctx->d_partials, cap * 16u) != hipSuccess)
+    if (hipMalloc(&ctx->d_partials, cap * 32u) != hipSuccess)
       return r3d_fail(ctx, R3D_ERR_NOMEM, "raycast: counter partials for %zu workgroups", cap);
     ctx->partials_cap = cap;
   }
@@ -1192,19 +1229,24 @@ static int raycast_impl(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_h
   const uint32_t *dtables = (const uint32_t *)((const uint8_t *)slot->dev + g.view_bytes());
   RayArgs a = base_args(ctx, vol, out, opts);
   if (bk) a.bk = BricksDev{bk->d_page, bk->d_atlas, bk->desc.abpa, bk->page_words, bk->nslots};
-  rc = launch_groups(ctx, a, dviews, dtables, g, ctx->tuning, ctx->d_partials, ctx->d_queue, counters, s,
+  if (measure && bk) return r3d_fail(ctx, R3D_ERR_UNSUPPORTED, "bricks_raycast: no measurement mode");
+  if (a.touched && !measure)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: a touched-brick bitmap needs R3D_RAYCAST_MEASURE");
+  rc = launch_groups(ctx, a, dviews, dtables, g, tuning, ctx->d_partials, ctx->d_queue, counters, ncount, s,
                      bk != nullptr);
   if (rc != R3D_OK) return rc;
   R3D_HIP_CHECK(ctx, hipEventRecord(slot->ev, s));
   slot->recorded = true;
   if (stats) {
-    unsigned long long h[4];
+    unsigned long long h[6];
     R3D_HIP_CHECK(ctx, hipMemcpyAsync(h, counters, sizeof h, hipMemcpyDeviceToHost, s));
     R3D_HIP_CHECK(ctx, hipStreamSynchronize(s));
     stats->samples = h[0];
     stats->skips = h[1];
     stats->shaded = h[2];
     stats->rays = h[3];
+    stats->blended = h[4];
+    stats->blended_shaded = h[5];
   }
   return R3D_OK;
 }
@@ -1247,6 +1289,8 @@ extern "C" int r3d_hip_raycast_plan_create(r3d_hip_ctx *ctx, const r3d_hip_volum
   if (!plan) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast plan: NULL output");
   *plan = nullptr;
   if (!vol || vol->ctx != ctx) return r3d_fail(ctx, R3D_ERR_INVALID, "raycast plan: bad volume");
+  if (opts && (opts->touched || (opts->flags & R3D_RAYCAST_MEASURE)))
+    return r3d_fail(ctx, R3D_ERR_UNSUPPORTED, "raycast plan: the measurement mode is for r3d_hip_raycast");
   int rc = validate_views(ctx, views, nviews, out);
   if (rc != R3D_OK) return rc;
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
@@ -1276,7 +1320,7 @@ extern "C" int r3d_hip_raycast_plan_create(r3d_hip_ctx *ctx, const r3d_hip_volum
   const size_t off_tab = (p->g.view_bytes() + 255u) & ~(size_t)255u;
   const size_t off_q = (off_tab + sizeof(uint32_t) * p->g.table_words() * NSHAPES + 255u) & ~(size_t)255u;
   const size_t off_part = off_q + 256u;
-  const size_t total = off_part + (size_t)slots * 16u;
+  const size_t total = off_part + (size_t)slots * 32u;
   if (hipMalloc(&p->d_mem, total) != hipSuccess) {
     free(buf);
     delete p;
@@ -1305,7 +1349,8 @@ extern "C" int r3d_hip_raycast_plan_launch(r3d_hip_raycast_plan *p, void *stream
   r3d_hip_ctx *ctx = p->ctx;
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   return launch_groups(ctx, base_args(ctx, p->vol, &p->out, &p->opts), p->d_views, p->d_tables, p->g, p->tuning,
-                       p->d_partials, p->d_queue, (unsigned long long *)p->opts.counters, (hipStream_t)stream);
+                       p->d_partials, p->d_queue, (unsigned long long *)p->opts.counters,
+                       4u, (hipStream_t)stream);
 }
 
 extern "C" int r3d_hip_raycast_plan_set_tuning(r3d_hip_raycast_plan *p, const r3d_raycast_tuning *t) {
@@ -1345,11 +1390,11 @@ extern "C" int r3d_hip_raycast_plan_tune(r3d_hip_raycast_plan *p, void *stream, 
   for (int sh = 0; sh < NSHAPES && rc == R3D_OK; sh++)
     for (int b = 0; b < 5 && rc == R3D_OK; b++) {
       r3d_raycast_tuning t{SHAPE_W[sh], SHAPE_H[sh], bpc[b], 0.0f};
-      rc = launch_groups(ctx, a, p->d_views, p->d_tables, p->g, t, p->d_partials, p->d_queue, nullptr, s);
+      rc = launch_groups(ctx, a, p->d_views, p->d_tables, p->g, t, p->d_partials, p->d_queue, nullptr, 4u, s);
       if (rc != R3D_OK) break;
       if (hipEventRecord(e0, s) != hipSuccess) rc = R3D_ERR_DEVICE;
       for (int r = 0; r < 3 && rc == R3D_OK; r++)
-        rc = launch_groups(ctx, a, p->d_views, p->d_tables, p->g, t, p->d_partials, p->d_queue, nullptr, s);
+        rc = launch_groups(ctx, a, p->d_views, p->d_tables, p->g, t, p->d_partials, p->d_queue, nullptr, 4u, s);
       float ms = 0.0f;
       if (rc == R3D_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))

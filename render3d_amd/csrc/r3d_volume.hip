@@ -832,3 +832,10 @@ extern "C" int r3d_hip_synth_sheets(r3d_hip_ctx *ctx, uint8_t *dst, uint32_t nx,
   R3D_HIP_CHECK(ctx, hipGetLastError());
   return R3D_OK;
 }
+
+extern "C" uint64_t r3d_hip_volume_brick_count(const r3d_hip_volume *v) {
+  if (!v) return 0;
+  uint64_t n = 0;
+  for (uint32_t l = 0; l < v->nlev; l++) n += (uint64_t)v->bricks[l][0] * v->bricks[l][1] * v->bricks[l][2];
+  return n;
+}

@@ -249,6 +249,39 @@ def test_raycast_with_skips_matches_oracle(ctx, sparse96, case):
         assert cnt["shaded"] > 0
 
 
+@pytest.mark.parametrize("case", ["full", "mip", "flat", "lodbias", "half", "multi"])
+def test_measurement_mode_matches_oracle(ctx, scene96, case):
+    """R3D_RAYCAST_MEASURE: the measurement instances render the production
+    kernel's image bit for bit, count blended samples as the oracle does, and
+    mark exactly the bricks whose footprints the oracle reads."""
+    v, ov = scene96
+    W, H = 120, 88
+    lut, views = _views(W, H)
+    vs = views[case]
+    ctx.set_transfer(lut)
+    prod = ctx.raycast(v, vs, W, H, alpha_cutoff=0.99, out=torch.zeros((H, W, 4), device="cuda"), stats=True)
+    touched = torch.zeros((v.brick_count + 31) // 32, dtype=torch.int32, device="cuda")
+    counters = torch.zeros(6, dtype=torch.int64, device="cuda")
+    res = ctx.raycast(v, vs, W, H, alpha_cutoff=0.99, out=torch.zeros((H, W, 4), device="cuda"), stats=True,
+                      measure=True, touched=touched)
+    ctx.raycast(v, vs, W, H, alpha_cutoff=0.99, counters=counters, measure=True)
+    assert torch.equal(res["rgba"], prod["rgba"])
+    bits = ov.track_touched()
+    try:
+        _, cnt, _ = orc.raycast(ov, scene.tf_unorm(lut), vs, W, H, alpha_cutoff=0.99)
+    finally:
+        ov.track_touched(False)
+    st = res["stats"]
+    want = [cnt[k] for k in ("samples", "skips", "shaded", "rays", "blended", "blended_shaded")]
+    assert [st[k] for k in ("samples", "skips", "shaded", "rays", "blended", "blended_shaded")] == want
+    assert counters.cpu().numpy().tolist() == want
+    if case in ("full", "lodbias", "multi"):
+        assert cnt["blended"] > 0 and cnt["blended_shaded"] > 0
+    got = touched.cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, bits), (int(np.unpackbits(got.view(np.uint8)).sum()),
+                                       int(np.unpackbits(bits.view(np.uint8)).sum()))
+
+
 def test_raycast_f32_tf_and_empty_volume(ctx):
     n = 48
     v = ctx.volume(n, n, n)
