@@ -187,11 +187,20 @@ __device__ __forceinline__ void blinn_phong(V3 g, V3 lvec, V3 hvec, float &cr, f
   }
 }
 
+// Whether a sample of value v can contribute: opq holds one bit per TF
+// interval [i, i + 1], set unless both entries have zero alpha. A sample in
+// a zero interval gets alpha lerp(0, 0, f) = 0, so its opacity
+// 1 - pow(1 - 0, k) is 0 and it adds nothing; i0 is tf_sample's index.
+__device__ __forceinline__ bool tf_visible(const uint32_t *opq, float v) {
+  const uint32_t i0 = (uint32_t)fclamp_sel(v * 256.0f - 0.5f, 0.0f, 255.0f);
+  return ((opq[i0 >> 5] >> (i0 & 31u)) & 1u) != 0u;
+}
+
 // The per-ray march (MEASURE: also count blended samples, for the
 // measurement instances only). KIND fixes the compositing / shading branches at compile
 // time, as the reference builds one pipeline per mode (src/shaders/shaders.cmake:17-30).
 template <int KIND, bool MEASURE>
-__device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const float4 *tf,
+__device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const float4 *tf, const uint32_t *opq,
                                         const r3d_frame_params &pc, float cutoff, uint32_t px, uint32_t py,
                                         float *coop) {
   RayOut R;
@@ -314,27 +323,30 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
     if (KIND == KIND_OTHER && first_hit < 0.0f && v > 0.05f) first_hit = t;
     if (SHADED_KIND<KIND>) {
       // every active lane takes part in the taps, so the TF and opacity are
-      // evaluated for all of them (a cut sample simply does not use them)
-      constexpr int NTAP = KIND_TAPS<KIND>;
-      const float4 s = tf_sample(tf, v);
-      const float a = 1.0f - pow_unit(1.0f - s.w, opac_k * lod_step);
-      const bool lit_here = !cut && a > 0.003f;
-      float gt[NTAP];
-      coop_taps<NTAP>([&](V3 q, float ql) { return S.sample(q, ql); }, coop, lit_here, p,
-                      tap_radius<NTAP>(voxel * lod_step), lod, gt);
-      if (lit_here) {
-        float cr = s.x, cg = s.y, cb = s.z;
-        shaded_n++;
-        if (MEASURE) blends_sh += blend ? 1u : 0u;
-        blinn_phong(tap_gradient<NTAP>(gt), lvec, hvec, cr, cg, cb);
-        const float wa = (1.0f - aa) * a;
-        ar = ar + wa * cr;
-        ag = ag + wa * cg;
-        ab = ab + wa * cb;
-        aa = aa + wa;
-        if (aa >= cutoff) return 0;
+      // evaluated for all of them (a cut sample simply does not use them);
+      // when no lane's sample can contribute, the wave skips all of it
+      if (__ballot(!cut && tf_visible(opq, v)) != 0) {
+        constexpr int NTAP = KIND_TAPS<KIND>;
+        const float4 s = tf_sample(tf, v);
+        const float a = 1.0f - pow_unit(1.0f - s.w, opac_k * lod_step);
+        const bool lit_here = !cut && a > 0.003f;
+        float gt[NTAP];
+        coop_taps<NTAP>([&](V3 q, float ql) { return S.sample(q, ql); }, coop, lit_here, p,
+                        tap_radius<NTAP>(voxel * lod_step), lod, gt);
+        if (lit_here) {
+          float cr = s.x, cg = s.y, cb = s.z;
+          shaded_n++;
+          if (MEASURE) blends_sh += blend ? 1u : 0u;
+          blinn_phong(tap_gradient<NTAP>(gt), lvec, hvec, cr, cg, cb);
+          const float wa = (1.0f - aa) * a;
+          ar = ar + wa * cr;
+          ag = ag + wa * cg;
+          ab = ab + wa * cb;
+          aa = aa + wa;
+          if (aa >= cutoff) return 0;
+        }
       }
-    } else if (composite && !cut) {
+    } else if (composite && !cut && tf_visible(opq, v)) {
       const float4 s = tf_sample(tf, v);
       const float a = 1.0f - pow_unit(1.0f - s.w, opac_k * lod_step);
       if (a > 0.003f) {
@@ -564,8 +576,8 @@ __device__ __forceinline__ BCtx locate_cached(const BricksDev &B, const uint32_t
 
 template <int KIND>
 __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_t *hdr, const float4 *tf,
-                                               const r3d_frame_params &pc, float cutoff, uint32_t px, uint32_t py,
-                                               float *coop) {
+                                               const uint32_t *opq, const r3d_frame_params &pc, float cutoff,
+                                               uint32_t px, uint32_t py, float *coop) {
   RayOut R;
   R.steps = R.skips = R.shaded = R.blends = R.blends_shaded = 0;  // one level per sample: no blends
   const uint32_t mode = SHADED_KIND<KIND> ? MODE_FULL : KIND == KIND_MIP ? MODE_MIP : KIND == KIND_FLAT ? MODE_FLAT : pc.mode;
@@ -718,26 +730,28 @@ __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_
     if (mode == MODE_MIP && mip_max >= 0.999f) return 0;
     if (KIND == KIND_OTHER && first_hit < 0.0f && v > 0.05f) first_hit = t;
     if (SHADED_KIND<KIND>) {  // every active lane takes part in the taps (see march)
-      constexpr int NTAP = KIND_TAPS<KIND>;
-      const float4 s = tf_sample(tf, v);
-      const float a = 1.0f - pow_unit(1.0f - s.w, opac_k * lod_step);
-      const bool lit_here = !cut && a > 0.003f;
-      float gt[NTAP];
-      // taps reuse the located brick (raycast.slang:933-937, 964-973); x + -ev is x - ev
-      coop_taps<NTAP>([&](V3 q, float slot) { return brick_fetch(B.atlas, __builtin_bit_cast(uint32_t, slot), q); },
-                      coop, lit_here, bc.bl, tap_radius<NTAP>(bc.grad_vox), __builtin_bit_cast(float, bc.slot), gt);
-      if (lit_here) {
-        float cr = s.x, cg = s.y, cb = s.z;
-        shaded_n++;
-        blinn_phong(tap_gradient<NTAP>(gt), lvec, hvec, cr, cg, cb);
-        const float wa = (1.0f - aa) * a;
-        ar = ar + wa * cr;
-        ag = ag + wa * cg;
-        ab = ab + wa * cb;
-        aa = aa + wa;
-        if (aa >= cutoff) return 0;
+      if (__ballot(!cut && tf_visible(opq, v)) != 0) {
+        constexpr int NTAP = KIND_TAPS<KIND>;
+        const float4 s = tf_sample(tf, v);
+        const float a = 1.0f - pow_unit(1.0f - s.w, opac_k * lod_step);
+        const bool lit_here = !cut && a > 0.003f;
+        float gt[NTAP];
+        // taps reuse the located brick (raycast.slang:933-937, 964-973); x + -ev is x - ev
+        coop_taps<NTAP>([&](V3 q, float slot) { return brick_fetch(B.atlas, __builtin_bit_cast(uint32_t, slot), q); },
+                        coop, lit_here, bc.bl, tap_radius<NTAP>(bc.grad_vox), __builtin_bit_cast(float, bc.slot), gt);
+        if (lit_here) {
+          float cr = s.x, cg = s.y, cb = s.z;
+          shaded_n++;
+          blinn_phong(tap_gradient<NTAP>(gt), lvec, hvec, cr, cg, cb);
+          const float wa = (1.0f - aa) * a;
+          ar = ar + wa * cr;
+          ag = ag + wa * cg;
+          ab = ab + wa * cb;
+          aa = aa + wa;
+          if (aa >= cutoff) return 0;
+        }
       }
-    } else if (composite && !cut) {
+    } else if (composite && !cut && tf_visible(opq, v)) {
       const float4 s = tf_sample(tf, v);
       const float a = 1.0f - pow_unit(1.0f - s.w, opac_k * lod_step);
       if (a > 0.003f) {
@@ -834,6 +848,7 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
   __shared__ uint32_t s_ts[R3D_MAX_VIEWS_PER_CALL + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint4 s_red[NW][2];
+  __shared__ uint32_t s_opq[8];  // TF intervals that can be visible (tf_visible)
   __shared__ float s_coop[SHADED_KIND<KIND> ? NW * COOP_FLOATS : 1];  // per wave: gradient taps
   const uint32_t lane = threadIdx.x;
   for (uint32_t i = lane; i < 256u; i += NT)
@@ -842,6 +857,18 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
   if (BRICKS)  // nlev held to 1..15: the per-level words stay inside the header
     for (uint32_t i = lane; i < 64u; i += NT) s_hdr[i] = i ? a.bk.page[i] : min(max(a.bk.page[0], 1u), 15u);
   for (uint32_t i = lane; i <= a.nviews; i += NT) s_ts[i] = a.tile_start[i];
+  __syncthreads();
+  if (lane < 64u) {  // wave 0: one bit per TF interval [i, i + 1] whose alpha is not zero at both ends
+#pragma unroll
+    for (uint32_t c = 0; c < 4u; c++) {
+      const uint32_t i = c * 64u + lane, i1 = i < 255u ? i + 1u : 255u;
+      const uint64_t m = __ballot(s_tf[i].w != 0.0f || s_tf[i1].w != 0.0f);
+      if (lane == 0) {
+        s_opq[2u * c] = (uint32_t)m;
+        s_opq[2u * c + 1u] = (uint32_t)(m >> 32);
+      }
+    }
+  }
   __syncthreads();
   const Sampler S{a.vol.base, s_lev, a.vol.nlev, MEASURE ? a.touched : nullptr};
   uint32_t st = 0, sk = 0, sh = 0, nr = 0, bl = 0, bs = 0;
@@ -873,8 +900,8 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
     if (tx < RW && ty < RH) {
       const uint32_t lx = half ? tx * 2u : tx, ly = half ? ty * 2u : ty;
       float *coop = s_coop + (SHADED_KIND<KIND> ? (lane >> 6) * COOP_FLOATS : 0u);
-      const RayOut R = BRICKS ? march_bricks<KIND>(a.bk, s_hdr, s_tf, pc, a.cutoff, lx, ly, coop)
-                              : march<KIND, MEASURE>(S, a.vol, s_tf, pc, a.cutoff, lx, ly, coop);
+      const RayOut R = BRICKS ? march_bricks<KIND>(a.bk, s_hdr, s_tf, s_opq, pc, a.cutoff, lx, ly, coop)
+                              : march<KIND, MEASURE>(S, a.vol, s_tf, s_opq, pc, a.cutoff, lx, ly, coop);
       st += R.steps;
       sk += R.skips;
       sh += R.shaded;
