@@ -599,6 +599,62 @@ int r3d_hip_bricks_raycast(r3d_hip_ctx *ctx, const r3d_hip_bricks *b, const r3d_
                            const r3d_hip_image *out, const r3d_raycast_opts *opts, r3d_hip_stats *stats,
                            void *stream);
 
+/* ---- streaming residency for bricks-LOD volumes ---------------------------
+ * The bricks pump of the reference (src/render/render.h:115-132;
+ * src/vk/vkbackend.c:5244-5599) over a handle's page table, for volumes whose
+ * pyramid has more bricks than the atlas has slots. The warm tier is the
+ * pyramid of the last r3d_hip_bricks_build, kept in HBM (level 0 is the
+ * caller's volume: it must stay valid while streaming; the build's resident
+ * flags seed the atlas, and the coarsest level should be resident, as the
+ * reference makes it). Per frame: begin, then any number of box / point
+ * collects, then submit; r3d_hip_bricks_stream is begin + the view-cone
+ * collect + submit. Requested bricks that are resident are marked wanted;
+ * the others become candidates, parent before child, then nearest first.
+ * Bricks whose largest voxel is below the gate (at least 5/255, the codec
+ * noise floor, vkbackend.c:2348) never take a slot. A submit fills at most
+ * min(budget, 32) bricks into free slots, else into the least recently
+ * wanted slots not wanted this frame (the evicted bricks' entries become
+ * absent first), and publishes their entries (slot | max << 24). The fills
+ * and entry writes are queued on `stream`, ordered after earlier frames on
+ * it: use one stream for the renders and the stream calls (or order them).
+ * Between builds the streamer owns the table: do not mix it with writes
+ * through r3d_hip_bricks_tables. Geometry is in the renderer's volume space
+ * (base voxels / the largest base dim). */
+
+/* r3d_bricks_stream_begin (render.h:127): opens a collect pass; returns 1, or
+ * 0 when the handle has not been built (nothing is collected then). */
+int r3d_hip_bricks_stream_begin(r3d_hip_bricks *b);
+/* r3d_bricks_stream_box (render.h:128-129): the bricks of the box lo..hi at
+ * the level whose voxel pitch matches pixel_cone (coarser until at most
+ * 2,048 bricks), each with its parent. */
+void r3d_hip_bricks_stream_box(r3d_hip_bricks *b, const float lo[3], const float hi[3], float pixel_cone,
+                               float gate);
+/* r3d_bricks_stream_point (render.h:130-131): the brick holding p at level,
+ * and its parent. */
+void r3d_hip_bricks_stream_point(r3d_hip_bricks *b, const float p[3], uint32_t level, float gate);
+/* r3d_bricks_stream_submit (render.h:132): fills up to min(budget, 32)
+ * candidates on `stream`. */
+int r3d_hip_bricks_stream_submit(r3d_hip_ctx *ctx, r3d_hip_bricks *b, uint32_t budget, void *stream);
+/* r3d_bricks_stream (render.h:115-124): begin, the view-cone collect (eye and
+ * fwd in volume space, half_tan = tan(fov_y / 2) x the frustum diagonal
+ * factor, pixel_cone = the ray-cone growth per unit distance; slice_depth > 0
+ * limits z to [slice_z0, slice_z0 + slice_depth) base voxels), submit. */
+int r3d_hip_bricks_stream(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const float eye[3], const float fwd[3], float half_tan,
+                          float pixel_cone, uint32_t slice_z0, uint32_t slice_depth, float gate, uint32_t budget,
+                          void *stream);
+/* r3d_bricks_get_stats (render.h:146-158), the fields a GPU-resident warm
+ * tier has. */
+typedef struct r3d_bricks_stream_stats {
+  uint32_t nb, hot, hot_cap;  /* page entries; resident slots; slots */
+  uint32_t inflight;          /* bricks filled by this pass's submit (0 from begin until it fills) */
+  uint32_t evicted_last;      /* bricks evicted by this pass's submit */
+  uint32_t nlevels;
+  uint32_t lod_wanted[8];     /* the last pass's desired bricks per level */
+  uint64_t lod_requests[8];   /* cumulative */
+  uint64_t decoded, evicted, jobs; /* cumulative fills, evictions, submits that filled */
+} r3d_bricks_stream_stats;
+int r3d_hip_bricks_stream_stats(const r3d_hip_bricks *b, r3d_bricks_stream_stats *st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -159,6 +159,13 @@ class BricksDesc(C.Structure):
 
 BR_LOD_FLAG = 0x20000
 
+
+class BricksStreamStats(C.Structure):
+    _fields_ = [("nb", C.c_uint32), ("hot", C.c_uint32), ("hot_cap", C.c_uint32), ("inflight", C.c_uint32),
+                ("evicted_last", C.c_uint32), ("nlevels", C.c_uint32), ("lod_wanted", C.c_uint32 * 8),
+                ("lod_requests", C.c_uint64 * 8), ("decoded", C.c_uint64), ("evicted", C.c_uint64),
+                ("jobs", C.c_uint64)]
+
 SHARD_GRID = 64
 SHARD_INDEX_BYTES = SHARD_GRID ** 3 * 16 + 4
 
@@ -228,6 +235,13 @@ SIGNATURES = {
     "r3d_hip_bricks_raycast": (C.c_int, [P, P, C.POINTER(FrameParams), C.c_uint32, C.POINTER(Image),
                                          C.POINTER(RaycastOpts), C.POINTER(Stats), P]),
     "r3d_hip_deblock_chunks_u8": (C.c_int, [P, C.c_uint32, P, P, C.c_float, C.c_float, P, P]),
+    "r3d_hip_bricks_stream_begin": (C.c_int, [P]),
+    "r3d_hip_bricks_stream_box": (None, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.c_float]),
+    "r3d_hip_bricks_stream_point": (None, [P, C.POINTER(C.c_float), C.c_uint32, C.c_float]),
+    "r3d_hip_bricks_stream_submit": (C.c_int, [P, P, C.c_uint32, P]),
+    "r3d_hip_bricks_stream": (C.c_int, [P, P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.c_float,
+                                        C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, P]),
+    "r3d_hip_bricks_stream_stats": (C.c_int, [P, C.POINTER(BricksStreamStats)]),
 }
 
 _lib = None

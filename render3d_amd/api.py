@@ -564,6 +564,36 @@ class Bricks:
             raise ValueError("bricks write_slots: whole slots inside the atlas")
         _hip_copy(atlas + first * 128 ** 3, v.ctypes.data, v.nbytes)
 
+    # ---- streaming residency (r3d_hip_bricks_stream_*) --------------------------
+    def stream_begin(self):
+        """r3d_bricks_stream_begin: opens a collect pass (True) or not (False)."""
+        return bool(self.ctx.lib.r3d_hip_bricks_stream_begin(self.h))
+
+    def stream_box(self, lo, hi, pixel_cone, gate):
+        f3 = C.c_float * 3
+        self.ctx.lib.r3d_hip_bricks_stream_box(self.h, f3(*lo), f3(*hi), pixel_cone, gate)
+
+    def stream_point(self, p, level, gate):
+        self.ctx.lib.r3d_hip_bricks_stream_point(self.h, (C.c_float * 3)(*p), level, gate)
+
+    def stream_submit(self, budget, stream=None):
+        self.ctx.check(self.ctx.lib.r3d_hip_bricks_stream_submit(self.ctx.h, self.h, budget, _stream(stream)))
+
+    def stream(self, eye, fwd, half_tan, pixel_cone, gate, budget, slice_z0=0, slice_depth=0, stream=None):
+        """r3d_bricks_stream: begin + the view-cone collect + submit."""
+        f3 = C.c_float * 3
+        self.ctx.check(self.ctx.lib.r3d_hip_bricks_stream(self.ctx.h, self.h, f3(*eye), f3(*fwd), half_tan,
+                                                          pixel_cone, slice_z0, slice_depth, gate, budget,
+                                                          _stream(stream)))
+
+    def stream_stats(self):
+        st = _lib.BricksStreamStats()
+        self.ctx.check(self.ctx.lib.r3d_hip_bricks_stream_stats(self.h, C.byref(st)))
+        return {"nb": st.nb, "hot": st.hot, "hot_cap": st.hot_cap, "inflight": st.inflight,
+                "evicted_last": st.evicted_last, "nlevels": st.nlevels, "lod_wanted": list(st.lod_wanted),
+                "lod_requests": list(st.lod_requests), "decoded": st.decoded, "evicted": st.evicted,
+                "jobs": st.jobs}
+
     def raycast(self, views, width, height, *, alpha_cutoff=0.98, rgba=True, rgba8=False, tf=None, counters=None,
                 px_steps=False, stats=False, out=None, stream=None):
         """Render bricks-LOD views (brick_mode = self.brick_mode); as Context.raycast."""

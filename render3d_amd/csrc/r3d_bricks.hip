@@ -14,6 +14,10 @@
 // into temporary linear levels, then one workgroup per resident brick copies
 // its 128^3 voxels into its slot (zeros past the level's edge) and writes its
 // entry with the brick's maximum.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -169,6 +173,60 @@ __global__ __launch_bounds__(256) void k_bk_entries(const BkJob *jobs, uint32_t 
   if (j < njobs) page[jobs[j].word] = jobs[j].slot | (maxv[j] << 24);
 }
 
+
+// The entries of evicted bricks become absent before their slots are refilled
+// (queue-ordered on the stream, so a frame rendered earlier still reads the
+// old entry and a later one the new).
+__global__ __launch_bounds__(256) void k_bk_evict(const uint32_t *words, uint32_t n, uint32_t *page) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) page[words[i]] = 0xFFFFFFFFu;
+}
+
+struct BkMax {
+  const uint8_t *lev[R3D_MAX_LEVELS];
+  uint32_t dims[R3D_MAX_LEVELS][3];
+  uint32_t nb[R3D_MAX_LEVELS][3];
+  uint32_t off[R3D_MAX_LEVELS];  // page word of each level's first entry
+  uint32_t nlev;
+  uint8_t *bmax;                 // per entry (word - 64)
+};
+
+// One workgroup per brick of every level: its largest voxel (the streamer's
+// gate test: bricks below the gate never take a slot). Rows of 128 voxels,
+// one per lane, 16 bytes at a time where the row lies whole in the level.
+__global__ __launch_bounds__(256) void k_bk_bmax(BkMax a) {
+  __shared__ uint32_t s_max[4];
+  const uint32_t e = blockIdx.x;
+  uint32_t l = 0;
+  while (l + 1u < a.nlev && e + 64u >= a.off[l + 1u]) l++;
+  const uint32_t k = e + 64u - a.off[l];
+  const uint32_t cx = k % a.nb[l][0], cy = (k / a.nb[l][0]) % a.nb[l][1], cz = k / (a.nb[l][0] * a.nb[l][1]);
+  const uint32_t nx = a.dims[l][0], ny = a.dims[l][1], nz = a.dims[l][2];
+  const uint8_t *L = a.lev[l];
+  const uint32_t x0 = cx * BK, x1 = min(x0 + BK, nx);
+  const bool vec = x1 - x0 == BK && nx % 16u == 0u && ((uintptr_t)L & 15u) == 0u;
+  uint32_t mx = 0;
+  for (uint32_t r = threadIdx.x; r < BK * BK; r += 256u) {
+    const uint32_t gy = cy * BK + (r & 127u), gz = cz * BK + (r >> 7);
+    if (gy >= ny || gz >= nz) continue;
+    const uint8_t *row = L + ((uint64_t)gz * ny + gy) * nx;
+    if (vec) {
+      for (uint32_t x = x0; x < x1; x += 16u) {
+        const uint4 q = *(const uint4 *)(row + x);
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int i = 0; i < 16; i++) mx = max(mx, (w[i >> 2] >> (8 * (i & 3))) & 255u);
+      }
+    } else {
+      for (uint32_t x = x0; x < x1; x++) mx = max(mx, (uint32_t)row[x]);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, o));
+  if ((threadIdx.x & 63u) == 0u) s_max[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) a.bmax[e] = (uint8_t)max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+}
+
 }  // namespace r3d
 
 using namespace r3d;
@@ -211,7 +269,8 @@ extern "C" int r3d_hip_bricks_create(r3d_hip_ctx *ctx, const r3d_bricks_desc *d,
   // the build's scratch, allocated once so that a build allocates nothing
   const uint64_t entries = words - 64u;
   b->max_jobs = (uint32_t)(entries < b->nslots ? entries : b->nslots);
-  bool ok = hipMalloc(&b->d_jobs, (sizeof(BkJob) + 4u) * (uint64_t)b->max_jobs) == hipSuccess;
+  bool ok = hipMalloc(&b->d_jobs, (sizeof(BkJob) + 4u) * (uint64_t)b->max_jobs) == hipSuccess &&
+            hipMalloc(&b->d_bmax, entries) == hipSuccess;
   for (uint32_t l = 1; l < d->nlev && ok; l++)
     ok = hipMalloc(&b->d_lev[l], (uint64_t)b->dims[l][0] * b->dims[l][1] * b->dims[l][2]) == hipSuccess;
   if (!ok) {
@@ -243,6 +302,7 @@ extern "C" void r3d_hip_bricks_destroy(r3d_hip_bricks *b) {
   if (b->d_page) hipFree(b->d_page);
   if (b->d_atlas) hipFree(b->d_atlas);
   if (b->d_jobs) hipFree(b->d_jobs);
+  if (b->d_bmax) hipFree(b->d_bmax);
   for (uint32_t l = 0; l < R3D_MAX_LEVELS; l++)
     if (b->d_lev[l]) hipFree(b->d_lev[l]);
   delete b;
@@ -326,5 +386,374 @@ extern "C" int r3d_hip_bricks_build(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const u
   const hipError_t sync = hipStreamSynchronize(st);
   R3D_HIP_CHECK(ctx, err);
   R3D_HIP_CHECK(ctx, sync);
+  // the streamer's view of the table: the built residents in their slots
+  const uint32_t nent = b->page_words - 64u;
+  b->src0 = vol;
+  b->stream_ready = false;
+  b->stream_open = false;
+  b->frame = 0;
+  b->brick_slot.assign(nent, 0xFFFFFFFFu);
+  b->brick_want.assign(nent, 0u);
+  b->slot_brick.assign(b->nslots, 0xFFFFFFFFu);
+  b->slot_use.assign(b->nslots, 0u);
+  for (const BkJob &j : jobs) {
+    b->brick_slot[j.word - 64u] = j.slot;
+    b->slot_brick[j.slot] = j.word - 64u;
+  }
+  return R3D_OK;
+}
+
+// ---- streaming residency ----------------------------------------------------
+// The bricks pump of src/vk/vkbackend.c:5244-5599 (r3d_bricks_stream_begin,
+// _box, _point, _submit and the cone collect r3d_bricks_stream) over this
+// handle's page table. The warm tier is the pyramid of the last build, kept
+// in HBM (level 0 is the caller's volume, levels 1.. the handle's scratch),
+// so a requested brick is always available and is filled on the GPU at
+// submit, queue-ordered on the stream after the evictions it causes: no
+// worker thread and no drain. The policy is the reference's: per frame,
+// candidates are the requested bricks that are not resident (parent before
+// child, priority 0 before 1, then nearest first), bricks whose maximum is
+// below the gate (at least 5, the codec noise floor) never take a slot, the
+// coarsest level's resident bricks are kept, a free slot is taken first, else
+// the least recently wanted slot not wanted this frame is evicted, and a
+// submit fills at most min(budget, 32) bricks.
+
+namespace {
+
+constexpr uint32_t BR_INV = 0xFFFFFFFFu;
+constexpr uint32_t BR_MAX_BATCH = 32u;  // vkbackend.c:2349
+constexpr int BR_NOISE_FLOOR = 5;       // vkbackend.c:2348
+
+uint32_t base_maxdim(const r3d_hip_bricks *b) {
+  return std::max(b->desc.nx, std::max(b->desc.ny, b->desc.nz));
+}
+
+// Bricks below it never take a slot: max(gate * 255 rounded, the noise floor).
+int gate8(float gate) {
+  const int g8 = (int)(gate * 255.0f + 0.5f);
+  return g8 < BR_NOISE_FLOOR ? BR_NOISE_FLOOR : g8;
+}
+
+// r3d_lod_pick (src/core/lod.h:8-17).
+uint32_t lod_pick(float distance, float pixel_cone, float base_voxels, uint32_t nlevels) {
+  if (!nlevels) return 0;
+  const float footprint = fmaxf(distance * pixel_cone * base_voxels, 1.0f);
+  int level = (int)floorf(log2f(footprint));
+  if (level < 0) level = 0;
+  if (level >= (int)nlevels) level = (int)nlevels - 1;
+  return (uint32_t)level;
+}
+
+// Bricks along one axis whose centres lie within radius of eye.
+void axis_bounds(float eye, float radius, float edge, uint32_t n, uint32_t *lo, uint32_t *hi) {
+  int a = (int)ceilf((eye - radius) / edge - 0.5f);
+  int c = (int)floorf((eye + radius) / edge - 0.5f) + 1;
+  a = a < 0 ? 0 : (a > (int)n ? (int)n : a);
+  c = c < 0 ? 0 : (c > (int)n ? (int)n : c);
+  *lo = (uint32_t)a;
+  *hi = (uint32_t)c;
+}
+
+uint32_t entry_of(const r3d_hip_bricks *b, uint32_t l, uint32_t bx, uint32_t by, uint32_t bz) {
+  return b->off[l] - 64u + (bz * b->nb[l][1] + by) * b->nb[l][0] + bx;
+}
+
+// Keeps the coarsest level's resident bricks from eviction (their LRU stamps
+// follow the frame).
+void touch_coarsest(r3d_hip_bricks *b) {
+  const uint32_t l = b->desc.nlev - 1u, n = b->nb[l][0] * b->nb[l][1] * b->nb[l][2];
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t s = b->brick_slot[b->off[l] - 64u + i];
+    if (s != BR_INV) b->slot_use[s] = b->frame;
+  }
+}
+
+// A requested brick: resident ones are stamped as wanted, the others become
+// candidates once per frame.
+void candidate(r3d_hip_bricks *b, uint32_t e, float d2, uint32_t priority, int g8) {
+  if ((int)b->bmax[e] < g8) return;
+  const uint32_t s = b->brick_slot[e];
+  if (s != BR_INV) {
+    b->slot_use[s] = b->frame;
+    return;
+  }
+  if (b->brick_want[e] == b->frame) return;
+  b->brick_want[e] = b->frame;
+  b->cands.push_back({d2, e, priority});
+}
+
+// A free slot, else the least recently wanted one not wanted this frame.
+uint32_t pick_slot(const r3d_hip_bricks *b) {
+  uint32_t best = BR_INV, bu = 0xFFFFFFFFu;
+  for (uint32_t s = 0; s < b->nslots; s++) {
+    if (b->slot_brick[s] == BR_INV) return s;
+    if (b->slot_use[s] != b->frame && b->slot_use[s] < bu) {
+      bu = b->slot_use[s];
+      best = s;
+    }
+  }
+  return best;
+}
+
+void count_wanted(r3d_hip_bricks *b, uint32_t level) {
+  if (level < 8u) {
+    b->lod_wanted[level]++;
+    b->lod_requests[level]++;
+  }
+}
+
+// The per-brick maxima of the built pyramid (once per build).
+int stream_ready(r3d_hip_bricks *b) {
+  if (b->stream_ready) return R3D_OK;
+  r3d_hip_ctx *ctx = b->ctx;
+  if (!b->src0) return r3d_fail(ctx, R3D_ERR_INVALID, "bricks_stream: build the handle first");
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  BkMax a{};
+  a.lev[0] = b->src0;
+  a.nlev = b->desc.nlev;
+  for (uint32_t l = 0; l < a.nlev; l++) {
+    if (l) a.lev[l] = b->d_lev[l];
+    for (int k = 0; k < 3; k++) {
+      a.dims[l][k] = b->dims[l][k];
+      a.nb[l][k] = b->nb[l][k];
+    }
+    a.off[l] = b->off[l];
+  }
+  a.bmax = b->d_bmax;
+  const uint32_t nent = b->page_words - 64u;
+  hipLaunchKernelGGL(k_bk_bmax, dim3(nent), dim3(256), 0, 0, a);
+  R3D_HIP_CHECK(ctx, hipGetLastError());
+  b->bmax.assign(nent, 0);
+  R3D_HIP_CHECK(ctx, hipMemcpy(b->bmax.data(), b->d_bmax, nent, hipMemcpyDeviceToHost));
+  b->stream_ready = true;
+  return R3D_OK;
+}
+
+}  // namespace
+
+extern "C" int r3d_hip_bricks_stream_begin(r3d_hip_bricks *b) {
+  if (!b || !b->ctx) return 0;
+  b->stream_open = false;
+  b->last_filled = b->last_evicted = 0;
+  b->cands.clear();
+  if (stream_ready(b) != R3D_OK) return 0;
+  b->frame++;
+  memset(b->lod_wanted, 0, sizeof b->lod_wanted);
+  b->stream_open = true;
+  return 1;
+}
+
+extern "C" void r3d_hip_bricks_stream_box(r3d_hip_bricks *b, const float lo[3], const float hi[3], float pixel_cone,
+                                          float gate) {
+  if (!b || !b->stream_open || !lo || !hi) return;
+  const int g8 = gate8(gate);
+  const uint32_t nlev = b->desc.nlev;
+  const float maxdim = (float)base_maxdim(b);
+  const float vpp = fmaxf(pixel_cone * maxdim, 1.0f);  // base voxels per pixel
+  uint32_t desired = (uint32_t)floorf(log2f(vpp));
+  if (desired >= nlev) desired = nlev - 1u;
+  touch_coarsest(b);
+  uint32_t rng[3][2];
+  float edge;
+  for (;;) {  // coarser until the box holds at most 2,048 bricks
+    edge = (float)(BK << desired) / maxdim;
+    uint64_t count = 1;
+    for (int a = 0; a < 3; a++) {
+      int64_t a0 = (int64_t)floorf(lo[a] / edge), a1 = (int64_t)floorf(hi[a] / edge) + 1;
+      a0 = a0 < 0 ? 0 : a0;
+      a1 = a1 > (int64_t)b->nb[desired][a] ? (int64_t)b->nb[desired][a] : a1;
+      a1 = a1 < a0 ? a0 : a1;
+      rng[a][0] = (uint32_t)a0;
+      rng[a][1] = (uint32_t)a1;
+      count *= (uint64_t)(a1 - a0);
+    }
+    if (count <= 2048u || desired + 1u >= nlev) break;
+    desired++;
+  }
+  const float cx0 = (lo[0] + hi[0]) * 0.5f, cy0 = (lo[1] + hi[1]) * 0.5f, cz0 = (lo[2] + hi[2]) * 0.5f;
+  const bool parent = desired + 1u < nlev;
+  for (uint32_t bz = rng[2][0]; bz < rng[2][1]; bz++)
+    for (uint32_t by = rng[1][0]; by < rng[1][1]; by++)
+      for (uint32_t bx = rng[0][0]; bx < rng[0][1]; bx++) {
+        const float dx = ((float)bx + 0.5f) * edge - cx0;
+        const float dy = ((float)by + 0.5f) * edge - cy0;
+        const float dz = ((float)bz + 0.5f) * edge - cz0;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        count_wanted(b, desired);
+        if (parent) candidate(b, entry_of(b, desired + 1u, bx >> 1u, by >> 1u, bz >> 1u), d2, 0u, g8);
+        candidate(b, entry_of(b, desired, bx, by, bz), d2, 1u, g8);
+      }
+}
+
+extern "C" void r3d_hip_bricks_stream_point(r3d_hip_bricks *b, const float p[3], uint32_t level, float gate) {
+  if (!b || !b->stream_open || !p) return;
+  const int g8 = gate8(gate);
+  const uint32_t nlev = b->desc.nlev;
+  if (level >= nlev) level = nlev - 1u;
+  const float edge = (float)(BK << level) / (float)base_maxdim(b);
+  uint32_t bc[3];
+  for (int a = 0; a < 3; a++) {
+    int64_t i = (int64_t)(p[a] / edge);
+    i = i < 0 ? 0 : (i >= (int64_t)b->nb[level][a] ? (int64_t)b->nb[level][a] - 1 : i);
+    bc[a] = (uint32_t)i;
+  }
+  if (level + 1u < nlev) candidate(b, entry_of(b, level + 1u, bc[0] >> 1u, bc[1] >> 1u, bc[2] >> 1u), 0.0f, 0u, g8);
+  count_wanted(b, level);
+  candidate(b, entry_of(b, level, bc[0], bc[1], bc[2]), 0.0f, 1u, g8);
+}
+
+extern "C" int r3d_hip_bricks_stream_submit(r3d_hip_ctx *ctx, r3d_hip_bricks *b, uint32_t budget, void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!b || b->ctx != ctx) return r3d_fail(ctx, R3D_ERR_INVALID, "bricks_stream_submit: bad handle");
+  if (!b->stream_open) return R3D_OK;
+  b->stream_open = false;
+  if (b->cands.empty() || budget == 0) {
+    b->cands.clear();
+    return R3D_OK;
+  }
+  if (budget > BR_MAX_BATCH) budget = BR_MAX_BATCH;
+  // parent (priority 0) before child, then nearest first; equal keys keep
+  // their collect order
+  std::stable_sort(b->cands.begin(), b->cands.end(), [](const r3d_hip_bricks::Cand &x, const r3d_hip_bricks::Cand &y) {
+    return x.priority != y.priority ? x.priority < y.priority : x.d2 < y.d2;
+  });
+  // staged for the device: [evicted words][jobs]
+  uint32_t evict[BR_MAX_BATCH];
+  BkJob jobs[BR_MAX_BATCH];
+  uint32_t n = 0, nevict = 0;
+  for (size_t k = 0; k < b->cands.size() && n < budget; k++) {
+    const uint32_t e = b->cands[k].b;
+    if (b->brick_slot[e] != BR_INV) continue;  // requested by two collects
+    const uint32_t s = pick_slot(b);
+    if (s == BR_INV) break;  // every slot is wanted this frame
+    const uint32_t old = b->slot_brick[s];
+    if (old != BR_INV) {
+      b->brick_slot[old] = BR_INV;
+      evict[nevict++] = old + 64u;
+    }
+    b->slot_brick[s] = e;
+    b->slot_use[s] = b->frame;
+    b->brick_slot[e] = s;
+    uint32_t l = 0;
+    while (l + 1u < b->desc.nlev && e + 64u >= b->off[l + 1u]) l++;
+    const uint32_t k2 = e + 64u - b->off[l];
+    jobs[n++] = BkJob{l, k2 % b->nb[l][0], (k2 / b->nb[l][0]) % b->nb[l][1], k2 / (b->nb[l][0] * b->nb[l][1]),
+                      e + 64u, s};
+  }
+  b->cands.clear();
+  if (!n) return R3D_OK;
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = (hipStream_t)stream;
+  uint8_t buf[sizeof evict + sizeof jobs];
+  memcpy(buf, evict, sizeof evict);
+  memcpy(buf + sizeof evict, jobs, sizeof(BkJob) * n);
+  ViewSlot *slot = nullptr;
+  int rc = stage(ctx, buf, sizeof evict + sizeof(BkJob) * n, st, &slot);
+  if (rc != R3D_OK) return rc;
+  const uint32_t *d_evict = (const uint32_t *)slot->dev;
+  const BkJob *d_jobs = (const BkJob *)((const uint8_t *)slot->dev + sizeof evict);
+  if (nevict) hipLaunchKernelGGL(k_bk_evict, dim3(1), dim3(256), 0, st, d_evict, nevict, b->d_page);
+  BkFill f{};
+  f.lev[0] = b->src0;
+  for (uint32_t l = 0; l < b->desc.nlev; l++) {
+    if (l) f.lev[l] = b->d_lev[l];
+    for (int a = 0; a < 3; a++) f.dims[l][a] = b->dims[l][a];
+  }
+  f.jobs = d_jobs;
+  f.njobs = n;
+  f.atlas = b->d_atlas;
+  f.page = b->d_page;
+  f.maxv = (uint32_t *)((BkJob *)b->d_jobs + b->max_jobs);
+  hipLaunchKernelGGL(k_bk_zero, dim3(1), dim3(256), 0, st, f.maxv, n);
+  hipLaunchKernelGGL(k_bk_fill, dim3(n * BK_PARTS), dim3(256), 0, st, f);
+  hipLaunchKernelGGL(k_bk_entries, dim3(1), dim3(256), 0, st, d_jobs, n, (const uint32_t *)f.maxv, f.page);
+  R3D_HIP_CHECK(ctx, hipGetLastError());
+  R3D_HIP_CHECK(ctx, hipEventRecord(slot->ev, st));
+  slot->recorded = true;
+  b->last_filled = n;
+  b->last_evicted = nevict;
+  b->filled += n;
+  b->evicted += nevict;
+  b->submits++;
+  return R3D_OK;
+}
+
+extern "C" int r3d_hip_bricks_stream(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const float eye[3], const float fwd[3],
+                                     float half_tan, float pixel_cone, uint32_t slice_z0, uint32_t slice_depth,
+                                     float gate, uint32_t budget, void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!b || b->ctx != ctx || !eye || !fwd) return r3d_fail(ctx, R3D_ERR_INVALID, "bricks_stream: bad argument");
+  if (!r3d_hip_bricks_stream_begin(b)) return b->src0 ? R3D_ERR_DEVICE : r3d_fail(ctx, R3D_ERR_INVALID,
+                                                                                  "bricks_stream: build first");
+  if (budget == 0) {
+    b->stream_open = false;
+    return R3D_OK;
+  }
+  const float tanw = half_tan * 1.15f + 1e-3f;  // the cone, slightly widened
+  const int g8 = gate8(gate);
+  const uint32_t nlev = b->desc.nlev;
+  touch_coarsest(b);
+  const float maxdim = (float)base_maxdim(b);
+  const float lod_factor = fmaxf(pixel_cone * maxdim, 1e-6f);
+  // level li is wanted only inside its distance shell: walk that shell's box
+  // (the coarsest level is resident and kept)
+  for (uint32_t li = 0; li + 1u < nlev; li++) {
+    const float edge = (float)(BK << li) / maxdim;
+    const float brad = 0.8660254f * edge;
+    const float outer = exp2f((float)li + 1.0f) / lod_factor + brad;
+    uint32_t x0, x1, y0, y1, z0, z1;
+    axis_bounds(eye[0], outer, edge, b->nb[li][0], &x0, &x1);
+    axis_bounds(eye[1], outer, edge, b->nb[li][1], &y0, &y1);
+    axis_bounds(eye[2], outer, edge, b->nb[li][2], &z0, &z1);
+    if (slice_depth) {
+      uint64_t zlast = (uint64_t)slice_z0 + slice_depth;
+      if (zlast > b->desc.nz) zlast = b->desc.nz;
+      const uint32_t zb = BK << li;
+      const uint32_t sz0 = slice_z0 / zb, sz1 = (uint32_t)((zlast + zb - 1u) / zb);
+      z0 = z0 < sz0 ? sz0 : z0;
+      z1 = z1 > sz1 ? sz1 : z1;
+    }
+    for (uint32_t bz = z0; bz < z1; bz++)
+      for (uint32_t by = y0; by < y1; by++)
+        for (uint32_t bx = x0; bx < x1; bx++) {
+          const float cx = ((float)bx + 0.5f) * edge - eye[0];
+          const float cy = ((float)by + 0.5f) * edge - eye[1];
+          const float cz = ((float)bz + 0.5f) * edge - eye[2];
+          const float d2 = cx * cx + cy * cy + cz * cz;
+          const float along = cx * fwd[0] + cy * fwd[1] + cz * fwd[2];
+          bool vis = d2 < (edge + brad) * (edge + brad);
+          if (!vis && along > 0.0f) {
+            const float perp2 = d2 - along * along;
+            const float rad = along * tanw + brad;
+            vis = perp2 < rad * rad;
+          }
+          if (!vis) continue;
+          const uint32_t desired = lod_pick(sqrtf(d2), pixel_cone, maxdim, nlev);
+          if (li != desired) continue;
+          count_wanted(b, desired);
+          // the parent first: a stable, spatially matching fallback while refining
+          candidate(b, entry_of(b, li + 1u, bx >> 1u, by >> 1u, bz >> 1u), d2, 0u, g8);
+          candidate(b, entry_of(b, li, bx, by, bz), d2, 1u, g8);
+        }
+  }
+  return r3d_hip_bricks_stream_submit(ctx, b, budget, stream);
+}
+
+extern "C" int r3d_hip_bricks_stream_stats(const r3d_hip_bricks *b, r3d_bricks_stream_stats *st) {
+  if (!b || !st) return R3D_ERR_INVALID;
+  memset(st, 0, sizeof *st);
+  st->nb = b->page_words - 64u;
+  st->hot_cap = b->nslots;
+  for (uint32_t s = 0; s < b->nslots && s < b->slot_brick.size(); s++) st->hot += b->slot_brick[s] != BR_INV;
+  st->inflight = b->last_filled;
+  st->evicted_last = b->last_evicted;
+  st->decoded = b->filled;
+  st->evicted = b->evicted;
+  st->jobs = b->submits;
+  st->nlevels = b->desc.nlev;
+  for (int l = 0; l < 8; l++) {
+    st->lod_wanted[l] = b->lod_wanted[l];
+    st->lod_requests[l] = b->lod_requests[l];
+  }
   return R3D_OK;
 }

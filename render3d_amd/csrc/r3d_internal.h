@@ -6,6 +6,8 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <vector>
+
 #include "r3d_hip.h"
 
 #define R3D_BRICK 16u
@@ -102,6 +104,25 @@ struct r3d_hip_bricks {
   uint8_t *d_lev[R3D_MAX_LEVELS] = {};
   void *d_jobs = nullptr;
   uint32_t max_jobs = 0;
+  // streaming (r3d_hip_bricks_stream_*): the source pyramid of the last build
+  // (level 0 is the caller's volume) and the residency policy's host state
+  const uint8_t *src0 = nullptr;
+  bool stream_ready = false;             // bmax computed for the current build
+  uint8_t *d_bmax = nullptr;             // per page entry: the brick's largest voxel
+  std::vector<uint8_t> bmax;
+  std::vector<uint32_t> brick_slot;      // per page entry: its slot, or ~0
+  std::vector<uint32_t> brick_want;      // per page entry: the frame it was last requested in
+  std::vector<uint32_t> slot_brick;      // per slot: its page entry (index after the header), or ~0
+  std::vector<uint32_t> slot_use;        // per slot: the frame it was last wanted in
+  struct Cand {
+    float d2;
+    uint32_t b, priority;
+  };
+  std::vector<Cand> cands;
+  uint32_t frame = 0;
+  bool stream_open = false;
+  uint32_t lod_wanted[8] = {}, last_filled = 0, last_evicted = 0;
+  uint64_t lod_requests[8] = {}, filled = 0, evicted = 0, submits = 0;
 };
 
 // error helpers
