@@ -280,6 +280,13 @@ typedef struct r3d_hip_stats {
   /* with R3D_RAYCAST_MEASURE only (0 otherwise): */
   uint64_t blended;        /* samples that blended two mip levels (two trilinear footprints) */
   uint64_t blended_shaded; /* shaded samples among them (their taps blend too) */
+  /* r3d_frame_stats (render_types.h:107-119): gpu_raycast_ns is the GPU time
+   * of this call's launches (HIP events on the call's stream), panes_drawn
+   * the views it rendered (pane-cache misses). The counters above cover the
+   * drawn views only. */
+  uint64_t gpu_raycast_ns;
+  uint32_t panes_drawn;
+  uint32_t reserved;
 } r3d_hip_stats;
 
 #define R3D_MAX_VIEWS_PER_CALL 256u
@@ -289,6 +296,17 @@ typedef struct r3d_hip_stats {
  * rays, blended, blended_shaded, as r3d_hip_stats) and touched, when given,
  * gets the bricks the footprints read. Not for timing. */
 #define R3D_RAYCAST_MEASURE 1u
+/* flags: the pane cache (vkbackend.c:7066-7079, 7144-7159). A view is drawn
+ * only when its parameters (frame_index aside: a cached view keeps its last
+ * jitter), the layout of all views (view_org, viewport), the outputs, the
+ * options or the scene changed since view index v was last drawn with this
+ * flag; the others keep the pixels already in the output. The scene changes
+ * with every volume build or decode, bricks build or streamed fill, transfer
+ * function, quality or tuning set; call r3d_hip_invalidate_panes after writes
+ * the library does not see (the contents of opts->tf, the bricks tables
+ * written directly, the output written by anything else). Any render without
+ * the flag (raycast, plan launch, diff forward) makes every view redraw. */
+#define R3D_RAYCAST_PANE_CACHE 2u
 typedef struct r3d_raycast_opts {
   float alpha_cutoff;     /* early termination: raycast.slang:124 uses 0.98 */
   uint32_t flags;
@@ -320,6 +338,10 @@ int r3d_hip_raycast(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_frame
  * src/vk/vkbackend.c:2075-2077 does. The differentiable path always uses six
  * taps. */
 void r3d_hip_set_quality(r3d_hip_ctx *ctx, uint32_t quality);
+/* The reference bumps its scene generation where its own data changes
+ * (vkbackend.c, scene_gen++); here data the caller writes directly must be
+ * announced: every pane-cached view redraws on its next call. */
+void r3d_hip_invalidate_panes(r3d_hip_ctx *ctx);
 
 /* Launch shape of the raycast kernel: the workgroup renders tile_w x
  * tile_h-pixel tiles (8x8, 16x4, 16x8 or 16x16); blocks_per_cu > 0 makes

@@ -113,10 +113,12 @@ class Image(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("skips", C.c_uint64), ("shaded", C.c_uint64), ("rays", C.c_uint64),
-                ("blended", C.c_uint64), ("blended_shaded", C.c_uint64)]
+                ("blended", C.c_uint64), ("blended_shaded", C.c_uint64), ("gpu_raycast_ns", C.c_uint64),
+                ("panes_drawn", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 RAYCAST_MEASURE = 1
+RAYCAST_PANE_CACHE = 2
 
 
 class RaycastOpts(C.Structure):
@@ -206,6 +208,7 @@ SIGNATURES = {
                                   C.POINTER(RaycastOpts), C.POINTER(Stats), P]),
     "r3d_hip_set_raycast_tuning": (C.c_int, [P, C.POINTER(RaycastTuning)]),
     "r3d_hip_set_quality": (None, [P, C.c_uint32]),
+    "r3d_hip_invalidate_panes": (None, [P]),
     "r3d_hip_raycast_plan_create": (C.c_int, [P, P, C.POINTER(FrameParams), C.c_uint32, C.POINTER(Image),
                                               C.POINTER(RaycastOpts), C.POINTER(P)]),
     "r3d_hip_raycast_plan_launch": (C.c_int, [P, P]),

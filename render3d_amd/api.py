@@ -279,28 +279,38 @@ class Context:
         return Volume(self, nx, ny, nz, nlevels)
 
     def raycast(self, vol, views, width, height, *, alpha_cutoff=0.98, rgba=True, rgba8=False, tf=None,
-                counters=None, px_steps=False, stats=False, out=None, stream=None, measure=False, touched=None):
+                counters=None, px_steps=False, stats=False, out=None, stream=None, measure=False, touched=None,
+                pane_cache=False, out8=None):
         """Render views into a [height, width, 4] float32 image (and/or RGBA8).
         measure=True runs the measurement instances (R3D_RAYCAST_MEASURE):
         stats and counters then include the blended-sample counts, and
         touched (an int32 device bitmap of vol.brick_count bits, zeroed)
-        receives the bricks the footprints read."""
+        receives the bricks the footprints read. pane_cache=True
+        (R3D_RAYCAST_PANE_CACHE) draws only the views whose inputs changed
+        since they were last drawn into the same out / out8; stats then
+        count the drawn views (panes_drawn)."""
         return self._raycast(self.lib.r3d_hip_raycast, vol.h, views, width, height, alpha_cutoff, rgba, rgba8, tf,
-                             counters, px_steps, stats, out, stream, measure, touched)
+                             counters, px_steps, stats, out, stream, measure, touched, pane_cache, out8)
+
+    def invalidate_panes(self):
+        """r3d_hip_invalidate_panes: every pane-cached view redraws next time."""
+        self.lib.r3d_hip_invalidate_panes(self.h)
 
     def _raycast(self, fn, handle, views, width, height, alpha_cutoff, rgba, rgba8, tf, counters, px_steps, stats,
-                 out, stream, measure=False, touched=None):
+                 out, stream, measure=False, touched=None, pane_cache=False, out8=None):
         if isinstance(views, FrameParams):
             views = [views]
         arr = (FrameParams * len(views))(*views)
         dev = f"cuda:{self.device}"
         img = out if out is not None else (torch.empty((height, width, 4), dtype=torch.float32, device=dev)
                                           if rgba else None)
-        img8 = torch.zeros((height, width, 4), dtype=torch.uint8, device=dev) if rgba8 else None
+        img8 = out8 if out8 is not None else (torch.zeros((height, width, 4), dtype=torch.uint8, device=dev)
+                                              if rgba8 else None)
         steps = torch.zeros((height, width), dtype=torch.int32, device=dev) if px_steps else None
         im = Image(img.data_ptr() if img is not None else None, img8.data_ptr() if img8 is not None else None,
                    width, height)
-        opts = RaycastOpts(alpha_cutoff, _lib.RAYCAST_MEASURE if measure else 0, None if tf is None else tf.data_ptr(),
+        flags = (_lib.RAYCAST_MEASURE if measure else 0) | (_lib.RAYCAST_PANE_CACHE if pane_cache else 0)
+        opts = RaycastOpts(alpha_cutoff, flags, None if tf is None else tf.data_ptr(),
                            None if counters is None else counters.data_ptr(),
                            None if steps is None else steps.data_ptr(),
                            None if touched is None else touched.data_ptr())
@@ -309,7 +319,8 @@ class Context:
                       _stream(stream)))
         res = {"rgba": img, "rgba8": img8, "px_steps": steps}
         if stats:
-            res["stats"] = {"samples": st.samples, "skips": st.skips, "shaded": st.shaded, "rays": st.rays}
+            res["stats"] = {"samples": st.samples, "skips": st.skips, "shaded": st.shaded, "rays": st.rays,
+                            "gpu_raycast_ns": st.gpu_raycast_ns, "panes_drawn": st.panes_drawn}
             if measure:
                 res["stats"].update(blended=st.blended, blended_shaded=st.blended_shaded)
         return res
@@ -595,10 +606,11 @@ class Bricks:
                 "jobs": st.jobs}
 
     def raycast(self, views, width, height, *, alpha_cutoff=0.98, rgba=True, rgba8=False, tf=None, counters=None,
-                px_steps=False, stats=False, out=None, stream=None):
+                px_steps=False, stats=False, out=None, stream=None, pane_cache=False, out8=None):
         """Render bricks-LOD views (brick_mode = self.brick_mode); as Context.raycast."""
         return self.ctx._raycast(self.ctx.lib.r3d_hip_bricks_raycast, self.h, views, width, height, alpha_cutoff,
-                                 rgba, rgba8, tf, counters, px_steps, stats, out, stream)
+                                 rgba, rgba8, tf, counters, px_steps, stats, out, stream, pane_cache=pane_cache,
+                                 out8=out8)
 
 
 _hip = None

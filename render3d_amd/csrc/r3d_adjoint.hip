@@ -552,6 +552,7 @@ extern "C" int r3d_hip_raycast_diff(r3d_hip_ctx *ctx, const r3d_hip_volume *vol,
   Plan plan;
   int rc = validate_diff(ctx, vol, views, nviews, dp, &plan, false);
   if (rc != R3D_OK) return rc;
+  r3d_panes_reset(ctx);
   if (!out || !out->rgba || out->width == 0 || out->height == 0)
     return r3d_fail(ctx, R3D_ERR_INVALID, "diff: FP32 output image required");
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));

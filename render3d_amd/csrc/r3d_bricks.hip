@@ -384,6 +384,7 @@ extern "C" int r3d_hip_bricks_build(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const u
     }
   }
   const hipError_t sync = hipStreamSynchronize(st);
+  r3d_scene_changed(ctx);
   R3D_HIP_CHECK(ctx, err);
   R3D_HIP_CHECK(ctx, sync);
   // the streamer's view of the table: the built residents in their slots
@@ -642,6 +643,7 @@ extern "C" int r3d_hip_bricks_stream_submit(r3d_hip_ctx *ctx, r3d_hip_bricks *b,
   }
   b->cands.clear();
   if (!n) return R3D_OK;
+  r3d_scene_changed(ctx);  // residency changes what bricks-LOD views render
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   hipStream_t st = (hipStream_t)stream;
   uint8_t buf[sizeof evict + sizeof jobs];

@@ -52,6 +52,8 @@ extern "C" int r3d_hip_init(r3d_hip_ctx **out, int device) {
   if (rc == R3D_OK) rc = r3d::upload_blob_tables(c);
   if (rc == R3D_OK) rc = r3d::upload_crc_tables(c);
   if (rc == R3D_OK) rc = r3d::parse_noalloc_setup(c);
+  if (rc == R3D_OK && (hipEventCreate(&c->ev_ray[0]) != hipSuccess || hipEventCreate(&c->ev_ray[1]) != hipSuccess))
+    rc = R3D_ERR_DEVICE;
   if (rc == R3D_OK) {
     // default transfer function: the grayscale identity ramp (r3d_tf_preset 0,
     // src/core/transfer.c:28-32), as the reference uploads before any TF is set
@@ -79,6 +81,8 @@ extern "C" void r3d_hip_destroy(r3d_hip_ctx *c) {
     if (s.host) hipHostFree(s.host);
     if (s.dev) hipFree(s.dev);
   }
+  for (hipEvent_t e : c->ev_ray)
+    if (e) hipEventDestroy(e);
   if (c->d_tf) hipFree(c->d_tf);
   if (c->d_counters) hipFree(c->d_counters);
   if (c->d_queue) hipFree(c->d_queue);
@@ -113,6 +117,7 @@ extern "C" int r3d_hip_set_transfer(r3d_hip_ctx *ctx, const uint8_t rgba[256][4]
     for (int ch = 0; ch < 4; ch++) t[i * 4 + ch] = (float)rgba[i][ch] * (1.0f / 255.0f);
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   R3D_HIP_CHECK(ctx, hipMemcpy(ctx->d_tf, t, sizeof t, hipMemcpyHostToDevice));
+  r3d_scene_changed(ctx);
   return R3D_OK;
 }
 
@@ -122,5 +127,12 @@ extern "C" int r3d_hip_set_transfer_f32(r3d_hip_ctx *ctx, const float *rgba) {
     if (!isfinite(rgba[i])) return r3d_fail(ctx, R3D_ERR_INVALID, "transfer entry %d is not finite", i);
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   R3D_HIP_CHECK(ctx, hipMemcpy(ctx->d_tf, rgba, 256 * 4 * sizeof(float), hipMemcpyHostToDevice));
+  r3d_scene_changed(ctx);
   return R3D_OK;
+}
+
+// For writes the library cannot see (a caller's LUT passed as opts->tf, the
+// bricks tables written directly): every pane-cached view redraws.
+extern "C" void r3d_hip_invalidate_panes(r3d_hip_ctx *ctx) {
+  if (ctx) r3d_scene_changed(ctx);
 }

@@ -71,7 +71,20 @@ struct r3d_hip_ctx {
   void *retired[48] = {};
   uint32_t nretired = 0;
   void *d_noalloc = nullptr;  // this device's fixed parse scratch (r3d_hip_parse_blobs_noalloc)
+  // pane cache (R3D_RAYCAST_PANE_CACHE): the scene generation, bumped by every
+  // call that changes what a view renders, and the key each view index was
+  // last drawn under (0: draw)
+  uint64_t scene_gen = 1;
+  uint64_t pane_key[R3D_MAX_VIEWS_PER_CALL] = {};
+  hipEvent_t ev_ray[2] = {nullptr, nullptr};  // the launches' GPU time, for stats
 };
+
+// What a view renders changed (data, transfer function, quality, launch shape).
+inline void r3d_scene_changed(r3d_hip_ctx *c) { c->scene_gen++; }
+// Outputs were written outside the pane cache: every cached view redraws.
+inline void r3d_panes_reset(r3d_hip_ctx *c) {
+  for (uint32_t i = 0; i < R3D_MAX_VIEWS_PER_CALL; i++) c->pane_key[i] = 0;
+}
 
 struct r3d_hip_volume {
   uint8_t *d_flat = nullptr;  // staging for r3d_hip_volume_decode: nchunks x 4096

@@ -1216,6 +1216,49 @@ struct r3d_hip_raycast_plan {
 };
 
 // r3d_hip_raycast and r3d_hip_bricks_raycast: one of vol / bk is given.
+// FNV-1a over bytes, continuing from k.
+static uint64_t fnv1a(uint64_t k, const void *p, size_t n) {
+  const uint8_t *b = (const uint8_t *)p;
+  for (size_t i = 0; i < n; i++) k = (k ^ b[i]) * 1099511628211ull;
+  return k;
+}
+
+// The pane cache (vkbackend.c:7066-7079, 7144-7159): the key of view v is
+// every parameter byte but frame_index (a cached view keeps its last jitter),
+// the scene generation, the handle, the outputs, the options and the layout
+// of all views (a layout change redraws every view). Returns how many views
+// changed; their indices go to draw[], their keys to keys[].
+static uint32_t pane_select(const r3d_hip_ctx *ctx, const void *handle, const r3d_frame_params *views,
+                            uint32_t nviews, const r3d_hip_image *out, const r3d_raycast_opts *opts,
+                            uint32_t *draw, uint64_t *keys) {
+  uint64_t lay = 1469598103934665603ull ^ nviews;
+  for (uint32_t v = 0; v < nviews; v++) {
+    lay = fnv1a(lay, &views[v].view_org, sizeof views[v].view_org);
+    lay = fnv1a(lay, views[v].viewport, sizeof views[v].viewport);
+  }
+  struct {
+    uint64_t gen, lay;
+    const void *handle, *rgba, *rgba8, *tf, *px_steps, *touched;
+    uint32_t width, height, flags;
+    float alpha_cutoff;
+  } common{ctx->scene_gen, lay, handle, out->rgba, out->rgba8, opts ? opts->tf : nullptr,
+           opts ? opts->px_steps : nullptr, opts ? opts->touched : nullptr, out->width, out->height,
+           opts ? opts->flags : 0u, opts ? opts->alpha_cutoff : 0.98f};
+  const uint64_t k0 = fnv1a(1469598103934665603ull, &common, sizeof common);
+  uint32_t n = 0;
+  for (uint32_t v = 0; v < nviews; v++) {
+    r3d_frame_params hp = views[v];
+    hp.frame_index = 0;
+    uint64_t k = fnv1a(k0, &hp, sizeof hp);
+    k = (k ^ v) * 1099511628211ull;
+    if (k == 0) k = 1;
+    if (ctx->pane_key[v] == k) continue;  // unchanged: the output keeps its pixels
+    draw[n] = v;
+    keys[n++] = k;
+  }
+  return n;
+}
+
 static int raycast_impl(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_hip_bricks *bk,
                         const r3d_frame_params *views, uint32_t nviews, const r3d_hip_image *out,
                         const r3d_raycast_opts *opts, r3d_hip_stats *stats, void *stream) {
@@ -1223,6 +1266,25 @@ static int raycast_impl(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_h
   if (rc != R3D_OK) return rc;
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   hipStream_t s = (hipStream_t)stream;
+  if (stats) *stats = r3d_hip_stats{};
+  // the pane cache: only views whose inputs changed are drawn
+  const bool cache = opts && (opts->flags & R3D_RAYCAST_PANE_CACHE);
+  uint32_t draw[R3D_MAX_VIEWS_PER_CALL];
+  uint64_t keys[R3D_MAX_VIEWS_PER_CALL];
+  std::vector<r3d_frame_params> sel;
+  if (cache) {
+    const uint32_t n = pane_select(ctx, bk ? (const void *)bk : (const void *)vol, views, nviews, out, opts, draw,
+                                   keys);
+    if (n == 0) return R3D_OK;
+    if (n < nviews) {
+      sel.resize(n);
+      for (uint32_t i = 0; i < n; i++) sel[i] = views[draw[i]];
+      views = sel.data();
+      nviews = n;
+    }
+  } else {
+    r3d_panes_reset(ctx);  // this call may overwrite what cached views left
+  }
   uint8_t buf[sizeof(r3d_frame_params) * R3D_MAX_VIEWS_PER_CALL +
               sizeof(uint32_t) * (R3D_MAX_VIEWS_PER_CALL + NKINDS) * NSHAPES];
   ViewGroups g;
@@ -1259,21 +1321,29 @@ static int raycast_impl(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_h
   if (measure && bk) return r3d_fail(ctx, R3D_ERR_UNSUPPORTED, "bricks_raycast: no measurement mode");
   if (a.touched && !measure)
     return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: a touched-brick bitmap needs R3D_RAYCAST_MEASURE");
+  if (stats) R3D_HIP_CHECK(ctx, hipEventRecord(ctx->ev_ray[0], s));
   rc = launch_groups(ctx, a, dviews, dtables, g, tuning, ctx->d_partials, ctx->d_queue, counters, ncount, s,
                      bk != nullptr);
   if (rc != R3D_OK) return rc;
+  if (stats) R3D_HIP_CHECK(ctx, hipEventRecord(ctx->ev_ray[1], s));
   R3D_HIP_CHECK(ctx, hipEventRecord(slot->ev, s));
   slot->recorded = true;
+  if (cache)
+    for (uint32_t i = 0; i < nviews; i++) ctx->pane_key[draw[i]] = keys[i];
   if (stats) {
     unsigned long long h[6];
     R3D_HIP_CHECK(ctx, hipMemcpyAsync(h, counters, sizeof h, hipMemcpyDeviceToHost, s));
     R3D_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    float ms = 0.0f;
+    R3D_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_ray[0], ctx->ev_ray[1]));
     stats->samples = h[0];
     stats->skips = h[1];
     stats->shaded = h[2];
     stats->rays = h[3];
     stats->blended = h[4];
     stats->blended_shaded = h[5];
+    stats->gpu_raycast_ns = (uint64_t)((double)ms * 1e6 + 0.5);
+    stats->panes_drawn = nviews;
   }
   return R3D_OK;
 }
@@ -1297,7 +1367,9 @@ extern "C" int r3d_hip_bricks_raycast(r3d_hip_ctx *ctx, const r3d_hip_bricks *b,
 // r3d_set_quality (src/render/render.h:241-242; src/vk/vkbackend.c:2075-2077):
 // an unknown quality selects FULL.
 extern "C" void r3d_hip_set_quality(r3d_hip_ctx *ctx, uint32_t quality) {
-  if (ctx) ctx->quality = quality == R3D_QUALITY_FAST ? R3D_QUALITY_FAST : R3D_QUALITY_FULL;
+  if (!ctx) return;
+  ctx->quality = quality == R3D_QUALITY_FAST ? R3D_QUALITY_FAST : R3D_QUALITY_FULL;
+  r3d_scene_changed(ctx);
 }
 
 extern "C" int r3d_hip_set_raycast_tuning(r3d_hip_ctx *ctx, const r3d_raycast_tuning *t) {
@@ -1305,6 +1377,7 @@ extern "C" int r3d_hip_set_raycast_tuning(r3d_hip_ctx *ctx, const r3d_raycast_tu
   const int rc = check_tuning(ctx, t);
   if (rc != R3D_OK) return rc;
   ctx->tuning = *t;
+  r3d_scene_changed(ctx);
   return R3D_OK;
 }
 
@@ -1375,6 +1448,7 @@ extern "C" int r3d_hip_raycast_plan_launch(r3d_hip_raycast_plan *p, void *stream
   if (!p) return R3D_ERR_INVALID;
   r3d_hip_ctx *ctx = p->ctx;
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  r3d_panes_reset(ctx);
   return launch_groups(ctx, base_args(ctx, p->vol, &p->out, &p->opts), p->d_views, p->d_tables, p->g, p->tuning,
                        p->d_partials, p->d_queue, (unsigned long long *)p->opts.counters,
                        4u, (hipStream_t)stream);

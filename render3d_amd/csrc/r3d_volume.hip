@@ -658,6 +658,7 @@ static dim3 brick_grid(const r3d_hip_volume *v, uint32_t l) { return dim3(nbrick
 // Level 1 (from level 0, or only its apron when the level-0 pass already wrote
 // its payload), levels 2.., and the occupancy dilation.
 static int build_tail(r3d_hip_ctx *ctx, r3d_hip_volume *v, hipStream_t s, bool l1_payload_done) {
+  r3d_scene_changed(ctx);  // every build ends here
   if (v->nlev > 1) {
     if (l1_payload_done)
       hipLaunchKernelGGL(k_apron, brick_grid(v, 1), dim3(256), 0, s, geom(v, 1));
