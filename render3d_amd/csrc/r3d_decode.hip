@@ -133,47 +133,6 @@ constexpr bool basis_neg(int k, int n) {
   return (((2 * n + 1) * k) & 63) > 16 && (((2 * n + 1) * k) & 63) < 48;
 }
 
-// dct3d.c:61-68 on a register-resident line. Every k term is added: a zero
-// input contributes an exact +-0, so the sums equal the reference's, which
-// skips zero inputs (the only possible difference is the sign of an exact
-// zero, which the + dc that follows removes).
-__device__ __forceinline__ void idct16(const float in[16], float out[16]) {
-  // Each sum starts from its first term instead of 0 + term: that differs from
-  // the reference (which starts at +0 and skips zero inputs) only in the sign
-  // of an exact zero, which the + dc after the last pass removes.
-  float e[8], o[8];
-#pragma unroll
-  for (int n = 0; n < 8; n++) {
-    const float p = c_mag[basis_j(0, n)] * in[0];
-    e[n] = basis_neg(0, n) ? -p : p;
-    const float q = c_mag[basis_j(1, n)] * in[1];
-    o[n] = basis_neg(1, n) ? -q : q;
-  }
-#pragma unroll
-  for (int k = 2; k < 16; k += 2) {
-    const float v = in[k];
-#pragma unroll
-    for (int n = 0; n < 8; n++) {
-      const float p = c_mag[basis_j(k, n)] * v;
-      e[n] = basis_neg(k, n) ? e[n] - p : e[n] + p;
-    }
-  }
-#pragma unroll
-  for (int k = 3; k < 16; k += 2) {
-    const float v = in[k];
-#pragma unroll
-    for (int n = 0; n < 8; n++) {
-      const float p = c_mag[basis_j(k, n)] * v;
-      o[n] = basis_neg(k, n) ? o[n] - p : o[n] + p;
-    }
-  }
-#pragma unroll
-  for (int n = 0; n < 8; n++) {
-    out[n] = e[n] + o[n];
-    out[15 - n] = e[n] - o[n];
-  }
-}
-
 // dct3d.c:52-59 (forward, for the workload encoder).
 __device__ __forceinline__ void fdct16(const float in[16], float out[16]) {
   float s[8], d[8];
@@ -250,36 +209,75 @@ __device__ __forceinline__ void store_row16(uint8_t *dst, const uint32_t w[4]) {
   }
 }
 
-// Three line passes with the transposing LDS write between them. On entry v
-// holds line t of the dequantised coefficients; on exit tile holds norm in
-// natural (z*16+y)*16+x order.
-__device__ __forceinline__ void idct3_tile(float v[16], float *tile, uint32_t t, float dc) {
-  float o[16];
-  idct16(v, o);
+// dct3d.c:61-68 on a register-resident line, over the inputs k <= K only (K
+// uniform across the workgroup; a compile-time 15 for dense chunks): the
+// inputs past K are zero, so their terms are exact +-0 that the reference
+// skips, and here they may hold anything. Every term up to K is added (a
+// zero input contributes an exact +-0 too), each sum in ascending k, and each
+// sum starts from its first term instead of 0 + term: both differ from the
+// reference only in the sign of an exact zero, which the + dc after the last
+// pass removes.
+__device__ __forceinline__ void idct16k(const float in[16], float out[16], uint32_t K) {
+  float e[8], o[8];
 #pragma unroll
-  for (int k = 0; k < 16; k++) tile[k * 256 + t] = o[k];
-  __syncthreads();
+  for (int n = 0; n < 8; n++) {
+    const float p = c_mag[basis_j(0, n)] * in[0];
+    e[n] = basis_neg(0, n) ? -p : p;
+  }
+  if (K >= 1u) {
 #pragma unroll
-  for (int k = 0; k < 16; k++) v[k] = tile[t * 16 + k];
-  __syncthreads();
-  idct16(v, o);
+    for (int n = 0; n < 8; n++) {
+      const float q = c_mag[basis_j(1, n)] * in[1];
+      o[n] = basis_neg(1, n) ? -q : q;
+    }
+  } else {
 #pragma unroll
-  for (int k = 0; k < 16; k++) tile[k * 256 + t] = o[k];
-  __syncthreads();
+    for (int n = 0; n < 8; n++) o[n] = 0.0f;
+  }
 #pragma unroll
-  for (int k = 0; k < 16; k++) v[k] = tile[t * 16 + k];
-  __syncthreads();
-  idct16(v, o);
+  for (int k = 2; k < 16; k++) {
+    if ((uint32_t)k <= K) {
+      const float v = in[k];
 #pragma unroll
-  for (int k = 0; k < 16; k++) tile[k * 256 + t] = o[k] + dc;
-  __syncthreads();
+      for (int n = 0; n < 8; n++) {
+        const float p = c_mag[basis_j(k, n)] * v;
+        if (k & 1)
+          o[n] = basis_neg(k, n) ? o[n] - p : o[n] + p;
+        else
+          e[n] = basis_neg(k, n) ? e[n] - p : e[n] + p;
+      }
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    out[n] = e[n] + o[n];
+    out[15 - n] = e[n] - o[n];
+  }
 }
 
-template <bool F32OUT>
+// The wave-wide maximum of v in [0, 31] (a binary search over ballots).
+__device__ __forceinline__ uint32_t wave_max5(uint32_t v) {
+  uint32_t m = 0;
+#pragma unroll
+  for (uint32_t b = 16; b; b >>= 1)
+    if (__ballot(v >= m + b)) m += b;
+  return m;
+}
+
+// A chunk's frequency box, packed per wave as (kx+1) | (ky+1) << 5 | (kz+1) << 10
+// (0 where the wave saw no nonzero level).
+__device__ __forceinline__ uint32_t box_pack(uint32_t xm, uint32_t ym, uint32_t zm) {
+  return xm | ym << 5 | zm << 10;
+}
+
+// SPARSE: the pairs format, with the passes bounded by the chunk's frequency
+// box; dense levels take every pass whole (their box is nearly always full).
+template <bool F32OUT, bool SPARSE>
 __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
   __shared__ float tile[4096];
   __shared__ float steps[3][48];  // step, 0.80f * step, 0.40f * step
-  const uint32_t t = threadIdx.x;
+  __shared__ uint32_t s_box[4];
+  const uint32_t t = threadIdx.x, wv = t >> 6, lane = t & 63u;
   const uint32_t z = t >> 4, y = t & 15u;
   for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
     const r3d_chunk_header h = a.hdr[c];
@@ -304,12 +302,40 @@ __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
       steps[1][t] = 0.80f * st;
       steps[2][t] = 0.40f * st;
     }
-    float v[16];
-    if (a.fmt == R3D_COEF_DENSE_I16) {
+    // the frequency box: every nonzero level has x <= kx, y <= ky, z <= kz
+    int32_t w[8];
+    uint64_t p0 = 0, p1 = 0;
+    uint32_t kx = 15u, ky = 15u, kz = 15u;
+    if (!SPARSE) {
       const int4 *src = (const int4 *)(a.levels + (size_t)c * 4096u + t * 16u);
       const int4 q0 = src[0], q1 = src[1];
-      const int32_t w[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+      w[0] = q0.x, w[1] = q0.y, w[2] = q0.z, w[3] = q0.w, w[4] = q1.x, w[5] = q1.y, w[6] = q1.z, w[7] = q1.w;
       __syncthreads();
+    } else {
+      pair_range(a, c, &p0, &p1);
+      uint32_t xm = 0, ym = 0, zm = 0;
+      for (uint64_t p = p0 + t; p < p1; p += 256u) {
+        const uint32_t pos = a.pairs[p] & 4095u;
+        xm = max(xm, (pos & 15u) + 1u);
+        ym = max(ym, ((pos >> 4) & 15u) + 1u);
+        zm = max(zm, (pos >> 8) + 1u);
+      }
+      const uint32_t bx = wave_max5(xm), by = wave_max5(ym), bz = wave_max5(zm);
+      if (lane == 0) s_box[wv] = box_pack(bx, by, bz);
+      __syncthreads();
+      uint32_t bxm = 0, bym = 0, bzm = 0;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const uint32_t b = __builtin_amdgcn_readfirstlane(s_box[i]);
+        bxm = max(bxm, b & 31u);
+        bym = max(bym, (b >> 5) & 31u);
+        bzm = max(bzm, b >> 10);
+      }
+      kx = bxm ? bxm - 1u : 0u, ky = bym ? bym - 1u : 0u, kz = bzm ? bzm - 1u : 0u;
+    }
+    const bool p1_wave = 4u * wv <= kz;  // pass 1: the waves holding lines z <= kz
+    float v[16], o[16];
+    if (!SPARSE) {
 #pragma unroll
       for (int k = 0; k < 16; k++) {
         const int16_t l = (int16_t)((uint32_t)w[k >> 1] >> ((k & 1) * 16));
@@ -317,11 +343,13 @@ __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
         v[k] = deq_tab((float)l, steps[0][f], steps[1][f], steps[2][f]);
       }
     } else {
+      // zeros for the box's rows, then the pairs
+      if (z <= kz && y <= ky) {
+        float4 *row = (float4 *)&tile[t * 16u];
 #pragma unroll
-      for (int k = 0; k < 16; k++) tile[k * 256 + t] = 0.0f;
+        for (int i = 0; i < 4; i++) row[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
       __syncthreads();
-      uint64_t p0, p1;
-      pair_range(a, c, &p0, &p1);
       for (uint64_t p = p0 + t; p < p1; p += 256u) {
         const uint32_t pr = a.pairs[p];
         const uint32_t pos = pr & 4095u;
@@ -330,11 +358,43 @@ __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
         tile[pos] = deq_tab(l, steps[0][f], steps[1][f], steps[2][f]);
       }
       __syncthreads();
+      if (p1_wave) {
 #pragma unroll
-      for (int k = 0; k < 16; k++) v[k] = tile[t * 16 + k];
+        for (int k = 0; k < 16; k++) v[k] = tile[t * 16 + k];
+      }
       __syncthreads();
     }
-    idct3_tile(v, tile, t, dc);
+    // pass 1 (along x): line t = (z, y); writes tile[x*256 + t] (lines outside
+    // the box hold anything: the next pass reads none of them)
+    if (p1_wave) {
+      idct16k(v, o, kx);
+#pragma unroll
+      for (int k = 0; k < 16; k++) tile[k * 256 + t] = o[k];
+    }
+    __syncthreads();
+    // pass 2 (along y): the lines (x, z <= kz) packed into the first lanes
+    const uint32_t d2 = kz + 1u, m2 = (65536u + d2 - 1u) / d2;
+    const uint32_t x2 = (t * m2) >> 16, z2 = t - x2 * d2;
+    const bool p2 = t < 16u * d2;
+    if (p2) {
+#pragma unroll
+      for (int k = 0; k < 16; k++) v[k] = tile[x2 * 256u + z2 * 16u + k];
+    }
+    __syncthreads();
+    if (p2) {
+      idct16k(v, o, ky);
+#pragma unroll
+      for (int k = 0; k < 16; k++) tile[k * 256 + x2 * 16u + z2] = o[k];
+    }
+    __syncthreads();
+    // pass 3 (along z): every line t = (y, x); writes natural order + dc
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = tile[t * 16 + k];
+    __syncthreads();
+    idct16k(v, o, kz);
+#pragma unroll
+    for (int k = 0; k < 16; k++) tile[k * 256 + t] = o[k] + dc;
+    __syncthreads();
     if (a.corr_off) {  // dct3d.c:572-587: serial, in stream order
       if (t == 0) {
         uint64_t c0, n;
@@ -355,10 +415,10 @@ __global__ __launch_bounds__(256) void k_decode_exact(DecodeArgs a) {
         fd[x >> 2] = make_float4(tile[t * 16 + x] * inv + h.vmin, tile[t * 16 + x + 1] * inv + h.vmin,
                                  tile[t * 16 + x + 2] * inv + h.vmin, tile[t * 16 + x + 3] * inv + h.vmin);
     } else {
-      uint32_t w[4] = {0, 0, 0, 0};
+      uint32_t wo[4] = {0, 0, 0, 0};
 #pragma unroll
-      for (int x = 0; x < 16; x++) w[x >> 2] |= round_u8(tile[t * 16 + x] * inv + h.vmin) << ((x & 3) * 8);
-      store_row(dst, w);
+      for (int x = 0; x < 16; x++) wo[x >> 2] |= round_u8(tile[t * 16 + x] * inv + h.vmin) << ((x & 3) * 8);
+      store_row(dst, wo);
     }
     if (a.status && t == 0) a.status[c] = 1;
     __syncthreads();
@@ -956,8 +1016,11 @@ static int decode_impl(r3d_hip_ctx *ctx, const r3d_chunk_batch *b, const r3d_dec
   hipStream_t st = (hipStream_t)stream;
   if (variant == R3D_DECODE_EXACT_F32) {
     const uint32_t grid = grid_for(ctx, b->nchunks, 1u << 16);
-    if (f32out) hipLaunchKernelGGL(k_decode_exact<true>, dim3(grid), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL(k_decode_exact<false>, dim3(grid), dim3(256), 0, st, a);
+    const bool sp = b->coef_format == R3D_COEF_SPARSE;
+    if (f32out && sp) hipLaunchKernelGGL((k_decode_exact<true, true>), dim3(grid), dim3(256), 0, st, a);
+    else if (f32out) hipLaunchKernelGGL((k_decode_exact<true, false>), dim3(grid), dim3(256), 0, st, a);
+    else if (sp) hipLaunchKernelGGL((k_decode_exact<false, true>), dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_decode_exact<false, false>), dim3(grid), dim3(256), 0, st, a);
   } else {
     // four chunks (waves) per workgroup; 3 waves per SIMD are resident
     const uint32_t grid = grid_for(ctx, (b->nchunks + 3u) / 4u, 8u);
