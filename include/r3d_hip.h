@@ -643,8 +643,33 @@ int r3d_hip_bricks_raycast(r3d_hip_ctx *ctx, const r3d_hip_bricks *b, const r3d_
  * through r3d_hip_bricks_tables. Geometry is in the renderer's volume space
  * (base voxels / the largest base dim). */
 
+/* One level's 3ddct shard store: the level's shard files resident in device
+ * memory (a shard not listed is absent), the store's volume being the level's
+ * dims (x, y, z of the pyramid the handle was created for). */
+typedef struct r3d_bricks_level_src {
+  const r3d_shard_src *shards;
+  uint32_t nshards;
+  uint32_t reserved0;
+} r3d_bricks_level_src;
+/* The reference's warm tier in compressed form (its bricks come from a
+ * pyramid of 3ddct stores): instead of a built pyramid, the source of the
+ * fills is one shard store per level (nlev entries; the lists are copied, the
+ * shard bytes must stay valid while streaming). Each fill decodes the
+ * brick's 128^3 box of its level (r3d_hip_shard_decode_region) straight into
+ * the slot; its maximum is measured on the GPU, the entry is published only
+ * at 5 or more (the noise floor, vkbackend.c:5265-5269), and the gate learns
+ * it at the next begin (a brick's maximum is unknown, so not gated, until it
+ * has been decoded once). Bricks whose resident flag is set (one per page
+ * entry, page order, or NULL for all) are filled now, in slots in page order;
+ * the call returns when they are. While a submit's fills are in flight,
+ * r3d_hip_bricks_stream_begin returns 0 (stats.inflight holds their count),
+ * as the reference's pump skips a frame while its worker runs. */
+int r3d_hip_bricks_build_from_shards(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const r3d_bricks_level_src *levels,
+                                     uint32_t nlev, const uint8_t *resident, void *stream);
+
 /* r3d_bricks_stream_begin (render.h:127): opens a collect pass; returns 1, or
- * 0 when the handle has not been built (nothing is collected then). */
+ * 0 when the handle has not been built (nothing is collected then), or while
+ * the last submit's fills from shard stores are in flight. */
 int r3d_hip_bricks_stream_begin(r3d_hip_bricks *b);
 /* r3d_bricks_stream_box (render.h:128-129): the bricks of the box lo..hi at
  * the level whose voxel pitch matches pixel_cone (coarser until at most

@@ -10,7 +10,11 @@ It restates the reference's pump, src/vk/vkbackend.c:
   - r3d_bricks_stream_box 5330-5382, r3d_bricks_stream_point 5384-5414,
     the cone collect of r3d_bricks_stream 5416-5517 and
     r3d_bricks_stream_submit 5519-5595 (every brick is in the warm tier);
-  - r3d_lod_pick, src/core/lod.h:8-17; bricks_axis_bounds 3096-3107.
+  - r3d_lod_pick, src/core/lod.h:8-17; bricks_axis_bounds 3096-3107;
+  - with a compressed warm tier (known=...): a brick's maximum is unknown,
+    so not gated, until a fill has measured it (brick_maxk = -1,
+    vkbackend.c:3084), and an entry is published only at the noise floor or
+    above (5265-5269).
 Arithmetic is FP32 in the C's operation order (numpy float32 scalars), so the
 decisions equal the library's; log2 is numpy's float32 log2.
 """
@@ -61,7 +65,9 @@ def axis_bounds(eye, radius, edge, n):
 
 
 class StreamPolicy:
-    def __init__(self, level_dims, nslots, bmax, resident=None):
+    def __init__(self, level_dims, nslots, bmax, resident=None, known=None, floor=0):
+        """bmax: every brick's maximum; known: None (all known) or a flag per
+        brick (the residents' are set); floor: the entries' publish floor."""
         self.dims = [tuple(d) for d in level_dims]
         self.nlev = len(self.dims)
         self.nb = bricks_per_axis(self.dims)
@@ -81,6 +87,8 @@ class StreamPolicy:
         self.open = False
         self.lod_wanted = [0] * 8
         self.lod_requests = [0] * 8
+        self.known = None if known is None else [bool(k) for k in known]
+        self.floor = floor
         nxt = 0
         for e in range(self.nent):  # the build's residents, slots in page order
             if resident is None or resident[e]:
@@ -115,7 +123,7 @@ class StreamPolicy:
                 self.slot_use[s] = self.frame
 
     def candidate(self, e, d2, priority, g8):
-        if self.bmax[e] < g8:
+        if (self.known is None or self.known[e]) and self.bmax[e] < g8:
             return
         s = self.brick_slot[e]
         if s != INV:
@@ -253,6 +261,9 @@ class StreamPolicy:
                 evicted.append(old)
             self.slot_brick[s], self.slot_use[s], self.brick_slot[e] = e, self.frame, s
             filled.append((e, s))
+        if self.known is not None:
+            for e, _ in filled:
+                self.known[e] = True
         return filled, evicted
 
     def stream(self, eye, fwd, half_tan, pixel_cone, gate, budget, slice_z0=0, slice_depth=0):
@@ -267,6 +278,6 @@ class StreamPolicy:
         """Expected page entries (after the 64-word header)."""
         out = np.full(self.nent, INV, np.uint32)
         for e, s in enumerate(self.brick_slot):
-            if s != INV:
+            if s != INV and self.bmax[e] >= self.floor:
                 out[e] = s | (int(self.bmax[e]) << 24)
         return out

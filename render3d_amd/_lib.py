@@ -148,6 +148,10 @@ class ShardSrc(C.Structure):
                 ("sx", C.c_uint32), ("reserved0", C.c_uint32)]
 
 
+class BricksLevelSrc(C.Structure):
+    _fields_ = [("shards", C.c_void_p), ("nshards", C.c_uint32), ("reserved0", C.c_uint32)]
+
+
 class ShardRegion(C.Structure):
     _fields_ = [("nz", C.c_uint64), ("ny", C.c_uint64), ("nx", C.c_uint64), ("z0", C.c_uint64), ("y0", C.c_uint64),
                 ("x0", C.c_uint64), ("dz", C.c_uint32), ("dy", C.c_uint32), ("dx", C.c_uint32),
@@ -234,6 +238,7 @@ SIGNATURES = {
     "r3d_hip_bricks_create": (C.c_int, [P, C.POINTER(BricksDesc), C.POINTER(P)]),
     "r3d_hip_bricks_destroy": (None, [P]),
     "r3d_hip_bricks_build": (C.c_int, [P, P, P, P, P]),
+    "r3d_hip_bricks_build_from_shards": (C.c_int, [P, P, P, C.c_uint32, P, P]),
     "r3d_hip_bricks_tables": (C.c_int, [P, C.POINTER(P), C.POINTER(C.c_uint32), C.POINTER(P), C.POINTER(C.c_uint32)]),
     "r3d_hip_bricks_raycast": (C.c_int, [P, P, C.POINTER(FrameParams), C.c_uint32, C.POINTER(Image),
                                          C.POINTER(RaycastOpts), C.POINTER(Stats), P]),

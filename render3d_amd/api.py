@@ -538,6 +538,30 @@ class Bricks:
                                                          None if res is None else res.ctypes.data_as(C.c_void_p),
                                                          _stream(stream)))
 
+    def build_from_shards(self, levels, resident=None, stream=None):
+        """r3d_hip_bricks_build_from_shards: levels = one shard list per level,
+        [(uint8 device tensor: the whole shard file, sz, sy, sx)], the store's
+        volume being that level's dims; streaming then decodes bricks from
+        them. Bricks whose resident flag is set are decoded now."""
+        if len(levels) != self.nlev:
+            raise ValueError(f"bricks build_from_shards: {len(levels)} level stores for {self.nlev} levels")
+        self._shard_keep = []
+        lv = (_lib.BricksLevelSrc * self.nlev)()
+        for l, shards in enumerate(levels):
+            arr = (_lib.ShardSrc * max(1, len(shards)))()
+            for i, (t, sz, sy, sx) in enumerate(shards):
+                arr[i] = _lib.ShardSrc(t.data_ptr(), int(t.numel()), int(sz), int(sy), int(sx), 0)
+            self._shard_keep.append((arr, list(shards)))
+            lv[l] = _lib.BricksLevelSrc(C.cast(arr, C.c_void_p), len(shards), 0)
+        res = None
+        if resident is not None:
+            res = np.ascontiguousarray(resident, np.uint8)
+            if res.size != self.tables()[1] - 64:
+                raise ValueError("bricks build_from_shards: one resident flag per page entry")
+        self.ctx.check(self.ctx.lib.r3d_hip_bricks_build_from_shards(
+            self.ctx.h, self.h, lv, self.nlev, None if res is None else res.ctypes.data_as(C.c_void_p),
+            _stream(stream)))
+
     def read_page(self):
         """The page table as uint32 on the host (synchronous)."""
         page, words, _, _ = self.tables()

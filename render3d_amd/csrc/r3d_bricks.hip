@@ -166,11 +166,31 @@ __global__ __launch_bounds__(256) void k_bk_zero(uint32_t *p, uint32_t n) {
   if (i < n) p[i] = 0u;
 }
 
-// The entries of the filled bricks, once every part has folded its maximum.
+// The entries of the filled bricks, once every part has folded its maximum;
+// a brick whose maximum is below floor stays absent (vkbackend.c:5265-5269).
 __global__ __launch_bounds__(256) void k_bk_entries(const BkJob *jobs, uint32_t njobs, const uint32_t *maxv,
-                                                    uint32_t *page) {
+                                                    uint32_t *page, uint32_t floor) {
   const uint32_t j = blockIdx.x * 256u + threadIdx.x;
-  if (j < njobs) page[jobs[j].word] = jobs[j].slot | (maxv[j] << 24);
+  if (j < njobs) page[jobs[j].word] = maxv[j] >= floor ? jobs[j].slot | (maxv[j] << 24) : 0xFFFFFFFFu;
+}
+
+// The largest voxel of each filled slot: BK_PARTS workgroups per job, 256
+// rows of 128 voxels each, 16 bytes per load (a slot is 2 MiB aligned).
+__global__ __launch_bounds__(256) void k_bk_slotmax(const BkJob *jobs, const uint8_t *atlas, uint32_t *maxv) {
+  __shared__ uint32_t s_max[4];
+  const uint32_t j = blockIdx.x / BK_PARTS, part = blockIdx.x % BK_PARTS;
+  const uint4 *S = (const uint4 *)(atlas + (uint64_t)jobs[j].slot * BK_VOX + (uint64_t)part * 256u * 128u);
+  uint32_t mx = 0;
+  for (uint32_t i = threadIdx.x; i < 256u * 128u / 16u; i += 256u) {
+    const uint4 q = S[i];
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int k = 0; k < 16; k++) mx = max(mx, (w[k >> 2] >> (8 * (k & 3))) & 255u);
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, o));
+  if ((threadIdx.x & 63u) == 0u) s_max[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(&maxv[j], max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3])));
 }
 
 
@@ -305,6 +325,11 @@ extern "C" void r3d_hip_bricks_destroy(r3d_hip_bricks *b) {
   if (b->d_bmax) hipFree(b->d_bmax);
   for (uint32_t l = 0; l < R3D_MAX_LEVELS; l++)
     if (b->d_lev[l]) hipFree(b->d_lev[l]);
+  if (b->ev_fill) {
+    hipEventSynchronize(b->ev_fill);
+    hipEventDestroy(b->ev_fill);
+  }
+  if (b->h_maxv) hipHostFree(b->h_maxv);
   delete b;
 }
 
@@ -379,7 +404,7 @@ extern "C" int r3d_hip_bricks_build(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const u
       hipLaunchKernelGGL(k_bk_zero, dim3((f.njobs + 255u) / 256u), dim3(256), 0, st, f.maxv, f.njobs);
       hipLaunchKernelGGL(k_bk_fill, dim3(f.njobs * BK_PARTS), dim3(256), 0, st, f);
       hipLaunchKernelGGL(k_bk_entries, dim3((f.njobs + 255u) / 256u), dim3(256), 0, st, (const BkJob *)djobs,
-                         f.njobs, (const uint32_t *)f.maxv, f.page);
+                         f.njobs, (const uint32_t *)f.maxv, f.page, 0u);
       err = hipGetLastError();
     }
   }
@@ -390,6 +415,8 @@ extern "C" int r3d_hip_bricks_build(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const u
   // the streamer's view of the table: the built residents in their slots
   const uint32_t nent = b->page_words - 64u;
   b->src0 = vol;
+  b->from_shards = false;
+  b->pend.clear();
   b->stream_ready = false;
   b->stream_open = false;
   b->frame = 0;
@@ -472,7 +499,7 @@ void touch_coarsest(r3d_hip_bricks *b) {
 // A requested brick: resident ones are stamped as wanted, the others become
 // candidates once per frame.
 void candidate(r3d_hip_bricks *b, uint32_t e, float d2, uint32_t priority, int g8) {
-  if ((int)b->bmax[e] < g8) return;
+  if (b->bmax_known[e] && (int)b->bmax[e] < g8) return;  // an unknown maximum is not gated
   const uint32_t s = b->brick_slot[e];
   if (s != BR_INV) {
     b->slot_use[s] = b->frame;
@@ -505,7 +532,7 @@ void count_wanted(r3d_hip_bricks *b, uint32_t level) {
 
 // The per-brick maxima of the built pyramid (once per build).
 int stream_ready(r3d_hip_bricks *b) {
-  if (b->stream_ready) return R3D_OK;
+  if (b->stream_ready || b->from_shards) return R3D_OK;
   r3d_hip_ctx *ctx = b->ctx;
   if (!b->src0) return r3d_fail(ctx, R3D_ERR_INVALID, "bricks_stream: build the handle first");
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
@@ -526,11 +553,132 @@ int stream_ready(r3d_hip_bricks *b) {
   R3D_HIP_CHECK(ctx, hipGetLastError());
   b->bmax.assign(nent, 0);
   R3D_HIP_CHECK(ctx, hipMemcpy(b->bmax.data(), b->d_bmax, nent, hipMemcpyDeviceToHost));
+  b->bmax_known.assign(nent, 1u);
   b->stream_ready = true;
   return R3D_OK;
 }
 
+// Decodes each job's brick from its level's shard store straight into its
+// slot: the brick's 128^3 box as a region of the level (r3d_hip_shard_decode_region,
+// the reference reader shardio.c:269-306), zeros past the level's edge and in
+// missing or corrupt chunks.
+int shard_fill(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const BkJob *jobs, uint32_t n, void *stream) {
+  for (uint32_t j = 0; j < n; j++) {
+    const BkJob &q = jobs[j];
+    const std::vector<r3d_shard_src> &sh = b->lev_shards[q.level];
+    r3d_shard_region r{};
+    r.nz = b->dims[q.level][2];
+    r.ny = b->dims[q.level][1];
+    r.nx = b->dims[q.level][0];
+    r.z0 = (uint64_t)q.cz * BK;
+    r.y0 = (uint64_t)q.cy * BK;
+    r.x0 = (uint64_t)q.cx * BK;
+    r.dz = r.dy = r.dx = BK;
+    const int rc = r3d_hip_shard_decode_region(ctx, sh.empty() ? nullptr : sh.data(), (uint32_t)sh.size(), &r,
+                                               b->d_atlas + (uint64_t)q.slot * BK_VOX, nullptr, nullptr, stream);
+    if (rc != R3D_OK) return rc;
+  }
+  return R3D_OK;
+}
+
 }  // namespace
+
+// The handle's source as one 3ddct shard store per level (the level's dims are
+// the store's), for streaming without a decoded pyramid: the bricks whose
+// resident flag is set are decoded into slots in page order, the others
+// start absent with unknown maxima.
+extern "C" int r3d_hip_bricks_build_from_shards(r3d_hip_ctx *ctx, r3d_hip_bricks *b,
+                                                const r3d_bricks_level_src *levels, uint32_t nlev,
+                                                const uint8_t *resident, void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!b || b->ctx != ctx || !levels)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "bricks_build_from_shards: bad handle or level list");
+  if (nlev != b->desc.nlev)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "bricks_build_from_shards: %u level stores for %u levels", nlev,
+                    b->desc.nlev);
+  for (uint32_t l = 0; l < nlev; l++) {
+    if (levels[l].nshards && !levels[l].shards)
+      return r3d_fail(ctx, R3D_ERR_INVALID, "bricks_build_from_shards: level %u has no shard list", l);
+    for (uint32_t k = 0; k < levels[l].nshards; k++)
+      if (levels[l].shards[k].nbytes && !levels[l].shards[k].bytes)
+        return r3d_fail(ctx, R3D_ERR_INVALID, "bricks_build_from_shards: level %u shard %u has no bytes", l, k);
+  }
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = (hipStream_t)stream;
+  const uint32_t nent = b->page_words - 64u;
+  std::vector<BkJob> jobs;
+  for (uint32_t l = 0; l < nlev; l++) {
+    const uint32_t n = b->nb[l][0] * b->nb[l][1] * b->nb[l][2];
+    for (uint32_t k = 0; k < n; k++) {
+      const uint32_t w = b->off[l] + k;
+      if (resident && !resident[w - 64u]) continue;
+      if (jobs.size() >= b->nslots)
+        return r3d_fail(ctx, R3D_ERR_RANGE, "bricks_build_from_shards: more resident bricks than the %u slots",
+                        b->nslots);
+      jobs.push_back(BkJob{l, k % b->nb[l][0], (k / b->nb[l][0]) % b->nb[l][1], k / (b->nb[l][0] * b->nb[l][1]), w,
+                           (uint32_t)jobs.size()});
+    }
+  }
+  if (!b->h_maxv && hipHostMalloc((void **)&b->h_maxv, 4ull * (b->max_jobs > 32u ? b->max_jobs : 32u)) != hipSuccess) {
+    b->h_maxv = nullptr;
+    return r3d_fail(ctx, R3D_ERR_NOMEM, "bricks_build_from_shards: pinned maxima");
+  }
+  if (!b->ev_fill && hipEventCreate(&b->ev_fill) != hipSuccess) {
+    b->ev_fill = nullptr;
+    return r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build_from_shards: event");
+  }
+  // wait for an earlier submit's copy into h_maxv before reusing it
+  R3D_HIP_CHECK(ctx, hipEventSynchronize(b->ev_fill));
+  for (uint32_t l = 0; l < nlev; l++)
+    b->lev_shards[l].assign(levels[l].shards, levels[l].shards + levels[l].nshards);
+  for (uint32_t l = nlev; l < R3D_MAX_LEVELS; l++) b->lev_shards[l].clear();
+  b->from_shards = true;
+  b->src0 = nullptr;
+  b->stream_ready = false;
+  b->stream_open = false;
+  b->pend.clear();
+  b->frame = 0;
+  b->bmax.assign(nent, 0);
+  b->bmax_known.assign(nent, 0);
+  b->brick_slot.assign(nent, 0xFFFFFFFFu);
+  b->brick_want.assign(nent, 0u);
+  b->slot_brick.assign(b->nslots, 0xFFFFFFFFu);
+  b->slot_use.assign(b->nslots, 0u);
+  std::vector<uint32_t> page(nent, 0xFFFFFFFFu);
+  R3D_HIP_CHECK(ctx, hipMemcpyAsync(b->d_page + 64, page.data(), 4ull * nent, hipMemcpyHostToDevice, st));
+  int rc = R3D_OK;
+  const uint32_t n = (uint32_t)jobs.size();
+  if (n) {
+    rc = shard_fill(ctx, b, jobs.data(), n, stream);
+    BkJob *djobs = (BkJob *)b->d_jobs;
+    uint32_t *maxv = (uint32_t *)(djobs + b->max_jobs);
+    if (rc == R3D_OK && hipMemcpyAsync(djobs, jobs.data(), sizeof(BkJob) * n, hipMemcpyHostToDevice, st) != hipSuccess)
+      rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build_from_shards: job upload");
+    if (rc == R3D_OK) {
+      hipLaunchKernelGGL(k_bk_zero, dim3((n + 255u) / 256u), dim3(256), 0, st, maxv, n);
+      hipLaunchKernelGGL(k_bk_slotmax, dim3(n * BK_PARTS), dim3(256), 0, st, (const BkJob *)djobs,
+                         (const uint8_t *)b->d_atlas, maxv);
+      hipLaunchKernelGGL(k_bk_entries, dim3((n + 255u) / 256u), dim3(256), 0, st, (const BkJob *)djobs, n,
+                         (const uint32_t *)maxv, b->d_page, (uint32_t)BR_NOISE_FLOOR);
+      if (hipGetLastError() != hipSuccess) rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build_from_shards: launch");
+    }
+    if (rc == R3D_OK && hipMemcpyAsync(b->h_maxv, maxv, 4ull * n, hipMemcpyDeviceToHost, st) != hipSuccess)
+      rc = r3d_fail(ctx, R3D_ERR_DEVICE, "bricks_build_from_shards: maxima copy");
+  }
+  // the page and job copies read host vectors that end with this call
+  const hipError_t sync = hipStreamSynchronize(st);
+  r3d_scene_changed(ctx);
+  if (rc != R3D_OK) return rc;
+  R3D_HIP_CHECK(ctx, sync);
+  for (const BkJob &j : jobs) {
+    const uint32_t e = j.word - 64u, idx = (uint32_t)(&j - jobs.data());
+    b->brick_slot[e] = j.slot;
+    b->slot_brick[j.slot] = e;
+    b->bmax[e] = (uint8_t)b->h_maxv[idx];
+    b->bmax_known[e] = 1u;
+  }
+  return R3D_OK;
+}
 
 extern "C" int r3d_hip_bricks_stream_begin(r3d_hip_bricks *b) {
   if (!b || !b->ctx) return 0;
@@ -538,6 +686,19 @@ extern "C" int r3d_hip_bricks_stream_begin(r3d_hip_bricks *b) {
   b->last_filled = b->last_evicted = 0;
   b->cands.clear();
   if (stream_ready(b) != R3D_OK) return 0;
+  if (!b->pend.empty()) {  // the last fills' maxima (vkbackend.c:5256-5282): applied once they are back
+    const hipError_t q = hipEventQuery(b->ev_fill);
+    if (q == hipErrorNotReady) {
+      b->last_filled = (uint32_t)b->pend.size();  // still in flight: no collect this frame
+      return 0;
+    }
+    if (q != hipSuccess) return 0;
+    for (size_t j = 0; j < b->pend.size(); j++) {
+      b->bmax[b->pend[j]] = (uint8_t)b->h_maxv[j];
+      b->bmax_known[b->pend[j]] = 1u;
+    }
+    b->pend.clear();
+  }
   b->frame++;
   memset(b->lod_wanted, 0, sizeof b->lod_wanted);
   b->stream_open = true;
@@ -646,6 +807,13 @@ extern "C" int r3d_hip_bricks_stream_submit(r3d_hip_ctx *ctx, r3d_hip_bricks *b,
   r3d_scene_changed(ctx);  // residency changes what bricks-LOD views render
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   hipStream_t st = (hipStream_t)stream;
+  // shard stores: decode into the slots first (the region decodes stage their
+  // own tables), then the jobs go up; all of it is ordered on the stream, so
+  // no render on it sees a slot between its eviction and its entry
+  if (b->from_shards) {
+    const int rc = shard_fill(ctx, b, jobs, n, stream);
+    if (rc != R3D_OK) return rc;
+  }
   uint8_t buf[sizeof evict + sizeof jobs];
   memcpy(buf, evict, sizeof evict);
   memcpy(buf + sizeof evict, jobs, sizeof(BkJob) * n);
@@ -655,21 +823,35 @@ extern "C" int r3d_hip_bricks_stream_submit(r3d_hip_ctx *ctx, r3d_hip_bricks *b,
   const uint32_t *d_evict = (const uint32_t *)slot->dev;
   const BkJob *d_jobs = (const BkJob *)((const uint8_t *)slot->dev + sizeof evict);
   if (nevict) hipLaunchKernelGGL(k_bk_evict, dim3(1), dim3(256), 0, st, d_evict, nevict, b->d_page);
-  BkFill f{};
-  f.lev[0] = b->src0;
-  for (uint32_t l = 0; l < b->desc.nlev; l++) {
-    if (l) f.lev[l] = b->d_lev[l];
-    for (int a = 0; a < 3; a++) f.dims[l][a] = b->dims[l][a];
+  uint32_t *maxv = (uint32_t *)((BkJob *)b->d_jobs + b->max_jobs);
+  hipLaunchKernelGGL(k_bk_zero, dim3(1), dim3(256), 0, st, maxv, n);
+  if (b->from_shards) {
+    // the maxima: the gate learns them (applied at the next begin), and a
+    // brick below the noise floor is not published (vkbackend.c:5265-5269)
+    hipLaunchKernelGGL(k_bk_slotmax, dim3(n * BK_PARTS), dim3(256), 0, st, d_jobs, (const uint8_t *)b->d_atlas, maxv);
+    hipLaunchKernelGGL(k_bk_entries, dim3(1), dim3(256), 0, st, d_jobs, n, (const uint32_t *)maxv, b->d_page,
+                       (uint32_t)BR_NOISE_FLOOR);
+    R3D_HIP_CHECK(ctx, hipGetLastError());
+    R3D_HIP_CHECK(ctx, hipMemcpyAsync(b->h_maxv, maxv, 4ull * n, hipMemcpyDeviceToHost, st));
+    R3D_HIP_CHECK(ctx, hipEventRecord(b->ev_fill, st));
+    b->pend.resize(n);
+    for (uint32_t j = 0; j < n; j++) b->pend[j] = jobs[j].word - 64u;
+  } else {
+    BkFill f{};
+    f.lev[0] = b->src0;
+    for (uint32_t l = 0; l < b->desc.nlev; l++) {
+      if (l) f.lev[l] = b->d_lev[l];
+      for (int a = 0; a < 3; a++) f.dims[l][a] = b->dims[l][a];
+    }
+    f.jobs = d_jobs;
+    f.njobs = n;
+    f.atlas = b->d_atlas;
+    f.page = b->d_page;
+    f.maxv = maxv;
+    hipLaunchKernelGGL(k_bk_fill, dim3(n * BK_PARTS), dim3(256), 0, st, f);
+    hipLaunchKernelGGL(k_bk_entries, dim3(1), dim3(256), 0, st, d_jobs, n, (const uint32_t *)f.maxv, f.page, 0u);
+    R3D_HIP_CHECK(ctx, hipGetLastError());
   }
-  f.jobs = d_jobs;
-  f.njobs = n;
-  f.atlas = b->d_atlas;
-  f.page = b->d_page;
-  f.maxv = (uint32_t *)((BkJob *)b->d_jobs + b->max_jobs);
-  hipLaunchKernelGGL(k_bk_zero, dim3(1), dim3(256), 0, st, f.maxv, n);
-  hipLaunchKernelGGL(k_bk_fill, dim3(n * BK_PARTS), dim3(256), 0, st, f);
-  hipLaunchKernelGGL(k_bk_entries, dim3(1), dim3(256), 0, st, d_jobs, n, (const uint32_t *)f.maxv, f.page);
-  R3D_HIP_CHECK(ctx, hipGetLastError());
   R3D_HIP_CHECK(ctx, hipEventRecord(slot->ev, st));
   slot->recorded = true;
   b->last_filled = n;

@@ -136,6 +136,15 @@ struct r3d_hip_bricks {
   bool stream_open = false;
   uint32_t lod_wanted[8] = {}, last_filled = 0, last_evicted = 0;
   uint64_t lod_requests[8] = {}, filled = 0, evicted = 0, submits = 0;
+  // streaming from 3ddct shard stores (r3d_hip_bricks_build_from_shards): one
+  // shard list per level (device-resident files); a brick's maximum is known
+  // once a fill has decoded it
+  bool from_shards = false;
+  std::vector<r3d_shard_src> lev_shards[R3D_MAX_LEVELS];
+  std::vector<uint8_t> bmax_known;
+  uint32_t *h_maxv = nullptr;     // pinned: the maxima of the last submit's fills
+  hipEvent_t ev_fill = nullptr;   // recorded after their copy
+  std::vector<uint32_t> pend;     // page entries (after the header) of those fills
 };
 
 // error helpers

@@ -241,3 +241,26 @@ def test_box_level_and_parent_first_by_hand():
     big.box((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.0, 0.0)
     assert big.lod_wanted[:3] == [0, 1000, 0]
     assert len(big.cands) == 1000  # the parents are resident
+
+
+def test_unknown_maxima_fill_once_then_gate():
+    """A compressed warm tier: a brick's maximum is unknown (not gated) until
+    a fill measures it; below the noise floor it stays unpublished, and once
+    known it is gated like any other."""
+    dims = [(256, 128, 128), (128, 64, 64)]  # entries 0, 1 (level 0), 2 (level 1, resident in slot 0)
+    pol = sp.StreamPolicy(dims, 2, np.array([3, 50, 60]), np.array([0, 0, 1]), known=[0, 0, 1], floor=5)
+    pol.begin()
+    pol.point((0.1, 0.1, 0.1), 0, 0.0)
+    assert pol.submit(8) == ([(0, 1)], [])  # unknown maximum: filled
+    assert pol.known[0] and pol.entries()[0] == sp.INV and pol.entries()[2] == 0 | 60 << 24
+    pol.begin()
+    pol.point((0.9, 0.1, 0.1), 0, 0.0)
+    assert pol.submit(8) == ([(1, 1)], [0])  # entry 0 is no longer wanted: its slot goes
+    pol.begin()
+    pol.point((0.1, 0.1, 0.1), 0, 0.0)
+    assert pol.submit(8) == ([], [])  # known below the gate: never again
+    # with every maximum known up front, entry 0 never takes a slot
+    pol = sp.StreamPolicy(dims, 2, np.array([3, 50, 60]), np.array([0, 0, 1]))
+    pol.begin()
+    pol.point((0.1, 0.1, 0.1), 0, 0.0)
+    assert pol.submit(8) == ([], [])
