@@ -555,6 +555,16 @@ int r3d_hip_shard_decode_region(r3d_hip_ctx *ctx, const r3d_shard_src *shards, u
                                 const r3d_shard_region *region, uint8_t *dst, int32_t *chunk_status,
                                 uint32_t *counts, void *stream);
 
+/* The shard path straight into a render volume: r3d_hip_shard_decode_region
+ * of region (whose size must be the volume's dims) into the volume's staging
+ * buffer, then the bricks, aprons, pyramid and occupancy built from it, on
+ * `stream` with no host round trip (the reference reads regions into host
+ * memory and uploads them, shardio.c:269-306 then r3d_upload_volume). The
+ * statuses and counts are those of the region decode. */
+int r3d_hip_volume_decode_shards(r3d_hip_ctx *ctx, r3d_hip_volume *vol, const r3d_shard_src *shards,
+                                 uint32_t nshards, const r3d_shard_region *region, int32_t *chunk_status,
+                                 uint32_t *counts, void *stream);
+
 /* CRC-32C (reflected Castagnoli polynomial, init and final xor 0xFFFFFFFF:
  * the zarr index trailer, shardio.c:39-52) of n device bytes into *crc
  * (device). */

@@ -239,6 +239,7 @@ SIGNATURES = {
     "r3d_hip_bricks_destroy": (None, [P]),
     "r3d_hip_bricks_build": (C.c_int, [P, P, P, P, P]),
     "r3d_hip_bricks_build_from_shards": (C.c_int, [P, P, P, C.c_uint32, P, P]),
+    "r3d_hip_volume_decode_shards": (C.c_int, [P, P, P, C.c_uint32, P, P, P, P]),
     "r3d_hip_bricks_tables": (C.c_int, [P, C.POINTER(P), C.POINTER(C.c_uint32), C.POINTER(P), C.POINTER(C.c_uint32)]),
     "r3d_hip_bricks_raycast": (C.c_int, [P, P, C.POINTER(FrameParams), C.c_uint32, C.POINTER(Image),
                                          C.POINTER(RaycastOpts), C.POINTER(Stats), P]),

@@ -463,6 +463,22 @@ class Volume:
         self.ctx.check(self.ctx.lib.r3d_hip_volume_decode(self.ctx.h, self.h, C.byref(batch), C.byref(opts),
                                                           _stream(stream)))
 
+    def decode_shards(self, shards, dims, origin=(0, 0, 0), counts=None, stream=None):
+        """r3d_hip_volume_decode_shards: the volume's box at origin (z0, y0, x0)
+        of a shard store of dims (nz, ny, nx), decoded and built. shards as
+        Context.shard_decode_region; returns the counts (int32 [4])."""
+        ctx = self.ctx
+        nx, ny, nz = self.dims_xyz
+        r = ctx._region(dims, tuple(origin) + (nz, ny, nx))
+        if counts is None:
+            counts = torch.zeros(4, dtype=torch.int32, device=f"cuda:{ctx.device}")
+        arr = (_lib.ShardSrc * max(1, len(shards)))()
+        for i, (t, sz, sy, sx) in enumerate(shards):
+            arr[i] = _lib.ShardSrc(t.data_ptr(), int(t.numel()), int(sz), int(sy), int(sx), 0)
+        ctx.check(ctx.lib.r3d_hip_volume_decode_shards(ctx.h, self.h, C.cast(arr, C.c_void_p), len(shards),
+                                                       C.byref(r), None, _ptr(counts), _stream(stream)))
+        return counts
+
     @property
     def brick_count(self):
         """16^3 bricks over all levels (r3d_hip_volume_brick_count)."""

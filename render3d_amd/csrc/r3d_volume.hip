@@ -763,6 +763,30 @@ extern "C" int r3d_hip_volume_upload(r3d_hip_ctx *ctx, r3d_hip_volume *v, const 
   return rc;
 }
 
+// r3d_shard_decode_region (shardio.c:269-306) of the volume's box of a shard
+// store, then the volume built from it: the region decode writes the volume's
+// staging buffer row by row and the brickify pass builds the bricks, aprons,
+// pyramid and occupancy from there, all on the GPU.
+extern "C" int r3d_hip_volume_decode_shards(r3d_hip_ctx *ctx, r3d_hip_volume *v, const r3d_shard_src *shards,
+                                            uint32_t nshards, const r3d_shard_region *region, int32_t *chunk_status,
+                                            uint32_t *counts, void *stream) {
+  if (!ctx) return R3D_ERR_INVALID;
+  if (!v || v->ctx != ctx || !region) return r3d_fail(ctx, R3D_ERR_INVALID, "volume_decode_shards: bad argument");
+  if (region->dz != v->desc.nz || region->dy != v->desc.ny || region->dx != v->desc.nx)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "volume_decode_shards: region %ux%ux%u for a volume of %ux%ux%u",
+                    region->dx, region->dy, region->dz, v->desc.nx, v->desc.ny, v->desc.nz);
+  R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const uint64_t nch = (uint64_t)v->bricks[0][0] * v->bricks[0][1] * v->bricks[0][2];  // >= the voxels / 4096
+  if (!v->d_flat && hipMalloc(&v->d_flat, nch * 4096u) != hipSuccess) {
+    v->d_flat = nullptr;
+    return r3d_fail(ctx, R3D_ERR_NOMEM, "volume_decode_shards: staging %llu bytes",
+                    (unsigned long long)(nch * 4096u));
+  }
+  const int rc = r3d_hip_shard_decode_region(ctx, shards, nshards, region, v->d_flat, chunk_status, counts, stream);
+  if (rc != R3D_OK) return rc;
+  return brickify_and_build<SRC_LINEAR>(ctx, v, v->d_flat, (hipStream_t)stream);
+}
+
 extern "C" int r3d_hip_volume_chunk_targets(r3d_hip_ctx *ctx, r3d_hip_volume *v, uint64_t *out_offsets,
                                             uint8_t **base, uint32_t *stride_y, uint32_t *stride_z,
                                             void *stream) {

@@ -165,3 +165,29 @@ def test_full_shard_equals_level_decode(ctx):
     dst, counts, _ = ctx.shard_decode_region([(shard, 0, 0, 0)], (n, n, n), r, status=False)
     torch.cuda.synchronize()
     assert torch.equal(dst, want[100:300, 333:483, 517:818])
+
+
+def test_volume_decode_shards_equals_region_then_upload(ctx):
+    """r3d_hip_volume_decode_shards: the region decode straight into a render
+    volume equals the region decode uploaded into one (every level and the
+    occupancy), with the same counts, on the reference's shards (aligned,
+    straddling shards, past the volume's y edge, odd sizes)."""
+    ds = dev_shards()
+    for i in (0, 3, 4, 5, 8, 9):
+        z0, y0, x0, dz, dy, dx = (int(v) for v in G["regions"][i])
+        v = ctx.volume(dx, dy, dz)
+        counts = v.decode_shards(ds, DIMS, (z0, y0, x0))
+        dst, c2, _ = ctx.shard_decode_region(ds, DIMS, (z0, y0, x0, dz, dy, dx), status=False)
+        w = ctx.volume(dx, dy, dz)
+        w.upload(dst.contiguous())
+        torch.cuda.synchronize()
+        assert np.array_equal(counts.cpu().numpy(), c2.cpu().numpy()), i
+        for l in range(v.nlevels):
+            assert np.array_equal(v.read_level(l), w.read_level(l)), (i, l)
+        assert np.array_equal(v.read_occupancy(), w.read_occupancy()), i
+        assert np.array_equal(v.read_level(0), dst.cpu().numpy().reshape(-1)), i
+    with pytest.raises(R3DError):
+        v = ctx.volume(16, 16, 16)
+        r = ctx._region(DIMS, (0, 0, 0, 16, 16, 32))  # the region must be the volume's size
+        import ctypes
+        ctx.check(ctx.lib.r3d_hip_volume_decode_shards(ctx.h, v.h, None, 0, ctypes.byref(r), None, None, None))
