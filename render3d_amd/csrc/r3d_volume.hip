@@ -720,6 +720,7 @@ extern "C" int r3d_hip_volume_decode(r3d_hip_ctx *ctx, r3d_hip_volume *v, const 
   if (batch->nchunks != nch)
     return r3d_fail(ctx, R3D_ERR_INVALID, "volume_decode: %u chunks for a volume of %llu", batch->nchunks,
                     (unsigned long long)nch);
+  if (opts && opts->flags) return r3d_fail(ctx, R3D_ERR_INVALID, "volume_decode: the volume takes u8 output");
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   if (!v->d_flat && hipMalloc(&v->d_flat, nch * 4096u) != hipSuccess)
     return r3d_fail(ctx, R3D_ERR_NOMEM, "volume_decode: staging %llu bytes", (unsigned long long)(nch * 4096u));

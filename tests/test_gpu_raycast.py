@@ -306,3 +306,35 @@ def test_unsupported_views_rejected(ctx, scene96):
     p.mode = 9
     with pytest.raises(r3d.R3DError):
         ctx.raycast(v, [p], 32, 32)
+
+
+@pytest.mark.parametrize("dims", [(128, 64, 96), (64, 48, 160), (64, 32, 16), (48, 64, 64)])
+def test_volume_decode_equals_decode_then_build(ctx, dims):
+    """r3d_hip_volume_decode (the chunk batch decoded into the volume's
+    staging, then built) on shapes with and without the four-brick build
+    groups: every level and the occupancy equal a separate decode + build of
+    the same chunks, dense and sparse, twice in a row on one volume."""
+    nx, ny, nz = dims
+    vol = ctx.synth_sheets(nx, ny, nz, 3, 17)
+    step = torch.from_numpy(orc.step_linear()).cuda()
+    levels, hdr = ctx.encode_chunks(vol, nx, ny, nz, step)
+    flat = ctx.decode_chunks(headers=hdr, step_tables=step, levels=levels)
+    ref = ctx.volume(nx, ny, nz, 3)
+    ref.build_from_chunks(flat)
+    lv = levels.cpu().numpy()
+    cnt = (lv != 0).sum(1)
+    offs = np.zeros(lv.shape[0] + 1, np.uint32)
+    offs[1:] = np.cumsum(cnt)
+    pos = np.nonzero(lv.reshape(-1))[0]
+    pairs = ((pos % 4096) | (lv.reshape(-1)[pos].view(np.uint16).astype(np.uint32) << 16)).astype(np.uint32)
+    batches = [ctx.make_batch(headers=hdr, step_tables=step, out=0, levels=levels),
+               ctx.make_batch(headers=hdr, step_tables=step, out=0, pairs=torch.from_numpy(pairs).cuda(),
+                              pair_offsets=torch.from_numpy(offs).cuda())]
+    for bt in batches:
+        v = ctx.volume(nx, ny, nz, 3)
+        for _ in range(2):  # twice: the second run reuses the staging
+            v.decode(bt)
+        torch.cuda.synchronize()
+        for l in range(3):
+            assert np.array_equal(v.read_level(l), ref.read_level(l)), (dims, l)
+        assert np.array_equal(v.read_occupancy(), ref.read_occupancy()), dims
