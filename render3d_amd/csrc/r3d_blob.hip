@@ -28,6 +28,7 @@
 namespace r3d {
 
 constexpr uint32_t BLOB_NT = 256;  // lanes (blobs) per workgroup
+constexpr uint32_t BLK_BYTES = 32;  // the blob read block (two uint4 members)
 constexpr uint32_t RC_TOP = 1u << 24;
 enum : uint32_t { CTX_SIG = 0, CTX_MAG = 32, CTX_EOB = 44, CTX_FLAG = 58, NCTX = 59 };
 
@@ -59,7 +60,9 @@ struct RDec {
   uint64_t w;        // the window: bytes [qn - wn, qn), stream order from bit 63 down
   uint32_t wn;
   uint32_t stg_k, stg_skip, stg_cnt;  // a staged dword (dword stg_k of blk), pending while stg_cnt > 0
-  uint4 blk;                          // the aligned 16-byte block the window is read from
+  // the aligned BLK_BYTES block the window is read from, as named members
+  // (an array member would move the whole decoder state to scratch)
+  uint4 b0, b1;
   uintptr_t blk_a;                    // its address (0: none yet)
   uint32_t code, range;
   uint16_t *ctx;  // this lane's contexts: ctx[i * BLOB_NT]
@@ -82,15 +85,18 @@ struct RDec {
   }
   __device__ __forceinline__ void merge() {
     if (stg_cnt) {
-      const uint32_t d = stg_k == 0u ? blk.x : stg_k == 1u ? blk.y : stg_k == 2u ? blk.z : blk.w;
+      const uint4 b = (stg_k >> 2) ? b1 : b0;
+      const uint32_t k = stg_k & 3u;
+      const uint32_t d = k == 0u ? b.x : k == 1u ? b.y : k == 2u ? b.z : b.w;
       append(d, stg_skip, stg_cnt);
       stg_cnt = 0;
     }
   }
   // Stage the dword after the window (when it has room for it). Bytes come
-  // from 16-byte aligned blocks, so a lane loads each block of its blob once
-  // instead of four dwords of it: a quarter of the loads, and a quarter of the
-  // line fetches when the L2 cannot hold every lane's current line. A block
+  // from BLK_BYTES aligned blocks, one cache line: a lane requests each line
+  // of its blob once, so the line cannot be evicted from the L2 (which cannot
+  // hold every lane's current line) between two requests of it and fetched
+  // again (with 16-byte blocks a line was fetched about 7 times). A block
   // holding a stream byte lies in the buffer's pages, like the dword did.
   __device__ __forceinline__ void refill() {
     if (stg_cnt == 0u && wn <= 4u) {
@@ -103,10 +109,11 @@ struct RDec {
         stg_skip = (uint32_t)(a & 3u);
         const uint32_t left = n - q;
         stg_cnt = 4u - stg_skip < left ? 4u - stg_skip : left;
-        stg_k = (uint32_t)(a >> 2) & 3u;
-        const uintptr_t ba = a & ~(uintptr_t)15u;
+        stg_k = (uint32_t)(a >> 2) & (BLK_BYTES / 4u - 1u);
+        const uintptr_t ba = a & ~(uintptr_t)(BLK_BYTES - 1u);
         if (ba != blk_a) {
-          blk = *(const uint4 *)ba;
+          const uint4 *bp = (const uint4 *)ba;
+          b0 = bp[0], b1 = bp[1];
           blk_a = ba;
         }
       }
