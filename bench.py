@@ -51,6 +51,7 @@ def parse():
     ap.add_argument("--no-deblock", action="store_true", help="skip the deblocking workload")
     ap.add_argument("--no-shard", action="store_true", help="skip the zarr shard region-decode workload")
     ap.add_argument("--no-bricks", action="store_true", help="skip the bricks-LOD (page table) workload")
+    ap.add_argument("--no-stream", action="store_true", help="skip the bricks streaming-residency workload")
     ap.add_argument("--no-blobs", action="store_true", help="skip the 3ddct blob (entropy stage) workload")
     ap.add_argument("--no-fit", action="store_true", help="skip the differentiable 256^3 fit (200 Adam steps)")
     ap.add_argument("--profile-decode", action="store_true",
@@ -239,6 +240,9 @@ def main():
     if shd is not None and not args.no_deblock:
         deb = deblock_stage(ctx, args, dev, stream, shd["_state"]["dst"], BLOB_QSTEP)
     brk = None if args.no_bricks else bricks_stage(ctx, args, dev, stream, n, nsheets, seed, lut, cam, W, H)
+    strm = None
+    if not args.no_stream:
+        strm = stream_stage(ctx, args, dev, stream, n, nsheets, seed, None if shd is None else shd["_state"]["shard"])
     fit = None if args.no_fit else fit_256(ctx, dev)
 
     cpu = None
@@ -256,6 +260,12 @@ def main():
             deb["cpu_baseline"], deb["parity_vs_reference"] = cpu_deblock(deb["_state"], args)
         if brk is not None:
             brk["cpu_baseline"], brk["parity_vs_oracle"] = cpu_bricks(brk["_state"], args)
+        if strm is not None and shd is not None and strm.get("shards") is not None:
+            c = shd["cpu_baseline"]
+            strm["shards"]["cpu_baseline"] = {
+                "value": c["value"] / 512.0, "unit": "bricks/s", "cores": c["cores"], "kind": c["kind"],
+                "sample": "the reference reader's chunk rate on a 64 x 1024 x 1024 slab of the same shard "
+                          "(shard.cpu_baseline) / 512 chunks per 128^3 brick"}
     if c0 is not None:
         c0.pop("_state", None)
     if blob is not None:
@@ -298,6 +308,7 @@ def main():
         "shard": shd,
         "deblock": deb,
         "bricks": brk,
+        "stream": strm,
         "samples_per_frame": samples,
         "skips_per_frame": skips,
         "shaded_per_frame": shaded,
@@ -649,6 +660,90 @@ def shard_stage(ctx, args, dev, stream, n, nsheets, seed):
            "compressed_GBps": nbytes / (ms / 1e3) / 1e9, "voxel_GBps": n ** 3 / (ms / 1e3) / 1e9,
            "counts_decoded_missing_corrupt_wide": c, "matches_level_decode": same}
     res["_state"] = {"shard": shard, "dst": dst}
+    return res
+
+
+def stream_stage(ctx, args, dev, stream, n, nsheets, seed, shard):
+    """Streaming residency (r3d_hip_bricks_stream) at n^3: a 4^3 = 64-slot
+    atlas for the args.levels pyramid (585 bricks at 1024^3), only the
+    coarsest level resident at build, then 48 frames of the view-cone collect
+    along an orbit of radius 0.3 inside the volume (128-pixel-high view, 60
+    degree FOV, gate 0, budget 32): each frame fills the nearest missing
+    bricks and evicts the least recently wanted ones. Two warm tiers: the
+    decoded pyramid in HBM (fill = a 2 MiB copy per brick), and, with the
+    shard workload's 1024^3 level-0 shard, the shard store (fill = the brick's
+    512 chunks decoded from the shard into the slot; the coarser levels' stores
+    are empty). A filling frame's GPU time is taken by HIP events around the
+    call on the stream; its host time (policy + launches) by the host clock."""
+    import math
+
+    import numpy as np
+    import torch
+
+    import render3d_amd as r3d
+
+    lin = ctx.synth_sheets(n, n, n, nsheets, seed)
+    nlev = args.levels
+    b = r3d.Bricks(ctx, n, n, n, nlev, 4)
+    nent = b.tables()[1] - 64
+    res_flags = np.zeros(nent, np.uint8)
+    res_flags[-((max(n >> (nlev - 1), 1) + 127) // 128) ** 3:] = 1  # the coarsest level
+    ht = math.tan(math.radians(30.0))
+    pc = float(np.float32(2.0 * ht / 128.0))
+    frames = 48
+
+    def run(source):
+        if source == "pyramid":
+            b.build(lin, res_flags, stream=stream)
+        else:
+            b.build_from_shards([[(shard, 0, 0, 0)]] + [[] for _ in range(nlev - 1)], res_flags, stream=stream)
+        torch.cuda.synchronize()
+        gpu_ms, host_ms, fills, evicts = [], [], 0, 0
+        for i in range(frames):
+            a = 2.0 * math.pi * i / frames
+            eye = [0.5 + 0.3 * math.cos(a), 0.5, 0.5 + 0.3 * math.sin(a)]
+            fwd = [-math.sin(a), 0.0, math.cos(a)]
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            t0 = time.perf_counter()
+            b.stream(eye, fwd, ht, pc, 0.0, 32, stream=stream)
+            t1 = time.perf_counter()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            st = b.stream_stats()
+            if st["inflight"]:
+                gpu_ms.append(e0.elapsed_time(e1))
+                host_ms.append((t1 - t0) * 1e3)
+                fills += st["inflight"]
+                evicts += st["evicted_last"]
+        g = float(np.sum(gpu_ms))
+        return {"frames": frames, "filling_frames": len(gpu_ms), "fills": fills, "evictions": evicts,
+                "gpu_ms_per_filling_frame": float(np.mean(gpu_ms)) if gpu_ms else None,
+                "host_ms_per_filling_frame": float(np.mean(host_ms)) if host_ms else None,
+                "bricks_per_s": fills / (g / 1e3) if g else None, "_gpu_ms": g}
+
+    res = {"slots": 64, "bricks": nent, "view": "128 px high, 60 deg FOV, orbit radius 0.3 inside the volume"}
+    pyr = run("pyramid")
+    fill_bytes = 2 * 128 ** 3  # a brick read from its level and written to its slot
+    pyr["GBps"] = pyr["fills"] * fill_bytes / (pyr.pop("_gpu_ms") / 1e3) / 1e9
+    pyr["roofline"] = {"kernel": "k_bk_fill", "bound": "hbm", "achieved": pyr["GBps"], "peak": HBM_PEAK_GBPS,
+                       "unit": "GB/s", "frac": pyr["GBps"] / HBM_PEAK_GBPS,
+                       "bytes_per_fill": fill_bytes}
+    res["pyramid"] = pyr
+    del lin
+    if shard is not None and n == 1024:
+        sh = run("shards")
+        g = sh.pop("_gpu_ms")
+        nbytes = int(shard.numel())
+        per_brick = nbytes / 512.0 + 128 ** 3  # the brick's share of the compressed shard + the slot written
+        sh["GBps"] = sh["fills"] * per_brick / (g / 1e3) / 1e9 if g else None
+        sh["roofline"] = {"kernel": "shard region decode per brick", "bound": "valu (decode)",
+                          "achieved_GBps_of_algorithmic_bytes": sh["GBps"], "peak_hbm": HBM_PEAK_GBPS,
+                          "frac_of_hbm": (sh["GBps"] or 0.0) / HBM_PEAK_GBPS,
+                          "bytes_per_fill": per_brick,
+                          "note": "each fill also re-reads its shard's 4 MiB index for the crc check"}
+        res["shards"] = sh
+    b.close()
     return res
 
 

@@ -559,24 +559,42 @@ int stream_ready(r3d_hip_bricks *b) {
 }
 
 // Decodes each job's brick from its level's shard store straight into its
-// slot: the brick's 128^3 box as a region of the level (r3d_hip_shard_decode_region,
-// the reference reader shardio.c:269-306), zeros past the level's edge and in
-// missing or corrupt chunks.
+// slot: the brick's 128^3 box of its level, zeros past the level's edge and
+// in missing or corrupt chunks (the reference reader, shardio.c:269-306).
+// The bricks of a level go in one batched pass (one index check per shard,
+// one parse and one decode for all of them); a level whose dims are not
+// multiples of 16 takes one region decode per brick.
 int shard_fill(r3d_hip_ctx *ctx, r3d_hip_bricks *b, const BkJob *jobs, uint32_t n, void *stream) {
-  for (uint32_t j = 0; j < n; j++) {
-    const BkJob &q = jobs[j];
-    const std::vector<r3d_shard_src> &sh = b->lev_shards[q.level];
-    r3d_shard_region r{};
-    r.nz = b->dims[q.level][2];
-    r.ny = b->dims[q.level][1];
-    r.nx = b->dims[q.level][0];
-    r.z0 = (uint64_t)q.cz * BK;
-    r.y0 = (uint64_t)q.cy * BK;
-    r.x0 = (uint64_t)q.cx * BK;
-    r.dz = r.dy = r.dx = BK;
-    const int rc = r3d_hip_shard_decode_region(ctx, sh.empty() ? nullptr : sh.data(), (uint32_t)sh.size(), &r,
-                                               b->d_atlas + (uint64_t)q.slot * BK_VOX, nullptr, nullptr, stream);
-    if (rc != R3D_OK) return rc;
+  for (uint32_t l = 0; l < b->desc.nlev; l++) {
+    const std::vector<r3d_shard_src> &sh = b->lev_shards[l];
+    const uint64_t dims[3] = {b->dims[l][2], b->dims[l][1], b->dims[l][0]};
+    const bool batched = dims[0] % 16u == 0u && dims[1] % 16u == 0u && dims[2] % 16u == 0u;
+    std::vector<BrickTarget> tg;
+    for (uint32_t j = 0; j < n; j++) {
+      const BkJob &q = jobs[j];
+      if (q.level != l) continue;
+      if (batched) {
+        tg.push_back(BrickTarget{q.cx, q.cy, q.cz, q.slot});
+        continue;
+      }
+      r3d_shard_region r{};
+      r.nz = dims[0];
+      r.ny = dims[1];
+      r.nx = dims[2];
+      r.z0 = (uint64_t)q.cz * BK;
+      r.y0 = (uint64_t)q.cy * BK;
+      r.x0 = (uint64_t)q.cx * BK;
+      r.dz = r.dy = r.dx = BK;
+      const int rc = r3d_hip_shard_decode_region(ctx, sh.empty() ? nullptr : sh.data(), (uint32_t)sh.size(), &r,
+                                                 b->d_atlas + (uint64_t)q.slot * BK_VOX, nullptr, nullptr, stream);
+      if (rc != R3D_OK) return rc;
+    }
+    for (size_t i = 0; i < tg.size(); i += 64u) {
+      const uint32_t m = (uint32_t)(tg.size() - i < 64u ? tg.size() - i : 64u);
+      const int rc = shard_decode_bricks(ctx, sh.empty() ? nullptr : sh.data(), (uint32_t)sh.size(), dims,
+                                         tg.data() + i, m, b->d_atlas, (hipStream_t)stream);
+      if (rc != R3D_OK) return rc;
+    }
   }
   return R3D_OK;
 }

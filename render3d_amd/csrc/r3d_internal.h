@@ -197,4 +197,14 @@ struct RegionDst {
 int decode_region(r3d_hip_ctx *ctx, const r3d_chunk_batch *b, const RegionDst &rg, hipStream_t stream);
 // Grow-only device workspace of the context (shard region decodes).
 int workspace(r3d_hip_ctx *ctx, size_t bytes, void **out);
+// 128^3 bricks of one shard store decoded into atlas slots in one pass (one
+// index check per shard, one gather per brick, one parse and one decode for
+// all of them): brick (bx, by, bz) of a store of dims (nz, ny, nx), each a
+// multiple of 16, into slot `slot` (z-major 128^3 u8); voxels past the store
+// are zero. At most 64 bricks.
+struct BrickTarget {
+  uint32_t bx, by, bz, slot;
+};
+int shard_decode_bricks(r3d_hip_ctx *ctx, const r3d_shard_src *shards, uint32_t nshards, const uint64_t dims[3],
+                        const BrickTarget *t, uint32_t n, uint8_t *atlas, hipStream_t st);
 }  // namespace r3d

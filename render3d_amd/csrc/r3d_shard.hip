@@ -10,6 +10,8 @@
 // k_crc_pieces + k_crc_combine. All of it is integer or byte work bound by
 // the parse (one lane per blob); the crc and the index lookups read 4 MiB per
 // shard.
+#include <string.h>
+
 #include "r3d_internal.h"
 
 namespace r3d {
@@ -339,6 +341,123 @@ extern "C" int r3d_hip_crc32c(r3d_hip_ctx *ctx, const uint8_t *data, uint64_t n,
   int rc = workspace(ctx, (size_t)(np ? np : 1u) * 4u, &ws);
   if (rc) return rc;
   return crc_launch(ctx, data, n, (uint32_t *)ws, crc, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int r3d::shard_decode_bricks(r3d_hip_ctx *ctx, const r3d_shard_src *shards, uint32_t nshards, const uint64_t dims[3],
+                             const BrickTarget *t, uint32_t n, uint8_t *atlas, hipStream_t st) {
+  const uint64_t ncz = dims[0] / 16u, ncy = dims[1] / 16u, ncx = dims[2] / 16u;
+  if (!n) return R3D_OK;
+  if (n > 64u || dims[0] % 16u || dims[1] % 16u || dims[2] % 16u)
+    return r3d_fail(ctx, R3D_ERR_INVALID, "shard bricks: %u bricks of a store of %llux%llux%llu", n,
+                    (unsigned long long)dims[2], (unsigned long long)dims[1], (unsigned long long)dims[0]);
+  // each brick's chunk box (8^3, clamped to the store), and where each chunk goes
+  struct JobBox {
+    uint64_t cz0, cy0, cx0;
+    uint32_t dz, dy, dx, first;
+  };
+  std::vector<JobBox> jb(n);
+  std::vector<uint64_t> outoff;
+  uint32_t N = 0;
+  for (uint32_t j = 0; j < n; j++) {
+    JobBox &b = jb[j];
+    b.cz0 = (uint64_t)t[j].bz * 8u, b.cy0 = (uint64_t)t[j].by * 8u, b.cx0 = (uint64_t)t[j].bx * 8u;
+    b.dz = b.cz0 < ncz ? (uint32_t)(ncz - b.cz0 < 8u ? ncz - b.cz0 : 8u) : 0u;
+    b.dy = b.cy0 < ncy ? (uint32_t)(ncy - b.cy0 < 8u ? ncy - b.cy0 : 8u) : 0u;
+    b.dx = b.cx0 < ncx ? (uint32_t)(ncx - b.cx0 < 8u ? ncx - b.cx0 : 8u) : 0u;
+    b.first = N;
+    const uint64_t slot = (uint64_t)t[j].slot * 128u * 128u * 128u;
+    for (uint32_t iz = 0; iz < b.dz; iz++)
+      for (uint32_t iy = 0; iy < b.dy; iy++)
+        for (uint32_t ix = 0; ix < b.dx; ix++)
+          outoff.push_back(slot + (uint64_t)iz * 16u * 16384u + (uint64_t)iy * 16u * 128u + (uint64_t)ix * 16u);
+    N += b.dz * b.dy * b.dx;
+  }
+  // the store's shard grid and its table; a shard listed twice: the last one counts
+  const uint64_t nsz = (ncz + SH_GRID - 1u) / SH_GRID, nsy = (ncy + SH_GRID - 1u) / SH_GRID,
+                 nsx = (ncx + SH_GRID - 1u) / SH_GRID;
+  const uint64_t nsb = nsz * nsy * nsx;
+  std::vector<ShardDev> tab(nsb);
+  const uint8_t *base = nullptr;
+  for (uint32_t k = 0; k < nshards; k++) {
+    const r3d_shard_src &q = shards[k];
+    if (q.sz >= nsz || q.sy >= nsy || q.sx >= nsx) continue;
+    ShardDev &d = tab[((uint64_t)q.sz * nsy + q.sy) * nsx + q.sx];
+    d.bytes = q.bytes;
+    d.nbytes = q.nbytes;
+    d.present = 1;
+    d.ok = 0;
+    if (q.nbytes >= SH_INDEX && (!base || q.bytes < base)) base = q.bytes;
+  }
+  if (!base) base = atlas;  // no readable shard: every length is 0, nothing is read
+  SlabWs w;
+  const size_t slab_bytes = (slab_ws(N ? N : 1u, nullptr, &w) + 255u) & ~(size_t)255u;
+  const size_t crc_bytes = (size_t)((SH_N * 16ull + CRC_PIECE - 1u) / CRC_PIECE) * 4u;
+  void *ws = nullptr;
+  int rc = workspace(ctx, slab_bytes + crc_bytes, &ws);
+  if (rc) return rc;
+  slab_ws(N ? N : 1u, (uint8_t *)ws, &w);
+  uint32_t *crc_part = (uint32_t *)((uint8_t *)ws + slab_bytes);
+  // one staging slot: [shard table][chunk destinations]
+  const size_t tb = ((sizeof(ShardDev) * nsb + 15u) & ~(size_t)15u);
+  std::vector<uint8_t> buf(tb + 8u * (size_t)N);
+  memcpy(buf.data(), tab.data(), sizeof(ShardDev) * nsb);
+  if (N) memcpy(buf.data() + tb, outoff.data(), 8u * (size_t)N);
+  ViewSlot *sl = nullptr;
+  rc = stage(ctx, buf.data(), buf.size(), st, &sl);
+  if (rc) return rc;
+  ShardDev *dtab = (ShardDev *)sl->dev;
+  const uint64_t *doff = (const uint64_t *)((const uint8_t *)sl->dev + tb);
+  for (uint64_t i = 0; i < nsb && rc == R3D_OK; i++) {
+    if (!tab[i].present || tab[i].nbytes < SH_INDEX) continue;
+    const uint8_t *index = tab[i].bytes + (tab[i].nbytes - SH_INDEX);
+    rc = crc_launch(ctx, index, (uint64_t)SH_N * 16u, crc_part, nullptr, index + (uint64_t)SH_N * 16u, &dtab[i].ok,
+                    st);
+  }
+  for (uint32_t j = 0; j < n && rc == R3D_OK; j++) {  // bricks reaching past the store: zeros first
+    const JobBox &b = jb[j];
+    if (b.dz < 8u || b.dy < 8u || b.dx < 8u)
+      hipLaunchKernelGGL(k_zero, dim3(512), dim3(256), 0, st, atlas + (uint64_t)t[j].slot * 128u * 128u * 128u,
+                         (uint64_t)128u * 128u * 128u);
+    const uint32_t nch = b.dz * b.dy * b.dx;
+    if (!nch) continue;
+    GatherArgs g{nch,   0u, b.dy, b.dx,  b.cz0, b.cy0, b.cx0, 0u,        0u,         0u,
+                 (uint32_t)nsy, (uint32_t)nsx, dtab, base, w.offsets + b.first, w.lengths + b.first,
+                 w.state + b.first};
+    hipLaunchKernelGGL(k_shard_gather, dim3((nch + 255u) / 256u), dim3(256), 0, st, g);
+  }
+  if (rc == R3D_OK && hipGetLastError() != hipSuccess) rc = r3d_fail(ctx, R3D_ERR_DEVICE, "shard bricks: launch");
+  if (rc == R3D_OK && N) {
+    r3d_blob_batch in{N, 1u, SH_CAP, 0u, base, w.offsets, w.lengths};
+    r3d_blob_parse_out po{w.hdr,     w.steps,    w.pair_off, w.pair_cnt,  w.pairs,
+                          w.corr_off, w.corr_cnt, w.corr_pos, w.corr_delta, w.parse_st};
+    rc = r3d_hip_parse_blobs(ctx, &in, &po, (void *)st);
+    if (rc == R3D_OK) {
+      r3d_chunk_batch cb{};
+      cb.nchunks = N;
+      cb.coef_format = R3D_COEF_SPARSE;
+      cb.pair_offsets = w.pair_off;
+      cb.pairs = w.pairs;
+      cb.npairs = (uint64_t)N * SH_CAP;
+      cb.headers = w.hdr;
+      cb.step_tables = w.steps;
+      cb.ntables = N;
+      cb.corr_offsets = w.corr_off;
+      cb.corr_pos = w.corr_pos;
+      cb.corr_delta = w.corr_delta;
+      cb.ncorr = (uint64_t)N * SH_CAP;
+      cb.out = atlas;
+      cb.out_offsets = doff;
+      cb.out_stride_y = 128u;
+      cb.out_stride_z = 16384u;
+      cb.pair_counts = w.pair_cnt;
+      cb.corr_counts = w.corr_cnt;
+      const r3d_decode_opts o{R3D_DECODE_EXACT_F32, 0u};
+      rc = r3d_hip_decode_chunks(ctx, &cb, &o, (void *)st);
+    }
+  }
+  // the staged table and destinations are read up to the decode
+  if (hipEventRecord(sl->ev, st) == hipSuccess) sl->recorded = true;
+  return rc;
 }
 
 extern "C" int r3d_hip_shard_decode_region(r3d_hip_ctx *ctx, const r3d_shard_src *shards, uint32_t nshards,

@@ -196,3 +196,37 @@ def test_shard_source_rejections(ctx, zscene):
     page = b.read_page()
     assert (page[64:] == 0xFFFFFFFF).all() and not b.read_slot(0).any()
     b.close()
+
+
+def test_build_from_shards_levels_off_the_chunk_grid(ctx):
+    """Levels whose dims are not multiples of 16 (their fills take one region
+    decode per brick; the others one batched decode per level): every brick
+    of a 200 x 136 x 120 three-level pyramid, made resident at build, equals
+    the oracle reader's decode of its box, zero past the level."""
+    dims = (200, 136, 120)
+    nx, ny, nz = dims
+    vol = orc.synth_sheets(nx, ny, nz, 4, 13)
+    pyr = orc.pyramid(vol, nx, ny, nz, 3)
+    shards, decoded = [], []
+    for (x, y, z), lev in pyr:
+        px, py, pz = -(-x // 16) * 16, -(-y // 16) * 16, -(-z // 16) * 16  # encoded padded to whole chunks
+        pad = np.zeros((pz, py, px), np.uint8)
+        pad[:z, :y, :x] = lev.reshape(z, y, x)
+        sh = _shard_of_level(ctx, pad.reshape(-1), (px, py, pz))
+        rc, dst, counts, _ = orc.shard_decode_region([(sh, 0, 0, 0)], (z, y, x), (0, 0, 0, z, y, x))
+        assert rc == 0
+        shards.append(torch.from_numpy(sh).cuda())
+        decoded.append(((x, y, z), dst.reshape(-1)))
+    ldims = [d for d, _ in decoded]
+    pol = sp.StreamPolicy(ldims, 27, sp.brick_maxima(decoded))
+    b = r3d.Bricks(ctx, nx, ny, nz, 3, 3)
+    b.build_from_shards([[(s, 0, 0, 0)] for s in shards])
+    torch.cuda.synchronize()
+    z = {"decoded": decoded}
+    slots = b.read_slots(0, pol.nent)
+    page = b.read_page()
+    for e in range(pol.nent):
+        assert np.array_equal(slots[e], _brick(z, e, pol)), e
+        m = int(_brick(z, e, pol).max())
+        assert page[64 + e] == (e | m << 24 if m >= sp.NOISE_FLOOR else 0xFFFFFFFF), e
+    b.close()
