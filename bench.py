@@ -125,8 +125,8 @@ def read_traffic(path, key="traffic_bytes_per_launch"):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        print(f"bench.py: one GPU per process; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU sharding is a design, not built)", file=sys.stderr)
         return 2
     import numpy as np
     import torch

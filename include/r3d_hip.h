@@ -33,7 +33,7 @@ enum {
   R3D_ERR_INVALID = -1,     /* bad argument or malformed batch; nothing launched */
   R3D_ERR_NOMEM = -2,       /* device or host allocation failed */
   R3D_ERR_DEVICE = -3,      /* a HIP runtime call failed */
-  R3D_ERR_UNSUPPORTED = -4, /* a view kind / variant this build does not implement */
+  R3D_ERR_UNSUPPORTED = -4, /* a view kind / variant the library does not implement */
   R3D_ERR_RANGE = -5,       /* a size beyond a documented limit */
 };
 

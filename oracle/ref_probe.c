@@ -8,7 +8,8 @@
  *   - the dequantised coefficients (dct3d.c:565-566),
  *   - the inverse transform (dct3d.c:113-118),
  *   - the pre-round values and the u8 output (dct3d.c:568-593, 625-639).
- * Nothing here is part of the product; it is never built on the GPU box.
+ * Nothing here is part of the product; it is built only where the reference
+ * sources are present.
  */
 #include DCT3D_SRC
 

@@ -2,7 +2,7 @@
 //
 // The reference shader calls the GPU's own log2/exp2/pow
 // (src/shaders/raycast.slang:776-779, 906, 993), whose ulp behaviour is
-// vendor-defined. This build pins one definition so its host transcription
+// vendor-defined. The library pins one definition so its host transcription
 // and these kernels agree bit for bit:
 //
 //   log2f: x = 2^e * m, m in [1,2); if m > 1.41421356 then m *= 0.5, e += 1;
