@@ -87,6 +87,7 @@ extern "C" void r3d_hip_destroy(r3d_hip_ctx *c) {
   if (c->d_counters) hipFree(c->d_counters);
   if (c->d_queue) hipFree(c->d_queue);
   if (c->d_partials) hipFree(c->d_partials);
+  if (c->d_order) hipFree(c->d_order);
   if (c->d_gtf) hipFree(c->d_gtf);
   if (c->d_ws) hipFree(c->d_ws);
   if (c->d_dblim) hipFree(c->d_dblim);

@@ -54,6 +54,8 @@ struct r3d_hip_ctx {
   unsigned long long *d_counters = nullptr;  // 8 x u64 scratch for stats
   uint32_t *d_partials = nullptr;            // per-tile raycast counter partials
   size_t partials_cap = 0;                   // in tiles
+  uint32_t *d_order = nullptr;               // raycast launch order (bucket counters, tile codes, order)
+  size_t order_cap = 0;                      // in words
   float *d_gtf = nullptr;                    // adjoint: per-block TF gradient partials
   size_t gtf_cap = 0;                        // bytes
   r3d::ViewSlot slots[4];

@@ -56,6 +56,7 @@ struct RayArgs {
   uint32_t ntiles;     // tiles of this launch
   uint32_t *queue;     // tile counter of a persistent launch, or NULL
   uint32_t *touched;   // measurement: one bit per brick a footprint reads, or NULL
+  const uint32_t *order;  // launch order: the i-th workgroup (or queue entry) renders tile order[i]; NULL: tile i
 };
 
 // Samples a ray issues per memory round trip, per kernel kind.
@@ -881,6 +882,7 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
       __syncthreads();
     }
     if (tile >= a.ntiles) break;
+    if (a.order) tile = a.order[tile];
     // tile -> view: the last v with tile_start[v] <= tile
     uint32_t lo = 0, hi = a.nviews;
     while (hi - lo > 1u) {
@@ -995,6 +997,160 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
   if (threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
+// ---- launch order: the costliest tiles first ---------------------------
+// Workgroups start in blockIdx order, and a frame ends when its slowest tile
+// does. On the 1080p frame a tile's time follows its longest ray (up to ~490
+// samples, ~0.6 ms for one tile), and in raster order the long central tiles
+// start a third of the way into the frame, so the frame's second half runs
+// a few of them on an almost idle chip. Sorting the launch by an estimate of
+// each tile's march length starts them first; the short and empty tiles fill
+// in behind. Every tile is still rendered exactly once and no pixel depends
+// on the order.
+constexpr uint32_t ORDER_BUCKETS = 256u;
+// Tiles below this count (per launch) keep raster order: the two extra
+// launches would cost more than the tail they remove.
+constexpr uint32_t ORDER_MIN_TILES = 4096u;
+constexpr uint32_t ORDER_MAX_TILES = 1u << 24;  // a tile's rank inside its bucket takes 24 bits
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Estimated samples of the ray through pixel (px, py): the box entry and exit
+// of the march's own setup, then the step control's integral dt / (dt0 *
+// lod_step(t)) by an 8-point midpoint rule with hardware log2/exp2 (an
+// ordering key only; it shapes no pixel). ext: the volume box, md: its
+// largest dimension in voxels, budget: the step budget of the span step.
+__device__ float ray_cost(const r3d_frame_params &pc, float px, float py, V3 ext, float md, float max_lod,
+                          float budget) {
+  const V3 fwd = mk(pc.cam_forward[0], pc.cam_forward[1], pc.cam_forward[2]);
+  const V3 right = mk(pc.cam_right[0], pc.cam_right[1], pc.cam_right[2]);
+  const V3 up = mk(pc.cam_up[0], pc.cam_up[1], pc.cam_up[2]);
+  const bool ortho = (pc.view_flags & 1u) != 0u;
+  const float ndx = (px + 0.5f) / (float)pc.viewport[0] * 2.0f - 1.0f;
+  const float ndy = 1.0f - (py + 0.5f) / (float)pc.viewport[1] * 2.0f;
+  V3 dir = ortho ? vnorm(fwd) : vnorm(vadd(vadd(fwd, vscale(right, ndx)), vscale(up, ndy)));
+  V3 org = mk(pc.cam_origin[0], pc.cam_origin[1], pc.cam_origin[2]);
+  if (ortho) org = vadd(vadd(org, vscale(right, ndx)), vscale(up, ndy));
+  const V3 ctr = vscale(ext, 0.5f);
+  const V3 c0 = mk(pc.vol_r0[0], pc.vol_r1[0], pc.vol_r2[0]);
+  const V3 c1 = mk(pc.vol_r0[1], pc.vol_r1[1], pc.vol_r2[1]);
+  const V3 c2 = mk(pc.vol_r0[2], pc.vol_r1[2], pc.vol_r2[2]);
+  const V3 q = vsub(vsub(org, mk(pc.vol_tx, pc.vol_ty, pc.vol_tz)), ctr);
+  const V3 ro = vadd(mk(vdot(c0, q), vdot(c1, q), vdot(c2, q)), ctr);
+  dir = mk(vdot(c0, dir), vdot(c1, dir), vdot(c2, dir));
+  const V3 inv = mk(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+  const V3 t0 = mk((0.0f - ro.x) * inv.x, (0.0f - ro.y) * inv.y, (0.0f - ro.z) * inv.z);
+  const V3 t1 = mk((ext.x - ro.x) * inv.x, (ext.y - ro.y) * inv.y, (ext.z - ro.z) * inv.z);
+  const float hx = fmaxf(fmaxf(fminf(t0.x, t1.x), fminf(t0.y, t1.y)), fminf(t0.z, t1.z));
+  const float hy = fminf(fminf(fmaxf(t0.x, t1.x), fmaxf(t0.y, t1.y)), fmaxf(t0.z, t1.z));
+  const float tn = fmaxf(hx, 0.0f), tfar = fminf(hy, 1e30f);
+  if (!(tn < tfar)) return 0.0f;
+  const float voxel = 1.0f / md, dt = fmaxf(pc.step_voxels, 1e-6f) * voxel;
+  const float cone = 2.0f * vlen(up) / (float)pc.viewport[1];
+  const float span_step = fmaxf((tfar - tn) / (budget * dt), 1.0f);
+  const float h = (tfar - tn) * 0.125f;
+  float n = 0.0f;
+  for (int i = 0; i < 8; i++) {
+    const float tt = tn + ((float)i + 0.5f) * h;
+    const float fv = (ortho ? cone : tt * cone) * md;
+    const float lod = fminf(fmaxf(__log2f(fmaxf(fv, 1.0f)) + pc.lod_bias, 0.0f), max_lod);
+    n += h / (dt * fmaxf(exp2f(lod), span_step));
+  }
+  return n;
+}
+
+// A tile's bucket: the largest estimate over its corners and centre, log scale.
+template <int TW, int TH, bool BRICKS>
+__device__ uint32_t tile_bucket(const RayArgs &a, uint32_t tile) {
+  uint32_t lo = 0, hi = a.nviews;
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a.tile_start[mid] <= tile) lo = mid;
+    else hi = mid;
+  }
+  const r3d_frame_params &pc = a.views[lo];
+  const bool half = (pc.view_flags & 32u) != 0u;
+  const uint32_t W = pc.viewport[0], H = pc.viewport[1];
+  const uint32_t RW = half ? (W + 1u) / 2u : W, RH = half ? (H + 1u) / 2u : H;
+  const uint32_t tiles_x = (RW + TW - 1u) / TW;
+  const uint32_t lt = tile - a.tile_start[lo];
+  const uint32_t x0 = (lt % tiles_x) * TW, y0 = (lt / tiles_x) * TH;
+  const uint32_t x1 = min(x0 + TW - 1u, RW - 1u), y1 = min(y0 + TH - 1u, RH - 1u);
+  V3 ext = mk(1.0f, 1.0f, 1.0f);
+  float md, max_lod, budget;
+  if (BRICKS) {
+    const uint32_t nl = min(max(a.bk.page[0], 1u), 15u);
+    const uint32_t sx = a.bk.page[1], sy = a.bk.page[2], sz = a.bk.page[3];
+    const uint32_t m = max(max(sx, sy), max(sz, 1u));
+    md = (float)m;
+    ext = mk((float)sx / md, (float)sy / md, (float)sz / md);
+    max_lod = (float)(nl - 1u);
+    budget = pc.mode == MODE_FULL ? 192.0f : 256.0f;
+  } else {
+    md = (float)max(a.vol.maxdim, 1u);
+    max_lod = fminf(pc.max_mip, (float)(a.vol.nlev > 0u ? a.vol.nlev - 1u : 0u));
+    budget = (float)MAX_STEPS;
+  }
+  const float sc = half ? 2.0f : 1.0f;
+  const float xs[5] = {(float)x0, (float)x1, (float)x0, (float)x1, (float)((x0 + x1) / 2u)};
+  const float ys[5] = {(float)y0, (float)y0, (float)y1, (float)y1, (float)((y0 + y1) / 2u)};
+  float c = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 5; i++) c = fmaxf(c, ray_cost(pc, xs[i] * sc, ys[i] * sc, ext, md, max_lod, budget));
+  return c > 0.0f ? min((uint32_t)(__log2f(1.0f + c) * 24.0f), ORDER_BUCKETS - 1u) : 0u;
+}
+
+// Pass 1 of the order: each tile's key (the largest estimate over its
+// corners and centre, on a log scale in ORDER_BUCKETS buckets) and its rank
+// inside its bucket. code[tile] = bucket << 24 | rank. Ranks come from an
+// LDS histogram per workgroup plus one global atomic per occupied bucket (a
+// global atomic per tile serialises: most tiles of a frame share the empty
+// bucket).
+template <int TW, int TH, bool BRICKS>
+__global__ __launch_bounds__(256) void k_tile_cost(RayArgs a, uint32_t *code, uint32_t *counts) {
+  __shared__ uint32_t hist[ORDER_BUCKETS], base[ORDER_BUCKETS];
+  const uint32_t tile = blockIdx.x * 256u + threadIdx.x;
+  hist[threadIdx.x] = 0u;
+  __syncthreads();
+  uint32_t b = 0u, lr = 0u;
+  if (tile < a.ntiles) {
+    b = tile_bucket<TW, TH, BRICKS>(a, tile);
+    lr = atomicAdd(&hist[b], 1u);
+  }
+  __syncthreads();
+  const uint32_t hc = hist[threadIdx.x];
+  if (hc) base[threadIdx.x] = atomicAdd(&counts[threadIdx.x], hc);
+  __syncthreads();
+  if (tile < a.ntiles) code[tile] = b << 24 | (base[b] + lr);
+}
+
+// Pass 2: order[off(bucket) + rank] = tile, the buckets from the costliest
+// down (each workgroup rebuilds the 256 bucket offsets in LDS).
+__global__ __launch_bounds__(256) void k_tile_scatter(const uint32_t *code, const uint32_t *counts,
+                                                      uint32_t ntiles, uint32_t *order) {
+  __shared__ uint32_t sc[ORDER_BUCKETS];
+  const uint32_t t = threadIdx.x;
+  const uint32_t v = counts[ORDER_BUCKETS - 1u - t];  // descending bucket order
+  sc[t] = v;
+  __syncthreads();
+  for (uint32_t d = 1; d < ORDER_BUCKETS; d <<= 1) {
+    const uint32_t x = t >= d ? sc[t - d] : 0u;
+    __syncthreads();
+    sc[t] += x;
+    __syncthreads();
+  }
+  const uint32_t excl = sc[t] - v;  // tiles in costlier buckets than bucket 255 - t
+  __syncthreads();
+  sc[ORDER_BUCKETS - 1u - t] = excl;
+  __syncthreads();
+  const uint32_t tile = blockIdx.x * 256u + t;
+  if (tile < ntiles) {
+    const uint32_t c = code[tile];
+    order[sc[c >> 24] + (c & 0xffffffu)] = tile;
+  }
+}
+
 typedef void (*RayKernel)(RayArgs);
 #define R3D_RAY_SHAPES(K, B)                                                                     \
   { k_raycast<K, 8, 8, B>, k_raycast<K, 16, 4, B>, k_raycast<K, 16, 8, B>, k_raycast<K, 16, 16, B> }
@@ -1010,6 +1166,10 @@ static const RayKernel ray_kernels[2][NKINDS][NSHAPES] = {
     {R3D_RAY_SHAPES(KIND_FULL, true), R3D_RAY_SHAPES(KIND_MIP, true), R3D_RAY_SHAPES(KIND_FLAT, true),
      R3D_RAY_SHAPES(KIND_OTHER, true), R3D_RAY_SHAPES(KIND_FAST, true)}};
 #undef R3D_RAY_SHAPES
+typedef void (*CostKernel)(RayArgs, uint32_t *, uint32_t *);
+static const CostKernel cost_kernels[2][NSHAPES] = {
+    {k_tile_cost<8, 8, false>, k_tile_cost<16, 4, false>, k_tile_cost<16, 8, false>, k_tile_cost<16, 16, false>},
+    {k_tile_cost<8, 8, true>, k_tile_cost<16, 4, true>, k_tile_cost<16, 8, true>, k_tile_cost<16, 16, true>}};
 
 static int kind_of(uint32_t mode, uint32_t quality) {
   switch (mode) {
@@ -1088,21 +1248,51 @@ static int check_tuning(r3d_hip_ctx *ctx, const r3d_raycast_tuning *t) {
   return R3D_OK;
 }
 
+// Words of launch-order scratch a launch of `g` at any tuning needs: the
+// bucket counters of every kind, then each tile's code and its order entry.
+static size_t order_words(const ViewGroups &g) {
+  uint32_t most = 0;
+  for (int s = 0; s < NSHAPES; s++) {
+    uint32_t n = 0;
+    for (int k = 0; k < NKINDS; k++) n += g.gtiles[s][k];
+    most = n > most ? n : most;
+  }
+  return (size_t)NKINDS * ORDER_BUCKETS + 2u * (size_t)most;
+}
+
 // Kernel launches only (capture-safe): queue reset, one launch per kind,
 // counter reduction into ncount counters. `partials` holds partial_slots()
 // entries of two uint4. ncount 6 runs the 8x8 measurement instances (t must
-// then be 8x8, one workgroup per tile).
+// then be 8x8, one workgroup per tile). `order` (order_words(g) words, or
+// NULL for raster order) holds the launch order of kinds with at least
+// ORDER_MIN_TILES tiles, built by two small kernels ahead of each launch.
 static int launch_groups(r3d_hip_ctx *ctx, RayArgs a, const r3d_frame_params *dviews, const uint32_t *dtables,
                          const ViewGroups &g, const r3d_raycast_tuning &t, uint32_t *partials, uint32_t *queue,
-                         unsigned long long *counters, uint32_t ncount, hipStream_t s, bool bricks = false) {
+                         uint32_t *order, unsigned long long *counters, uint32_t ncount, hipStream_t s,
+                         bool bricks = false) {
   const bool measure = ncount == 6u;
   const int sh = shape_index(t.tile_w, t.tile_h);
   const uint32_t nt = SHAPE_W[sh] * SHAPE_H[sh];
   if (t.blocks_per_cu) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, queue, (uint32_t)NKINDS);
-  uint32_t pbase = 0;
+  uint32_t total = 0;
+  bool ordered = false;
+  for (int k = 0; k < NKINDS; k++) {
+    total += g.gtiles[sh][k];
+    ordered = ordered || (order && g.gstart[k + 1] > g.gstart[k] && g.gtiles[sh][k] >= ORDER_MIN_TILES &&
+                          g.gtiles[sh][k] <= ORDER_MAX_TILES);
+  }
+  uint32_t *codes = order ? order + NKINDS * ORDER_BUCKETS : nullptr;
+  uint32_t *perm = order ? codes + total : nullptr;
+  if (ordered) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, order, NKINDS * ORDER_BUCKETS);
+    R3D_HIP_CHECK(ctx, hipGetLastError());
+  }
+  uint32_t pbase = 0, tbase = 0;
   for (int k = 0; k < NKINDS; k++) {
     const uint32_t nv = g.gstart[k + 1] - g.gstart[k];
     const uint32_t tiles = g.gtiles[sh][k];
+    const uint32_t tb = tbase;
+    tbase += tiles;
     if (nv == 0 || tiles == 0) continue;
     uint32_t grid = tiles;
     if (t.blocks_per_cu) {
@@ -1116,6 +1306,16 @@ static int launch_groups(r3d_hip_ctx *ctx, RayArgs a, const r3d_frame_params *dv
     a.nviews = nv;
     a.ntiles = tiles;
     a.queue = t.blocks_per_cu ? queue + k : nullptr;
+    a.order = nullptr;
+    if (order && tiles >= ORDER_MIN_TILES && tiles <= ORDER_MAX_TILES) {
+      uint32_t *ck = order + (size_t)k * ORDER_BUCKETS;
+      const uint32_t og = (tiles + 255u) / 256u;
+      hipLaunchKernelGGL(cost_kernels[bricks ? 1 : 0][sh], dim3(og), dim3(256), 0, s, a, codes + tb, ck);
+      hipLaunchKernelGGL(k_tile_scatter, dim3(og), dim3(256), 0, s, (const uint32_t *)(codes + tb),
+                         (const uint32_t *)ck, tiles, perm + tb);
+      R3D_HIP_CHECK(ctx, hipGetLastError());
+      a.order = perm + tb;
+    }
     hipLaunchKernelGGL(measure ? measure_kernels[k] : ray_kernels[bricks ? 1 : 0][k][sh], dim3(grid), dim3(nt), 0,
                        s, a);
     R3D_HIP_CHECK(ctx, hipGetLastError());
@@ -1210,6 +1410,7 @@ struct r3d_hip_raycast_plan {
   uint32_t *d_tables = nullptr;
   uint32_t *d_queue = nullptr;
   uint32_t *d_partials = nullptr;
+  uint32_t *d_order = nullptr;
   r3d_hip_image out{};
   r3d_raycast_opts opts{};
   r3d_raycast_tuning tuning{8u, 8u, 0u, 0.0f};
@@ -1314,6 +1515,20 @@ static int raycast_impl(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_h
       return r3d_fail(ctx, R3D_ERR_NOMEM, "raycast: counter partials for %zu workgroups", cap);
     ctx->partials_cap = cap;
   }
+  const size_t owords = order_words(g);
+  if (ctx->order_cap < owords) {
+    if (ctx->d_order) {
+      R3D_HIP_CHECK(ctx, hipStreamSynchronize(s));
+      uint32_t *old = ctx->d_order;
+      ctx->d_order = nullptr;
+      ctx->order_cap = 0;
+      R3D_HIP_CHECK(ctx, hipFree(old));
+    }
+    const size_t cap = owords < (size_t)1 << 20 ? (size_t)1 << 20 : owords;
+    if (hipMalloc(&ctx->d_order, cap * 4u) != hipSuccess)
+      return r3d_fail(ctx, R3D_ERR_NOMEM, "raycast: launch order for %zu words", cap);
+    ctx->order_cap = cap;
+  }
   const r3d_frame_params *dviews = (const r3d_frame_params *)slot->dev;
   const uint32_t *dtables = (const uint32_t *)((const uint8_t *)slot->dev + g.view_bytes());
   RayArgs a = base_args(ctx, vol, out, opts);
@@ -1322,7 +1537,8 @@ static int raycast_impl(r3d_hip_ctx *ctx, const r3d_hip_volume *vol, const r3d_h
   if (a.touched && !measure)
     return r3d_fail(ctx, R3D_ERR_INVALID, "raycast: a touched-brick bitmap needs R3D_RAYCAST_MEASURE");
   if (stats) R3D_HIP_CHECK(ctx, hipEventRecord(ctx->ev_ray[0], s));
-  rc = launch_groups(ctx, a, dviews, dtables, g, tuning, ctx->d_partials, ctx->d_queue, counters, ncount, s,
+  rc = launch_groups(ctx, a, dviews, dtables, g, tuning, ctx->d_partials, ctx->d_queue, ctx->d_order, counters,
+                     ncount, s,
                      bk != nullptr);
   if (rc != R3D_OK) return rc;
   if (stats) R3D_HIP_CHECK(ctx, hipEventRecord(ctx->ev_ray[1], s));
@@ -1420,7 +1636,8 @@ extern "C" int r3d_hip_raycast_plan_create(r3d_hip_ctx *ctx, const r3d_hip_volum
   const size_t off_tab = (p->g.view_bytes() + 255u) & ~(size_t)255u;
   const size_t off_q = (off_tab + sizeof(uint32_t) * p->g.table_words() * NSHAPES + 255u) & ~(size_t)255u;
   const size_t off_part = off_q + 256u;
-  const size_t total = off_part + (size_t)slots * 32u;
+  const size_t off_order = (off_part + (size_t)slots * 32u + 255u) & ~(size_t)255u;
+  const size_t total = off_order + order_words(p->g) * 4u;
   if (hipMalloc(&p->d_mem, total) != hipSuccess) {
     free(buf);
     delete p;
@@ -1430,6 +1647,7 @@ extern "C" int r3d_hip_raycast_plan_create(r3d_hip_ctx *ctx, const r3d_hip_volum
   p->d_tables = (uint32_t *)((uint8_t *)p->d_mem + off_tab);
   p->d_queue = (uint32_t *)((uint8_t *)p->d_mem + off_q);
   p->d_partials = (uint32_t *)((uint8_t *)p->d_mem + off_part);
+  p->d_order = (uint32_t *)((uint8_t *)p->d_mem + off_order);
   hipError_t e = hipMemcpy(p->d_views, buf, p->g.view_bytes(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(p->d_tables, buf + p->g.view_bytes(), sizeof(uint32_t) * p->g.table_words() * NSHAPES,
@@ -1450,7 +1668,7 @@ extern "C" int r3d_hip_raycast_plan_launch(r3d_hip_raycast_plan *p, void *stream
   R3D_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   r3d_panes_reset(ctx);
   return launch_groups(ctx, base_args(ctx, p->vol, &p->out, &p->opts), p->d_views, p->d_tables, p->g, p->tuning,
-                       p->d_partials, p->d_queue, (unsigned long long *)p->opts.counters,
+                       p->d_partials, p->d_queue, p->d_order, (unsigned long long *)p->opts.counters,
                        4u, (hipStream_t)stream);
 }
 
@@ -1491,11 +1709,13 @@ extern "C" int r3d_hip_raycast_plan_tune(r3d_hip_raycast_plan *p, void *stream, 
   for (int sh = 0; sh < NSHAPES && rc == R3D_OK; sh++)
     for (int b = 0; b < 5 && rc == R3D_OK; b++) {
       r3d_raycast_tuning t{SHAPE_W[sh], SHAPE_H[sh], bpc[b], 0.0f};
-      rc = launch_groups(ctx, a, p->d_views, p->d_tables, p->g, t, p->d_partials, p->d_queue, nullptr, 4u, s);
+      rc = launch_groups(ctx, a, p->d_views, p->d_tables, p->g, t, p->d_partials, p->d_queue, p->d_order, nullptr,
+                         4u, s);
       if (rc != R3D_OK) break;
       if (hipEventRecord(e0, s) != hipSuccess) rc = R3D_ERR_DEVICE;
       for (int r = 0; r < 3 && rc == R3D_OK; r++)
-        rc = launch_groups(ctx, a, p->d_views, p->d_tables, p->g, t, p->d_partials, p->d_queue, nullptr, 4u, s);
+        rc = launch_groups(ctx, a, p->d_views, p->d_tables, p->g, t, p->d_partials, p->d_queue, p->d_order, nullptr,
+                         4u, s);
       float ms = 0.0f;
       if (rc == R3D_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
