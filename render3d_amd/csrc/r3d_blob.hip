@@ -306,14 +306,25 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
     // sector) when the capacity and the array keep such groups aligned: a
     // scattered 4-byte store per pair made the memory system fetch and write
     // whole sectors, and on this ISA every store also holds up the next wait
-    // for a window refill. A lane stages its pairs 0..6 of a group in LDS, in
-    // the EOB context rows (44..57: 14 x 256 halfwords = 7 x 256 dwords),
-    // which the level loop and everything after it never touch; the eighth
-    // completes the group from a register. (A register shift of 8 costs 8
-    // moves in nearly every iteration, since some lane of the wave is nearly
-    // always at a sign; groups of 4 in registers wrote each sector twice.)
+    // for a window refill. A lane stages its pairs 0..6 of a group in its own
+    // EOB context halfwords (rows 44..57), which its level loop and
+    // everything after it never touch: slot j's low half in row 44 + 2j, its
+    // high half in row 45 + 2j. Only the lane's own halfwords: the other
+    // waves of the workgroup are still initialising or reading their
+    // contexts in the same rows (a 32-bit slot would cover two other lanes'
+    // halfwords). The eighth pair completes the group from a register. (A
+    // register shift of 8 costs 8 moves in nearly every iteration, since some
+    // lane of the wave is nearly always at a sign; groups of 4 in registers
+    // wrote each sector twice.)
     const bool vec8 = (a.cap & 7u) == 0u && ((uintptr_t)a.pairs & 31u) == 0u;
-    uint32_t *stg = (uint32_t *)(s_ctx + CTX_EOB * BLOB_NT) + t;  // slot j at stg[j * BLOB_NT]
+    uint16_t *stg = d.ctx + CTX_EOB * BLOB_NT;  // slot j: stg[2j * BLOB_NT] (low), stg[(2j + 1) * BLOB_NT] (high)
+    const auto stg_put = [&](uint32_t j, uint32_t v) {
+      stg[2u * j * BLOB_NT] = (uint16_t)v;
+      stg[(2u * j + 1u) * BLOB_NT] = (uint16_t)(v >> 16);
+    };
+    const auto stg_get = [&](uint32_t j) -> uint32_t {
+      return (uint32_t)stg[2u * j * BLOB_NT] | (uint32_t)stg[(2u * j + 1u) * BLOB_NT] << 16;
+    };
     uint32_t e = s_scan[0];
     uint32_t ph = eob == 0u ? PH_DONE : eob == 1u ? PH_MAG : PH_SIG;
     if (eob == 1u) recent = 1u;
@@ -346,10 +357,10 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
             const uint32_t slot = np & 7u;
             if (slot == 7u) {
               uint4 *q = (uint4 *)(a.pairs + base + np - 7u);
-              q[0] = make_uint4(stg[0], stg[BLOB_NT], stg[2u * BLOB_NT], stg[3u * BLOB_NT]);
-              q[1] = make_uint4(stg[4u * BLOB_NT], stg[5u * BLOB_NT], stg[6u * BLOB_NT], pr);
+              q[0] = make_uint4(stg_get(0), stg_get(1), stg_get(2), stg_get(3));
+              q[1] = make_uint4(stg_get(4), stg_get(5), stg_get(6), pr);
             } else {
-              stg[slot * BLOB_NT] = pr;
+              stg_put(slot, pr);
             }
           } else {
             a.pairs[base + np] = pr;
@@ -382,7 +393,7 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
     }
     if (vec8) {  // the last np % 8 pairs are staged in slots 0 .. np % 8 - 1
       const uint32_t r = np & 7u;
-      for (uint32_t k = 0; k < r; k++) a.pairs[base + np - r + k] = stg[k * BLOB_NT];
+      for (uint32_t k = 0; k < r; k++) a.pairs[base + np - r + k] = stg_get(k);
     }
     if (has_corr) {  // dct3d.c:572-587
       uint32_t cnt = d.eg() + 1u, pos = 0;

@@ -265,6 +265,29 @@ def test_damaged_blobs_in_length_order_match_the_unsorted_parse(ctx):
     assert {-1, 0, 1} <= set(big["status"].tolist())
 
 
+def test_repeated_parses_agree_with_the_reference(ctx):
+    """Every lane stages its pair groups in its own LDS halfwords only: the
+    other waves of a workgroup are still writing their priors and EOB
+    contexts in the same rows, at a timing that differs from run to run (a
+    32-bit staging slot over two other lanes' halfwords once turned a -1
+    level into +1). Repeated parses of mixed batches, lane-sorted and not,
+    must all equal the reference's levels."""
+    n = len(GOLD["blob_names"])
+    byts, offs = GOLD["blob_bytes"], GOLD["blob_offsets"].astype(np.int64)
+    for rep, zero_every in ((52, 0), (12, 3)):  # 2,080 entries in length order; 640 with zero-length entries mixed in
+        data, starts, lens = _tiled(byts, offs, rep, zero_every)
+        want = np.tile(GOLD["blob_levels"], (rep, 1))
+        real = lens > 0
+        for _ in range(8):
+            got = _parse_np(ctx, data, starts, lens)
+            assert np.all(got["status"][real] == 1) and np.all(got["status"][~real] == 0)
+            assert np.array_equal(got["levels"][real], want)
+    data, starts, lens = _tiled(byts, offs, 11)  # 440 entries: lanes in chunk order
+    for _ in range(8):
+        got = _parse_np(ctx, data, starts, lens)
+        assert np.array_equal(got["levels"], np.tile(GOLD["blob_levels"], (11, 1)))
+
+
 def test_captured_parse_survives_a_larger_eager_parse():
     """A HIP graph holding a length-ordered parse keeps its lane-order buffer
     when a later eager parse on the same context needs a larger one (the old
