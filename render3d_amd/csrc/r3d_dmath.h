@@ -31,21 +31,22 @@ __device__ __forceinline__ float fmax_sel(float a, float b) { return a > b ? a :
 __device__ __forceinline__ float fclamp_sel(float x, float lo, float hi) {
   return fmin_sel(fmax_sel(x, lo), hi);
 }
-__device__ __forceinline__ float fsat(float x) { return fclamp_sel(x, 0.0f, 1.0f); }
+// fclamp_sel for lo <= hi in one v_med3_f32: the median of (x, lo, hi) is
+// the select form's result for every x (NaN included: both give lo; checked
+// bit for bit on the GPU over ordinary values, +-0, +-inf, NaN and random
+// bit patterns).
+__device__ __forceinline__ float fclamp_ord(float x, float lo, float hi) {
+  return __builtin_amdgcn_fmed3f(x, lo, hi);
+}
+__device__ __forceinline__ float fsat(float x) { return fclamp_ord(x, 0.0f, 1.0f); }
 // lerp pinned as fma(t, b - a, a)
 __device__ __forceinline__ float flerp(float a, float b, float t) {
   return __builtin_fmaf(t, b - a, a);
 }
 
-__device__ __forceinline__ float log2_pinned(float x) {
-  if (x != x || x < 0.0f) return __builtin_nanf("");
-  if (x == 0.0f) return -__builtin_inff();
-  if (x == __builtin_inff()) return x;
-  int32_t e0 = 0;
-  if (x < 0x1p-126f) {
-    x = x * 0x1p23f;
-    e0 = -23;
-  }
+// log2_pinned's arithmetic for a positive, finite, normal x (e0: the
+// exponent bias of a pre-scaled subnormal).
+__device__ __forceinline__ float log2_core(float x, int32_t e0 = 0) {
   uint32_t u = __float_as_uint(x);
   int32_t e = (int32_t)((u >> 23) & 0xffu) - 127 + e0;
   float m = __uint_as_float((u & 0x7fffffu) | 0x3f800000u);
@@ -67,10 +68,23 @@ __device__ __forceinline__ float log2_pinned(float x) {
   return (float)e + f * p;
 }
 
-__device__ __forceinline__ float exp2_pinned(float x) {
-  if (x != x) return x;
-  if (x < -125.0f) return 0.0f;
-  if (x >= 127.5f) return __builtin_inff();
+__device__ __forceinline__ float log2_pinned(float x) {
+  if (x != x || x < 0.0f) return __builtin_nanf("");
+  if (x == 0.0f) return -__builtin_inff();
+  if (x == __builtin_inff()) return x;
+  const bool sub = x < 0x1p-126f;
+  return log2_core(sub ? x * 0x1p23f : x, sub ? -23 : 0);
+}
+
+// log2_pinned for x >= 1 (NaN excluded): only +inf needs its own result,
+// taken with a select instead of a branch.
+__device__ __forceinline__ float log2_ge1(float x) {
+  const float r = log2_core(x);
+  return x < __builtin_inff() ? r : x;
+}
+
+// exp2_pinned's arithmetic for -125 <= x < 127.5.
+__device__ __forceinline__ float exp2_core(float x) {
   float k = __builtin_floorf(x + 0.5f);
   float r = x - k;
   float q = 0x1.446c7ep-13f;
@@ -84,10 +98,19 @@ __device__ __forceinline__ float exp2_pinned(float x) {
   return q * __uint_as_float((uint32_t)(ki + 127) << 23);
 }
 
+__device__ __forceinline__ float exp2_pinned(float x) {
+  if (x != x) return x;
+  if (x < -125.0f) return 0.0f;
+  if (x >= 127.5f) return __builtin_inff();
+  return exp2_core(x);
+}
+
 __device__ __forceinline__ float pow_unit(float b, float y) {
   if (!(b > 0.0f)) return 0.0f;
   if (b >= 1.0f) return 1.0f;
-  return exp2_pinned(y * log2_pinned(b));
+  // 0 < b < 1: log2_pinned's special cases reduce to the subnormal scaling
+  const bool sub = b < 0x1p-126f;
+  return exp2_pinned(y * log2_core(sub ? b * 0x1p23f : b, sub ? -23 : 0));
 }
 
 __device__ __forceinline__ float pow24(float x) {

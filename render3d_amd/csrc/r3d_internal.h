@@ -18,12 +18,14 @@
 
 namespace r3d {
 
-// Per-level descriptor as the kernels read it (32 bytes, staged in LDS).
+// Per-level descriptor as the kernels read it (48 bytes, staged in LDS).
 struct LevelDev {
-  uint32_t off_lo, off_hi;  // byte offset of the level's first brick
-  uint32_t bx, by;          // bricks along x and y
   float fx, fy, fz;         // level dims as floats
-  uint32_t bz;
+  uint32_t off_lo;          // byte offset of the level's first brick: low word
+  float hx, hy, hz;         // fx - 1, fy - 1, fz - 1: a footprint origin's upper clamp
+  uint32_t off_hi;          // and high word
+  uint32_t bx, by, bz;      // bricks along x, y and z
+  uint32_t pad;
 };
 
 struct VolDev {
@@ -33,6 +35,7 @@ struct VolDev {
   uint32_t nlev;
   uint32_t maxdim;
   uint32_t ox, oy, oz;
+  uint32_t narrow;          // the pyramid's bytes fit a 32-bit offset
 };
 
 struct ViewSlot {

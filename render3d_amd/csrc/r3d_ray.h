@@ -36,7 +36,19 @@ __device__ __forceinline__ V3 vnorm(V3 a) {
   return mk(a.x / l, a.y / l, a.z / l);
 }
 
-struct Sampler {
+// A trilinear footprint: its four x-pairs of voxels as loaded (low byte at
+// x0, high byte at x0 + 1) and the fractions. The bytes are taken apart
+// only when the footprint is used, so a footprint can stay in flight.
+struct TriFoot {
+  uint32_t pr[4];
+  float fx, fy, fz;
+};
+
+// WIDE: 64-bit byte offsets into the pyramid; otherwise (a pyramid below
+// 4 GiB) 32-bit offsets on the scalar base, so every read is one
+// base + 32-bit-offset load and the brick arithmetic stays in 32 bits.
+template <bool WIDE>
+struct SamplerT {
   const uint8_t *base;
   const LevelDev *lev;  // LDS
   uint32_t nlev;
@@ -44,13 +56,7 @@ struct Sampler {
   // footprint read (a compile-time null in the rendering kernels)
   uint32_t *touched = nullptr;
 
-  // A trilinear footprint: its four x-pairs of voxels as loaded (low byte at
-  // x0, high byte at x0 + 1) and the fractions. The bytes are taken apart
-  // only when the footprint is used, so a footprint can stay in flight.
-  struct Foot {
-    uint32_t pr[4];
-    float fx, fy, fz;
-  };
+  using Foot = TriFoot;
 
   __device__ __forceinline__ static uint32_t pair_at(const uint8_t *q) {
     uint16_t v;
@@ -59,22 +65,27 @@ struct Sampler {
   }
 
   __device__ __forceinline__ Foot load(uint32_t l, V3 p) const {
-    const LevelDev L = lev[l];
-    const float tx = fclamp_sel(p.x * L.fx - 0.5f, 0.0f, L.fx - 1.0f);
-    const float ty = fclamp_sel(p.y * L.fy - 0.5f, 0.0f, L.fy - 1.0f);
-    const float tz = fclamp_sel(p.z * L.fz - 0.5f, 0.0f, L.fz - 1.0f);
+    const LevelDev &L = lev[l];
+    const float tx = fclamp_ord(p.x * L.fx - 0.5f, 0.0f, L.hx);
+    const float ty = fclamp_ord(p.y * L.fy - 0.5f, 0.0f, L.hy);
+    const float tz = fclamp_ord(p.z * L.fz - 0.5f, 0.0f, L.hz);
     const uint32_t x0 = (uint32_t)tx, y0 = (uint32_t)ty, z0 = (uint32_t)tz;
     Foot f;
     f.fx = tx - (float)x0;
     f.fy = ty - (float)y0;
     f.fz = tz - (float)z0;
     const uint32_t b = ((z0 >> 4) * L.by + (y0 >> 4)) * L.bx + (x0 >> 4);
-    const uint64_t off = ((uint64_t)L.off_hi << 32) | L.off_lo;
-    const uint8_t *q = base + off + (uint64_t)b * R3D_BRICK_BYTES +
-                       (((z0 & 15u) * 17u + (y0 & 15u)) * 17u + (x0 & 15u));
-    if (touched) {  // levels start 256-byte aligned, less than a brick past their predecessors' bricks
-      const uint64_t gb = off / R3D_BRICK_BYTES + b;
-      atomicOr(touched + (gb >> 5), 1u << (gb & 31u));
+    const uint32_t in = ((z0 & 15u) * 17u + (y0 & 15u)) * 17u + (x0 & 15u);
+    const uint8_t *q;
+    if (WIDE) {
+      const uint64_t off = ((uint64_t)L.off_hi << 32) | L.off_lo;
+      q = base + off + (uint64_t)b * R3D_BRICK_BYTES + in;
+      if (touched) {  // levels start 256-byte aligned, less than a brick past their predecessors' bricks
+        const uint64_t gb = off / R3D_BRICK_BYTES + b;
+        atomicOr(touched + (gb >> 5), 1u << (gb & 31u));
+      }
+    } else {
+      q = base + (uint64_t)(L.off_lo + b * R3D_BRICK_BYTES + in);
     }
     f.pr[0] = pair_at(q);
     f.pr[1] = pair_at(q + 17);
@@ -121,21 +132,24 @@ struct Sampler {
   }
 };
 
-// The occupancy byte of the 8^3 block holding p (clamped to the grid).
+using Sampler = SamplerT<true>;
+
+// The occupancy byte of the 8^3 block holding p (clamped to the grid; the
+// grid dims are at least 1).
 __device__ __forceinline__ uint32_t occ_load(const VolDev &V, V3 p) {
   const float ox = __builtin_floorf(p.x * (float)V.ox);
   const float oy = __builtin_floorf(p.y * (float)V.oy);
   const float oz = __builtin_floorf(p.z * (float)V.oz);
-  const uint32_t ix = (uint32_t)fclamp_sel(ox, 0.0f, (float)(V.ox - 1u));
-  const uint32_t iy = (uint32_t)fclamp_sel(oy, 0.0f, (float)(V.oy - 1u));
-  const uint32_t iz = (uint32_t)fclamp_sel(oz, 0.0f, (float)(V.oz - 1u));
-  return V.occ[((size_t)iz * V.oy + iy) * V.ox + ix];
+  const uint32_t ix = (uint32_t)fclamp_ord(ox, 0.0f, (float)(V.ox - 1u));
+  const uint32_t iy = (uint32_t)fclamp_ord(oy, 0.0f, (float)(V.oy - 1u));
+  const uint32_t iz = (uint32_t)fclamp_ord(oz, 0.0f, (float)(V.oz - 1u));
+  return V.occ[(iz * V.oy + iy) * V.ox + ix];  // fewer than 2^32 blocks in any volume that fits in memory
 }
 
 __device__ __forceinline__ float occ_sample(const VolDev &V, V3 p) { return (float)occ_load(V, p) * K255; }
 
 __device__ __forceinline__ float4 tf_sample(const float4 *tf, float v) {
-  const float t = fclamp_sel(v * 256.0f - 0.5f, 0.0f, 255.0f);
+  const float t = fclamp_ord(v * 256.0f - 0.5f, 0.0f, 255.0f);
   const uint32_t i0 = (uint32_t)t;
   const uint32_t i1 = i0 + 1u < 256u ? i0 + 1u : 255u;
   const float f = t - (float)i0;

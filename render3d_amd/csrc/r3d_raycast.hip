@@ -27,6 +27,8 @@
 
 #include <new>
 
+#include <type_traits>
+
 #include "r3d_ray.h"
 
 namespace r3d {
@@ -57,6 +59,7 @@ struct RayArgs {
   uint32_t *queue;     // tile counter of a persistent launch, or NULL
   uint32_t *touched;   // measurement: one bit per brick a footprint reads, or NULL
   const uint32_t *order;  // launch order: the i-th workgroup (or queue entry) renders tile order[i]; NULL: tile i
+  uint32_t tier2;      // k_raycast_tiers: ranks below tier2 take two lanes per ray
 };
 
 // Samples a ray issues per memory round trip, per kernel kind.
@@ -193,20 +196,23 @@ __device__ __forceinline__ void blinn_phong(V3 g, V3 lvec, V3 hvec, float &cr, f
 // a zero interval gets alpha lerp(0, 0, f) = 0, so its opacity
 // 1 - pow(1 - 0, k) is 0 and it adds nothing; i0 is tf_sample's index.
 __device__ __forceinline__ bool tf_visible(const uint32_t *opq, float v) {
-  const uint32_t i0 = (uint32_t)fclamp_sel(v * 256.0f - 0.5f, 0.0f, 255.0f);
+  const uint32_t i0 = (uint32_t)fclamp_ord(v * 256.0f - 0.5f, 0.0f, 255.0f);
   return ((opq[i0 >> 5] >> (i0 & 31u)) & 1u) != 0u;
 }
 
-// The per-ray march (MEASURE: also count blended samples, for the
-// measurement instances only). KIND fixes the compositing / shading branches at compile
-// time, as the reference builds one pipeline per mode (src/shaders/shaders.cmake:17-30).
-template <int KIND, bool MEASURE>
-__device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const float4 *tf, const uint32_t *opq,
-                                        const r3d_frame_params &pc, float cutoff, uint32_t px, uint32_t py,
-                                        float *coop) {
-  RayOut R;
-  R.steps = R.skips = R.shaded = R.blends = R.blends_shaded = 0;
-  const uint32_t mode = SHADED_KIND<KIND> ? MODE_FULL : KIND == KIND_MIP ? MODE_MIP : KIND == KIND_FLAT ? MODE_FLAT : pc.mode;
+// A cube-mode ray up to its march loop: ray setup, model transform, box hit,
+// cone, step control and jitter (raycast.slang:469-495, 677-728, 743-748).
+struct RaySetup {
+  V3 dir, ro, lvec, hvec;
+  float tn, tfar, t0, voxel, fdim, max_lod, cone, dt, span_step, bs, occ_gate, opac_k, sgx, sgy, sgz;
+  bool ortho, dim_p2;
+};
+
+// Fills G and returns true when the ray marches; otherwise R holds its pixel
+// (the RAYDIR colour, or the background of a ray that misses the box).
+template <int KIND>
+__device__ __forceinline__ bool ray_setup(const VolDev &V, const r3d_frame_params &pc, uint32_t mode, uint32_t px,
+                                          uint32_t py, RaySetup &G, RayOut &R) {
   const V3 fwd = mk(pc.cam_forward[0], pc.cam_forward[1], pc.cam_forward[2]);
   const V3 right = mk(pc.cam_right[0], pc.cam_right[1], pc.cam_right[2]);
   const V3 up = mk(pc.cam_up[0], pc.cam_up[1], pc.cam_up[2]);
@@ -220,13 +226,14 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
     R.c[1] = dir.y * 0.5f + 0.5f;
     R.c[2] = dir.z * 0.5f + 0.5f;
     R.c[3] = 1.0f;
-    return R;
+    return false;
   }
-  const float voxel = 1.0f / (float)V.maxdim;
+  G.ortho = ortho;
+  G.voxel = 1.0f / (float)V.maxdim;
   // with a power-of-two largest dim, voxel is exact and x / voxel == x * maxdim
-  const bool dim_p2 = (V.maxdim & (V.maxdim - 1u)) == 0u;
-  const float fdim = (float)V.maxdim;
-  const float max_lod = fmin_sel(pc.max_mip, (float)(V.nlev - 1u));
+  G.dim_p2 = (V.maxdim & (V.maxdim - 1u)) == 0u;
+  G.fdim = (float)V.maxdim;
+  G.max_lod = fmin_sel(pc.max_mip, (float)(V.nlev - 1u));
   V3 org = mk(pc.cam_origin[0], pc.cam_origin[1], pc.cam_origin[2]);
   if (ortho) org = vadd(vadd(org, vscale(right, ndx)), vscale(up, ndy));
   // common.slang:45-52 with ext = (1,1,1), center 0.5
@@ -249,30 +256,108 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
     R.c[1] = 0.02f;
     R.c[2] = 0.03f;
     R.c[3] = 1.0f;
-    return R;
+    return false;
   }
-  const float cone = 2.0f * vlen(up) / (float)pc.viewport[1];
-  const float dt = pc.step_voxels * voxel;
-  float t = tn + hash12(px, py, pc.frame_index) * dt;
+  G.dir = dir;
+  G.ro = ro;
+  G.tn = tn;
+  G.tfar = tfar;
+  G.cone = 2.0f * vlen(up) / (float)pc.viewport[1];
+  G.dt = pc.step_voxels * G.voxel;
+  G.t0 = tn + hash12(px, py, pc.frame_index) * G.dt;
+  G.span_step = fmax_sel((tfar - tn) / ((float)MAX_STEPS * G.dt), 1.0f);
+  G.bs = 8.0f * G.voxel;
+  G.occ_gate = fmax_sel(pc.skip_gate, 0.5f / 255.0f);
+  G.opac_k = pc.density * pc.step_voxels;
+  G.lvec = vscale(dir, -1.0f);
+  G.hvec = vnorm(vsub(G.lvec, dir));
+  G.sgx = dir.x > 0.0f ? 1.0f : 0.0f;
+  G.sgy = dir.y > 0.0f ? 1.0f : 0.0f;
+  G.sgz = dir.z > 0.0f ? 1.0f : 0.0f;
+  return true;
+}
+
+// The lod and lod step of a sample at t (raycast.slang:767-780); fv >= 1 and
+// max_lod in [0, 15], so log2 and exp2 need none of their special cases.
+__device__ __forceinline__ void lod_at(const RaySetup &G, float lod_bias, float tt, float &lod, float &lod_step) {
+  const float foot = G.ortho ? G.cone : tt * G.cone;
+  const float fv = G.dim_p2 ? foot * G.fdim : foot / G.voxel;
+  lod = fclamp_ord(log2_ge1(fmax_sel(fv, 1.0f)) + lod_bias, 0.0f, G.max_lod);
+  lod_step = fmax_sel(exp2_core(lod), G.span_step);
+}
+
+// Where the empty-space skip of a sample at (t, p) resumes: the next 8^3
+// block face along the ray plus 1% of a block (raycast.slang:865-868).
+__device__ __forceinline__ float skip_target(const RaySetup &G, V3 p, float t) {
+  const float bs = G.bs;
+  const float nx = (__builtin_floorf(p.x / bs) + G.sgx) * bs;
+  const float ny = (__builtin_floorf(p.y / bs) + G.sgy) * bs;
+  const float nz = (__builtin_floorf(p.z / bs) + G.sgz) * bs;
+  const float ex = __builtin_fabsf(G.dir.x) > 1e-8f ? (nx - G.ro.x) / G.dir.x : 1e30f;
+  const float ey = __builtin_fabsf(G.dir.y) > 1e-8f ? (ny - G.ro.y) / G.dir.y : 1e30f;
+  const float ez = __builtin_fabsf(G.dir.z) > 1e-8f ? (nz - G.ro.z) / G.dir.z : 1e30f;
+  return fmax_sel(fmin_sel(fmin_sel(ex, ey), ez), t) + bs * 0.01f;
+}
+
+// The pixel of a finished march (raycast.slang:1023-1038).
+template <int KIND>
+__device__ __forceinline__ void ray_finish(RayOut &R, uint32_t mode, const r3d_frame_params &pc, uint32_t steps,
+                                           float ar, float ag, float ab, float aa, float mip_max, float first_hit) {
+  const bool composite = mode == MODE_FULL || mode == MODE_FLAT;
+  if (KIND == KIND_OTHER && mode == MODE_HEATMAP) {
+    const float x = (float)steps / 1024.0f;
+    R.c[0] = x;
+    R.c[1] = 1.0f - __builtin_fabsf(x - 0.5f) * 2.0f;
+    R.c[2] = 1.0f - x;
+  } else if (KIND == KIND_OTHER && mode == MODE_DEPTH) {
+    const float d = first_hit < 0.0f ? 0.0f : 1.0f - fsat(first_hit / 2.0f);
+    R.c[0] = d;
+    R.c[1] = d;
+    R.c[2] = d;
+  } else if (composite) {
+    const float ta = 1.0f - aa;
+    R.c[0] = ar + ta * 0.02f;
+    R.c[1] = ag + ta * 0.02f;
+    R.c[2] = ab + ta * 0.03f;
+  } else {
+    const float m = mip_max * pc.density;
+    R.c[0] = m;
+    R.c[1] = m;
+    R.c[2] = m;
+  }
+  R.c[3] = 1.0f;
+}
+
+template <int KIND>
+constexpr uint32_t KIND_MODE = SHADED_KIND<KIND> ? MODE_FULL : KIND == KIND_MIP ? MODE_MIP : MODE_FLAT;
+
+// The per-ray march (MEASURE: also count blended samples, for the
+// measurement instances only). KIND fixes the compositing / shading branches at compile
+// time, as the reference builds one pipeline per mode (src/shaders/shaders.cmake:17-30).
+template <int KIND, bool MEASURE, class SamplerK>
+__device__ __forceinline__ RayOut march(const SamplerK &S, const VolDev &V, const float4 *tf, const uint32_t *opq,
+                                        const r3d_frame_params &pc, float cutoff, uint32_t px, uint32_t py,
+                                        float *coop) {
+  RayOut R;
+  R.steps = R.skips = R.shaded = R.blends = R.blends_shaded = 0;
+  const uint32_t mode = KIND == KIND_OTHER ? pc.mode : KIND_MODE<KIND>;
+  RaySetup G;
+  if (!ray_setup<KIND>(V, pc, mode, px, py, G, R)) return R;
+  const V3 dir = G.dir, ro = G.ro, lvec = G.lvec, hvec = G.hvec;
+  const float tfar = G.tfar, voxel = G.voxel, dt = G.dt;
+  const float occ_gate = G.occ_gate, opac_k = G.opac_k;
+  float t = G.t0;
   float mip_max = 0.0f, first_hit = -1.0f;
   float ar = 0.0f, ag = 0.0f, ab = 0.0f, aa = 0.0f;
   uint32_t steps = 0, skips = 0, shaded_n = 0, blends = 0, blends_sh = 0;
   const bool composite = mode == MODE_FULL || mode == MODE_FLAT;
-  const float span_step = fmax_sel((tfar - tn) / ((float)MAX_STEPS * dt), 1.0f);
-  const float bs = 8.0f * voxel;
-  const float occ_gate = fmax_sel(pc.skip_gate, 0.5f / 255.0f);
-  const float opac_k = pc.density * pc.step_voxels;
-  const V3 lvec = vscale(dir, -1.0f);
-  const V3 hvec = vnorm(vsub(lvec, dir));
-  const float sgx = dir.x > 0.0f ? 1.0f : 0.0f, sgy = dir.y > 0.0f ? 1.0f : 0.0f,
-              sgz = dir.z > 0.0f ? 1.0f : 0.0f;
   // One sample's position, lod and reads (both levels' footprints and the
   // occupancy byte). Reads are clamped into the volume, so a sample issued
   // past the end of the ray is always valid to read and simply dropped.
   struct Pending {
     float lod, lod_step;
     V3 p;
-    Sampler::Foot f0, f1;
+    TriFoot f0, f1;
     uint32_t occ;
   };
   // view fields the loop reads, held in registers (a read of the view in the
@@ -280,10 +365,7 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
   const float lod_bias = pc.lod_bias, threshold = pc.threshold;
   const auto issue = [&](float tt) {
     Pending q;
-    const float foot = ortho ? cone : tt * cone;
-    const float fv = dim_p2 ? foot * fdim : foot / voxel;
-    q.lod = fclamp_sel(log2_pinned(fmax_sel(fv, 1.0f)) + lod_bias, 0.0f, max_lod);
-    q.lod_step = fmax_sel(exp2_pinned(q.lod), span_step);
+    lod_at(G, lod_bias, tt, q.lod, q.lod_step);
     q.p = vadd(ro, vscale(dir, tt));
     const uint32_t d = (uint32_t)q.lod;
     const bool blend = q.lod - (float)d > 0.0f && d + 1u < S.nlev;
@@ -301,7 +383,7 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
     const float t_next = t + dt * lod_step;
     const uint32_t d = (uint32_t)lod;
     const float w = lod - (float)d;
-    const float s0 = Sampler::lerp3(cur.f0), s1 = Sampler::lerp3(cur.f1);
+    const float s0 = S.lerp3(cur.f0), s1 = S.lerp3(cur.f1);
     const bool blend = w > 0.0f && d + 1u < S.nlev;
     if (MEASURE) blends += blend ? 1u : 0u;
     float v = (blend ? flerp(s0, s1, w) : s0) * K255;
@@ -309,13 +391,7 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
     const bool cut = v < threshold;
     if (cut) v = 0.0f;
     if ((v < occ_gate) & (occ < occ_gate)) {  // raycast.slang:866-869
-      const float nx = (__builtin_floorf(p.x / bs) + sgx) * bs;
-      const float ny = (__builtin_floorf(p.y / bs) + sgy) * bs;
-      const float nz = (__builtin_floorf(p.z / bs) + sgz) * bs;
-      const float ex = __builtin_fabsf(dir.x) > 1e-8f ? (nx - ro.x) / dir.x : 1e30f;
-      const float ey = __builtin_fabsf(dir.y) > 1e-8f ? (ny - ro.y) / dir.y : 1e30f;
-      const float ez = __builtin_fabsf(dir.z) > 1e-8f ? (nz - ro.z) / dir.z : 1e30f;
-      t = fmax_sel(fmin_sel(fmin_sel(ex, ey), ez), t) + bs * 0.01f;
+      t = skip_target(G, p, t);
       skips++;
       return 2;
     }
@@ -389,28 +465,143 @@ __device__ __forceinline__ RayOut march(const Sampler &S, const VolDev &V, const
   R.shaded = shaded_n;
   R.blends = blends;
   R.blends_shaded = blends_sh;
-  if (KIND == KIND_OTHER && mode == MODE_HEATMAP) {
-    const float x = (float)steps / 1024.0f;
-    R.c[0] = x;
-    R.c[1] = 1.0f - __builtin_fabsf(x - 0.5f) * 2.0f;
-    R.c[2] = 1.0f - x;
-  } else if (KIND == KIND_OTHER && mode == MODE_DEPTH) {
-    const float d = first_hit < 0.0f ? 0.0f : 1.0f - fsat(first_hit / 2.0f);
-    R.c[0] = d;
-    R.c[1] = d;
-    R.c[2] = d;
-  } else if (composite) {
-    const float ta = 1.0f - aa;
-    R.c[0] = ar + ta * 0.02f;
-    R.c[1] = ag + ta * 0.02f;
-    R.c[2] = ab + ta * 0.03f;
-  } else {
-    const float m = mip_max * pc.density;
-    R.c[0] = m;
-    R.c[1] = m;
-    R.c[2] = m;
+  ray_finish<KIND>(R, mode, pc, steps, ar, ag, ab, aa, mip_max, first_hit);
+  return R;
+}
+
+// ---- lane-sharded march ------------------------------------------------------
+// A long ray's march is a chain of dependent samples. For the costliest tiles
+// a ray gets a pair of adjacent lanes: each iteration the pair takes the
+// ray's next two samples (lane j the j-th: both lanes walk the step-control
+// chain from the ray's t and keep their own position), reads and evaluates
+// them side by side, and then folds them into the ray's state in sample order
+// through pair broadcasts. Both lanes apply the same fold, so they hold the
+// same state. A sample is the same computation as in march() and the fold
+// runs march()'s operations in its order; a sample after one that ends the
+// ray or skips is dropped, so image, counters and per-pixel steps are
+// march()'s. The extra chain step and the fold cost about a fifth of a
+// shaded sample's instructions and halve the longest rays' chains, which
+// bound the frame; unshaded modes (MIP, FLAT) lose more than they gain, and
+// four lanes per ray measured slower than two.
+constexpr int LPR = 2;
+
+// v from lane SRC of this lane's pair (a quad permutation).
+template <int SRC>
+__device__ __forceinline__ uint32_t pair_bcast(uint32_t v) {
+  constexpr int ctrl = SRC | SRC << 2 | (SRC + 2) << 4 | (SRC + 2) << 6;
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, ctrl, 0xf, 0xf, false);
+}
+template <int SRC>
+__device__ __forceinline__ float pair_bcast(float v) {
+  return __uint_as_float(pair_bcast<SRC>(__float_as_uint(v)));
+}
+
+template <int KIND, class SamplerK>
+__device__ __forceinline__ RayOut march_pair(const SamplerK &S, const VolDev &V, const float4 *tf,
+                                             const uint32_t *opq, const r3d_frame_params &pc, float cutoff,
+                                             uint32_t px, uint32_t py, uint32_t j, float *coop) {
+  static_assert(SHADED_KIND<KIND>, "the lane-sharded march renders the shaded modes");
+  RayOut R;
+  R.steps = R.skips = R.shaded = R.blends = R.blends_shaded = 0;
+  constexpr uint32_t mode = KIND_MODE<KIND>;
+  RaySetup G;
+  if (!ray_setup<KIND>(V, pc, mode, px, py, G, R)) return R;
+  const float lod_bias = pc.lod_bias, threshold = pc.threshold;
+  float T = G.t0;  // the ray's next sample
+  float ar = 0.0f, ag = 0.0f, ab = 0.0f, aa = 0.0f;
+  uint32_t steps = 0, skips = 0, shaded_n = 0;
+  bool live = true;
+  while (live && T < G.tfar && steps < MAX_STEPS) {
+    // this lane's sample: the j-th of the chain from T
+    float tj = T, lodj = 0.0f, stepj = 1.0f, tc = T;
+#pragma unroll
+    for (int i = 0; i < LPR; i++) {
+      float lod, ls;
+      lod_at(G, lod_bias, tc, lod, ls);
+      if ((uint32_t)i == j) {
+        tj = tc;
+        lodj = lod;
+        stepj = ls;
+      }
+      tc = tc + G.dt * ls;
+    }
+    const V3 p = vadd(G.ro, vscale(G.dir, tj));
+    const uint32_t d = (uint32_t)lodj;
+    const float w = lodj - (float)d;
+    const bool blend = w > 0.0f && d + 1u < S.nlev;
+    const TriFoot f0 = S.load(d, p), f1 = S.load(blend ? d + 1u : d, p);
+    const uint32_t occb = occ_load(V, p);
+    const bool valid = tj < G.tfar && steps + j < MAX_STEPS;
+    const float s0 = S.lerp3(f0), s1 = S.lerp3(f1);
+    float v = (blend ? flerp(s0, s1, w) : s0) * K255;
+    const float occ = (float)occb * K255;
+    const bool cut = v < threshold;
+    if (cut) v = 0.0f;
+    const bool skip = (v < G.occ_gate) & (occ < G.occ_gate);
+    // the state after this sample: the skip target, or the next t
+    const float tnext = skip ? skip_target(G, p, tj) : tj + G.dt * stepj;
+    float a = 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    bool lit = false;
+    const bool open = valid && !skip;
+    if (__ballot(open && !cut && tf_visible(opq, v)) != 0) {
+      constexpr int NTAP = KIND_TAPS<KIND>;
+      const float4 s = tf_sample(tf, v);
+      a = 1.0f - pow_unit(1.0f - s.w, G.opac_k * stepj);
+      lit = open && !cut && a > 0.003f;
+      float gt[NTAP];
+      coop_taps<NTAP>([&](V3 q, float ql) { return S.sample(q, ql); }, coop, lit, p,
+                      tap_radius<NTAP>(G.voxel * stepj), lodj, gt);
+      if (lit) {
+        cr = s.x;
+        cg = s.y;
+        cb = s.z;
+        blinn_phong(tap_gradient<NTAP>(gt), G.lvec, G.hvec, cr, cg, cb);
+      }
+    }
+    // fold the pair's samples into the ray's state, in sample order
+    const uint32_t flags = (valid ? 1u : 0u) | (skip ? 2u : 0u) | (lit ? 4u : 0u);
+    bool more = true;  // no sample of this iteration has ended the ray or skipped yet
+    const auto fold = [&](auto src) {
+      constexpr int SRC = decltype(src)::value;
+      const uint32_t f = pair_bcast<SRC>(flags);
+      const float tn = pair_bcast<SRC>(tnext);
+      const float as = pair_bcast<SRC>(a), rs = pair_bcast<SRC>(cr), gs = pair_bcast<SRC>(cg),
+                  bs = pair_bcast<SRC>(cb);
+      if (!more) return;
+      if (!(f & 1u)) {  // t reached tfar or the step budget: the loop ends
+        live = false;
+        more = false;
+        return;
+      }
+      steps++;
+      if (f & 2u) {  // skipped: the next iteration starts at the skip target
+        T = tn;
+        skips++;
+        more = false;
+        return;
+      }
+      if (f & 4u) {
+        shaded_n++;
+        const float wa = (1.0f - aa) * as;
+        ar = ar + wa * rs;
+        ag = ag + wa * gs;
+        ab = ab + wa * bs;
+        aa = aa + wa;
+        if (aa >= cutoff) {
+          live = false;
+          more = false;
+          return;
+        }
+      }
+      T = tn;
+    };
+    fold(std::integral_constant<int, 0>{});
+    fold(std::integral_constant<int, 1>{});
   }
-  R.c[3] = 1.0f;
+  R.steps = steps;
+  R.skips = skips;
+  R.shaded = shaded_n;
+  ray_finish<KIND>(R, mode, pc, steps, ar, ag, ab, aa, 0.0f, -1.0f);
   return R;
 }
 
@@ -840,7 +1031,7 @@ static int shape_index(uint32_t w, uint32_t h) {
 // blended samples, blended shaded samples, 0, 0). MEASURE (8x8 cube-mode
 // measurement instances only) counts the blended samples and marks every
 // brick a footprint reads in a.touched (when set), one bit per brick.
-template <int KIND, int TW, int TH, bool BRICKS, bool MEASURE = false>
+template <int KIND, int TW, int TH, bool BRICKS, bool MEASURE = false, bool WIDE = true>
 __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
   constexpr uint32_t NT = TW * TH, NW = NT / 64u;
   __shared__ float4 s_tf[256];
@@ -871,7 +1062,7 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
     }
   }
   __syncthreads();
-  const Sampler S{a.vol.base, s_lev, a.vol.nlev, MEASURE ? a.touched : nullptr};
+  const SamplerT<WIDE || MEASURE> S{a.vol.base, s_lev, a.vol.nlev, MEASURE ? a.touched : nullptr};
   uint32_t st = 0, sk = 0, sh = 0, nr = 0, bl = 0, bs = 0;
   uint32_t tile = blockIdx.x;
   for (;;) {
@@ -957,6 +1148,132 @@ __global__ __launch_bounds__(TW * TH) void k_raycast(RayArgs a) {
   }
 }
 
+// An ordered 8x8 cube-mode launch of FULL, FAST, MIP or FLAT views, four
+// waves per workgroup: ranks below tier2 (the costliest tiles of a shaded
+// launch) render two tiles per workgroup with a lane pair per ray (each wave
+// 32 rays, march_pair); the rest one tile per wave with one lane per ray.
+// Four tiles share a workgroup's staged transfer function and level table.
+// Pixels and counters are those of k_raycast.
+template <int KIND, bool WIDE>
+__global__ __launch_bounds__(256) void k_raycast_tiers(RayArgs a) {
+  constexpr uint32_t NT = 256u, NW = 4u;
+  __shared__ float4 s_tf[256];
+  __shared__ LevelDev s_lev[R3D_MAX_LEVELS];
+  __shared__ uint32_t s_ts[R3D_MAX_VIEWS_PER_CALL + 1];
+  __shared__ uint4 s_red[NW][2];
+  __shared__ uint32_t s_opq[8];
+  __shared__ float s_coop[SHADED_KIND<KIND> ? NW * COOP_FLOATS : 1];
+  const uint32_t lane = threadIdx.x, wv = lane >> 6, wl = lane & 63u;
+  for (uint32_t i = lane; i < 256u; i += NT)
+    s_tf[i] = make_float4(a.tf[i * 4 + 0], a.tf[i * 4 + 1], a.tf[i * 4 + 2], a.tf[i * 4 + 3]);
+  if (lane < a.vol.nlev) s_lev[lane] = a.vol.levels[lane];
+  for (uint32_t i = lane; i <= a.nviews; i += NT) s_ts[i] = a.tile_start[i];
+  __syncthreads();
+  if (lane < 64u) {  // the TF intervals whose alpha is not zero at both ends (as k_raycast)
+#pragma unroll
+    for (uint32_t c = 0; c < 4u; c++) {
+      const uint32_t i = c * 64u + lane, i1 = i < 255u ? i + 1u : 255u;
+      const uint64_t m = __ballot(s_tf[i].w != 0.0f || s_tf[i1].w != 0.0f);
+      if (lane == 0) {
+        s_opq[2u * c] = (uint32_t)m;
+        s_opq[2u * c + 1u] = (uint32_t)(m >> 32);
+      }
+    }
+  }
+  __syncthreads();
+  const SamplerT<WIDE> S{a.vol.base, s_lev, a.vol.nlev, nullptr};
+  // this wave's tile rank, lanes per ray and first ray of the tile
+  const uint32_t w = blockIdx.x, w2 = (a.tier2 + 1u) / 2u;
+  uint32_t rank, lpr, ray0;
+  if (w < w2) {
+    rank = 2u * w + (wv >> 1);
+    if (rank >= a.tier2) rank = a.ntiles;  // an odd tier: the last workgroup's second tile is not there
+    lpr = 2u;
+    ray0 = 32u * (wv & 1u);
+  } else {
+    rank = a.tier2 + 4u * (w - w2) + wv;
+    lpr = 1u;
+    ray0 = 0u;
+  }
+  uint32_t st = 0, sk = 0, sh = 0, nr = 0;
+  if (rank < a.ntiles) {
+    const uint32_t tile = a.order[rank];
+    uint32_t lo = 0, hi = a.nviews;
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_ts[mid] <= tile) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t v = __builtin_amdgcn_readfirstlane(lo);
+    const r3d_frame_params &pc = a.views[v];
+    const bool half = (pc.view_flags & 32u) != 0u;
+    const uint32_t W = pc.viewport[0], H = pc.viewport[1];
+    const uint32_t RW = half ? (W + 1u) / 2u : W, RH = half ? (H + 1u) / 2u : H;
+    const uint32_t tiles_x = (RW + 7u) / 8u;
+    const uint32_t lt = tile - s_ts[v];
+    const uint32_t ray = ray0 + wl / lpr, j = wl % lpr;
+    const uint32_t tx = (lt % tiles_x) * 8u + (ray & 7u);
+    const uint32_t ty = (lt / tiles_x) * 8u + (ray >> 3);
+    if (tx < RW && ty < RH) {
+      const uint32_t lx = half ? tx * 2u : tx, ly = half ? ty * 2u : ty;
+      float *coop = s_coop + (SHADED_KIND<KIND> ? wv * COOP_FLOATS : 0u);
+      RayOut R;
+      if constexpr (SHADED_KIND<KIND>) {
+        R = lpr == 2u ? march_pair<KIND>(S, a.vol, s_tf, s_opq, pc, a.cutoff, lx, ly, j, coop)
+                      : march<KIND, false>(S, a.vol, s_tf, s_opq, pc, a.cutoff, lx, ly, coop);
+      } else {
+        R = march<KIND, false>(S, a.vol, s_tf, s_opq, pc, a.cutoff, lx, ly, coop);
+      }
+      if (j == 0u) {
+        st += R.steps;
+        sk += R.skips;
+        sh += R.shaded;
+        nr += 1u;
+        const uint32_t ox = pc.view_org & 0xffffu, oy = pc.view_org >> 16;
+        const uint32_t rep = half ? 2u : 1u;
+        for (uint32_t dy = 0; dy < rep; dy++)
+          for (uint32_t dx = 0; dx < rep; dx++) {
+            const uint32_t qx = lx + dx, qy = ly + dy;
+            if (qx >= W || qy >= H) continue;
+            const uint32_t gx = ox + qx, gy = oy + qy;
+            if (gx >= a.img_w || gy >= a.img_h) continue;
+            const size_t pi = (size_t)gy * a.img_w + gx;
+            if (a.img) ((float4 *)a.img)[pi] = make_float4(R.c[0], R.c[1], R.c[2], R.c[3]);
+            if (a.img8) {
+              uint32_t w8 = 0;
+#pragma unroll
+              for (int c = 0; c < 4; c++) w8 |= (uint32_t)(fsat(R.c[c]) * 255.0f + 0.5f) << (8 * c);
+              ((uint32_t *)a.img8)[pi] = w8;
+            }
+            if (a.px_steps) a.px_steps[pi] = R.steps;
+          }
+      }
+    }
+  }
+  if (a.partials) {  // two plain stores per workgroup; k_reduce_counters sums them
+    st = wave_sum(st);
+    sk = wave_sum(sk);
+    sh = wave_sum(sh);
+    nr = wave_sum(nr);
+    if (wl == 0u) {
+      s_red[wv][0] = make_uint4(st, sk, sh, nr);
+      s_red[wv][1] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    __syncthreads();
+    if (lane < 2u) {
+      uint4 t = s_red[0][lane];
+#pragma unroll
+      for (uint32_t i = 1; i < NW; i++) {
+        t.x += s_red[i][lane].x;
+        t.y += s_red[i][lane].y;
+        t.z += s_red[i][lane].z;
+        t.w += s_red[i][lane].w;
+      }
+      ((uint4 *)a.partials)[2u * blockIdx.x + lane] = t;
+    }
+  }
+}
+
 // Sums `n` workgroup partials (two uint4 each) into counters[0..ncount):
 // each workgroup reduces a strided share and adds it with 64-bit integer
 // atomics (exact in any order).
@@ -1010,6 +1327,16 @@ constexpr uint32_t ORDER_BUCKETS = 256u;
 // Tiles below this count (per launch) keep raster order: the two extra
 // launches would cost more than the tail they remove.
 constexpr uint32_t ORDER_MIN_TILES = 4096u;
+// The costliest R3D_PAIR_TIER_NUM sixteenths of an ordered 8x8 FULL or FAST
+// launch take a lane pair per ray (k_raycast_tiers), at most
+// R3D_PAIR_TILES_PER_CU tiles per CU: the tiles that start first and bound
+// the launch's tail; beyond them the pairs' extra work only costs time.
+#ifndef R3D_PAIR_TIER_NUM
+#define R3D_PAIR_TIER_NUM 4u
+#endif
+#ifndef R3D_PAIR_TILES_PER_CU
+#define R3D_PAIR_TILES_PER_CU 8u
+#endif
 constexpr uint32_t ORDER_MAX_TILES = 1u << 24;  // a tile's rank inside its bucket takes 24 bits
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t n) {
@@ -1152,20 +1479,37 @@ __global__ __launch_bounds__(256) void k_tile_scatter(const uint32_t *code, cons
 }
 
 typedef void (*RayKernel)(RayArgs);
-#define R3D_RAY_SHAPES(K, B)                                                                     \
-  { k_raycast<K, 8, 8, B>, k_raycast<K, 16, 4, B>, k_raycast<K, 16, 8, B>, k_raycast<K, 16, 16, B> }
+#define R3D_RAY_SHAPES(K, B, W)                                                                         \
+  {                                                                                                     \
+    k_raycast<K, 8, 8, B, false, W>, k_raycast<K, 16, 4, B, false, W>, k_raycast<K, 16, 8, B, false, W>, \
+        k_raycast<K, 16, 16, B, false, W>                                                               \
+  }
 // The 8x8 cube-mode measurement instances (blend counts, touched bricks).
 static const RayKernel measure_kernels[NKINDS] = {k_raycast<KIND_FULL, 8, 8, false, true>,
                                                 k_raycast<KIND_MIP, 8, 8, false, true>,
                                                 k_raycast<KIND_FLAT, 8, 8, false, true>,
                                                 k_raycast<KIND_OTHER, 8, 8, false, true>,
                                                 k_raycast<KIND_FAST, 8, 8, false, true>};
-static const RayKernel ray_kernels[2][NKINDS][NSHAPES] = {
-    {R3D_RAY_SHAPES(KIND_FULL, false), R3D_RAY_SHAPES(KIND_MIP, false), R3D_RAY_SHAPES(KIND_FLAT, false),
-     R3D_RAY_SHAPES(KIND_OTHER, false), R3D_RAY_SHAPES(KIND_FAST, false)},
-    {R3D_RAY_SHAPES(KIND_FULL, true), R3D_RAY_SHAPES(KIND_MIP, true), R3D_RAY_SHAPES(KIND_FLAT, true),
-     R3D_RAY_SHAPES(KIND_OTHER, true), R3D_RAY_SHAPES(KIND_FAST, true)}};
+// Rows: cube mode with 32-bit offsets (a pyramid below 4 GiB), bricks-LOD,
+// cube mode with 64-bit offsets.
+static const RayKernel ray_kernels[3][NKINDS][NSHAPES] = {
+    {R3D_RAY_SHAPES(KIND_FULL, false, false), R3D_RAY_SHAPES(KIND_MIP, false, false),
+     R3D_RAY_SHAPES(KIND_FLAT, false, false), R3D_RAY_SHAPES(KIND_OTHER, false, false),
+     R3D_RAY_SHAPES(KIND_FAST, false, false)},
+    {R3D_RAY_SHAPES(KIND_FULL, true, true), R3D_RAY_SHAPES(KIND_MIP, true, true),
+     R3D_RAY_SHAPES(KIND_FLAT, true, true), R3D_RAY_SHAPES(KIND_OTHER, true, true),
+     R3D_RAY_SHAPES(KIND_FAST, true, true)},
+    {R3D_RAY_SHAPES(KIND_FULL, false, true), R3D_RAY_SHAPES(KIND_MIP, false, true),
+     R3D_RAY_SHAPES(KIND_FLAT, false, true), R3D_RAY_SHAPES(KIND_OTHER, false, true),
+     R3D_RAY_SHAPES(KIND_FAST, false, true)}};
 #undef R3D_RAY_SHAPES
+// k_raycast_tiers by offset width (32-bit, 64-bit) and kind; none for the
+// remaining modes, which take k_raycast.
+static const RayKernel tier_kernels[2][NKINDS] = {
+    {k_raycast_tiers<KIND_FULL, false>, k_raycast_tiers<KIND_MIP, false>, k_raycast_tiers<KIND_FLAT, false>,
+     nullptr, k_raycast_tiers<KIND_FAST, false>},
+    {k_raycast_tiers<KIND_FULL, true>, k_raycast_tiers<KIND_MIP, true>, k_raycast_tiers<KIND_FLAT, true>, nullptr,
+     k_raycast_tiers<KIND_FAST, true>}};
 typedef void (*CostKernel)(RayArgs, uint32_t *, uint32_t *);
 static const CostKernel cost_kernels[2][NSHAPES] = {
     {k_tile_cost<8, 8, false>, k_tile_cost<16, 4, false>, k_tile_cost<16, 8, false>, k_tile_cost<16, 16, false>},
@@ -1294,7 +1638,22 @@ static int launch_groups(r3d_hip_ctx *ctx, RayArgs a, const r3d_frame_params *dv
     const uint32_t tb = tbase;
     tbase += tiles;
     if (nv == 0 || tiles == 0) continue;
-    uint32_t grid = tiles;
+    const bool ordered_k = order && tiles >= ORDER_MIN_TILES && tiles <= ORDER_MAX_TILES;
+    const RayKernel tier = ordered_k && !measure && !bricks && !t.blocks_per_cu && sh == 0
+                               ? tier_kernels[a.vol.narrow ? 0 : 1][k]
+                               : nullptr;
+    uint32_t grid = tiles, threads = nt;
+    if (tier) {  // shaded launches: the costliest ranks with a lane pair per ray (k_raycast_tiers)
+      uint32_t pairs = 0;
+      if (k == KIND_FULL || k == KIND_FAST) {
+        const uint64_t frac = ((uint64_t)tiles * R3D_PAIR_TIER_NUM + 15u) / 16u;
+        const uint64_t cap = (uint64_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * R3D_PAIR_TILES_PER_CU;
+        pairs = (uint32_t)(frac < cap ? frac : cap);
+      }
+      a.tier2 = pairs;
+      grid = (a.tier2 + 1u) / 2u + (tiles - a.tier2 + 3u) / 4u;
+      threads = 256u;
+    }
     if (t.blocks_per_cu) {
       const uint32_t cap = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * t.blocks_per_cu;
       if (cap < grid) grid = cap;
@@ -1307,7 +1666,7 @@ static int launch_groups(r3d_hip_ctx *ctx, RayArgs a, const r3d_frame_params *dv
     a.ntiles = tiles;
     a.queue = t.blocks_per_cu ? queue + k : nullptr;
     a.order = nullptr;
-    if (order && tiles >= ORDER_MIN_TILES && tiles <= ORDER_MAX_TILES) {
+    if (ordered_k) {
       uint32_t *ck = order + (size_t)k * ORDER_BUCKETS;
       const uint32_t og = (tiles + 255u) / 256u;
       hipLaunchKernelGGL(cost_kernels[bricks ? 1 : 0][sh], dim3(og), dim3(256), 0, s, a, codes + tb, ck);
@@ -1316,8 +1675,9 @@ static int launch_groups(r3d_hip_ctx *ctx, RayArgs a, const r3d_frame_params *dv
       R3D_HIP_CHECK(ctx, hipGetLastError());
       a.order = perm + tb;
     }
-    hipLaunchKernelGGL(measure ? measure_kernels[k] : ray_kernels[bricks ? 1 : 0][k][sh], dim3(grid), dim3(nt), 0,
-                       s, a);
+    const int row = bricks ? 1 : a.vol.narrow ? 0 : 2;
+    hipLaunchKernelGGL(measure ? measure_kernels[k] : tier ? tier : ray_kernels[row][k][sh], dim3(grid),
+                       dim3(threads), 0, s, a);
     R3D_HIP_CHECK(ctx, hipGetLastError());
   }
   if (counters && pbase) {

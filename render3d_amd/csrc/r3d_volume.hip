@@ -562,6 +562,7 @@ VolDev vol_dev(const r3d_hip_volume *v) {
   d.ox = v->odim[0];
   d.oy = v->odim[1];
   d.oz = v->odim[2];
+  d.narrow = v->bytes <= ((uint64_t)1 << 32) ? 1u : 0u;
   return d;
 }
 
@@ -609,6 +610,9 @@ extern "C" int r3d_hip_volume_create(r3d_hip_ctx *ctx, const r3d_volume_desc *de
     tab[l].fx = (float)v->dims[l][0];
     tab[l].fy = (float)v->dims[l][1];
     tab[l].fz = (float)v->dims[l][2];
+    tab[l].hx = tab[l].fx - 1.0f;
+    tab[l].hy = tab[l].fy - 1.0f;
+    tab[l].hz = tab[l].fz - 1.0f;
   }
   v->bytes = off;
   for (int a = 0; a < 3; a++) v->odim[a] = ((&desc->nx)[a] + 7u) / 8u;
