@@ -261,18 +261,20 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
   for (uint32_t i = t; i < 4096u; i += BLOB_NT) s_scan[i] = c_scan[i];
   __syncthreads();
   const uint32_t lane = blockIdx.x * BLOB_NT + t;
-  if (lane >= a.n) return;
-  const uint32_t c = a.order ? a.order[lane] : lane;
+  // lanes past the batch stay to the workgroup barrier below, doing nothing
+  const bool in = lane < a.n;
+  const uint32_t c = in ? (a.order ? a.order[lane] : lane) : 0u;
   RDec d;
   d.ctx = s_ctx + t;
   const uint64_t base = (uint64_t)c * a.cap;
-  const uint8_t *b = a.bytes + a.offsets[c];
-  const uint32_t len = a.lengths[c];
+  const uint8_t *b = in ? a.bytes + a.offsets[c] : nullptr;
+  const uint32_t len = in ? a.lengths[c] : 0u;
   int32_t st = 1;
   uint32_t np = 0, nc = 0;
   float vmin = 0.0f, vspan = 0.0f, qstep = 0.0f;
   int32_t dcq = 0;
-  if (len < 2u || b[0] != 0xD3u || b[1] != a.dtype) {
+  uint32_t has_corr = 0, eob = 0;
+  if (!in || len < 2u || b[0] != 0xD3u || b[1] != a.dtype) {
     st = 0;
   } else {
     d.init(b + 2, len - 2u);
@@ -285,12 +287,18 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
   }
   if (st == 1) {
     for (uint32_t i = 0; i < NCTX; i++) d.ctx[i * BLOB_NT] = c_prior[i];
-    const uint32_t has_corr = d.bit(CTX_FLAG);
+    has_corr = d.bit(CTX_FLAG);
     // EOB (dct3d.c:365-371): up to 13 context bits, then k - 1 raw bits
     uint32_t k = 0;
     while (k < 13u && d.bit(CTX_EOB + k)) k++;
-    uint32_t eob = k <= 1u ? k : ((1u << (k - 1u)) | d.bits((int)k - 1));
+    eob = k <= 1u ? k : ((1u << (k - 1u)) | d.bits((int)k - 1));
     if (eob > 4096u) eob = 4096u;
+  }
+  // Every lane of the workgroup is past its EOB from here on, and nothing
+  // after reads the EOB contexts again: their rows become the pair staging
+  // area, in slots that cover other lanes' halfwords.
+  __syncthreads();
+  if (st == 1) {
     // Levels (dct3d.c:390-405) as a per-lane state machine that makes exactly
     // one binary decision per iteration: significance in context (band,
     // min(3, popcount of the last 16 decisions)), the unary magnitude ladder
@@ -306,25 +314,19 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
     // sector) when the capacity and the array keep such groups aligned: a
     // scattered 4-byte store per pair made the memory system fetch and write
     // whole sectors, and on this ISA every store also holds up the next wait
-    // for a window refill. A lane stages its pairs 0..6 of a group in its own
-    // EOB context halfwords (rows 44..57), which its level loop and
-    // everything after it never touch: slot j's low half in row 44 + 2j, its
-    // high half in row 45 + 2j. Only the lane's own halfwords: the other
-    // waves of the workgroup are still initialising or reading their
-    // contexts in the same rows (a 32-bit slot would cover two other lanes'
-    // halfwords). The eighth pair completes the group from a register. (A
-    // register shift of 8 costs 8 moves in nearly every iteration, since some
-    // lane of the wave is nearly always at a sign; groups of 4 in registers
-    // wrote each sector twice.)
+    // for a window refill. A lane stages its pairs 0..6 of a group in LDS, in
+    // the EOB context rows (44..57: 14 x 256 halfwords = 7 x 256 dwords),
+    // free for every lane since the barrier above (a slot covers the
+    // halfwords of two other lanes, which may sit in another wave: without
+    // the barrier a wave that was still at its priors or its EOB turned a
+    // staged -1 level into +1); the eighth pair completes the group from a
+    // register. (A register shift of 8 costs 8 moves in nearly every
+    // iteration, since some lane of the wave is nearly always at a sign;
+    // groups of 4 in registers wrote each sector twice.)
     const bool vec8 = (a.cap & 7u) == 0u && ((uintptr_t)a.pairs & 31u) == 0u;
-    uint16_t *stg = d.ctx + CTX_EOB * BLOB_NT;  // slot j: stg[2j * BLOB_NT] (low), stg[(2j + 1) * BLOB_NT] (high)
-    const auto stg_put = [&](uint32_t j, uint32_t v) {
-      stg[2u * j * BLOB_NT] = (uint16_t)v;
-      stg[(2u * j + 1u) * BLOB_NT] = (uint16_t)(v >> 16);
-    };
-    const auto stg_get = [&](uint32_t j) -> uint32_t {
-      return (uint32_t)stg[2u * j * BLOB_NT] | (uint32_t)stg[(2u * j + 1u) * BLOB_NT] << 16;
-    };
+    uint32_t *stg = (uint32_t *)(s_ctx + CTX_EOB * BLOB_NT) + t;  // slot j at stg[j * BLOB_NT]
+    const auto stg_put = [&](uint32_t j, uint32_t v) { stg[j * BLOB_NT] = v; };
+    const auto stg_get = [&](uint32_t j) -> uint32_t { return stg[j * BLOB_NT]; };
     uint32_t e = s_scan[0];
     uint32_t ph = eob == 0u ? PH_DONE : eob == 1u ? PH_MAG : PH_SIG;
     if (eob == 1u) recent = 1u;
@@ -416,6 +418,7 @@ __global__ __launch_bounds__(BLOB_NT) void k_parse_blobs(ParseArgs a) {
       }
     }
   }
+  if (!in) return;
   r3d_chunk_header h;
   h.vmin = vmin;
   h.vspan = st == 1 ? vspan : 0.0f;  // vspan 0 makes the decoder zero-fill the chunk (status 0)
