@@ -351,7 +351,7 @@ def raycast_roofline(ctx, vol, view, W, H, img, ray_ms, dev):
     compulsory = uniq * 4928 + out_bytes
     comp_gbps = compulsory / (ray_ms / 1e3) / 1e9
     prof = os.path.join(ROOT, "profiles", "raycast_1080p.md")
-    return {"kernel": "k_raycast<FULL,8x8>", "bound": "hbm", "achieved": comp_gbps, "peak": HBM_PEAK_GBPS,
+    return {"kernel": "k_raycast_tiers<FULL>", "bound": "hbm", "achieved": comp_gbps, "peak": HBM_PEAK_GBPS,
             "unit": "GB/s", "frac": comp_gbps / HBM_PEAK_GBPS,
             "traffic": read_traffic(prof) if os.path.exists(prof) else None,
             "compulsory_bytes": compulsory, "unique_bricks": uniq, "bricks_in_pyramid": vol.brick_count,
