@@ -130,6 +130,24 @@ def test_tiers_with_skips_match_oracle(ctx, sparse96, case):
     _check(ctx, v, ov, lut, views["full" if case == "fast" else case], W, H, quality, True)
 
 
+@pytest.mark.parametrize("case", ["half", "ragged"])
+def test_tiers_half_and_ragged_views(ctx, dense96, case):
+    """A half-resolution view (each ray fills 2x2 pixels) of 1024^2, and a
+    600x520 view whose last tile column and row are partial: both launches
+    have 4,096 tiles or more."""
+    v, ov = dense96
+    lut = scene.tf_ramp_40_200()
+    gate = scene.skip_gate(lut)
+    cam = scene.orbit_camera(60.0, -15.0, 1.9)
+    if case == "half":
+        w, h = 2 * W, 2 * H
+        vs = [scene.frame_params(cam, w, h, view_flags=r3d.VIEW_HALF, step_voxels=0.5, gate=gate)]
+    else:
+        w, h = 600, 520
+        vs = [scene.frame_params(cam, w, h, step_voxels=0.5, gate=gate)]
+    _check(ctx, v, ov, lut, vs, w, h, r3d.QUALITY_FULL, False)
+
+
 def test_tiers_multi_view(ctx, dense96):
     v, ov = dense96
     lut, vs = _multi()
