@@ -122,9 +122,9 @@ struct GridTap {
 // sampler's sequence on an FP32 grid
 __device__ __forceinline__ GridTap grid_tap(const float *G, uint32_t gx, uint32_t gy, uint32_t gz, V3 p) {
   GridTap T;
-  const float tx = fclamp_sel(p.x * (float)gx - 0.5f, 0.0f, (float)(gx - 1u));
-  const float ty = fclamp_sel(p.y * (float)gy - 0.5f, 0.0f, (float)(gy - 1u));
-  const float tz = fclamp_sel(p.z * (float)gz - 0.5f, 0.0f, (float)(gz - 1u));
+  const float tx = fclamp_ord(p.x * (float)gx - 0.5f, 0.0f, (float)(gx - 1u));
+  const float ty = fclamp_ord(p.y * (float)gy - 0.5f, 0.0f, (float)(gy - 1u));
+  const float tz = fclamp_ord(p.z * (float)gz - 0.5f, 0.0f, (float)(gz - 1u));
   const uint32_t x0 = (uint32_t)tx, y0 = (uint32_t)ty, z0 = (uint32_t)tz;
   T.fx = tx - (float)x0;
   T.fy = ty - (float)y0;
@@ -157,21 +157,21 @@ struct SampleEval {
 // One loop iteration's parameter-dependent evaluation at t (raycast.slang
 // 770-1019 without the empty-space skip). `shade_now` computes the 6-tap
 // shading only when the sample contributes.
-template <bool SHADED>
-__device__ __forceinline__ SampleEval eval_sample(const Sampler &S, const DiffArgs &A, const float4 *tf,
+template <bool SHADED, class SamplerK>
+__device__ __forceinline__ SampleEval eval_sample(const SamplerK &S, const DiffArgs &A, const float4 *tf,
                                                   const Ray &R, const r3d_frame_params &pc, float t,
                                                   float *lod_step_out) {
   SampleEval E;
   const float foot = R.ortho ? R.cone : t * R.cone;
-  const float lod = fclamp_sel(log2_pinned(fmax_sel(foot / R.voxel, 1.0f)) + pc.lod_bias, 0.0f, R.max_lod);
-  const float lod_step = fmax_sel(exp2_pinned(lod), R.span_step);
+  const float lod = fclamp_ord(log2_ge1(fmax_sel(foot / R.voxel, 1.0f)) + pc.lod_bias, 0.0f, R.max_lod);
+  const float lod_step = fmax_sel(exp2_core(lod), R.span_step);
   *lod_step_out = lod_step;
   const V3 p = vadd(R.ro, vscale(R.dir, t));
   E.p = p;
   const float v = S.sample(p, lod);
   E.contrib = false;
   if (v < pc.threshold) return E;  // low cut: the sample is void
-  const float tt = fclamp_sel(v * 256.0f - 0.5f, 0.0f, 255.0f);
+  const float tt = fclamp_ord(v * 256.0f - 0.5f, 0.0f, 255.0f);
   E.i0 = (uint32_t)tt;
   E.i1 = E.i0 + 1u < 256u ? E.i0 + 1u : 255u;
   E.f = tt - (float)E.i0;
@@ -236,7 +236,7 @@ __device__ __forceinline__ bool tile_pixel(const DiffArgs &A, uint32_t tile, uin
 
 // REC: write the checkpoints (a forward that an adjoint will follow); without
 // them the kernel renders the same image and keeps nothing.
-template <bool SHADED, bool REC>
+template <bool SHADED, bool REC, bool WIDE>
 __global__ __launch_bounds__(64) void k_diff_forward(DiffArgs A) {
   __shared__ float4 s_tf[256];
   __shared__ LevelDev s_lev[R3D_MAX_LEVELS];
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(64) void k_diff_forward(DiffArgs A) {
   uint32_t v, px, py, ray;
   if (!tile_pixel(A, blockIdx.x, lane, &v, &px, &py, &ray)) return;
   const r3d_frame_params &pc = A.views[v];
-  const Sampler S{A.vol.base, s_lev, A.vol.nlev};
+  const SamplerT<WIDE> S{A.vol.base, s_lev, A.vol.nlev};
   const Ray R = setup_ray(A.vol, pc, px, py);
   RayRecord *rec = REC ? A.rec + ray : nullptr;
   float ar = 0.0f, ag = 0.0f, ab = 0.0f, aa = 0.0f;
@@ -331,7 +331,7 @@ struct GridAcc {
 
 // Persistent: each 256-lane workgroup walks groups of four 8x8 tiles (one per
 // wave), so the per-block TF partials stay few.
-template <bool SHADED>
+template <bool SHADED, bool WIDE>
 __global__ __launch_bounds__(256) void k_adjoint(DiffArgs A) {
   __shared__ float4 s_tf[256];
   __shared__ float s_gtf[1024];
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(256) void k_adjoint(DiffArgs A) {
   for (uint32_t i = tid; i < 1024u; i += 256u) s_gtf[i] = 0.0f;
   if (tid < A.vol.nlev) s_lev[tid] = A.vol.levels[tid];
   __syncthreads();
-  const Sampler S{A.vol.base, s_lev, A.vol.nlev};
+  const SamplerT<WIDE> S{A.vol.base, s_lev, A.vol.nlev};
   const uint32_t ngroups = (A.ntiles + 3u) / 4u;
   for (uint32_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     const uint32_t tile = grp * 4u + (tid >> 6);
@@ -382,12 +382,12 @@ __global__ __launch_bounds__(256) void k_adjoint(DiffArgs A) {
       uint32_t nc = 0;
       for (uint32_t i = i_begin; i < i_end; i++) {
         const float foot = R.ortho ? R.cone : t * R.cone;
-        const float lod = fclamp_sel(log2_pinned(fmax_sel(foot / R.voxel, 1.0f)) + pc.lod_bias, 0.0f, R.max_lod);
-        const float lod_step = fmax_sel(exp2_pinned(lod), R.span_step);
+        const float lod = fclamp_ord(log2_ge1(fmax_sel(foot / R.voxel, 1.0f)) + pc.lod_bias, 0.0f, R.max_lod);
+        const float lod_step = fmax_sel(exp2_core(lod), R.span_step);
         const V3 p = vadd(R.ro, vscale(R.dir, t));
         const float sv = S.sample(p, lod);
         if (!(sv < pc.threshold)) {
-          const float tt = fclamp_sel(sv * 256.0f - 0.5f, 0.0f, 255.0f);
+          const float tt = fclamp_ord(sv * 256.0f - 0.5f, 0.0f, 255.0f);
           const uint32_t i0 = (uint32_t)tt;
           const uint32_t i1 = i0 + 1u < 256u ? i0 + 1u : 255u;
           const float f = tt - (float)i0;
@@ -576,9 +576,14 @@ extern "C" int r3d_hip_raycast_diff(r3d_hip_ctx *ctx, const r3d_hip_volume *vol,
   A.img_h = out->height;
   A.rec = (RayRecord *)dp->workspace;
   typedef void (*FwdKernel)(DiffArgs);
-  static const FwdKernel fwd[2][2] = {{k_diff_forward<false, false>, k_diff_forward<false, true>},
-                                      {k_diff_forward<true, false>, k_diff_forward<true, true>}};
-  hipLaunchKernelGGL(fwd[plan.shaded ? 1 : 0][A.rec ? 1 : 0], dim3(plan.ntiles), dim3(64), 0, s, A);
+  // [64-bit offsets][shaded][checkpoints]: 32-bit offsets for a pyramid below 4 GiB
+  static const FwdKernel fwd[2][2][2] = {
+      {{k_diff_forward<false, false, false>, k_diff_forward<false, true, false>},
+       {k_diff_forward<true, false, false>, k_diff_forward<true, true, false>}},
+      {{k_diff_forward<false, false, true>, k_diff_forward<false, true, true>},
+       {k_diff_forward<true, false, true>, k_diff_forward<true, true, true>}}};
+  hipLaunchKernelGGL(fwd[A.vol.narrow ? 0 : 1][plan.shaded ? 1 : 0][A.rec ? 1 : 0], dim3(plan.ntiles), dim3(64), 0,
+                     s, A);
   R3D_HIP_CHECK(ctx, hipGetLastError());
   R3D_HIP_CHECK(ctx, hipEventRecord(slot->ev, s));
   slot->recorded = true;
@@ -631,10 +636,10 @@ extern "C" int r3d_hip_raycast_adjoint(r3d_hip_ctx *ctx, const r3d_hip_volume *v
   A.rec = (RayRecord *)dp->workspace;
   A.gtf_part = ctx->d_gtf;
   A.dgrid = dL_dgrid;
-  if (plan.shaded)
-    hipLaunchKernelGGL(k_adjoint<true>, dim3(nblocks), dim3(256), 0, s, A);
-  else
-    hipLaunchKernelGGL(k_adjoint<false>, dim3(nblocks), dim3(256), 0, s, A);
+  typedef void (*AdjKernel)(DiffArgs);
+  static const AdjKernel adj[2][2] = {{k_adjoint<false, false>, k_adjoint<true, false>},
+                                      {k_adjoint<false, true>, k_adjoint<true, true>}};
+  hipLaunchKernelGGL(adj[A.vol.narrow ? 0 : 1][plan.shaded ? 1 : 0], dim3(nblocks), dim3(256), 0, s, A);
   hipLaunchKernelGGL(k_reduce_tf, dim3(1024), dim3(256), 0, s, (const float *)ctx->d_gtf, nblocks, dL_dtf);
   R3D_HIP_CHECK(ctx, hipGetLastError());
   R3D_HIP_CHECK(ctx, hipEventRecord(slot->ev, s));
