@@ -637,9 +637,9 @@ struct BFoot {
 };
 
 __device__ __forceinline__ BFoot brick_load(const uint8_t *atlas, uint32_t slot, V3 bl) {
-  const float tx = fclamp_sel(bl.x, 0.5f, 127.5f) - 0.5f;
-  const float ty = fclamp_sel(bl.y, 0.5f, 127.5f) - 0.5f;
-  const float tz = fclamp_sel(bl.z, 0.5f, 127.5f) - 0.5f;
+  const float tx = fclamp_ord(bl.x, 0.5f, 127.5f) - 0.5f;
+  const float ty = fclamp_ord(bl.y, 0.5f, 127.5f) - 0.5f;
+  const float tz = fclamp_ord(bl.z, 0.5f, 127.5f) - 0.5f;
   const uint32_t x0 = (uint32_t)tx, y0 = (uint32_t)ty, z0 = (uint32_t)tz;
   const uint32_t dx = x0 < 127u ? 1u : 0u, dy = y0 < 127u ? 128u : 0u, dz = z0 < 127u ? 16384u : 0u;
   BFoot f;
@@ -672,8 +672,8 @@ __device__ __forceinline__ float brick_fetch(const uint8_t *atlas, uint32_t slot
 
 __device__ __forceinline__ V3 base_voxel(const uint32_t *hdr, V3 p, float md) {
   const float sx = (float)hdr[1], sy = (float)hdr[2], sz = (float)hdr[3];
-  return mk(fclamp_sel(p.x * md, 0.0f, fmax_sel(sx - 0.001f, 0.0f)), fclamp_sel(p.y * md, 0.0f, fmax_sel(sy - 0.001f, 0.0f)),
-            fclamp_sel(p.z * md, 0.0f, fmax_sel(sz - 0.001f, 0.0f)));
+  return mk(fclamp_ord(p.x * md, 0.0f, fmax_sel(sx - 0.001f, 0.0f)), fclamp_ord(p.y * md, 0.0f, fmax_sel(sy - 0.001f, 0.0f)),
+            fclamp_ord(p.z * md, 0.0f, fmax_sel(sz - 0.001f, 0.0f)));
 }
 
 // bricks_lod_locate_ex: desired level, +1, +2, then the coarsest. hdr is the
@@ -695,7 +695,7 @@ __device__ __forceinline__ BCtx locate_ex(const BricksDev &B, const uint32_t *hd
   const uint32_t nlev = hdr[0];
   const V3 bv = base_voxel(hdr, p, md);
   c.world_brick = BR_SLOT * (float)(1u << (nlev - 1u)) / md;
-  const int desired = (int)fclamp_sel(__builtin_floorf(lod), 0.0f, (float)(nlev - 1u));
+  const int desired = (int)fclamp_ord(__builtin_floorf(lod), 0.0f, (float)(nlev - 1u));
   const int last = (int)nlev - 1;
   for (uint32_t step = 0; step < 4u; step++) {
     const int li = step < 3u ? desired + (int)step : last;
@@ -714,7 +714,7 @@ __device__ __forceinline__ BCtx locate_ex(const BricksDev &B, const uint32_t *hd
     c.maxv = (float)(e >> 24) / 255.0f;
     c.slot = e & 0xFFFFFFu;
     c.bl = mk(lv.x - (float)(bcx << 7), lv.y - (float)(bcy << 7), lv.z - (float)(bcz << 7));
-    c.grad_vox = scales_p2 ? exp2_pinned(lod) * inv : exp2_pinned(lod) / fs;
+    c.grad_vox = scales_p2 ? exp2_core(lod) * inv : exp2_core(lod) / fs;  // lod in [0, 14]
     c.world_brick = BR_SLOT * fs / md;
     *scale_out = scale;
     *inv_out = inv;
@@ -734,7 +734,7 @@ __device__ __forceinline__ BCtx locate_ex(const BricksDev &B, const uint32_t *hd
 __device__ __forceinline__ bool cache_probe(const uint32_t *hdr, float md, V3 p, float lod, bool scales_p2,
                                             const LodCache &lc, BCtx *c) {
   const uint32_t nlev = hdr[0];
-  const int desired = (int)fclamp_sel(__builtin_floorf(lod), 0.0f, (float)(nlev - 1u));
+  const int desired = (int)fclamp_ord(__builtin_floorf(lod), 0.0f, (float)(nlev - 1u));
   if (!(lc.valid && lc.desired == desired)) return false;
   const V3 bv = base_voxel(hdr, p, md);
   const float fs = (float)lc.scale, inv = lc.inv;
@@ -751,7 +751,7 @@ __device__ __forceinline__ BCtx locate_cached(const BricksDev &B, const uint32_t
   BCtx hit;
   if (cache_probe(hdr, md, p, lod, scales_p2, lc, &hit)) return hit;
   const uint32_t nlev = hdr[0];
-  const int desired = (int)fclamp_sel(__builtin_floorf(lod), 0.0f, (float)(nlev - 1u));
+  const int desired = (int)fclamp_ord(__builtin_floorf(lod), 0.0f, (float)(nlev - 1u));
   uint32_t scale, bx, by, bz;
   float inv;
   const BCtx c = locate_ex(B, hdr, md, p, lod, scales_p2, &scale, &inv, &bx, &by, &bz);
@@ -887,8 +887,9 @@ __device__ __forceinline__ RayOut march_bricks(const BricksDev &B, const uint32_
   const auto at_t = [&](float tt, BPend &q) {
     const float foot = ortho ? cone : tt * cone;
     const float fv = md_p2 ? foot * md : foot / voxel;
-    q.lod = fclamp_sel(log2_pinned(fmax_sel(fv, 1.0f)) + pc.lod_bias, 0.0f, max_lod);
-    q.lod_step = fmax_sel(exp2_pinned(q.lod), span_step);
+    // fv >= 1 and max_lod in [0, 14]: log2 and exp2 without their special cases
+    q.lod = fclamp_ord(log2_ge1(fmax_sel(fv, 1.0f)) + pc.lod_bias, 0.0f, max_lod);
+    q.lod_step = fmax_sel(exp2_core(q.lod), span_step);
     q.p = vadd(ro, vscale(dir, tt));
   };
   // One march step on a located, issued sample at the current t: 0 ends the
