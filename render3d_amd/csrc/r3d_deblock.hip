@@ -14,24 +14,42 @@
 // clamp), so the output is bit-identical. HBM-bound byte work.
 #include <type_traits>
 
+#include "r3d_dmath.h"
 #include "r3d_internal.h"
 
 namespace r3d {
 
 constexpr uint32_t DB_N = 16u;
 
-// deblock.c:26-46: the nudge of the p0|q0 step, 0 for a real edge.
-__device__ __forceinline__ float db_delta(float p1, float p0, float q0, float q1, float alpha, float beta, float *d1) {
-  *d1 = 0.0f;
-  const float d = q0 - p0, ad = __builtin_fabsf(d);
-  if (ad >= alpha || ad == 0.0f) return 0.0f;
-  if (__builtin_fabsf(p1 - p0) >= beta) return 0.0f;
-  if (__builtin_fabsf(q1 - q0) >= beta) return 0.0f;
+// deblock.c:30-45 without the two activity tests: the nudges (d0, d1) of
+// a p0|q0 step d = q0 - p0, (0, 0) for a real edge or no step. d0 is never 0
+// otherwise (w > 0).
+__device__ __forceinline__ float2 db_step_nudge(float d, float alpha) {
+  const float ad = __builtin_fabsf(d);
+  if (ad >= alpha || ad == 0.0f) return make_float2(0.0f, 0.0f);
   float d0 = 0.40f * d;
   const float w = 1.0f - ad / alpha;
   d0 *= w;
-  *d1 = 0.4f * d0;
-  return d0;
+  return make_float2(d0, 0.4f * d0);
+}
+
+// deblock.c:26-46: the nudge of the p0|q0 step, 0 for a real edge. The three
+// tests have no side effects, so any order of them gives the same result.
+__device__ __forceinline__ float db_delta(float p1, float p0, float q0, float q1, float alpha, float beta, float *d1) {
+  *d1 = 0.0f;
+  if (__builtin_fabsf(p1 - p0) >= beta || __builtin_fabsf(q1 - q0) >= beta) return 0.0f;
+  const float2 n = db_step_nudge(q0 - p0, alpha);
+  *d1 = n.y;
+  return n.x;
+}
+
+// Voxels are u8, so the step q0 - p0 is one of the 511 integers -255 .. 255
+// and the nudges are a table of it: entry d + 255, computed with the
+// reference's own sequence (correctly rounded division included) once per
+// workgroup. One LDS read replaces the division and three products per sample.
+constexpr uint32_t DB_NSTEP = 511u;
+__device__ __forceinline__ void db_fill_nudges(float2 *tab, uint32_t t, uint32_t nt, float alpha) {
+  for (uint32_t i = t; i < DB_NSTEP; i += nt) tab[i] = db_step_nudge((float)((int32_t)i - 255), alpha);
 }
 
 __device__ __forceinline__ uint8_t db_u8(float v) {
@@ -197,19 +215,21 @@ __global__ __launch_bounds__(256) void k_db_pass(DbPass a) {
   }
 }
 
-// All three passes in one kernel for rows that are 16-byte aligned. One
-// workgroup per tile of 128 x 16 x 16 voxels shifted by -2 on every axis
-// (x range [128m - 2, 128m + 126), y [16j - 2, 16j + 14), z likewise): a tile
-// holds the whole two-layer neighbourhood of every seam it covers (8 along x,
-// one along y and z), every voxel belongs to one tile, and a tile reads no
-// byte that another tile writes (tiles run concurrently). Eight lanes share a
-// 128-byte row, 16 bytes each (coalesced loads); lane c of a row holds tile
-// bytes 16c .. 16c + 15, whose first four are the x seam 128m + 16c, so the x
-// seams are filtered in registers (the two bytes before each lane's load come
-// from its neighbour lane). The y and z seams follow through the tile in LDS,
-// a dword of four columns per lane, in the reference's order, and only
-// changed dwords go back. The volume is read once; the per-pass kernels read
-// it whole for the x pass (seams every 16 bytes touch every line) and a
+// All three passes in one kernel for rows that are 16-byte aligned, over
+// tiles of 128 x 16 x 16 voxels shifted by -2 on every axis (x range
+// [128m - 2, 128m + 126), y [16j - 2, 16j + 14), z likewise): a tile holds the
+// whole two-layer neighbourhood of every seam it covers (8 along x, one along
+// y and z), every voxel belongs to one tile, and a tile reads no byte that
+// another tile writes (tiles run concurrently, in any order). Eight lanes
+// share a 128-byte row, 16 bytes each (coalesced loads); lane c of a row
+// holds tile bytes 16c .. 16c + 15, whose first four are the x seam
+// 128m + 16c, so the x seams are filtered in registers (the two bytes before
+// each lane's load come from its neighbour lane). The y and z seams follow
+// through the tile in LDS, a dword of four columns per lane, in the
+// reference's order, and only changed dwords go back. Every sample takes its
+// nudges from the workgroup's table of the 511 possible steps and writes its
+// bytes with round_u8_into. The volume is read once; the per-pass kernels
+// read it whole for the x pass (seams every 16 bytes touch every line) and a
 // quarter again for each of y and z.
 struct DbTile {
   uint8_t *vol;
@@ -219,33 +239,42 @@ struct DbTile {
   const DbLim *flim;
 };
 
-__device__ __forceinline__ void db_sample(uint32_t &b0, uint32_t &b1, uint32_t &b2, uint32_t &b3, float alpha,
-                                          float beta) {
+// One seam sample from its four layers (u8 values) and the nudge table:
+// whether it is filtered, and its nudges.
+__device__ __forceinline__ bool db_nudge(uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3, float beta,
+                                         const float2 *tab, float2 *n) {
+  *n = tab[b2 + 255u - b1];
   const float p1 = (float)b0, p0 = (float)b1, q0 = (float)b2, q1 = (float)b3;
-  float d1;
-  const float d0 = db_delta(p1, p0, q0, q1, alpha, beta, &d1);
-  if (d0 != 0.0f) {
-    b1 = db_u8(p0 + d0);
-    b2 = db_u8(q0 - d0);
-    b0 = db_u8(p1 + d1);
-    b3 = db_u8(q1 - d1);
-  }
+  return n->x != 0.0f && __builtin_fabsf(p1 - p0) < beta && __builtin_fabsf(q1 - q0) < beta;
+}
+
+// An x seam sample held in one dword, bytes p1 | p0 | q0 | q1.
+__device__ __forceinline__ uint32_t db_sample_dword(uint32_t r, float beta, const float2 *tab) {
+  const uint32_t b0 = r & 255u, b1 = (r >> 8) & 255u, b2 = (r >> 16) & 255u, b3 = r >> 24;
+  float2 n;
+  if (!db_nudge(b0, b1, b2, b3, beta, tab, &n)) return r;
+  uint32_t o = round_u8_into((float)b0 + n.y, 0u, 0u);
+  o = round_u8_into((float)b1 + n.x, 1u, o);
+  o = round_u8_into((float)b2 - n.x, 2u, o);
+  return round_u8_into((float)b3 - n.y, 3u, o);
 }
 
 // Four columns packed in one dword per layer (p1 | p0 | q0 | q1 across the
 // seam); byte k is filtered when bit k of `valid` is set.
 __device__ __forceinline__ void db_sample4(uint32_t &w0, uint32_t &w1, uint32_t &w2, uint32_t &w3, uint32_t valid,
-                                           float alpha, float beta) {
-  uint32_t o0 = 0, o1 = 0, o2 = 0, o3 = 0;
+                                           float beta, const float2 *tab) {
+  uint32_t o0 = w0, o1 = w1, o2 = w2, o3 = w3;
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    uint32_t b0 = (w0 >> (8 * k)) & 255u, b1 = (w1 >> (8 * k)) & 255u, b2 = (w2 >> (8 * k)) & 255u,
-             b3 = (w3 >> (8 * k)) & 255u;
-    if ((valid >> k) & 1u) db_sample(b0, b1, b2, b3, alpha, beta);
-    o0 |= b0 << (8 * k);
-    o1 |= b1 << (8 * k);
-    o2 |= b2 << (8 * k);
-    o3 |= b3 << (8 * k);
+  for (uint32_t k = 0; k < 4u; k++) {
+    const uint32_t b0 = (w0 >> (8u * k)) & 255u, b1 = (w1 >> (8u * k)) & 255u, b2 = (w2 >> (8u * k)) & 255u,
+                   b3 = (w3 >> (8u * k)) & 255u;
+    float2 n;
+    if (((valid >> k) & 1u) && db_nudge(b0, b1, b2, b3, beta, tab, &n)) {
+      o0 = round_u8_into((float)b0 + n.y, k, o0);
+      o1 = round_u8_into((float)b1 + n.x, k, o1);
+      o2 = round_u8_into((float)b2 - n.x, k, o2);
+      o3 = round_u8_into((float)b3 - n.y, k, o3);
+    }
   }
   w0 = o0;
   w1 = o1;
@@ -253,10 +282,23 @@ __device__ __forceinline__ void db_sample4(uint32_t &w0, uint32_t &w1, uint32_t 
   w3 = o3;
 }
 
-constexpr uint32_t DBT_W = 128u;  // tile width in x
+// Tile width in x. 64-wide tiles (20 KiB of LDS, 8 workgroups per CU) read
+// faster but write slower, two workgroups dirtying each 128-byte line:
+// 0.58 ms against 0.49 on a 1024^3 region with 102 M changed voxels.
+constexpr uint32_t DBT_W = 128u;
 
+// Tile width in x: TW / 16 lanes share a row, each row of the tile's 256 (16 y
+// x 16 z) is held by one lane group.
+template <uint32_t TW>
 __global__ __launch_bounds__(256) void k_db_tiles(DbTile a) {
-  __shared__ __attribute__((aligned(16))) uint32_t s_t[16 * 16 * DBT_W / 4];  // [lz][ly][lx / 4]
+  constexpr uint32_t LPR = TW / 16u;    // lanes per row
+  constexpr uint32_t RPI = 256u / LPR;  // rows per pass of the workgroup
+  constexpr uint32_t NQ = 256u / RPI;   // rows per lane
+  constexpr uint32_t RW = TW / 4u;      // dwords per row
+  constexpr uint32_t NC = RW * 16u / 256u;  // y or z seam dword columns per lane
+  static_assert(TW % 64u == 0u && TW <= 256u, "whole rows per lane group and seam columns per lane");
+  __shared__ __attribute__((aligned(16))) uint32_t s_t[16 * 16 * RW];  // [lz][ly][lx / 4]
+  __shared__ float2 s_nudge[DB_NSTEP];
   const DbLim l = a.flim ? *a.flim : a.lim;
   if (!l.on) return;
   // XCD bands: a row's x tiles run on one XCD one after another, so the line
@@ -264,19 +306,19 @@ __global__ __launch_bounds__(256) void k_db_tiles(DbTile a) {
   // in blockIdx order it was fetched again, doubling the reads
   const uint32_t b = xcd_band(blockIdx.x, gridDim.x);
   const uint32_t tm = b % a.tx, tj = (b / a.tx) % a.ty, tk = b / (a.tx * a.ty);
-  const uint32_t t = threadIdx.x, c = t & 7u, rs = t >> 3;  // 16-byte column, first row
-  const uint32_t xb = DBT_W * tm;  // the tile starts at x = xb - 2
+  const uint32_t t = threadIdx.x, c = t % LPR, rs = t / LPR;  // 16-byte column, first row
+  const uint32_t xb = TW * tm;  // the tile starts at x = xb - 2
   const int64_t y0 = 16 * (int64_t)tj - 2, z0 = 16 * (int64_t)tk - 2;
   const uint32_t xc = xb + 16u * c;  // x of this lane's load
   const bool col_in = xc < a.nx, has_lo = c == 0u && tm > 0u;
-  // rows r = rs + 32q of the tile (r = ly + 16 lz)
-  uint4 W[8];
-  uint32_t lo[8];
-  bool rin[8];
-  uint8_t *rowp[8];
+  // rows r = rs + RPI q of the tile (r = ly + 16 lz)
+  uint4 W[NQ];
+  uint32_t lo[NQ];
+  bool rin[NQ];
+  uint8_t *rowp[NQ];
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const uint32_t r = rs + 32u * (uint32_t)q;
+  for (uint32_t q = 0; q < NQ; q++) {
+    const uint32_t r = rs + RPI * q;
     const int64_t gy = y0 + (r & 15u), gz = z0 + (r >> 4);
     rin[q] = gy >= 0 && gy < a.ny && gz >= 0 && gz < a.nz;
     rowp[q] = a.vol + ((uint64_t)(rin[q] ? gz : 0) * a.ny + (uint64_t)(rin[q] ? gy : 0)) * a.nx;
@@ -285,29 +327,27 @@ __global__ __launch_bounds__(256) void k_db_tiles(DbTile a) {
     if (rin[q] && col_in) W[q] = *(const uint4 *)(rowp[q] + xc);
     if (rin[q] && has_lo) lo[q] = *(const uint32_t *)(rowp[q] + xb - 4u);
   }
+  db_fill_nudges(s_nudge, t, 256u, l.alpha);
+  __syncthreads();
   // this lane's x seam is xc (xc + 1 < nx follows from col_in: nx is a multiple of 16)
   const bool seam_x = col_in && xc >= 16u;
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const uint32_t up = __shfl_up(W[q].w, 1u, 8);
+  for (uint32_t q = 0; q < NQ; q++) {
+    const uint32_t up = __shfl_up(W[q].w, 1u, LPR);
     const uint32_t pw = c == 0u ? lo[q] : up;
     uint32_t r0 = (pw >> 16) | (W[q].x << 16);
     const uint32_t r1 = (W[q].x >> 16) | (W[q].y << 16), r2 = (W[q].y >> 16) | (W[q].z << 16),
                    r3 = (W[q].z >> 16) | (W[q].w << 16);
-    if (rin[q] && seam_x) {
-      uint32_t b0 = r0 & 255u, b1 = (r0 >> 8) & 255u, b2 = (r0 >> 16) & 255u, b3 = r0 >> 24;
-      db_sample(b0, b1, b2, b3, l.alpha, l.beta);
-      r0 = b0 | b1 << 8 | b2 << 16 | b3 << 24;
-    }
-    const uint32_t r = rs + 32u * (uint32_t)q;
-    *(uint4 *)(s_t + r * (DBT_W / 4u) + 4u * c) = make_uint4(r0, r1, r2, r3);
+    if (rin[q] && seam_x) r0 = db_sample_dword(r0, l.beta, s_nudge);
+    const uint32_t r = rs + RPI * q;
+    *(uint4 *)(s_t + r * RW + 4u * c) = make_uint4(r0, r1, r2, r3);
   }
   __syncthreads();
-  // y seam at 16j: dword columns (x4, lz), 2 per lane
+  // y seam at 16j: dword columns (x4, lz)
   if (tj > 0u && 16u * tj + 1u < a.ny) {
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-      const uint32_t it = t + 256u * (uint32_t)q, x4 = it & 31u, cz = it >> 5;
+    for (uint32_t q = 0; q < NC; q++) {
+      const uint32_t it = t + 256u * q, x4 = it % RW, cz = it / RW;
       const int64_t gzc = z0 + cz;
       if (gzc >= 0 && gzc < a.nz) {
         uint32_t valid = 0;
@@ -316,22 +356,22 @@ __global__ __launch_bounds__(256) void k_db_tiles(DbTile a) {
           const int64_t gx = (int64_t)xb - 2 + 4 * x4 + k;
           valid |= (gx >= 0 && gx < a.nx ? 1u : 0u) << k;
         }
-        uint32_t *cc = s_t + cz * 16u * (DBT_W / 4u) + x4;
-        uint32_t w0 = cc[0], w1 = cc[DBT_W / 4u], w2 = cc[2u * (DBT_W / 4u)], w3 = cc[3u * (DBT_W / 4u)];
-        db_sample4(w0, w1, w2, w3, valid, l.alpha, l.beta);
+        uint32_t *cc = s_t + cz * 16u * RW + x4;
+        uint32_t w0 = cc[0], w1 = cc[RW], w2 = cc[2u * RW], w3 = cc[3u * RW];
+        db_sample4(w0, w1, w2, w3, valid, l.beta, s_nudge);
         cc[0] = w0;
-        cc[DBT_W / 4u] = w1;
-        cc[2u * (DBT_W / 4u)] = w2;
-        cc[3u * (DBT_W / 4u)] = w3;
+        cc[RW] = w1;
+        cc[2u * RW] = w2;
+        cc[3u * RW] = w3;
       }
     }
   }
   __syncthreads();
-  // z seam at 16k: dword columns (x4, ly), 2 per lane
+  // z seam at 16k: dword columns (x4, ly)
   if (tk > 0u && 16u * tk + 1u < a.nz) {
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-      const uint32_t it = t + 256u * (uint32_t)q, x4 = it & 31u, cy = it >> 5;
+    for (uint32_t q = 0; q < NC; q++) {
+      const uint32_t it = t + 256u * q, x4 = it % RW, cy = it / RW;
       const int64_t gyc = y0 + cy;
       if (gyc >= 0 && gyc < a.ny) {
         uint32_t valid = 0;
@@ -340,10 +380,10 @@ __global__ __launch_bounds__(256) void k_db_tiles(DbTile a) {
           const int64_t gx = (int64_t)xb - 2 + 4 * x4 + k;
           valid |= (gx >= 0 && gx < a.nx ? 1u : 0u) << k;
         }
-        uint32_t *cc = s_t + cy * (DBT_W / 4u) + x4;
-        constexpr uint32_t PL = 16u * (DBT_W / 4u);
+        uint32_t *cc = s_t + cy * RW + x4;
+        constexpr uint32_t PL = 16u * RW;
         uint32_t w0 = cc[0], w1 = cc[PL], w2 = cc[2u * PL], w3 = cc[3u * PL];
-        db_sample4(w0, w1, w2, w3, valid, l.alpha, l.beta);
+        db_sample4(w0, w1, w2, w3, valid, l.beta, s_nudge);
         cc[0] = w0;
         cc[PL] = w1;
         cc[2u * PL] = w2;
@@ -353,21 +393,21 @@ __global__ __launch_bounds__(256) void k_db_tiles(DbTile a) {
   }
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const uint32_t r = rs + 32u * (uint32_t)q;
-    const uint4 f = *(const uint4 *)(s_t + r * (DBT_W / 4u) + 4u * c);
-    const uint32_t nf = __shfl_down(f.x, 1u, 8);  // tile bytes 16c + 16, 16c + 17 (c < 7)
+  for (uint32_t q = 0; q < NQ; q++) {
+    const uint32_t r = rs + RPI * q;
+    const uint4 f = *(const uint4 *)(s_t + r * RW + 4u * c);
+    const uint32_t nf = __shfl_down(f.x, 1u, LPR);  // tile bytes 16c + 16, 16c + 17 (c < LPR - 1)
     if (!rin[q]) continue;
     uint8_t *row = rowp[q];
     // x = xb - 2, xb - 1: tile bytes 0, 1 (the previous tile owns xb - 4, xb - 3)
     if (has_lo && (f.x & 0xFFFFu) != (lo[q] >> 16)) *(uint16_t *)(row + xb - 2u) = (uint16_t)(f.x & 0xFFFFu);
     if (!col_in) continue;
-    // x = xc .. xc + 15: tile bytes 16c + 2 .. 16c + 17; for c = 7 the last two
-    // (x = xb + 126, 127) belong to the next tile
+    // x = xc .. xc + 15: tile bytes 16c + 2 .. 16c + 17; for the last column
+    // the last two (x = xb + TW - 2, TW - 1) belong to the next tile
     const uint32_t n0 = (f.x >> 16) | (f.y << 16), n1 = (f.y >> 16) | (f.z << 16), n2 = (f.z >> 16) | (f.w << 16),
                    n3 = (f.w >> 16) | (nf << 16);
     uint32_t *d = (uint32_t *)(row + xc);
-    if (c < 7u) {  // all 16 bytes are this tile's: one store when any changed
+    if (c + 1u < LPR) {  // all 16 bytes are this tile's: one store when any changed
       if (n0 != W[q].x || n1 != W[q].y || n2 != W[q].z || n3 != W[q].w) *(uint4 *)d = make_uint4(n0, n1, n2, n3);
     } else {
       if (n0 != W[q].x) d[0] = n0;
@@ -504,7 +544,7 @@ extern "C" int r3d_hip_deblock_u8(r3d_hip_ctx *ctx, uint8_t *vol, const uint64_t
     tl.flim = p.flim;
     const uint64_t tiles = (uint64_t)tl.tx * tl.ty * tl.tz;
     if (tiles <= 0x7FFFFFFFull) {
-      hipLaunchKernelGGL(k_db_tiles, dim3((uint32_t)tiles), dim3(256), 0, st, tl);
+      hipLaunchKernelGGL(k_db_tiles<DBT_W>, dim3((uint32_t)tiles), dim3(256), 0, st, tl);
       R3D_HIP_CHECK(ctx, hipGetLastError());
       return R3D_OK;
     }

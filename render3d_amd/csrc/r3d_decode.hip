@@ -120,14 +120,6 @@ __device__ __forceinline__ uint32_t round_u8(float v) {
   return (uint32_t)__builtin_fminf(s, 255.0f);
 }
 
-// round_u8 of v written into byte b of w: v_cvt_pk_u8_f32 rounds to nearest
-// even and saturates to [0, 255] (NaN to 0; checked on this GPU over 4.2 M
-// inputs), so on floor(v + 0.49999997), an integer, it is exactly the clamp
-// and truncation above, and it inserts the byte in the same instruction.
-__device__ __forceinline__ uint32_t round_u8_into(float v, uint32_t b, uint32_t w) {
-  return __builtin_amdgcn_cvt_pk_u8_f32(__builtin_floorf(v + 0.49999997f), b, w);
-}
-
 // The basis has 15 distinct magnitudes: cm[k][n] = +-M[j] with
 // j = |((2n+1)k mod 64)| folded to [0,16) (j = 0 is c_0 = 1/4, equal to
 // M[8]) and the sign negative for (2n+1)k mod 64 in (16, 48). The host

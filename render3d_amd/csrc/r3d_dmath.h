@@ -39,6 +39,19 @@ __device__ __forceinline__ float fclamp_ord(float x, float lo, float hi) {
   return __builtin_amdgcn_fmed3f(x, lo, hi);
 }
 __device__ __forceinline__ float fsat(float x) { return fclamp_ord(x, 0.0f, 1.0f); }
+// The u8 store of clamp(v, 0, 255) rounded half away from zero (roundf),
+// written into byte b of w. With c = 0.49999997f (the float below 0.5),
+// floor(v + c), the sum rounded to nearest even, is roundf(v) for every
+// v >= 0 (a tie k + 0.5 sums to within half an ulp of k + 1 and rounds up to
+// it; c < 0.5 keeps 0.5 - 2^-25 below 1), and at most 0 for v < 0.
+// v_cvt_pk_u8_f32 rounds to nearest even and saturates to [0, 255] (NaN to
+// 0; checked on this GPU over 4.2 M inputs), so on that integer it is exactly
+// the clamp, and it inserts the byte in the same instruction. Identical to
+// (uint8_t)roundf(clamp(v, 0, 255)) for every float (checked exhaustively
+// over [-1, 65536]; beyond, both saturate).
+__device__ __forceinline__ uint32_t round_u8_into(float v, uint32_t b, uint32_t w) {
+  return __builtin_amdgcn_cvt_pk_u8_f32(__builtin_floorf(v + 0.49999997f), b, w);
+}
 // lerp pinned as fma(t, b - a, a)
 __device__ __forceinline__ float flerp(float a, float b, float t) {
   return __builtin_fmaf(t, b - a, a);

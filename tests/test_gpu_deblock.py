@@ -83,6 +83,32 @@ def test_no_op_cases(ctx):
     assert torch.equal(small.cpu(), torch.from_numpy(D["a"][:17, :17, :17].copy()))
 
 
+def test_every_level_pair_across_a_seam(ctx):
+    """Every pair (p0, q0) of u8 values across an x seam, with flat sides
+    (p1 = p0, q1 = q0), at step 127.5 (alpha 255, beta 153), so every step
+    |q0 - p0| <= 254 is a seam: all 511 entries of the tile kernel's nudge
+    table and the byte rounding of every p0 + d0, plus the y and z seams the
+    pattern makes, against the oracle. The same bytes at an odd address take
+    the per-pass kernels."""
+    z, y = np.meshgrid(np.arange(256), np.arange(256), indexing="ij")
+    base = np.empty((256, 256, 32), np.uint8)
+    base[:, :, :16] = y[:, :, None]
+    base[:, :, 16:] = z[:, :, None]
+    want = orc.deblock(base, 127.5, 1.0)
+    seam = want[:, :, 14:18].astype(np.int32)
+    assert (seam[:, :, 1] != y).sum() > 60000  # nearly every pair was nudged
+    g = torch.from_numpy(base.copy()).cuda()
+    assert g.data_ptr() % 16 == 0
+    ctx.deblock(g, 127.5, 1.0)
+    buf = torch.zeros(base.size + 16, dtype=torch.uint8, device="cuda")
+    odd = buf[1:1 + base.size].view(base.shape)
+    odd.copy_(torch.from_numpy(base))
+    ctx.deblock(odd, 127.5, 1.0)
+    torch.cuda.synchronize()
+    assert np.array_equal(g.cpu().numpy(), want)
+    assert np.array_equal(odd.cpu().numpy(), want)
+
+
 def test_both_kernel_forms_against_the_oracle(ctx):
     """A row length that is a multiple of 128 (the tile pass's last tile holds
     only the two voxels before the end of each row) on a 16-byte aligned

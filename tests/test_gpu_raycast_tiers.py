@@ -1,8 +1,9 @@
 """Lane-sharded rays (k_raycast_tiers): a launch of at least 4,096 8x8 tiles
-renders its costliest tiles with four and two lanes per ray. Pixels, counters
-and per-pixel sample counts must equal the oracle's and the one-lane-per-ray
-kernel's (another tile shape takes k_raycast), in every mode the tiers render
-and with and without empty-space skips."""
+renders its costliest FULL/FAST tiles with a lane pair per ray and the rest
+with one lane per ray. Pixels, counters and per-pixel sample counts must
+equal the oracle's and the one-lane-per-ray kernel's (another tile shape
+takes k_raycast), in every mode the tiers render and with and without
+empty-space skips."""
 import math
 
 import numpy as np
