@@ -105,6 +105,7 @@ struct r3d_hip_volume {
   uint32_t odim[3] = {};
   uint8_t *d_occ = nullptr;
   uint8_t *d_occ_raw = nullptr;
+  bool wide = false;  // 64-bit brick offsets whatever the pyramid's size (R3D_WIDE_OFFSETS=1 at create)
 };
 
 struct r3d_hip_bricks {

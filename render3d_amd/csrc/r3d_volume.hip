@@ -20,6 +20,7 @@
 // Occupancy: max over each 8^3 block of level 0, then a 3^3 max dilation
 // (src/vk/vkbackend.c:794-870; occmax.comp, occdilate.comp).
 #include <new>
+#include <stdlib.h>
 #include <string.h>
 
 #include "r3d_dmath.h"
@@ -562,7 +563,7 @@ VolDev vol_dev(const r3d_hip_volume *v) {
   d.ox = v->odim[0];
   d.oy = v->odim[1];
   d.oz = v->odim[2];
-  d.narrow = v->bytes <= ((uint64_t)1 << 32) ? 1u : 0u;
+  d.narrow = !v->wide && v->bytes <= ((uint64_t)1 << 32) ? 1u : 0u;
   return d;
 }
 
@@ -591,6 +592,11 @@ extern "C" int r3d_hip_volume_create(r3d_hip_ctx *ctx, const r3d_volume_desc *de
   v->ctx = ctx;
   v->desc = *desc;
   v->nlev = nlevels;
+  // Kernels address the pyramid with 32-bit offsets when it fits 4 GiB;
+  // R3D_WIDE_OFFSETS=1 makes this volume take the 64-bit instances a larger
+  // pyramid takes (the same pixels), so they can be checked on small volumes.
+  const char *wide = getenv("R3D_WIDE_OFFSETS");
+  v->wide = wide && wide[0] == '1';
   uint64_t off = 0;
   LevelDev tab[R3D_MAX_LEVELS];
   memset(tab, 0, sizeof tab);
