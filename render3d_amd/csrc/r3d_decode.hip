@@ -1018,7 +1018,13 @@ static int decode_impl(r3d_hip_ctx *ctx, const r3d_chunk_batch *b, const r3d_dec
   }
   hipStream_t st = (hipStream_t)stream;
   if (variant == R3D_DECODE_EXACT_F32) {
-    const uint32_t grid = grid_for(ctx, b->nchunks, 1u << 16);
+    // At most 56 workgroups per CU (8 rounds of the 7 resident), each walking
+    // its chunks with a grid stride: the 1024^3 batch (262,144 chunks)
+    // 1.35-1.37 -> 1.26-1.28 ms and the 32,768-chunk sparse batch 0.145 ->
+    // 0.141 ms against one workgroup per chunk (same bytes). 7 per CU, exactly
+    // the resident count, was slower (1.43 ms): with shares that large the
+    // CUs do not finish together.
+    const uint32_t grid = grid_for(ctx, b->nchunks, 56u);
     const bool sp = b->coef_format == R3D_COEF_SPARSE;
     if (f32out && sp) hipLaunchKernelGGL((k_decode_exact<true, true>), dim3(grid), dim3(256), 0, st, a);
     else if (f32out) hipLaunchKernelGGL((k_decode_exact<true, false>), dim3(grid), dim3(256), 0, st, a);
