@@ -13,7 +13,8 @@
 //                                                          dct3d.c:568-591
 //   u8 = clamp(roundf(v), 0, 255), NaN -> 0                dct3d.c:625-635
 //
-// Layout: one 256-lane workgroup per chunk (grid-strided), lane t owns line t
+// Layout: one 256-lane workgroup per chunk at a time (at most 56 workgroups
+// per CU walk the chunks with a grid stride), lane t owns line t
 // of the current pass in registers; the rotate between passes is a
 // transposing write into a 16 KiB LDS tile (lane t writes element c of its
 // line to tile[c*256 + t], so the next pass's line t' = tile[t'*16 .. +15]).
